@@ -1,0 +1,224 @@
+// zone_dev.hpp — DBM zone algebra for gfx950, one zone per wavefront, staged in LDS.
+//
+// Each function is called by all 64 lanes of a one-wavefront workgroup with the zone in LDS
+// (`__shared__`); lanes own entries e = lane, lane+64, ... of the d*d row-major matrix. The
+// results are the canonical DBMs HDBM's operations produce (raw encoding and semantics as
+// restated in src/metal/netengine_metal.mm:76-335 and src/metal/engine_metal.mm:80-93):
+// a non-empty zone has exactly one canonical form, so a data-parallel schedule that reaches it
+// is bit-identical to the sequential one.
+//
+// Every function begins with a workgroup barrier, so callers may write the zone just before.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hh {
+
+constexpr int kInf = 1 << 30;
+constexpr int kLe0 = 1;
+constexpr int kAluNegInf = -(1 << 29);
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int r_le(int v) { return v * 2 + 1; }
+__device__ __forceinline__ int r_lt(int v) { return v * 2; }
+__device__ __forceinline__ int r_bnd(int r) { return r >> 1; }
+__device__ __forceinline__ bool r_strict(int r) { return (r & 1) == 0; }
+// Saturating bound addition: infinity absorbs; the sum is weak iff both operands are weak.
+__device__ __forceinline__ int radd(int a, int b) {
+  if (a >= kInf || b >= kInf) return kInf;
+  const int v = (a >> 1) + (b >> 1);
+  return v * 2 + (a & b & 1);
+}
+// The raw bound of the complement of x_i - x_j <| v, stated as a bound on x_j - x_i.
+__device__ __forceinline__ int r_neg(int r) {
+  const int v = r >> 1;
+  return r_strict(r) ? r_le(-v) : r_lt(-v);
+}
+
+__device__ __forceinline__ void wsync() { __syncthreads(); }
+
+__device__ __forceinline__ void z_copy(int* __restrict__ dst, const int* __restrict__ src, int d2) {
+  for (int e = threadIdx.x; e < d2; e += kWave) dst[e] = src[e];
+}
+
+__device__ __forceinline__ bool z_empty(const int* z, int d) {
+  wsync();
+  const int l = threadIdx.x;
+  const bool neg = l < d && z[l * d + l] < kLe0;
+  return __any(neg);
+}
+
+// Delay: every upper bound x_i - 0 dropped.
+__device__ __forceinline__ void z_up(int* z, int d) {
+  wsync();
+  const int l = threadIdx.x;
+  if (l >= 1 && l < d) z[l * d] = kInf;
+}
+
+// x_k := 0: row k takes row 0, column k takes column 0, z[k][k] = (<=, 0).
+__device__ __forceinline__ void z_reset(int* z, int d, int k) {
+  wsync();
+  const int l = threadIdx.x;
+  int r0 = 0, c0 = 0;
+  if (l < d) { r0 = z[l]; c0 = z[l * d]; }
+  wsync();
+  if (l < d) {
+    z[k * d + l] = (l == k) ? kLe0 : r0;
+    z[l * d + k] = (l == k) ? kLe0 : c0;
+  }
+}
+
+// Tighten x_a - x_b to r and re-close incrementally. Returns false iff the result is empty.
+// The input must be canonical and non-empty. The new system is empty exactly when the cycle
+// a -> b -> a closes negative, z[b][a] + r < (<=, 0); otherwise the closure of one added edge is
+// z[i][k] = min(z[i][k], z[i][b] + z[b][k]) after the column z[i][b] = min(z[i][b], z[i][a] + r).
+// On an empty result the zone's contents are left unspecified; callers discard it.
+__device__ __forceinline__ bool z_constrain(int* z, int d, int a, int b, int r) {
+  wsync();
+  const int zab = z[a * d + b];
+  if (r >= zab) return true;
+  if (radd(z[b * d + a], r) < kLe0) return false;
+  const int l = threadIdx.x;
+  int col = 0;
+  if (l < d) col = radd(z[l * d + a], r);
+  wsync();
+  if (l < d && col < z[l * d + b]) z[l * d + b] = col;
+  wsync();
+  const int d2 = d * d;
+  for (int e = l; e < d2; e += kWave) {
+    const int i = e / d, k = e - i * d;
+    const int ib = z[i * d + b];
+    if (ib >= kInf) continue;
+    const int v = radd(ib, z[b * d + k]);
+    if (v < z[e]) z[e] = v;
+  }
+  return true;
+}
+
+// Floyd-Warshall closure, one pivot per step. Within a step, row k and column k do not change on
+// a zone without negative cycles, so the parallel update equals the sequential one.
+__device__ __forceinline__ void z_close(int* z, int d) {
+  const int d2 = d * d, l = threadIdx.x;
+  for (int k = 0; k < d; ++k) {
+    wsync();
+    for (int e = l; e < d2; e += kWave) {
+      const int i = e / d, j = e - i * d;
+      const int ik = z[i * d + k];
+      if (ik >= kInf) continue;
+      const int v = radd(ik, z[k * d + j]);
+      if (v < z[e]) z[e] = v;
+    }
+  }
+}
+
+// Extra_LU+ (Behrmann, Bouyer, Larsen, Pelanek, STTT 2006) with row 0 read before any change,
+// then closure. `row0` is d ints of LDS scratch.
+__device__ __forceinline__ void z_extra_lu_plus(int* z, int d, const int* L, const int* U, int* row0) {
+  wsync();
+  const int l = threadIdx.x, d2 = d * d;
+  if (l < d) row0[l] = z[l];
+  wsync();
+  for (int e = l; e < d2; e += kWave) {
+    const int i = e / d, j = e - i * d;
+    if (i == j) continue;
+    const int v = z[e];
+    if (v >= kInf) continue;
+    int nv = v;
+    if (r_bnd(v) > L[i]) nv = kInf;
+    else if (-r_bnd(row0[i]) > L[i]) nv = kInf;
+    else if (i != 0 && -r_bnd(row0[j]) > U[j]) nv = kInf;
+    else if (v < r_lt(-U[j])) nv = r_lt(-U[j]);
+    z[e] = nv;
+  }
+  z_close(z, d);
+}
+
+// Z <=_aLU Zp (Herbreteau, Srivathsan, Walukiewicz, LICS 2012): Z is not simulated iff some pair
+// x != y has U_x, L_y finite, Z[0][x] >= (<=,-U_x), Zp[y][x] < Z[y][x] and
+// Zp[y][x] + (<,-L_y) < Z[0][x]. Lanes take the pairs (y, x) = entries of the matrix, so both
+// operands are read with unit stride; either operand may be in LDS or global memory.
+__device__ __forceinline__ bool z_alu_le(const int* Z, const int* Zp, int d, const int* L, const int* U) {
+  wsync();
+  const int l = threadIdx.x, d2 = d * d;
+  bool wit = false;
+  for (int e = l; e < d2; e += kWave) {
+    const int y = e / d, x = e - y * d;
+    if (x == y) continue;
+    const int ux = U[x], ly = L[y];
+    if (ux == kAluNegInf || ly == kAluNegInf) continue;
+    const int z0x = Z[x];
+    if (z0x < r_le(-ux)) continue;
+    const int zp = Zp[e];
+    if (zp < Z[e] && radd(zp, r_lt(-ly)) < z0x) wit = true;
+  }
+  return !__any(wit);
+}
+
+// Entrywise b <= a.
+__device__ __forceinline__ bool z_includes(const int* a, const int* b, int d) {
+  wsync();
+  const int l = threadIdx.x, d2 = d * d;
+  bool bad = false;
+  for (int e = l; e < d2; e += kWave)
+    if (b[e] > a[e]) bad = true;
+  return !__any(bad);
+}
+
+// Is a U b convex? If so a := hull (entrywise max) and return true.
+//
+// H \ a is the union over facets (i, j) of a with a_ij < H_ij (i.e. a_ij < b_ij) of
+// R = H /\ (x_j - x_i <| neg a_ij); R is never empty for such a facet, and its canonical form is
+// R[p][q] = min(H_pq, H_pj + neg + H_iq). R is inside b iff that is <= b_pq for every entry,
+// which holds trivially where H_pq = b_pq; so only the entries with a_pq > b_pq need the test.
+// Symmetrically for H \ b. With FA = {a < b} and PA = {a > b} the whole test is the pairs
+// FA x PA twice, checked here across lanes. `fa`/`pa` are d*d ints of LDS scratch each;
+// `cnt` is 2 ints of LDS scratch. Both a and b must be canonical, non-empty and in LDS.
+__device__ __forceinline__ bool z_try_union(int* a, const int* b, int d, int* fa, int* pa, int* cnt) {
+  wsync();
+  const int l = threadIdx.x, d2 = d * d;
+  if (l == 0) { cnt[0] = 0; cnt[1] = 0; }
+  wsync();
+  // ballot compaction of the differing entries, in index order
+  int nf = 0, np = 0;
+  for (int base = 0; base < d2; base += kWave) {
+    const int e = base + l;
+    const int av = e < d2 ? a[e] : 0, bv = e < d2 ? b[e] : 0;
+    const bool f = e < d2 && av < bv, p = e < d2 && av > bv;
+    const uint64_t mf = __ballot(f), mp = __ballot(p);
+    const uint64_t below = (l == 0) ? 0ull : (~0ull >> (64 - l));
+    if (f) fa[nf + __popcll(mf & below)] = e;
+    if (p) pa[np + __popcll(mp & below)] = e;
+    nf += __popcll(mf);
+    np += __popcll(mp);
+  }
+  wsync();
+  bool bad = false;
+  if (nf > 0 && np > 0) {
+    const int pairs = nf * np;
+    for (int t = l; t < pairs && !bad; t += kWave) {
+      // pass 1: facet of a from FA, entry from PA, compared against b
+      const int f = fa[t / np], e = pa[t % np];
+      const int i = f / d, j = f - i * d, p = e / d, q = e - p * d;
+      const int neg = r_neg(a[f]);
+      const int hpj = max(a[p * d + j], b[p * d + j]);
+      const int hiq = max(a[i * d + q], b[i * d + q]);
+      if (radd(radd(hpj, neg), hiq) > b[e]) bad = true;
+    }
+    for (int t = l; t < pairs && !bad; t += kWave) {
+      // pass 2: facet of b from PA, entry from FA, compared against a
+      const int f = pa[t / nf], e = fa[t % nf];
+      const int i = f / d, j = f - i * d, p = e / d, q = e - p * d;
+      const int neg = r_neg(b[f]);
+      const int hpj = max(a[p * d + j], b[p * d + j]);
+      const int hiq = max(a[i * d + q], b[i * d + q]);
+      if (radd(radd(hpj, neg), hiq) > a[e]) bad = true;
+    }
+  }
+  if (__any(bad)) return false;
+  wsync();
+  for (int e = l; e < d2; e += kWave) a[e] = max(a[e], b[e]);
+  return true;
+}
+
+}  // namespace hh

@@ -1,0 +1,68 @@
+"""The device search against the oracle's sequential engine: verdict and every counter equal.
+
+fischer: the reference's family (recorded counts are the golden pins); fischer_variant: the
+plain-reachability variant, order-sensitive from N = 7 (the parallel CPU engine's counts differ
+there, the sequential and device ones must not); token_ring: binary synchronisation.
+"""
+import json
+import os
+
+import pytest
+
+from oracle import binding
+from ptctcl_amd import hip, models
+from ptctcl_amd.network import Target
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reach_counts.json")))
+KEYS = ("explored", "generated", "stored_live", "keys", "inserted", "subsumed", "killed", "unions",
+        "chain_reads", "levels", "found", "emode_switch_level")
+
+
+def check(net, target):
+    r_o, st_o = binding.reach(net, target, 0)
+    with hip.HipContext(net, target) as ctx:
+        r_h, st_h = ctx.reach()
+    print(net.name, r_o, {k: (st_o[k], getattr(st_h, k)) for k in KEYS}, flush=True)
+    assert r_h == r_o
+    for k in KEYS:
+        assert getattr(st_h, k) == st_o[k], k
+    return r_h, st_h
+
+
+@pytest.mark.parametrize("N", [4, 5, 6, 7, 8])
+def test_fischer_counts(N):
+    net = models.fischer(N)
+    r, st = check(net, models.mutex_target(net))
+    assert not r and st.explored == GOLD["fischer_reach"][str(N)]["states"]
+
+
+@pytest.mark.parametrize("N", [4, 5, 6, 7])
+def test_variant_order_sensitive(N):
+    net = models.fischer_variant(N, k=N)
+    check(net, models.mutex_target(net))
+
+
+def test_variant_config0():
+    net = models.fischer_variant(4, k=2)
+    r, st = check(net, models.mutex_target(net))
+    assert not r
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_token_ring(seed):
+    net = models.token_ring(6, seed)
+    check(net, None)
+
+
+def test_reachable_target_early_stop():
+    net = models.fischer(6)
+    r, st = check(net, Target(loc_req=[net.resolve("P2", "cs")]))
+    assert r and st.found == 1
+
+
+def test_clock_target():
+    net = models.fischer(5)
+    # P0 in wait with x0 > 3 (x0 is clock 1)
+    t = Target(loc_req=[net.resolve("P0", "wait")], clock_cond=[(0, 1, -3 * 2)])
+    check(net, t)
