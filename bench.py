@@ -101,22 +101,22 @@ def main() -> int:
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     st = None
-    prof = {"expand_ms": 0.0, "merge_ms": 0.0, "other_ms": 0.0, "expand_bytes": 0.0, "merge_bytes": 0.0}
+    prof = None
     for _ in range(a.steps):
         reachable, st = ctx.reach()
         p = ctx.profile()
-        for k in prof:
-            prof[k] += p[k]
+        prof = p if prof is None else {k: prof[k] + p[k] for k in prof}
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1000.0 / a.steps
     value = st.explored / (ms / 1000.0)
 
-    # dominant kernel by device time; per-launch figures are whole-search totals over `levels`
-    launches = max(1, st.levels) * a.steps
-    kern = "k_merge" if prof["merge_ms"] >= prof["expand_ms"] else "k_expand"
-    kms = prof["merge_ms"] if kern == "k_merge" else prof["expand_ms"]
-    kbytes = prof["merge_bytes"] if kern == "k_merge" else prof["expand_bytes"]
+    # dominant kernel by device time; a launch is one BFS level's launch of that kernel
+    fams = {"k_expand": "expand", "k_probe": "probe", "k_merge": "merge"}
+    kern = max(fams, key=lambda k: prof[fams[k] + "_ms"])
+    f = fams[kern]
+    launches = max(1, int(prof[f + "_launches"]))
+    kms, kbytes = prof[f + "_ms"], prof[f + "_bytes"]
     achieved = (kbytes / launches) / ((kms / launches) / 1000.0) / 1e9 if kms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kern,
@@ -137,8 +137,7 @@ def main() -> int:
         "config": dict(cfg, explored=st.explored, generated=st.generated, verdict_A_not=not reachable,
                        levels=st.levels, stored_live=st.stored_live, keys=st.keys),
         "roofline": roofline,
-        "kernel_ms": {"expand": round(prof["expand_ms"] / a.steps, 3), "merge": round(prof["merge_ms"] / a.steps, 3),
-                      "hash_scatter": round(prof["other_ms"] / a.steps, 3)},
+        "kernel_ms_per_step": {k: round(prof[k + "_ms"] / a.steps, 3) for k in ("expand", "probe", "merge", "other")},
     }
     ctx.close()
     if not a.no_cpu:

@@ -151,11 +151,17 @@ const char* hora_hip_last_error(const hora_hip_ctx* ctx);
  * Writes 1 to *reachable if a target state is reachable, 0 otherwise. */
 int hora_hip_reach(hora_hip_ctx* ctx, int32_t* reachable, hora_reach_stats* stats);
 
-/* Per-level device timings of the last hora_hip_reach, for the roofline: kernel time of the
- * successor expansion and of the passed-list merge, in milliseconds, plus the algorithmic
- * bytes those kernels moved (int32 entries, 4 B each). Any pointer may be NULL. */
-int hora_hip_reach_profile(const hora_hip_ctx* ctx, double* expand_ms, double* merge_ms,
-                           double* other_ms, double* expand_bytes, double* merge_bytes);
+/* Device time and algorithmic HBM traffic of the last hora_hip_reach, per kernel family, summed
+ * over its BFS levels (HIP events on the context's stream). Bytes count int32 zone entries at
+ * 4 B each: expand = frontier zones read + successor zones written; probe = successor zones +
+ * the stored zones each was tested against; merge = kept successor zones + stored zones read by
+ * the folds + surviving zones copied to the arena (read and write). */
+typedef struct hora_reach_profile {
+  double expand_ms, probe_ms, merge_ms, other_ms;
+  double expand_bytes, probe_bytes, merge_bytes;
+  int64_t expand_launches, probe_launches, merge_launches;
+} hora_reach_profile;
+int hora_hip_reach_profile(const hora_hip_ctx* ctx, hora_reach_profile* out);
 
 /* ---- batched zone operations (zone-kernel microbenchmark; R1-R7, R11, R16-R18) ----
  * One call applies one operation to n zones of dimension d, stored back to back (n*d*d int32).

@@ -12,8 +12,10 @@
 //              invariant, delay unless frozen, invariant, inactive-clock reset, Extra_LU+ when on,
 //              emptiness, target test (close_succ_zone, include/hora/detail/semantics.hpp:522-568)
 //   scan       compaction of the non-empty successors, keeping (state, recipe) order
-//   k_hash     discrete key -> slot of the device-resident passed list (open addressing over a
-//              64-bit CAS), plus a ticket in that slot's per-level work list
+//   k_find     discrete key -> slot of the device-resident passed list (open addressing over a
+//              64-bit CAS)
+//   k_probe    cover pre-filter of each successor against its key's previous-level zones
+//   k_ticket   a ticket in the key's per-level work list for each successor the probe kept
 //   k_merge    one wavefront per touched discrete key: the key's new zones in (state, recipe)
 //              order, each through NetPassedT::add's folds (include/hora/detail/passed_set.hpp:
 //              287-444): a_LU cover test, retirement of covered stored zones, convex-union merge
@@ -28,6 +30,7 @@
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -49,6 +52,7 @@ constexpr int kSortLds = 1024;       // bucket work lists up to this length are 
 constexpr u32 kDead = 0x80000000u;   // tombstone bit on a chain entry
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;
+constexpr int kLightMax = 8;   // working-chain size up to which a key takes the light merge launch
 
 struct DevNet {
   int d, d2, np, ni, nclocks, lu_plus;
@@ -90,17 +94,50 @@ struct DevNet {
   const int* cc;
 };
 
+// Statistics are striped over cache lines by workgroup so that hundreds of thousands of
+// workgroups per launch do not serialise on one address; the host sums the stripes.
+constexpr int kStripes = 64;
+enum { C_SUB, C_KILL, C_UNI, C_READS, C_INS, C_KEYS, C_LIVE, C_PREADS, C_PSUB, C_NSTAT };
 struct Counters {
-  unsigned long long subsumed, killed, unions, reads, inserted, keys, live, zcount, ncount;
+  unsigned long long st[kStripes][16];
   int err;
   int first_target;
   int ntouched;
-  int pad;
+  int nlight, nheavy;
+  int pad[3];
 };
+
+__device__ __forceinline__ void stat_add(Counters* c, int k, unsigned long long v) {
+  if (v) atomicAdd(&c->st[blockIdx.x % kStripes][k], v);
+}
+
+// Wavefront-aggregated append: lanes with `pred` get consecutive indices from `*counter`, with one
+// atomic per wavefront. Every lane of the wavefront must call it.
+__device__ __forceinline__ int wave_append(int* counter, bool pred) {
+  const u64 b = __ballot(pred);
+  if (!b) return -1;
+  const int lane = threadIdx.x & 63;
+  const int leader = __builtin_ctzll(b);
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, __popcll(b));
+  base = __shfl(base, leader);
+  return pred ? base + __popcll(b & ((1ull << lane) - 1)) : -1;
+}
+
+// Wavefront-aggregated statistic: one atomic per wavefront. Every lane must call it.
+__device__ __forceinline__ void wave_stat(Counters* c, int k, unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) stat_add(c, k, v);
+}
 
 enum { ERR_NONE = 0, ERR_HASH_FULL = 1, ERR_ZONES_FULL = 2, ERR_NODES_FULL = 3 };
 
 // ---------------------------------------------------------------- discrete state helpers
+
+// A value every lane of the wavefront holds (the discrete state being expanded). Forcing it into a
+// scalar register (readfirstlane) was measured 5x slower on the expansion: the LDS read then has
+// to complete before each table lookup is issued, which serialises the loop. Left as a marker.
+__device__ __forceinline__ int uni(int v) { return v; }
 
 __device__ __forceinline__ bool atom_holds(int v, int op, int k) {
   switch (op) {
@@ -115,12 +152,12 @@ __device__ __forceinline__ bool atom_holds(int v, int op, int k) {
 
 __device__ __forceinline__ bool int_ok(const DevNet& N, int e, const int* ints) {
   for (int c = N.ig_off[e]; c < N.ig_off[e + 1]; ++c)
-    if (!atom_holds(ints[N.ig_data[3 * c]], N.ig_data[3 * c + 1], N.ig_data[3 * c + 2])) return false;
+    if (!atom_holds(uni(ints[N.ig_data[3 * c]]), N.ig_data[3 * c + 1], N.ig_data[3 * c + 2])) return false;
   return true;
 }
 
 __device__ __forceinline__ bool committed_at(const DevNet& N, int p, int l) {
-  return (N.loc_flags[N.locbase[p] + l] & 1) != 0;
+  return (N.loc_flags[N.locbase[p] + uni(l)] & 1) != 0;
 }
 
 // Lanes decode the packed key (DiscKeyCodecT::decode, engine.cpp:1018-1025) into LDS.
@@ -141,7 +178,7 @@ __device__ __forceinline__ u64 encode_key(const DevNet& N, const int* locs, cons
 __device__ bool urgent_sync_enabled(const DevNet& N, const int* locs, const int* ints) {
   if (!N.has_urgent_chan) return false;
   for (int i = 0; i < N.np; ++i) {
-    const int gi = N.locbase[i] + locs[i];
+    const int gi = N.locbase[i] + uni(locs[i]);
     for (int t = N.le_off[gi]; t < N.le_off[gi + 1]; ++t) {
       const int se = N.le_edges[t];
       const int ch = N.edge_chan[se];
@@ -150,7 +187,7 @@ __device__ bool urgent_sync_enabled(const DevNet& N, const int* locs, const int*
       if (N.chan_bcast[ch]) return true;
       for (int j = 0; j < N.np; ++j) {
         if (j == i) continue;
-        const int gj = N.locbase[j] + locs[j];
+        const int gj = N.locbase[j] + uni(locs[j]);
         for (int t2 = N.le_off[gj]; t2 < N.le_off[gj + 1]; ++t2) {
           const int re = N.le_edges[t2];
           if (N.edge_chan[re] != ch || N.edge_dir[re] != -1) continue;
@@ -165,7 +202,7 @@ __device__ bool urgent_sync_enabled(const DevNet& N, const int* locs, const int*
 // frozen_flags (semantics.hpp:160-169): bit0 committed, bit1 urgent
 __device__ int frozen_flags(const DevNet& N, const int* locs, const int* ints) {
   int f = 0;
-  for (int p = 0; p < N.np; ++p) f |= N.loc_flags[N.locbase[p] + locs[p]] & 3;
+  for (int p = 0; p < N.np; ++p) f |= N.loc_flags[N.locbase[p] + uni(locs[p])] & 3;
   if (!(f & 2) && urgent_sync_enabled(N, locs, ints)) f |= 2;
   return f;
 }
@@ -177,7 +214,7 @@ __device__ int frozen_flags(const DevNet& N, const int* locs, const int* ints) {
 template <class F>
 __device__ void enum_recipes(const DevNet& N, const int* locs, const int* ints, bool src_c, F&& fn) {
   for (int p = 0; p < N.np; ++p) {
-    const int lp = locs[p];
+    const int lp = uni(locs[p]);
     if (src_c && !committed_at(N, p, lp)) continue;
     const int g = N.locbase[p] + lp;
     for (int t = N.le_off[g]; t < N.le_off[g + 1]; ++t) {
@@ -188,7 +225,7 @@ __device__ void enum_recipes(const DevNet& N, const int* locs, const int* ints, 
     }
   }
   for (int i = 0; i < N.np; ++i) {
-    const int gi = N.locbase[i] + locs[i];
+    const int gi = N.locbase[i] + uni(locs[i]);
     const bool i_comm = committed_at(N, i, locs[i]);
     for (int t = N.le_off[gi]; t < N.le_off[gi + 1]; ++t) {
       const int se = N.le_edges[t];
@@ -197,7 +234,7 @@ __device__ void enum_recipes(const DevNet& N, const int* locs, const int* ints, 
       if (!int_ok(N, se, ints)) continue;
       for (int j = 0; j < N.np; ++j) {
         if (j == i) continue;
-        const int gj = N.locbase[j] + locs[j];
+        const int gj = N.locbase[j] + uni(locs[j]);
         for (int t2 = N.le_off[gj]; t2 < N.le_off[gj + 1]; ++t2) {
           const int re = N.le_edges[t2];
           if (N.edge_chan[re] != ch || N.edge_dir[re] != -1) continue;
@@ -213,7 +250,7 @@ __device__ void enum_recipes(const DevNet& N, const int* locs, const int* ints, 
 // Conjoin the combined invariant of `locs` (all_invariants + apply_all, semantics.hpp:82-86).
 __device__ bool apply_inv(const DevNet& N, const int* locs, int* z) {
   for (int p = 0; p < N.np; ++p) {
-    const int g = N.locbase[p] + locs[p];
+    const int g = N.locbase[p] + uni(locs[p]);
     for (int c = N.inv_off[g]; c < N.inv_off[g + 1]; ++c)
       if (!z_constrain(z, N.d, N.inv_data[3 * c], N.inv_data[3 * c + 1], N.inv_data[3 * c + 2])) return false;
   }
@@ -239,7 +276,7 @@ __device__ bool close_succ(const DevNet& N, const int* locs, const int* ints, in
     apply_inv(N, locs, z);
   }
   u64 a = N.qbits;
-  for (int p = 0; p < N.np; ++p) a |= N.amask[N.locbase[p] + locs[p]];
+  for (int p = 0; p < N.np; ++p) a |= N.amask[N.locbase[p] + uni(locs[p])];
   u64 dead = ~a & (((1ull << N.nclocks) - 1) << 1);
   while (dead) {
     const int c = __builtin_ctzll(dead);
@@ -449,12 +486,11 @@ struct Passed {
   u64* key;        // [H] packed discrete key or kEmptyKey
   u64* off;        // [H] chain segment offset into nodes
   u32* len;        // [H] live chain length
-  u32* ncnt;       // [H] successors landing here this level
+  u32* ncnt;       // [H] successors queued here this level
   u64* noff;       // [H] offset of this level's work list
+  u64* nseg;       // [H] chain segment reserved for this level
   u32* nodes;      // chain segments (zone ids, oldest first)
   int* zarena;     // stored zones
-  int* zlevel;     // [zones] BFS level a zone was stored at
-  int* zsrc;       // [zones] compacted successor index that stored it
   u64 hmask;
   int hshift;
   u64 zcap, ncap;
@@ -478,20 +514,107 @@ __device__ __forceinline__ long find_or_insert(Passed P, u64 dk, bool* inserted)
   return -1;
 }
 
-__global__ void k_hash(Passed P, const int* __restrict__ sj, const u64* __restrict__ sdk, long G,
-                       long* __restrict__ gbk, u32* __restrict__ gticket, u32* __restrict__ touched,
-                       Counters* ctr) {
+// successor -> passed-list slot of its discrete key (inserting new keys)
+__global__ void k_find(Passed P, const int* __restrict__ sj, const u64* __restrict__ sdk, long G,
+                       long* __restrict__ gbk, Counters* ctr) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= G) return;
-  const u64 dk = sdk[sj[g]];
   bool ins = false;
-  const long h = find_or_insert(P, dk, &ins);
-  if (h < 0) { atomicExch(&ctr->err, ERR_HASH_FULL); gbk[g] = -1; return; }
-  if (ins) atomicAdd(&ctr->keys, 1ull);
-  gbk[g] = h;
-  const u32 t = atomicAdd(&P.ncnt[h], 1u);
-  gticket[g] = t;
-  if (t == 0) touched[atomicAdd(&ctr->ntouched, 1)] = (u32)h;
+  if (g < G) {
+    const long h = find_or_insert(P, sdk[sj[g]], &ins);
+    if (h < 0) atomicExch(&ctr->err, ERR_HASH_FULL);
+    gbk[g] = h;
+  }
+  wave_stat(ctr, C_KEYS, ins ? 1ull : 0ull);
+}
+
+// Cover pre-filter. A successor covered (a_LU) by a zone its key held before this level is
+// dropped by the sequential add() whatever else the level does: a stored zone is only retired by
+// a zone that covers it (a kill) or that contains it (a convex-union hull), simulation is
+// transitive and inclusion implies it, so at the successor's turn the chain still holds a cover
+// (the argument of NetPassedT::probe_kill, include/hora/detail/passed_set.hpp:233-234). Such
+// successors are decided here, in parallel; the ordered merge sees only the rest.
+//
+// Four 16-lane groups per wavefront, one successor per group, lanes over matrix entries; the
+// group walks the key's previous-level chain newest first.
+constexpr int kGroup = 16;
+
+__global__ __launch_bounds__(64) void k_probe(DevNet N, Passed P, const long* __restrict__ gbk,
+                                              const int* __restrict__ sj, const u64* __restrict__ sdk,
+                                              const int* __restrict__ szone, long G, int* __restrict__ covered,
+                                              Counters* ctr) {
+  __shared__ int Lg[4][HORA_MAX_DIM], Ug[4][HORA_MAX_DIM], lg[4][HORA_MAX_PROCS];
+  const int lane = threadIdx.x, q = lane / kGroup, t = lane % kGroup;
+  const long g = (long)blockIdx.x * 4 + q;
+  const bool active = g < G;
+  const int d = N.d, d2 = N.d2;
+  long h = active ? gbk[g] : -1;
+  const u32 len = (h >= 0) ? P.len[h] : 0u;
+  // per-successor a_LU bounds from its locations (state_bounds, engine.cpp:1487-1497)
+  if (active && len > 0) {
+    const u64 dk = sdk[sj[g]];
+    for (int p = t; p < N.np; p += kGroup) lg[q][p] = (int)((dk >> N.kshift[p]) & N.kmask[p]);
+  }
+  __syncthreads();
+  if (active && len > 0) {
+    for (int c = t; c < d; c += kGroup) {
+      int lb = 0, ub = 0;
+      if (c > 0) {
+        lb = ub = kAluNegInf;
+        for (int p = 0; p < N.np; ++p) {
+          const size_t b = (size_t)(N.locbase[p] + lg[q][p]) * d + c;
+          lb = max(lb, N.lloc[b]);
+          ub = max(ub, N.uloc[b]);
+        }
+        lb = max(lb, N.Lq[c]);
+        ub = max(ub, N.Uq[c]);
+      }
+      Lg[q][c] = lb;
+      Ug[q][c] = ub;
+    }
+  }
+  __syncthreads();
+  const int* S = active ? szone + (size_t)sj[active ? g : 0] * d2 : szone;
+  const u64 gmask = 0xffffull << (q * kGroup);
+  bool cov = false;
+  int reads = 0;
+  const int* L = Lg[q];
+  const int* U = Ug[q];
+  for (int m = (int)len - 1; m >= 0; --m) {   // group-uniform loop
+    const int* M = P.zarena + (size_t)P.nodes[P.off[h] + m] * d2;
+    ++reads;
+    bool wit = false;
+    for (int e = t; e < d2; e += kGroup) {
+      const int y = e / d, x = e - y * d;
+      if (x == y) continue;
+      const int ux = U[x], ly = L[y];
+      if (ux == kAluNegInf || ly == kAluNegInf) continue;
+      const int s0x = S[x];
+      if (s0x < r_le(-ux)) continue;
+      const int mp = M[e];
+      if (mp < S[e] && radd(mp, r_lt(-ly)) < s0x) wit = true;
+    }
+    const u64 b = __ballot(wit) & gmask;
+    if (b == 0) { cov = true; break; }
+  }
+  if (active && t == 0) covered[g] = cov ? 1 : 0;
+  const bool lead = active && t == 0;
+  wave_stat(ctr, C_PREADS, lead ? (unsigned long long)reads : 0ull);
+  wave_stat(ctr, C_PSUB, (lead && cov) ? 1ull : 0ull);
+}
+
+// tickets in the per-key work lists, for the successors the pre-filter kept
+__global__ void k_ticket(Passed P, const long* __restrict__ gbk, const int* __restrict__ covered, long G,
+                         u32* __restrict__ gticket, u32* __restrict__ touched, Counters* ctr) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long h = g < G ? gbk[g] : -1;
+  bool first = false;
+  if (h >= 0 && !covered[g]) {
+    const u32 t = atomicAdd(&P.ncnt[h], 1u);
+    gticket[g] = t;
+    first = t == 0;
+  }
+  const int idx = wave_append(&ctr->ntouched, first);
+  if (first) touched[idx] = (u32)h;
 }
 
 __host__ __device__ inline u64 list_cap(u64 n) {
@@ -501,23 +624,33 @@ __host__ __device__ inline u64 list_cap(u64 n) {
   return c;
 }
 
-__global__ void k_touched_caps(Passed P, const u32* __restrict__ touched, int T, int* __restrict__ caps) {
+// Per touched key: its work-list capacity, and the chain segment it reserves for this level
+// (twice the working chain, so that a working chain longer than the LDS copy fits beside its
+// cache-slot table).
+__global__ void k_touched_caps(Passed P, const u32* __restrict__ touched, int T, int* __restrict__ caps,
+                               int* __restrict__ segs) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < T) caps[t] = (int)list_cap(P.ncnt[touched[t]]);
+  if (t >= T) return;
+  const u32 h = touched[t];
+  caps[t] = (int)list_cap(P.ncnt[h]);
+  segs[t] = 2 * (int)(P.len[h] + P.ncnt[h]);
 }
 
 __global__ void k_touched_offsets(Passed P, const u32* __restrict__ touched, int T,
-                                  const long long* __restrict__ off) {
+                                  const long long* __restrict__ off, const long long* __restrict__ soff,
+                                  u64 seg_base) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < T) P.noff[touched[t]] = (u64)off[t];
+  if (t >= T) return;
+  P.noff[touched[t]] = (u64)off[t];
+  P.nseg[touched[t]] = seg_base + (u64)soff[t];
 }
 
-__global__ void k_scatter(Passed P, const long* __restrict__ gbk, const u32* __restrict__ gticket, long G,
-                          int* __restrict__ wlist) {
+__global__ void k_scatter(Passed P, const long* __restrict__ gbk, const int* __restrict__ covered,
+                          const u32* __restrict__ gticket, long G, int* __restrict__ wlist) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= G) return;
   const long h = gbk[g];
-  if (h < 0) return;
+  if (h < 0 || covered[g]) return;
   wlist[P.noff[h] + gticket[g]] = (int)g;
 }
 
@@ -540,17 +673,101 @@ __device__ void bitonic_sort(int* a, int n) {   // n a power of two, ascending
   wsync();
 }
 
-// LDS layout of k_merge.
-__host__ __device__ constexpr int merge_lds_ints(int d2) {
-  return 4 * d2 + 2 * HORA_MAX_DIM + HORA_MAX_PROCS + HORA_MAX_INTS + 8 + kSortLds;
+constexpr u32 kNewTag = 0x40000000u;     // chain entry refers to a successor stored this level
+
+// Lane-serial a_LU test (same predicate as z_alu_le, one lane per pair of zones): Z <=_aLU Zp.
+__device__ __forceinline__ bool alu_le_lane(const int* Z, const int* Zp, int d, const int* L, const int* U) {
+  for (int x = 0; x < d; ++x) {
+    const int ux = U[x];
+    if (ux == kAluNegInf) continue;
+    const int z0x = Z[x];
+    if (z0x < r_le(-ux)) continue;
+    for (int y = 0; y < d; ++y) {
+      if (y == x) continue;
+      const int ly = L[y];
+      if (ly == kAluNegInf) continue;
+      const int zp = Zp[y * d + x];
+      if (zp < Z[y * d + x] && radd(zp, r_lt(-ly)) < z0x) return false;
+    }
+  }
+  return true;
 }
 
-__global__ __launch_bounds__(64) void k_merge(DevNet N, Passed P, const u32* __restrict__ touched,
+// Lane-serial convex-union test (the FA x PA formulation of z_try_union) for d <= 16: the entry
+// sets live in four 64-bit masks. A gap between the two zones' projections on one clock is checked
+// first: it makes the union non-convex, so it only shortens the answer.
+__device__ __forceinline__ bool union_convex_lane(const int* a, const int* b, int d) {
+  for (int i = 1; i < d; ++i) {
+    const int bu = b[i * d], al = a[i], au = a[i * d], bl = b[i];
+    if (bu < kInf && al < kInf && -r_bnd(al) > r_bnd(bu)) return false;
+    if (au < kInf && bl < kInf && -r_bnd(bl) > r_bnd(au)) return false;
+  }
+  const int d2 = d * d;
+  u64 fa[4] = {0, 0, 0, 0}, pa[4] = {0, 0, 0, 0};
+  for (int e = 0; e < d2; ++e) {
+    const int av = a[e], bv = b[e];
+    if (av < bv) fa[e >> 6] |= 1ull << (e & 63);
+    else if (av > bv) pa[e >> 6] |= 1ull << (e & 63);
+  }
+  for (int pass = 0; pass < 2; ++pass) {
+    const int* z = pass == 0 ? a : b;      // facets of z ...
+    const int* o = pass == 0 ? b : a;      // ... crossed inside the hull must land in o
+    const u64* F = pass == 0 ? fa : pa;
+    const u64* Pm = pass == 0 ? pa : fa;
+    for (int wf = 0; wf < 4; ++wf)
+      for (u64 mf = F[wf]; mf; mf &= mf - 1) {
+        const int f = wf * 64 + __builtin_ctzll(mf);
+        const int i = f / d, j = f - i * d;
+        const int neg = r_neg(z[f]);
+        for (int wp = 0; wp < 4; ++wp)
+          for (u64 mp = Pm[wp]; mp; mp &= mp - 1) {
+            const int e = wp * 64 + __builtin_ctzll(mp);
+            const int p = e / d, q = e - p * d;
+            const int hpj = max(a[p * d + j], b[p * d + j]);
+            const int hiq = max(a[i * d + q], b[i * d + q]);
+            if (radd(radd(hpj, neg), hiq) > o[e]) return false;
+          }
+      }
+  }
+  return true;
+}
+
+// LDS layout of k_merge, in ints: S, M, fa, pa (d2 each); L/U; decoded key; scratch; the sorted
+// work list; chain ids and cache slots; the chain-zone cache (cz zones at an odd stride).
+struct MergeLayout {
+  int d2, zs, cz, sort_cap, chain_cap;
+  __host__ __device__ int total() const {
+    return 4 * d2 + 2 * HORA_MAX_DIM + HORA_MAX_PROCS + HORA_MAX_INTS + 16 + 32 + sort_cap + 2 * chain_cap + cz * zs;
+  }
+};
+
+__host__ __device__ inline MergeLayout merge_layout(int d2, bool heavy) {
+  MergeLayout m;
+  m.d2 = d2;
+  m.zs = d2 | 1;   // odd stride: lanes reading different cached zones hit different banks
+  m.sort_cap = heavy ? kSortLds : 64;
+  m.chain_cap = heavy ? 2048 : 64;
+  const int budget = heavy ? 60 * 1024 / 4 : 12 * 1024 / 4;   // ints
+  const int fixed = 4 * d2 + 2 * HORA_MAX_DIM + HORA_MAX_PROCS + HORA_MAX_INTS + 16 + 32 + m.sort_cap + 2 * m.chain_cap;
+  m.cz = max(2, min(heavy ? 1024 : 16, (budget - fixed) / m.zs));
+  return m;
+}
+
+// One wavefront per touched key: the key's kept successors in (state, recipe) order through
+// add()'s folds (passed_set.hpp:327-415). Chain entries are previous-level zone ids, or
+// kNewTag|g for a zone stored this level (its matrix is in the successor slot g, rewritten there
+// when a union widened it). Chain zones are cached in LDS. Short chains test one stored zone at a
+// time with lanes over matrix entries; long chains test up to 64 stored zones at once, one per
+// lane, and evaluate the 32-zone union window in one round, taking the first success in
+// newest-first order as the sequential fold does. At the end the key's live new zones get zone
+// ids in one allocation, move to the arena, and are reported as this level's survivors.
+__global__ __launch_bounds__(64) void k_merge(DevNet N, Passed P, const u32* __restrict__ list, int nlist,
                                               int* __restrict__ wlist, const int* __restrict__ sj,
-                                              const int* __restrict__ szone, int* __restrict__ survived,
-                                              int level, Counters* ctr) {
+                                              int* __restrict__ szone, int* __restrict__ survived,
+                                              Counters* ctr, int heavy, u64 zbase) {
   extern __shared__ int sm[];
   const int d = N.d, d2 = N.d2, l = threadIdx.x;
+  const MergeLayout ml = merge_layout(d2, heavy != 0);
   int* S = sm;
   int* M = S + d2;
   int* fa = M + d2;
@@ -559,21 +776,33 @@ __global__ __launch_bounds__(64) void k_merge(DevNet N, Passed P, const u32* __r
   int* Ub = Lb + HORA_MAX_DIM;
   int* locs = Ub + HORA_MAX_DIM;
   int* ints = locs + HORA_MAX_PROCS;
-  int* misc = ints + HORA_MAX_INTS;          // [0..1] try_union scratch, [2] zone id, [3] flag
-  int* sbuf = misc + 8;
+  int* misc = ints + HORA_MAX_INTS;          // [0..1] try_union scratch, [2..7] allocations
+  int* cand = misc + 16;                      // union window positions
+  int* sbuf = cand + 32;
+  u32* chl = (u32*)(sbuf + ml.sort_cap);
+  int* cslot = (int*)(chl + ml.chain_cap);
+  int* cache = cslot + ml.chain_cap;
+  const int zs = ml.zs, cz = ml.cz;
+  if ((int)blockIdx.x >= nlist) return;
 
-  const u32 h = touched[blockIdx.x];
+  const u32 h = list[blockIdx.x];
   const u32 n = P.ncnt[h];
   int* lst = wlist + P.noff[h];
-  // order this key's new successors by compacted index = (state, recipe) order
-  const u64 cap = list_cap(n);
-  if (n <= (u32)kSortLds) {
+  // order the kept successors by compacted index = (state, recipe) order
+  if (n <= (u32)kWave) {
+    const int v = (int)l < (int)n ? lst[l] : 0x7fffffff;
+    int rank = 0;
+    for (int j = 0; j < kWave; ++j) rank += (__shfl(v, j) < v) ? 1 : 0;
+    if ((int)l < (int)n) sbuf[rank] = v;
+    lst = sbuf;
+  } else if (n <= (u32)ml.sort_cap) {
     int p2 = 1;
     while (p2 < (int)n) p2 <<= 1;
     for (int k = l; k < p2; k += kWave) sbuf[k] = k < (int)n ? lst[k] : 0x7fffffff;
     bitonic_sort(sbuf, p2);
     lst = sbuf;
   } else {
+    const u64 cap = list_cap(n);
     for (u64 k = n + l; k < cap; k += kWave) lst[k] = 0x7fffffff;
     bitonic_sort(lst, (int)cap);
   }
@@ -582,8 +811,9 @@ __global__ __launch_bounds__(64) void k_merge(DevNet N, Passed P, const u32* __r
   decode_key(N, P.key[h], locs, ints);
   wsync();
   if (l < d) {
-    int lb = kAluNegInf, ub = kAluNegInf;
+    int lb = 0, ub = 0;
     if (l > 0) {
+      lb = ub = kAluNegInf;
       for (int p = 0; p < N.np; ++p) {
         const size_t b = (size_t)(N.locbase[p] + locs[p]) * d + l;
         lb = max(lb, N.lloc[b]);
@@ -591,129 +821,211 @@ __global__ __launch_bounds__(64) void k_merge(DevNet N, Passed P, const u32* __r
       }
       lb = max(lb, N.Lq[l]);
       ub = max(ub, N.Uq[l]);
-    } else {
-      lb = ub = 0;
     }
     Lb[l] = lb;
     Ub[l] = ub;
   }
 
-  // working chain: a fresh segment holding the old live chain, room for every new zone
+  // working chain: previous level's live chain, then one entry per kept successor; it lives in
+  // LDS, or, when longer than the LDS copy, in the chain segment this key reserved for the level
   const u32 old_len = P.len[h];
   const u64 old_off = P.off[h];
-  if (l == 0) {
-    const u64 seg = atomicAdd(&ctr->ncount, (unsigned long long)(old_len + n));
-    misc[4] = (int)(seg & 0xffffffffu);
-    misc[5] = (int)(seg >> 32);
+  const u64 seg = P.nseg[h];
+  u32* ch = chl;
+  int* cs = cslot;
+  if (old_len + n > (u32)ml.chain_cap) {
+    ch = P.nodes + seg;
+    cs = (int*)(P.nodes + seg + old_len + n);
+  }
+  // cache the newest min(old_len, cz) stored zones; the rest of the slots start free
+  const int ncached = min((int)old_len, cz);
+  for (u32 k = l; k < old_len; k += kWave) {
+    ch[k] = P.nodes[old_off + k];
+    const int slot = (int)k - ((int)old_len - ncached);
+    cs[k] = slot >= 0 ? slot : -1;
   }
   wsync();
-  const u64 seg = (u64)(u32)misc[4] | ((u64)(u32)misc[5] << 32);
-  if (seg + old_len + n > P.ncap) {
-    if (l == 0) atomicExch(&ctr->err, ERR_NODES_FULL);
-    return;
+  for (int k = 0; k < ncached; ++k) {
+    const int* src = P.zarena + (size_t)ch[old_len - ncached + k] * d2;
+    for (int e = l; e < d2; e += kWave) cache[k * zs + e] = src[e];
   }
-  u32* ch = P.nodes + seg;
-  for (u32 k = l; k < old_len; k += kWave) ch[k] = P.nodes[old_off + k];
-  wsync();
+  // cache slots are handed out upwards to new members until the cache is full
+  int next_slot = ncached;
+  // new members are tagged with their position k in the sorted work list
+  auto zp = [&](int pos) -> const int* {
+    const int s = cs[pos];
+    if (s >= 0) return cache + s * zs;
+    const u32 id = ch[pos] & ~kDead;
+    return (id & kNewTag) ? szone + (size_t)sj[lst[id & (kNewTag - 1)]] * d2 : P.zarena + (size_t)id * d2;
+  };
   int len = (int)old_len, live = (int)old_len;
   unsigned long long n_sub = 0, n_kill = 0, n_uni = 0, n_reads = 0, n_ins = 0;
+  wsync();
 
   for (u32 k = 0; k < n; ++k) {
     const int g = lst[k];
-    const int* sz = szone + (size_t)sj[g] * d2;
+    int* sz = szone + (size_t)sj[g] * d2;
     wsync();
     for (int e = l; e < d2; e += kWave) S[e] = sz[e];
     wsync();
+    bool widened = false;
     if (live > 0) {
+      const bool wide = live > 8;
       // (1)+(2) covered by a live stored zone: drop
       bool covered = false;
-      for (int m = len - 1; m >= 0 && !covered; --m) {
-        const u32 id = ch[m];
-        if (id & kDead) continue;
-        ++n_reads;
-        covered = z_alu_le(S, P.zarena + (size_t)id * d2, d, Lb, Ub);
+      if (!wide) {
+        for (int m = len - 1; m >= 0 && !covered; --m) {
+          if (ch[m] & kDead) continue;
+          ++n_reads;
+          covered = z_alu_le(S, zp(m), d, Lb, Ub);
+        }
+      } else {
+        for (int base = len - 1; base >= 0 && !covered; base -= kWave) {
+          const int m = base - l;
+          bool c = false;
+          if (m >= 0 && !(ch[m] & kDead)) c = alu_le_lane(S, zp(m), d, Lb, Ub);
+          covered = __any(c);
+        }
+        n_reads += (unsigned long long)live;
       }
       if (covered) { ++n_sub; continue; }
       // (3) retire stored zones covered by S
-      for (int m = len - 1; m >= 0; --m) {
-        const u32 id = ch[m];
-        if (id & kDead) continue;
-        ++n_reads;
-        if (z_alu_le(P.zarena + (size_t)id * d2, S, d, Lb, Ub)) {
-          wsync();
-          if (l == 0) ch[m] = id | kDead;
-          --live;
-          ++n_kill;
+      if (!wide) {
+        for (int m = len - 1; m >= 0; --m) {
+          const u32 id = ch[m];
+          if (id & kDead) continue;
+          ++n_reads;
+          if (z_alu_le(zp(m), S, d, Lb, Ub)) {
+            wsync();
+            if (l == 0) ch[m] = id | kDead;
+            --live;
+            ++n_kill;
+          }
         }
+      } else {
+        for (int base = len - 1; base >= 0; base -= kWave) {
+          const int m = base - l;
+          bool kill = false;
+          if (m >= 0 && !(ch[m] & kDead)) kill = alu_le_lane(zp(m), S, d, Lb, Ub);
+          const u64 b = __ballot(kill);
+          wsync();
+          if (kill) ch[m] |= kDead;
+          live -= __popcll(b);
+          n_kill += __popcll(b);
+        }
+        n_reads += (unsigned long long)live;
       }
       // (4) convex union with the kMergeScan newest live zones, restarting after each merge
       for (bool merged = true; merged;) {
         merged = false;
-        int seen = 0;
-        for (int m = len - 1; m >= 0 && seen < kMergeScan; --m) {
-          wsync();
-          const u32 id = ch[m];
-          if (id & kDead) continue;
-          ++seen;
-          ++n_reads;
-          const int* mz = P.zarena + (size_t)id * d2;
-          for (int e = l; e < d2; e += kWave) M[e] = mz[e];
-          if (z_try_union(S, M, d, fa, pa, misc)) {
+        wsync();
+        // window positions, newest first
+        if (l == 0) {
+          int c = 0;
+          for (int m = len - 1; m >= 0 && c < kMergeScan; --m)
+            if (!(ch[m] & kDead)) cand[c++] = m;
+          misc[6] = c;
+        }
+        wsync();
+        const int nc = misc[6];
+        if (nc == 0) break;
+        n_reads += (unsigned long long)nc;
+        int win = -1;
+        if (wide && d <= 16) {
+          bool ok = false;
+          if (l < nc) ok = union_convex_lane(S, zp(cand[l]), d);
+          const u64 b = __ballot(ok);
+          if (b) win = __builtin_ctzll(b);
+          if (win >= 0) {
+            const int* mz = zp(cand[win]);
             wsync();
-            if (l == 0) ch[m] = id | kDead;
-            --live;
-            ++n_uni;
-            merged = true;
-            break;
+            for (int e = l; e < d2; e += kWave) S[e] = max(S[e], mz[e]);
           }
+        } else {
+          for (int c = 0; c < nc && win < 0; ++c) {
+            const int* mz = zp(cand[c]);
+            wsync();
+            for (int e = l; e < d2; e += kWave) M[e] = mz[e];
+            if (z_try_union(S, M, d, fa, pa, misc)) win = c;
+          }
+        }
+        if (win >= 0) {
+          const int m = cand[win];
+          wsync();
+          if (l == 0) ch[m] |= kDead;
+          --live;
+          ++n_uni;
+          widened = true;
+          merged = true;
         }
       }
     }
-    // (5) append S
+    // (5) store S as the newest member: in a free cache slot when one is left, and back in its
+    // successor slot when a union widened it
     wsync();
+    int slot = -1;
+    if (next_slot < cz) slot = next_slot++;
+    if (slot >= 0)
+      for (int e = l; e < d2; e += kWave) cache[slot * zs + e] = S[e];
+    if (widened)
+      for (int e = l; e < d2; e += kWave) sz[e] = S[e];
     if (l == 0) {
-      const unsigned long long zid = atomicAdd(&ctr->zcount, 1ull);
-      misc[2] = (int)zid;
-      misc[3] = zid < P.zcap ? 1 : 0;
-    }
-    wsync();
-    if (!misc[3]) {
-      if (l == 0) atomicExch(&ctr->err, ERR_ZONES_FULL);
-      return;
-    }
-    const u32 zid = (u32)misc[2];
-    int* dst = P.zarena + (size_t)zid * d2;
-    for (int e = l; e < d2; e += kWave) dst[e] = S[e];
-    if (l == 0) {
-      P.zlevel[zid] = level;
-      P.zsrc[zid] = g;
-      ch[len] = zid;
+      ch[len] = kNewTag | k;
+      cs[len] = slot;
     }
     ++len;
     ++live;
     ++n_ins;
     wsync();
   }
-  // compact the live chain in place, oldest first; report this level's survivors
+  // Live new zones move to the arena, each at the zone id reserved for its work-list position
+  // (zbase + noff + k); the chain is compacted oldest first into this key's reserved segment.
   wsync();
-  if (l == 0) {
-    int w = 0;
-    for (int m = 0; m < len; ++m) {
-      const u32 id = ch[m];
-      if (id & kDead) continue;
-      ch[w++] = id;
-      if (P.zlevel[id] == level) survived[P.zsrc[id]] = (int)id;
+  u32* out = P.nodes + seg;
+  const u64 zk = zbase + P.noff[h];
+  int w = 0;
+  for (int m = 0; m < len; ++m) {
+    const u32 id = ch[m];
+    if (id & kDead) continue;
+    if (id & kNewTag) {
+      const u32 kpos = id & (kNewTag - 1);
+      const u32 zid = (u32)(zk + kpos);
+      const int* src = zp(m);
+      int* dst = P.zarena + (size_t)zid * d2;
+      for (int e = l; e < d2; e += kWave) dst[e] = src[e];
+      wsync();
+      if (l == 0) { out[w] = zid; survived[lst[kpos]] = (int)zid; }
+    } else {
+      wsync();
+      if (l == 0) out[w] = id;
     }
+    ++w;
+  }
+  if (l == 0) {
     P.off[h] = seg;
     P.len[h] = (u32)w;
     P.ncnt[h] = 0;
-    atomicAdd(&ctr->subsumed, n_sub);
-    atomicAdd(&ctr->killed, n_kill);
-    atomicAdd(&ctr->unions, n_uni);
-    atomicAdd(&ctr->reads, n_reads);
-    atomicAdd(&ctr->inserted, n_ins);
-    atomicAdd(&ctr->live, (unsigned long long)((long long)w - (long long)old_len));
+    stat_add(ctr, C_SUB, n_sub);
+    stat_add(ctr, C_KILL, n_kill);
+    stat_add(ctr, C_UNI, n_uni);
+    stat_add(ctr, C_READS, n_reads);
+    stat_add(ctr, C_INS, n_ins);
+    stat_add(ctr, C_LIVE, (unsigned long long)((long long)w - (long long)old_len));
   }
+}
+
+// light/heavy classification of touched keys by working-chain size
+__global__ void k_classify(Passed P, const u32* __restrict__ touched, int T, u32* __restrict__ light,
+                           u32* __restrict__ heavy, Counters* ctr, int light_max) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 h = t < T ? touched[t] : 0u;
+  const bool ok = t < T;
+  const u32 w = ok ? P.len[h] + P.ncnt[h] : 0u;
+  const bool lt = ok && w <= (u32)light_max && P.ncnt[h] <= 64u;
+  const int il = wave_append(&ctr->nlight, lt);
+  const int ih = wave_append(&ctr->nheavy, ok && !lt);
+  if (lt) light[il] = h;
+  if (ok && !lt) heavy[ih] = h;
 }
 
 // next frontier from survivors, in compacted-successor order
@@ -913,22 +1225,23 @@ struct hora_hip_ctx {
   std::vector<u64> h_kmask;
   int has_target_clock = 0;
   // search state
-  DBuf<u64> ht_key, ht_off, ht_noff;
+  DBuf<u64> ht_key, ht_off, ht_noff, ht_nseg;
   DBuf<u32> ht_len, ht_ncnt;
   DBuf<u32> nodes;
-  DBuf<int> zarena, zlevel, zsrc;
+  DBuf<int> zarena;
   u64 H = 0;
   int hbits = 0;
   DBuf<u64> fr_dk, fr_dk2;
   DBuf<u32> fr_zid, fr_zid2;
-  DBuf<int> rec_cnt, rec_off, s_valid, s_zone, s_tgt, sj, caps, wlist, survived, flag_i, off_i;
+  DBuf<int> rec_cnt, rec_off, s_valid, s_zone, s_tgt, sj, caps, segs, wlist, survived, flag_i, off_i, covered;
   DBuf<u64> s_dk;
   DBuf<long> gbk;
-  DBuf<u32> gticket, touched;
-  DBuf<long long> pos, bsum, scan_total;
+  DBuf<u32> gticket, touched, klight, kheavy;
+  DBuf<long long> pos, pos2, bsum, scan_total;
   DBuf<Counters> ctr;
   // profile of the last search
-  double t_expand = 0, t_merge = 0, t_other = 0, b_expand = 0, b_merge = 0;
+  hora_reach_profile prof{};
+  bool trace = false;
   hipEvent_t ev[8] = {};
 };
 
@@ -962,13 +1275,12 @@ Passed passed_view(hora_hip_ctx* c) {
   P.len = c->ht_len.p;
   P.ncnt = c->ht_ncnt.p;
   P.noff = c->ht_noff.p;
+  P.nseg = c->ht_nseg.p;
   P.nodes = c->nodes.p;
   P.zarena = c->zarena.p;
-  P.zlevel = c->zlevel.p;
-  P.zsrc = c->zsrc.p;
   P.hmask = c->H - 1;
   P.hshift = 64 - c->hbits;
-  P.zcap = std::min<u64>(std::min<u64>(c->zarena.cap / (size_t)c->d2, c->zlevel.cap), c->zsrc.cap);
+  P.zcap = c->zarena.cap / (size_t)c->d2;
   P.ncap = c->nodes.cap;
   return P;
 }
@@ -979,6 +1291,7 @@ void alloc_table(hora_hip_ctx* c, int hbits) {
   c->ht_key.ensure(c->H);
   c->ht_off.ensure(c->H);
   c->ht_noff.ensure(c->H);
+  c->ht_nseg.ensure(c->H);
   c->ht_len.ensure(c->H);
   c->ht_ncnt.ensure(c->H);
   HIP_OK(hipMemsetAsync(c->ht_key.p, 0xff, c->H * sizeof(u64), c->stream));
@@ -997,6 +1310,7 @@ void grow_table(hora_hip_ctx* c) {
   Passed Pold = passed_view(c);
   Pold.key = okey.p; Pold.off = ooff.p; Pold.len = olen.p;
   c->ht_noff.release();
+  c->ht_nseg.release();
   c->ht_ncnt.release();
   alloc_table(c, c->hbits + 1);
   Passed Pnew = passed_view(c);
@@ -1010,26 +1324,65 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
+// Host view of the device counters: stripes summed.
+struct Tally {
+  unsigned long long v[C_NSTAT] = {};
+  int err = 0, first_target = 0x7fffffff, ntouched = 0, nlight = 0, nheavy = 0;
+  unsigned long long operator[](int k) const { return v[k]; }
+};
+
+Tally read_counters(hora_hip_ctx* c) {
+  Counters hc;
+  HIP_OK(hipMemcpyAsync(&hc, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  Tally t;
+  for (int s = 0; s < kStripes; ++s)
+    for (int k = 0; k < C_NSTAT; ++k) t.v[k] += hc.st[s][k];
+  t.err = hc.err;
+  t.first_target = hc.first_target;
+  t.ntouched = hc.ntouched;
+  t.nlight = hc.nlight;
+  t.nheavy = hc.nheavy;
+  return t;
+}
+
+// Per-level device time and algorithmic HBM bytes (int32 entries, 4 B each):
+//   probe: the successor's zone plus each previous-level chain zone it is tested against;
+//   merge: each kept successor's zone, each chain zone the folds read, and each surviving new
+//          zone copied into the arena (read + write).
+void account(hora_hip_ctx* c, const Tally& hc, const Tally& prev, long G, int T, long F2) {
+  const double z = 4.0 * c->d2;
+  c->prof.probe_ms += ev_ms(c->ev[5], c->ev[6]);
+  c->prof.probe_bytes += z * ((double)G + (double)(hc[C_PREADS] - prev[C_PREADS]));
+  c->prof.probe_launches += 1;
+  c->prof.merge_ms += ev_ms(c->ev[3], c->ev[4]);
+  const double kept = (double)G - (double)(hc[C_PSUB] - prev[C_PSUB]);
+  c->prof.merge_bytes += z * (kept + (double)(hc[C_READS] - prev[C_READS]) + 2.0 * (double)F2);
+  c->prof.merge_launches += T > 0 ? 1 : 0;
+  c->prof.other_ms += ev_ms(c->ev[2], c->ev[5]) + ev_ms(c->ev[6], c->ev[3]);
+}
+
 int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
   const auto t0 = std::chrono::steady_clock::now();
   std::memset(st, 0, sizeof(*st));
   st->emode_switch_level = -1;
   *reachable = 0;
-  c->t_expand = c->t_merge = c->t_other = c->b_expand = c->b_merge = 0;
+  c->prof = hora_reach_profile{};
+  Tally prev;
   const int d2 = c->d2;
   c->dn.lu_plus = 0;
 
-  // passed list, arenas
+  // passed list and arenas
   alloc_table(c, 20);
   c->zarena.ensure((size_t)d2 << 20);
-  c->zlevel.ensure(1 << 20);
-  c->zsrc.ensure(1 << 20);
   c->nodes.ensure(1 << 22);
   c->ctr.ensure(1);
   HIP_OK(hipMemsetAsync(c->ctr.p, 0, sizeof(Counters), c->stream));
-  Counters h{};
-  h.first_target = 0x7fffffff;
-  HIP_OK(hipMemcpyAsync(c->ctr.p, &h, sizeof(Counters), hipMemcpyHostToDevice, c->stream));
+  {
+    const int ft = 0x7fffffff;
+    HIP_OK(hipMemcpyAsync(&c->ctr.p->first_target, &ft, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  }
+  u64 zcount = 0, ncount = 0;   // zone ids and chain-node entries handed out so far
 
   // the initial state travels through the same successor pipeline as a one-slot level 0
   u64 dk0 = 0;
@@ -1042,9 +1395,7 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
   long F = 1;
   int level = 0;
   bool init = true;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> dummy;
   for (;;) {
-    Counters hc{};
     // ---- recipe counts and offsets
     long R = 1;
     c->rec_cnt.ensure(F);
@@ -1074,10 +1425,10 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     // ---- compaction of non-empty successors
     c->pos.ensure(R + 1);
     const long Gall = R > 0 ? scan_exclusive(c, c->s_valid.p, R, c->pos.p) : 0;
-    c->t_expand += ev_ms(c->ev[0], c->ev[1]);
-    c->b_expand += 4.0 * d2 * (double)(F + Gall);
-    HIP_OK(hipMemcpyAsync(&hc, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
+    c->prof.expand_ms += ev_ms(c->ev[0], c->ev[1]);
+    c->prof.expand_bytes += 4.0 * d2 * (double)((init ? 0 : F) + Gall);
+    c->prof.expand_launches += 1;
+    Tally hc = read_counters(c);
     if (!init) st->expanded += (u64)F;
     long G = Gall;
     const bool found = hc.first_target != 0x7fffffff;
@@ -1097,51 +1448,71 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     if (G == 0) break;
     c->sj.ensure(Gall);
     hipLaunchKernelGGL(k_compact_valid, dim3(blocks_for(R, 256)), dim3(256), 0, c->stream, c->s_valid.p, c->pos.p, R, c->sj.p);
-    // ---- capacity: hash load, zones, chain nodes
-    while ((double)(hc.keys + (u64)G) > 0.6 * (double)c->H) grow_table(c);
-    if (hc.zcount + (u64)G > passed_view(c).zcap) {
-      const size_t want = (size_t)(hc.zcount + (u64)G) * 2;
-      c->zarena.grow_keep(want * d2, c->stream);
-      c->zlevel.grow_keep(want, c->stream);
-      c->zsrc.grow_keep(want, c->stream);
-    }
-    if (hc.ncount + hc.live + (u64)G + 1 > c->nodes.cap) {
-      // chain segments are rewritten per level; compact by copying only live chains is left to
-      // growth here: every live chain is at most `live` entries and a level appends <= live + G
-      c->nodes.grow_keep((size_t)(hc.ncount + hc.live + (u64)G + 1) * 2, c->stream);
-    }
+    while ((double)(hc[C_KEYS] + (u64)G) > 0.6 * (double)c->H) grow_table(c);
     Passed P = passed_view(c);
-    // ---- hash + per-key work lists
+    // ---- keys, cover pre-filter, per-key work lists
     c->gbk.ensure(G);
     c->gticket.ensure(G);
     c->touched.ensure(G);
+    c->covered.ensure(G);
     HIP_OK(hipMemsetAsync(&c->ctr.p->ntouched, 0, sizeof(int), c->stream));
     HIP_OK(hipEventRecord(c->ev[2], c->stream));
-    hipLaunchKernelGGL(k_hash, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->sj.p, c->s_dk.p, G,
-                       c->gbk.p, c->gticket.p, c->touched.p, c->ctr.p);
+    hipLaunchKernelGGL(k_find, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->sj.p, c->s_dk.p, G,
+                       c->gbk.p, c->ctr.p);
+    HIP_OK(hipEventRecord(c->ev[5], c->stream));
+    hipLaunchKernelGGL(k_probe, dim3(blocks_for(G, 4)), dim3(64), 0, c->stream, c->dn, P, c->gbk.p, c->sj.p,
+                       c->s_dk.p, c->s_zone.p, G, c->covered.p, c->ctr.p);
+    HIP_OK(hipEventRecord(c->ev[6], c->stream));
+    hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p, G,
+                       c->gticket.p, c->touched.p, c->ctr.p);
     int T = 0;
     HIP_OK(hipMemcpyAsync(&T, &c->ctr.p->ntouched, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     c->caps.ensure(T);
-    hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T, c->caps.p);
+    c->segs.ensure(T);
+    hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                       c->caps.p, c->segs.p);
     c->pos.ensure(T + 1);
+    c->pos2.ensure(T + 1);
     const long long W = scan_exclusive(c, c->caps.p, T, c->pos.p);
-    hipLaunchKernelGGL(k_touched_offsets, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T, c->pos.p);
+    const long long Sseg = scan_exclusive(c, c->segs.p, T, c->pos2.p);
+    // capacity: one zone id per work-list position, the reserved chain segments
+    if (zcount + (u64)W > P.zcap) c->zarena.grow_keep((size_t)(zcount + (u64)W) * 2 * d2, c->stream);
+    if (ncount + (u64)Sseg > P.ncap) c->nodes.grow_keep((size_t)(ncount + (u64)Sseg) * 2, c->stream);
+    P = passed_view(c);
+    hipLaunchKernelGGL(k_touched_offsets, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                       c->pos.p, c->pos2.p, ncount);
     c->wlist.ensure(W + 1);
-    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->gticket.p, G, c->wlist.p);
-    // ---- merge
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p,
+                       c->gticket.p, G, c->wlist.p);
+    // ---- ordered merge of the kept successors: light keys, then heavy keys
     c->survived.ensure(G);
     HIP_OK(hipMemsetAsync(c->survived.p, 0xff, (size_t)G * sizeof(int), c->stream));
-    const size_t lds_m = merge_lds_ints(d2) * sizeof(int);
+    c->klight.ensure(T);
+    c->kheavy.ensure(T);
+    HIP_OK(hipMemsetAsync(&c->ctr.p->nlight, 0, 2 * sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_classify, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                       c->klight.p, c->kheavy.p, c->ctr.p, kLightMax);
+    int nlh[2] = {0, 0};
+    HIP_OK(hipMemcpyAsync(nlh, &c->ctr.p->nlight, sizeof(nlh), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    const size_t lds_l = merge_layout(d2, false).total() * sizeof(int);
+    const size_t lds_h = merge_layout(d2, true).total() * sizeof(int);
     HIP_OK(hipEventRecord(c->ev[3], c->stream));
-    hipLaunchKernelGGL(k_merge, dim3(T), dim3(64), lds_m, c->stream, c->dn, P, c->touched.p, c->wlist.p, c->sj.p,
-                       c->s_zone.p, c->survived.p, level, c->ctr.p);
+    if (nlh[0] > 0)
+      hipLaunchKernelGGL(k_merge, dim3(nlh[0]), dim3(64), lds_l, c->stream, c->dn, P, c->klight.p, nlh[0],
+                         c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 0, zcount);
+    if (nlh[1] > 0)
+      hipLaunchKernelGGL(k_merge, dim3(nlh[1]), dim3(64), lds_h, c->stream, c->dn, P, c->kheavy.p, nlh[1],
+                         c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 1, zcount);
     HIP_OK(hipEventRecord(c->ev[4], c->stream));
+    zcount += (u64)W;
+    ncount += (u64)Sseg;
     if (found) {
-      HIP_OK(hipMemcpyAsync(&hc, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
-      HIP_OK(hipStreamSynchronize(c->stream));
-      c->t_merge += ev_ms(c->ev[3], c->ev[4]);
+      hc = read_counters(c);
+      account(c, hc, prev, G, T, 0);
       if (hc.err) { c->err = "device capacity exceeded"; return HORA_HIP_E_MEMORY; }
+      prev = hc;
       break;
     }
     // ---- next frontier
@@ -1153,14 +1524,14 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     c->fr_zid2.ensure(F2);
     hipLaunchKernelGGL(k_build_frontier, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, c->survived.p, c->pos.p, G,
                        c->sj.p, c->s_dk.p, c->fr_dk2.p, c->fr_zid2.p);
-    HIP_OK(hipMemcpyAsync(&hc, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
-    const double reads_before = (double)st->chain_reads;
-    st->chain_reads = hc.reads;
-    c->t_merge += ev_ms(c->ev[3], c->ev[4]);
-    c->t_other += ev_ms(c->ev[2], c->ev[3]);
-    // merge bytes: new zones read, stored zones read by the folds, zones written
-    c->b_merge += 4.0 * d2 * ((double)G + ((double)hc.reads - reads_before)) + 4.0 * d2 * (double)F2;
+    hc = read_counters(c);
+    account(c, hc, prev, G, T, F2);
+    if (c->trace)
+      std::fprintf(stderr, "level %d F=%ld R=%ld G=%ld probe_cov=%llu T=%d (light %d heavy %d) W=%lld F2=%ld | exp %.3f probe %.3f merge %.3f ms | reads p=%llu m=%llu unions=%llu\n",
+                   level, F, R, G, hc[C_PSUB] - prev[C_PSUB], T, nlh[0], nlh[1], W, F2, ev_ms(c->ev[0], c->ev[1]),
+                   ev_ms(c->ev[5], c->ev[6]), ev_ms(c->ev[3], c->ev[4]), hc[C_PREADS] - prev[C_PREADS],
+                   hc[C_READS] - prev[C_READS], hc[C_UNI] - prev[C_UNI]);
+    prev = hc;
     if (hc.err) {
       c->err = hc.err == ERR_HASH_FULL ? "passed-list hash table full"
                : hc.err == ERR_ZONES_FULL ? "zone arena full" : "chain node arena full";
@@ -1171,22 +1542,21 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     F = F2;
     init = false;
     // adaptive Extra_LU+ switch (engine.cpp:2178-2184)
-    if (!c->dn.lu_plus && hc.keys && (double)hc.live >= 4.0 * (double)hc.keys) {
+    if (!c->dn.lu_plus && hc[C_KEYS] && (double)hc[C_LIVE] >= 4.0 * (double)hc[C_KEYS]) {
       c->dn.lu_plus = 1;
       st->emode_switch_level = level;
     }
     if (F == 0) break;
   }
-  Counters hc{};
-  HIP_OK(hipMemcpy(&hc, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost));
+  const Tally hc = read_counters(c);
   st->levels = level;
-  st->stored_live = hc.live;
-  st->keys = hc.keys;
-  st->inserted = hc.inserted;
-  st->subsumed = hc.subsumed;
-  st->killed = hc.killed;
-  st->unions = hc.unions;
-  st->chain_reads = hc.reads;
+  st->stored_live = hc[C_LIVE];
+  st->keys = hc[C_KEYS];
+  st->inserted = hc[C_INS];
+  st->subsumed = hc[C_SUB] + hc[C_PSUB];
+  st->killed = hc[C_KILL];
+  st->unions = hc[C_UNI];
+  st->chain_reads = hc[C_READS] + hc[C_PREADS];
   st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return 0;
 }
@@ -1269,6 +1639,10 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
   for (int ch = 0; ch < e->nchans; ++ch)
     if (e->chan_urgent && e->chan_urgent[ch]) has_urgent = 1;
 
+  {
+    const char* tr = std::getenv("HORA_HIP_TRACE");
+    c->trace = tr && *tr == '1';
+  }
   HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto& ev : c->ev) HIP_OK(hipEventCreate(&ev));
   const int ne = std::max(c->ne, 1);
@@ -1377,14 +1751,9 @@ int hora_hip_reach(hora_hip_ctx* ctx, int32_t* reachable, hora_reach_stats* stat
   }
 }
 
-int hora_hip_reach_profile(const hora_hip_ctx* ctx, double* expand_ms, double* merge_ms, double* other_ms,
-                           double* expand_bytes, double* merge_bytes) {
-  if (!ctx) return HORA_HIP_E_ARG;
-  if (expand_ms) *expand_ms = ctx->t_expand;
-  if (merge_ms) *merge_ms = ctx->t_merge;
-  if (other_ms) *other_ms = ctx->t_other;
-  if (expand_bytes) *expand_bytes = ctx->b_expand;
-  if (merge_bytes) *merge_bytes = ctx->b_merge;
+int hora_hip_reach_profile(const hora_hip_ctx* ctx, hora_reach_profile* out) {
+  if (!ctx || !out) return HORA_HIP_E_ARG;
+  *out = ctx->prof;
   return HORA_HIP_OK;
 }
 

@@ -39,6 +39,12 @@ class Backend(enum.Enum):
     Hip = 4
 
 
+class hora_reach_profile(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("expand_ms", "probe_ms", "merge_ms", "other_ms",
+                                          "expand_bytes", "probe_bytes", "merge_bytes")] + \
+               [(k, C.c_int64) for k in ("expand_launches", "probe_launches", "merge_launches")]
+
+
 class HipError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"hora_hip error {code}: {msg}")
@@ -66,7 +72,7 @@ def lib():
         L.hora_hip_last_error.argtypes = [C.c_void_p]
         L.hora_hip_last_error.restype = C.c_char_p
         L.hora_hip_reach.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(hora_reach_stats)]
-        L.hora_hip_reach_profile.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 5
+        L.hora_hip_reach_profile.argtypes = [C.c_void_p, C.POINTER(hora_reach_profile)]
         L.hora_hip_zone_op.argtypes = [C.c_int32, C.c_int32, C.c_int64] + [C.c_void_p] * 8
         _lib = L
     return _lib
@@ -119,10 +125,10 @@ class HipContext:
         return bool(r.value), ReachStats.from_c(st)
 
     def profile(self) -> dict:
-        v = [C.c_double() for _ in range(5)]
-        lib().hora_hip_reach_profile(self.h, *[C.byref(x) for x in v])
-        keys = ("expand_ms", "merge_ms", "other_ms", "expand_bytes", "merge_bytes")
-        return {k: x.value for k, x in zip(keys, v)}
+        """Per-kernel device time (ms) and algorithmic bytes of the last search."""
+        p = hora_reach_profile()
+        lib().hora_hip_reach_profile(self.h, C.byref(p))
+        return {k: getattr(p, k) for k, _ in p._fields_}
 
     def close(self):
         if self.h:

@@ -15,8 +15,10 @@ from ptctcl_amd.network import Target
 
 pytestmark = pytest.mark.gpu
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reach_counts.json")))
+# every counter with a meaning in the sequential engine; chain_reads counts work, which the
+# device's parallel cover pre-filter changes by design
 KEYS = ("explored", "generated", "stored_live", "keys", "inserted", "subsumed", "killed", "unions",
-        "chain_reads", "levels", "found", "emode_switch_level")
+        "levels", "found", "emode_switch_level")
 
 
 def check(net, target):
