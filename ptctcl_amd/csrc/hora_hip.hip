@@ -52,6 +52,7 @@ constexpr int kSortLds = 1024;       // bucket work lists up to this length are 
 constexpr u32 kDead = 0x80000000u;   // tombstone bit on a chain entry
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;
+constexpr int kStageMax = 8192;  // table blob (ints) staged in LDS by the walking kernels when it fits
 constexpr int kLightMax = 8;   // working-chain size up to which a key takes the light merge launch
 
 struct DevNet {
@@ -74,8 +75,8 @@ struct DevNet {
   const int* ig_data;
   const int* iu_off;
   const int* iu_data;
-  const unsigned char* chan_bcast;
-  const unsigned char* chan_urgent;
+  const int* chan_bcast;  // one int per channel
+  const int* chan_urgent;
   int has_urgent_chan;
   const int* Lx;          // extrapolation bounds [d] (global bounds widened by the query)
   const int* Ux;
@@ -92,7 +93,37 @@ struct DevNet {
   const int* lr;
   const int* ic;
   const int* cc;
+  // every table above except lloc/uloc lives in one blob, so that a workgroup can stage all of
+  // them in LDS with one copy and rebase the pointers
+  const int* blob;
+  int blob_ints;
+  int stage;              // 1: kernels that walk the tables stage the blob in LDS
 };
+
+// Rebase every blob pointer of N onto an LDS copy of the blob.
+__device__ __forceinline__ DevNet rebase(const DevNet& N, const int* lds) {
+  DevNet L = N;
+  const char* from = (const char*)N.blob;
+  const char* to = (const char*)lds;
+  auto rb = [&](auto p) { return (decltype(p))(to + ((const char*)p - from)); };
+  L.locbase = rb(N.locbase); L.loc_flags = rb(N.loc_flags); L.inv_off = rb(N.inv_off); L.inv_data = rb(N.inv_data);
+  L.le_off = rb(N.le_off); L.le_edges = rb(N.le_edges); L.edge_proc = rb(N.edge_proc); L.edge_tgt = rb(N.edge_tgt);
+  L.edge_chan = rb(N.edge_chan); L.edge_dir = rb(N.edge_dir); L.cg_off = rb(N.cg_off); L.cg_data = rb(N.cg_data);
+  L.rs_off = rb(N.rs_off); L.rs_clk = rb(N.rs_clk); L.ig_off = rb(N.ig_off); L.ig_data = rb(N.ig_data);
+  L.iu_off = rb(N.iu_off); L.iu_data = rb(N.iu_data); L.chan_bcast = rb(N.chan_bcast); L.chan_urgent = rb(N.chan_urgent);
+  L.Lx = rb(N.Lx); L.Ux = rb(N.Ux); L.Lq = rb(N.Lq); L.Uq = rb(N.Uq); L.amask = rb(N.amask); L.kshift = rb(N.kshift);
+  L.kmask = rb(N.kmask); L.int_lo = rb(N.int_lo); L.lr = rb(N.lr); L.ic = rb(N.ic); L.cc = rb(N.cc);
+  L.blob = lds;
+  return L;
+}
+
+// Stage the table blob in LDS (all lanes copy) when the context asked for it.
+__device__ __forceinline__ DevNet stage_tables(const DevNet& N, int* lds) {
+  if (!N.stage) return N;
+  for (int k = threadIdx.x; k < N.blob_ints; k += blockDim.x) lds[k] = N.blob[k];
+  __syncthreads();
+  return rebase(N, lds);
+}
 
 // Statistics are striped over cache lines by workgroup so that hundreds of thousands of
 // workgroups per launch do not serialise on one address; the host sums the stripes.
@@ -130,7 +161,8 @@ __device__ __forceinline__ void wave_stat(Counters* c, int k, unsigned long long
   if ((threadIdx.x & 63) == 0) stat_add(c, k, v);
 }
 
-enum { ERR_NONE = 0, ERR_HASH_FULL = 1, ERR_ZONES_FULL = 2, ERR_NODES_FULL = 3 };
+enum { ERR_NONE = 0, ERR_HASH_FULL = 1, ERR_ZONES_FULL = 2, ERR_NODES_FULL = 3, ERR_RECIPES = 4 };
+constexpr int kInvCap = 128;      // invariant constraints of one location vector held in LDS
 
 // ---------------------------------------------------------------- discrete state helpers
 
@@ -175,7 +207,7 @@ __device__ __forceinline__ u64 encode_key(const DevNet& N, const int* locs, cons
 }
 
 // urgent_sync_enabled (semantics.hpp:118-139)
-__device__ bool urgent_sync_enabled(const DevNet& N, const int* locs, const int* ints) {
+__device__ __forceinline__ bool urgent_sync_enabled(const DevNet& N, const int* locs, const int* ints) {
   if (!N.has_urgent_chan) return false;
   for (int i = 0; i < N.np; ++i) {
     const int gi = N.locbase[i] + uni(locs[i]);
@@ -200,7 +232,7 @@ __device__ bool urgent_sync_enabled(const DevNet& N, const int* locs, const int*
 }
 
 // frozen_flags (semantics.hpp:160-169): bit0 committed, bit1 urgent
-__device__ int frozen_flags(const DevNet& N, const int* locs, const int* ints) {
+__device__ __forceinline__ int frozen_flags(const DevNet& N, const int* locs, const int* ints) {
   int f = 0;
   for (int p = 0; p < N.np; ++p) f |= N.loc_flags[N.locbase[p] + uni(locs[p])] & 3;
   if (!(f & 2) && urgent_sync_enabled(N, locs, ints)) f |= 2;
@@ -212,7 +244,7 @@ __device__ int frozen_flags(const DevNet& N, const int* locs, const int* ints) {
 // receiver process and edge). `fn(se, re)` gets global edge ids (re = -1 for a local step).
 // Broadcast channels are refused at hora_hip_open.
 template <class F>
-__device__ void enum_recipes(const DevNet& N, const int* locs, const int* ints, bool src_c, F&& fn) {
+__device__ __forceinline__ void enum_recipes(const DevNet& N, const int* locs, const int* ints, bool src_c, F&& fn) {
   for (int p = 0; p < N.np; ++p) {
     const int lp = uni(locs[p]);
     if (src_c && !committed_at(N, p, lp)) continue;
@@ -248,7 +280,7 @@ __device__ void enum_recipes(const DevNet& N, const int* locs, const int* ints, 
 }
 
 // Conjoin the combined invariant of `locs` (all_invariants + apply_all, semantics.hpp:82-86).
-__device__ bool apply_inv(const DevNet& N, const int* locs, int* z) {
+__device__ __forceinline__ bool apply_inv(const DevNet& N, const int* locs, int* z) {
   for (int p = 0; p < N.np; ++p) {
     const int g = N.locbase[p] + uni(locs[p]);
     for (int c = N.inv_off[g]; c < N.inv_off[g + 1]; ++c)
@@ -257,18 +289,18 @@ __device__ bool apply_inv(const DevNet& N, const int* locs, int* z) {
   return true;
 }
 
-__device__ bool apply_edge_guard(const DevNet& N, int e, int* z) {
+__device__ __forceinline__ bool apply_edge_guard(const DevNet& N, int e, int* z) {
   for (int c = N.cg_off[e]; c < N.cg_off[e + 1]; ++c)
     if (!z_constrain(z, N.d, N.cg_data[3 * c], N.cg_data[3 * c + 1], N.cg_data[3 * c + 2])) return false;
   return true;
 }
 
-__device__ void apply_edge_resets(const DevNet& N, int e, int* z) {
+__device__ __forceinline__ void apply_edge_resets(const DevNet& N, int e, int* z) {
   for (int r = N.rs_off[e]; r < N.rs_off[e + 1]; ++r) z_reset(z, N.d, N.rs_clk[r]);
 }
 
 // close_succ_zone (semantics.hpp:522-568) followed by the caller's emptiness test.
-__device__ bool close_succ(const DevNet& N, const int* locs, const int* ints, int* z, int* row0) {
+__device__ __forceinline__ bool close_succ(const DevNet& N, const int* locs, const int* ints, int* z, int* row0) {
   if (!apply_inv(N, locs, z)) return false;
   const int fz = frozen_flags(N, locs, ints);
   if (!fz) {
@@ -288,7 +320,7 @@ __device__ bool close_succ(const DevNet& N, const int* locs, const int* ints, in
 }
 
 // reachable_net's match (engine.cpp:2275-2287); `tmp` is d*d LDS scratch.
-__device__ bool is_target(const DevNet& N, const int* locs, const int* ints, const int* z, int* tmp) {
+__device__ __forceinline__ bool is_target(const DevNet& N, const int* locs, const int* ints, const int* z, int* tmp) {
   if (N.never) return false;
   for (int k = 0; k < N.nlr; ++k)
     if (locs[N.lr[2 * k]] != N.lr[2 * k + 1]) return false;
@@ -311,31 +343,202 @@ struct ExpandOut {
   int* tgt;        // [R] 1 = matches the target
 };
 
-__global__ __launch_bounds__(64) void k_count(DevNet N, const u64* __restrict__ fr_dk, int F,
-                                              int* __restrict__ cnt) {
-  __shared__ int locs[HORA_MAX_PROCS], ints[HORA_MAX_INTS];
+// Wavefront-wide helpers. One lane per process (np <= 64): the per-process table lookups of a
+// location vector are issued together instead of as a chain of dependent loads.
+
+__device__ __forceinline__ u64 wave_or64(u64 v) {
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
+  return v;
+}
+
+__device__ __forceinline__ int wave_excl_scan(int v, int* total) {
+  const int l = threadIdx.x;
+  int x = v;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (l >= o) x += y;
+  }
+  *total = __shfl(x, kWave - 1);
+  return x - v;
+}
+
+// Packed key of a location vector and valuation (DiscKeyCodecT::encode, engine.cpp:1006-1013).
+__device__ __forceinline__ u64 wave_encode_key(const DevNet& N, const int* locs, const int* ints) {
+  const int l = threadIdx.x;
+  u64 k = 0;
+  if (l < N.np) k |= (u64)(unsigned)locs[l] << N.kshift[l];
+  if (l < N.ni) k |= (u64)(unsigned)(ints[l] - N.int_lo[l]) << N.kshift[N.np + l];
+  return wave_or64(k);
+}
+
+// The target side of close_succ_zone for a location vector: frozen flags (semantics.hpp:160-169),
+// the active-clock mask union (reset_inactive, semantics.hpp:195-205) and the combined invariant
+// gathered into `inv` as (i, j, raw) triples (all_invariants, semantics.hpp:82-86).
+__device__ __forceinline__ void wave_loc_info(const DevNet& N, const int* locs, const int* ints, int* frozen,
+                                              u64* amask, int* inv, int* ninv) {
+  const int l = threadIdx.x;
+  int f = 0, ia = 0, ib = 0;
+  u64 am = 0;
+  if (l < N.np) {
+    const int g = N.locbase[l] + locs[l];
+    f = N.loc_flags[g] & 3;
+    am = N.amask[g];
+    ia = N.inv_off[g];
+    ib = N.inv_off[g + 1];
+  }
+  int fr = (__ballot(f & 1) ? 1 : 0) | (__ballot(f & 2) ? 2 : 0);
+  if (!(fr & 2) && N.has_urgent_chan && urgent_sync_enabled(N, locs, ints)) fr |= 2;
+  *frozen = fr;
+  *amask = wave_or64(am);
+  int total = 0;
+  const int off = wave_excl_scan(ib - ia, &total);
+  for (int c = ia; c < ib; ++c) {
+    const int o = 3 * (off + c - ia);
+    inv[o] = N.inv_data[3 * c];
+    inv[o + 1] = N.inv_data[3 * c + 1];
+    inv[o + 2] = N.inv_data[3 * c + 2];
+  }
+  *ninv = total;
+}
+
+// Conjoin a list of constraints. Constraints that do not tighten the zone now never will (entries
+// only decrease), so a parallel check selects the ones that need the incremental closure; the
+// final zone does not depend on the order, and emptiness is detected either way.
+__device__ __forceinline__ bool z_apply_list(int* z, int d, const int* cs, int n) {
+  for (int base = 0; base < n; base += kWave) {
+    wsync();
+    const int k = base + (int)threadIdx.x;
+    bool cand = false;
+    if (k < n) cand = cs[3 * k + 2] < z[cs[3 * k] * d + cs[3 * k + 1]];
+    for (u64 b = __ballot(cand); b; b &= b - 1) {
+      const int c = base + __builtin_ctzll(b);
+      if (!z_constrain(z, d, cs[3 * c], cs[3 * c + 1], cs[3 * c + 2])) return false;
+    }
+  }
+  return true;
+}
+
+// Reset every clock in `mask` at once. Resetting x_k copies row 0 into row k and column 0 into
+// column k; for a set K that is z[i][j] := z[i'][j'] with i' = 0 if i is in K (else i), likewise
+// j' -- the composition of the single resets in any order. `tmp` is d*d LDS scratch.
+__device__ __forceinline__ void z_reset_many(int* z, int d, u64 mask, int* tmp) {
+  if (!mask) return;
+  wsync();
+  const int l = threadIdx.x, d2 = d * d;
+  for (int e = l; e < d2; e += kWave) {
+    const int i = e / d, j = e - i * d;
+    const int si = ((mask >> i) & 1) ? 0 : i, sj = ((mask >> j) & 1) ? 0 : j;
+    tmp[e] = z[si * d + sj];
+  }
+  wsync();
+  for (int e = l; e < d2; e += kWave) z[e] = tmp[e];
+}
+
+__device__ __forceinline__ u64 edge_reset_mask(const DevNet& N, int e) {
+  u64 m = 0;
+  for (int r = N.rs_off[e]; r < N.rs_off[e + 1]; ++r) m |= 1ull << N.rs_clk[r];
+  return m;
+}
+
+// Transition recipes of a discrete state, in gen_succ_reuse's order (engine.cpp:541-718), into
+// `rec` as (sender edge, receiver edge or -1) pairs: local edges in process order -- each lane
+// enumerates its own process, a prefix sum places them -- then binary synchronisations (sender
+// process and edge, then receiver process and edge). Broadcast channels are refused at open.
+// Returns the number of recipes, or -1 if more than `cap`.
+__device__ __forceinline__ int build_recipes(const DevNet& N, const int* locs, const int* ints, bool src_c,
+                                             int* rec, int cap) {
+  const int l = threadIdx.x;
+  int cnt = 0, g = 0;
+  bool comm = false;
+  if (l < N.np) {
+    g = N.locbase[l] + locs[l];
+    comm = (N.loc_flags[g] & 1) != 0;
+    if (!src_c || comm)
+      for (int t = N.le_off[g]; t < N.le_off[g + 1]; ++t) {
+        const int e = N.le_edges[t];
+        if (N.edge_chan[e] < 0 && int_ok(N, e, ints)) ++cnt;
+      }
+  }
+  int total = 0;
+  int off = wave_excl_scan(cnt, &total);
+  if (total > cap) return -1;
+  if (l < N.np && cnt > 0)
+    for (int t = N.le_off[g]; t < N.le_off[g + 1]; ++t) {
+      const int e = N.le_edges[t];
+      if (N.edge_chan[e] < 0 && int_ok(N, e, ints)) {
+        rec[2 * off] = e;
+        rec[2 * off + 1] = -1;
+        ++off;
+      }
+    }
+  // binary synchronisations: senders spread over lanes by process, receivers walked in order
+  const u64 has_send = __ballot(l < N.np && N.le_off[g + 1] > N.le_off[g]);
+  if (!has_send) return total;
+  int n = total;
+  for (int i = 0; i < N.np; ++i) {
+    const int gi = N.locbase[i] + locs[i];
+    const bool i_comm = committed_at(N, i, locs[i]);
+    for (int t = N.le_off[gi]; t < N.le_off[gi + 1]; ++t) {
+      const int se = N.le_edges[t];
+      const int ch = N.edge_chan[se];
+      if (ch < 0 || N.edge_dir[se] != 1 || N.chan_bcast[ch]) continue;
+      if (!int_ok(N, se, ints)) continue;
+      for (int j = 0; j < N.np; ++j) {
+        if (j == i) continue;
+        const int gj = N.locbase[j] + locs[j];
+        for (int t2 = N.le_off[gj]; t2 < N.le_off[gj + 1]; ++t2) {
+          const int re = N.le_edges[t2];
+          if (N.edge_chan[re] != ch || N.edge_dir[re] != -1) continue;
+          if (src_c && !(i_comm || committed_at(N, j, locs[j]))) continue;
+          if (!int_ok(N, re, ints)) continue;
+          if (n >= cap) return -1;
+          if (l == 0) { rec[2 * n] = se; rec[2 * n + 1] = re; }
+          ++n;
+        }
+      }
+    }
+  }
+  return n;
+}
+
+constexpr int kRecipeCap = 256;    // transitions of one discrete state held in LDS
+
+__global__ __launch_bounds__(64) void k_count(const DevNet* __restrict__ Np, const u64* __restrict__ fr_dk, int F,
+                                              int* __restrict__ cnt, Counters* ctr) {
+  extern __shared__ int sm[];
+  int* locs = sm;
+  int* ints = locs + HORA_MAX_PROCS;
+  int* rec = ints + HORA_MAX_INTS;
+  const DevNet& N = *Np;
   const int i = blockIdx.x;
   if (i >= F) return;
   decode_key(N, fr_dk[i], locs, ints);
   wsync();
-  const bool src_c = (frozen_flags(N, locs, ints) & 1) != 0;
-  int r = 0;
-  enum_recipes(N, locs, ints, src_c, [&](int, int) { ++r; });
-  if (threadIdx.x == 0) cnt[i] = r;
+  int fz;
+  u64 am;
+  int ninv;
+  int* inv = rec + 2 * kRecipeCap;
+  wave_loc_info(N, locs, ints, &fz, &am, inv, &ninv);
+  const int r = build_recipes(N, locs, ints, (fz & 1) != 0, rec, kRecipeCap);
+  if (threadIdx.x == 0) {
+    cnt[i] = r < 0 ? 0 : r;
+    if (r < 0) atomicExch(&ctr->err, ERR_RECIPES);
+  }
 }
 
-// LDS layout of k_expand: 3 zones + row-0 scratch + 2 discrete states.
+// LDS layout of k_expand: 3 zones + row-0 scratch + 2 discrete states + recipes + invariant list.
 __host__ __device__ constexpr int expand_lds_ints(int d2) {
-  return 3 * d2 + HORA_MAX_DIM + 2 * (HORA_MAX_PROCS + HORA_MAX_INTS);
+  return 3 * d2 + HORA_MAX_DIM + 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * kRecipeCap + 3 * kInvCap;
 }
 
 // Successors of frontier states [0, F) into slots rec_off[i] + r. With `init` set, the frontier
 // is the single initial state: its zone is the zero zone and its one recipe is "no transition".
-__global__ __launch_bounds__(64) void k_expand(DevNet N, const u64* __restrict__ fr_dk,
+__global__ __launch_bounds__(64) void k_expand(const DevNet* __restrict__ Np, const u64* __restrict__ fr_dk,
                                                const u32* __restrict__ fr_zid, const int* __restrict__ zarena,
                                                int F, const int* __restrict__ rec_off, ExpandOut out,
                                                Counters* ctr, int init) {
   extern __shared__ int sm[];
+  const DevNet& N = *Np;
   int* src = sm;
   int* z = src + N.d2;
   int* tmp = z + N.d2;
@@ -344,9 +547,11 @@ __global__ __launch_bounds__(64) void k_expand(DevNet N, const u64* __restrict__
   int* ints = locs + HORA_MAX_PROCS;
   int* nl = ints + HORA_MAX_INTS;
   int* niv = nl + HORA_MAX_PROCS;
+  int* rec = niv + HORA_MAX_INTS;
+  int* inv = rec + 2 * kRecipeCap;
   const int i = blockIdx.x;
   if (i >= F) return;
-  const int l = threadIdx.x, d2 = N.d2;
+  const int l = threadIdx.x, d = N.d, d2 = N.d2;
   decode_key(N, fr_dk[i], locs, ints);
   if (init) {
     for (int e = l; e < d2; e += kWave) src[e] = kLe0;
@@ -355,7 +560,7 @@ __global__ __launch_bounds__(64) void k_expand(DevNet N, const u64* __restrict__
     for (int e = l; e < d2; e += kWave) src[e] = zs[e];
   }
   wsync();
-  auto emit = [&](int slot, int se, int re) {
+  auto emit = [&](int slot, int se, int re) __attribute__((always_inline)) {
     wsync();
     if (l < N.np) nl[l] = locs[l];
     if (l < N.ni) niv[l] = ints[l];
@@ -370,21 +575,39 @@ __global__ __launch_bounds__(64) void k_expand(DevNet N, const u64* __restrict__
     z_copy(z, src, d2);
     bool ok = true;
     if (se >= 0) {
+      // guards (sender then receiver), then every reset of both edges at once
       ok = apply_edge_guard(N, se, z);
       if (ok && re >= 0) ok = apply_edge_guard(N, re, z);
-      if (ok) {
-        apply_edge_resets(N, se, z);
-        if (re >= 0) apply_edge_resets(N, re, z);
-      }
+      if (ok) z_reset_many(z, d, edge_reset_mask(N, se) | (re >= 0 ? edge_reset_mask(N, re) : 0ull), tmp);
     }
     wsync();
-    if (ok) ok = close_succ(N, nl, niv, z, row0);
+    int fz = 0, ninv = 0;
+    u64 am = 0;
+    wave_loc_info(N, nl, niv, &fz, &am, inv, &ninv);
+    if (ninv > kInvCap) {
+      if (l == 0) atomicExch(&ctr->err, ERR_RECIPES);
+      ok = false;
+    }
+    // close_succ_zone (semantics.hpp:522-568): invariant, delay unless frozen, invariant,
+    // inactive clocks reset, Extra_LU+ when on; then the caller's emptiness test
+    if (ok) ok = z_apply_list(z, d, inv, ninv);
+    if (ok) {
+      if (!fz) {
+        z_up(z, d);
+        z_apply_list(z, d, inv, ninv);
+      }
+      const u64 dead = ~(am | N.qbits) & (((1ull << N.nclocks) - 1) << 1);
+      z_reset_many(z, d, dead, tmp);
+      if (N.lu_plus) z_extra_lu_plus(z, d, N.Lx, N.Ux, row0);
+      ok = !z_empty(z, d);
+    }
     const bool tg = ok && is_target(N, nl, niv, z, tmp);
+    const u64 key = wave_encode_key(N, nl, niv);
     wsync();
     if (l == 0) {
       out.valid[slot] = ok ? 1 : 0;
       out.tgt[slot] = tg ? 1 : 0;
-      out.dk[slot] = ok ? encode_key(N, nl, niv) : 0;
+      out.dk[slot] = ok ? key : 0;
       if (tg) atomicMin(&ctr->first_target, slot);
     }
     if (ok) {
@@ -396,9 +619,14 @@ __global__ __launch_bounds__(64) void k_expand(DevNet N, const u64* __restrict__
     emit(0, -1, -1);
     return;
   }
-  const bool src_c = (frozen_flags(N, locs, ints) & 1) != 0;
-  int slot = rec_off[i];
-  enum_recipes(N, locs, ints, src_c, [&](int se, int re) { emit(slot++, se, re); });
+  int fz;
+  u64 am;
+  int ninv;
+  wave_loc_info(N, locs, ints, &fz, &am, inv, &ninv);
+  const int nr = build_recipes(N, locs, ints, (fz & 1) != 0, rec, kRecipeCap);
+  wsync();
+  const int base = rec_off[i];
+  for (int r = 0; r < nr; ++r) emit(base + r, rec[2 * r], rec[2 * r + 1]);
 }
 
 // ---------------------------------------------------------------- scans
@@ -538,11 +766,12 @@ __global__ void k_find(Passed P, const int* __restrict__ sj, const u64* __restri
 // group walks the key's previous-level chain newest first.
 constexpr int kGroup = 16;
 
-__global__ __launch_bounds__(64) void k_probe(DevNet N, Passed P, const long* __restrict__ gbk,
+__global__ __launch_bounds__(64) void k_probe(const DevNet* __restrict__ Np, Passed P, const long* __restrict__ gbk,
                                               const int* __restrict__ sj, const u64* __restrict__ sdk,
                                               const int* __restrict__ szone, long G, int* __restrict__ covered,
                                               Counters* ctr) {
   __shared__ int Lg[4][HORA_MAX_DIM], Ug[4][HORA_MAX_DIM], lg[4][HORA_MAX_PROCS];
+  const DevNet& N = *Np;
   const int lane = threadIdx.x, q = lane / kGroup, t = lane % kGroup;
   const long g = (long)blockIdx.x * 4 + q;
   const bool active = g < G;
@@ -656,11 +885,12 @@ __global__ void k_scatter(Passed P, const long* __restrict__ gbk, const int* __r
 
 // ---------------------------------------------------------------- passed list: merge
 
-__device__ void bitonic_sort(int* a, int n) {   // n a power of two, ascending
+template <bool kGlobal>
+__device__ __forceinline__ void bitonic_sort(int* a, int n) {   // n a power of two, ascending
   const int l = threadIdx.x;
   for (int k = 2; k <= n; k <<= 1)
     for (int j = k >> 1; j > 0; j >>= 1) {
-      wsync();
+      if (kGlobal) bsync(); else wsync();
       for (int i = l; i < n; i += kWave) {
         const int ixj = i ^ j;
         if (ixj > i) {
@@ -670,7 +900,7 @@ __device__ void bitonic_sort(int* a, int n) {   // n a power of two, ascending
         }
       }
     }
-  wsync();
+  if (kGlobal) bsync(); else wsync();
 }
 
 constexpr u32 kNewTag = 0x40000000u;     // chain entry refers to a successor stored this level
@@ -703,24 +933,37 @@ __device__ __forceinline__ bool union_convex_lane(const int* a, const int* b, in
     if (au < kInf && bl < kInf && -r_bnd(bl) > r_bnd(au)) return false;
   }
   const int d2 = d * d;
-  u64 fa[4] = {0, 0, 0, 0}, pa[4] = {0, 0, 0, 0};
-  for (int e = 0; e < d2; ++e) {
-    const int av = a[e], bv = b[e];
-    if (av < bv) fa[e >> 6] |= 1ull << (e & 63);
-    else if (av > bv) pa[e >> 6] |= 1ull << (e & 63);
+  // entry sets as four 64-bit words each; every word index below is a compile-time constant
+  // after unrolling, so the words stay in registers
+  u64 fa[4], pa[4];
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    u64 mf = 0, mp = 0;
+    for (int bit = 0; bit < 64; ++bit) {
+      const int e = w * 64 + bit;
+      if (e >= d2) break;
+      const int av = a[e], bv = b[e];
+      if (av < bv) mf |= 1ull << bit;
+      else if (av > bv) mp |= 1ull << bit;
+    }
+    fa[w] = mf;
+    pa[w] = mp;
   }
+  // pass 1: facets of a (FA) crossed into the hull must land in b, checked on PA entries;
+  // pass 2: facets of b (PA) must land in a, checked on FA entries
+#pragma unroll
   for (int pass = 0; pass < 2; ++pass) {
-    const int* z = pass == 0 ? a : b;      // facets of z ...
-    const int* o = pass == 0 ? b : a;      // ... crossed inside the hull must land in o
-    const u64* F = pass == 0 ? fa : pa;
-    const u64* Pm = pass == 0 ? pa : fa;
+    const int* z = pass == 0 ? a : b;
+    const int* o = pass == 0 ? b : a;
+#pragma unroll
     for (int wf = 0; wf < 4; ++wf)
-      for (u64 mf = F[wf]; mf; mf &= mf - 1) {
+      for (u64 mf = pass == 0 ? fa[wf] : pa[wf]; mf; mf &= mf - 1) {
         const int f = wf * 64 + __builtin_ctzll(mf);
         const int i = f / d, j = f - i * d;
         const int neg = r_neg(z[f]);
+#pragma unroll
         for (int wp = 0; wp < 4; ++wp)
-          for (u64 mp = Pm[wp]; mp; mp &= mp - 1) {
+          for (u64 mp = pass == 0 ? pa[wp] : fa[wp]; mp; mp &= mp - 1) {
             const int e = wp * 64 + __builtin_ctzll(mp);
             const int p = e / d, q = e - p * d;
             const int hpj = max(a[p * d + j], b[p * d + j]);
@@ -761,11 +1004,12 @@ __host__ __device__ inline MergeLayout merge_layout(int d2, bool heavy) {
 // lane, and evaluate the 32-zone union window in one round, taking the first success in
 // newest-first order as the sequential fold does. At the end the key's live new zones get zone
 // ids in one allocation, move to the arena, and are reported as this level's survivors.
-__global__ __launch_bounds__(64) void k_merge(DevNet N, Passed P, const u32* __restrict__ list, int nlist,
+__global__ __launch_bounds__(64) void k_merge(const DevNet* __restrict__ Np, Passed P, const u32* __restrict__ list, int nlist,
                                               int* __restrict__ wlist, const int* __restrict__ sj,
                                               int* __restrict__ szone, int* __restrict__ survived,
                                               Counters* ctr, int heavy, u64 zbase) {
   extern __shared__ int sm[];
+  const DevNet& N = *Np;
   const int d = N.d, d2 = N.d2, l = threadIdx.x;
   const MergeLayout ml = merge_layout(d2, heavy != 0);
   int* S = sm;
@@ -799,12 +1043,12 @@ __global__ __launch_bounds__(64) void k_merge(DevNet N, Passed P, const u32* __r
     int p2 = 1;
     while (p2 < (int)n) p2 <<= 1;
     for (int k = l; k < p2; k += kWave) sbuf[k] = k < (int)n ? lst[k] : 0x7fffffff;
-    bitonic_sort(sbuf, p2);
+    bitonic_sort<false>(sbuf, p2);
     lst = sbuf;
   } else {
     const u64 cap = list_cap(n);
     for (u64 k = n + l; k < cap; k += kWave) lst[k] = 0x7fffffff;
-    bitonic_sort(lst, (int)cap);
+    bitonic_sort<true>(lst, (int)cap);
   }
 
   // a_LU bounds of this key (state_bounds, engine.cpp:1487-1497)
@@ -833,10 +1077,13 @@ __global__ __launch_bounds__(64) void k_merge(DevNet N, Passed P, const u32* __r
   const u64 seg = P.nseg[h];
   u32* ch = chl;
   int* cs = cslot;
-  if (old_len + n > (u32)ml.chain_cap) {
+  const bool chg = old_len + n > (u32)ml.chain_cap;
+  if (chg) {
     ch = P.nodes + seg;
     cs = (int*)(P.nodes + seg + old_len + n);
   }
+  // ordering after a chain write: the chain may be in global memory
+  auto csync = [&]() __attribute__((always_inline)) { if (chg) bsync(); else wsync(); };
   // cache the newest min(old_len, cz) stored zones; the rest of the slots start free
   const int ncached = min((int)old_len, cz);
   for (u32 k = l; k < old_len; k += kWave) {
@@ -844,7 +1091,7 @@ __global__ __launch_bounds__(64) void k_merge(DevNet N, Passed P, const u32* __r
     const int slot = (int)k - ((int)old_len - ncached);
     cs[k] = slot >= 0 ? slot : -1;
   }
-  wsync();
+  csync();
   for (int k = 0; k < ncached; ++k) {
     const int* src = P.zarena + (size_t)ch[old_len - ncached + k] * d2;
     for (int e = l; e < d2; e += kWave) cache[k * zs + e] = src[e];
@@ -852,7 +1099,7 @@ __global__ __launch_bounds__(64) void k_merge(DevNet N, Passed P, const u32* __r
   // cache slots are handed out upwards to new members until the cache is full
   int next_slot = ncached;
   // new members are tagged with their position k in the sorted work list
-  auto zp = [&](int pos) -> const int* {
+  auto zp = [&](int pos) __attribute__((always_inline)) -> const int* {
     const int s = cs[pos];
     if (s >= 0) return cache + s * zs;
     const u32 id = ch[pos] & ~kDead;
@@ -898,6 +1145,7 @@ __global__ __launch_bounds__(64) void k_merge(DevNet N, Passed P, const u32* __r
           if (z_alu_le(zp(m), S, d, Lb, Ub)) {
             wsync();
             if (l == 0) ch[m] = id | kDead;
+            csync();
             --live;
             ++n_kill;
           }
@@ -910,6 +1158,7 @@ __global__ __launch_bounds__(64) void k_merge(DevNet N, Passed P, const u32* __r
           const u64 b = __ballot(kill);
           wsync();
           if (kill) ch[m] |= kDead;
+          csync();
           live -= __popcll(b);
           n_kill += __popcll(b);
         }
@@ -953,6 +1202,7 @@ __global__ __launch_bounds__(64) void k_merge(DevNet N, Passed P, const u32* __r
           const int m = cand[win];
           wsync();
           if (l == 0) ch[m] |= kDead;
+          csync();
           --live;
           ++n_uni;
           widened = true;
@@ -967,8 +1217,10 @@ __global__ __launch_bounds__(64) void k_merge(DevNet N, Passed P, const u32* __r
     if (next_slot < cz) slot = next_slot++;
     if (slot >= 0)
       for (int e = l; e < d2; e += kWave) cache[slot * zs + e] = S[e];
-    if (widened)
+    if (widened) {
       for (int e = l; e < d2; e += kWave) sz[e] = S[e];
+      bsync();   // the slot is read back by other lanes when the zone is not cached
+    }
     if (l == 0) {
       ch[len] = kNewTag | k;
       cs[len] = slot;
@@ -976,7 +1228,7 @@ __global__ __launch_bounds__(64) void k_merge(DevNet N, Passed P, const u32* __r
     ++len;
     ++live;
     ++n_ins;
-    wsync();
+    csync();
   }
   // Live new zones move to the arena, each at the zone id reserved for its work-list position
   // (zbase + noff + k); the chain is compacted oldest first into this key's reserved segment.
@@ -1215,12 +1467,9 @@ struct hora_hip_ctx {
   hipStream_t stream = nullptr;
   int d = 0, d2 = 0, np = 0, ni = 0, nclocks = 0, tl = 0, ne = 0;
   // static tables
-  DBuf<int> locbase, loc_flags, inv_off, inv_data, le_off, le_edges, edge_proc, edge_tgt, edge_chan, edge_dir,
-      cg_off, cg_data, rs_off, rs_clk, ig_off, ig_data, iu_off, iu_data, Lx, Ux, lloc, uloc, Lq, Uq, kshift,
-      int_lo, lr, ic, cc;
-  DBuf<unsigned char> chan_bcast, chan_urgent;
-  DBuf<u64> amask, kmask;
+  DBuf<int> blob, lloc, uloc;
   DevNet dn{};
+  DevNet* dnp = nullptr;   // device copy of dn, refreshed when dn changes
   std::vector<int> h_loc_init, h_int_init, h_kshift, h_int_lo;
   std::vector<u64> h_kmask;
   int has_target_clock = 0;
@@ -1371,6 +1620,7 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
   Tally prev;
   const int d2 = c->d2;
   c->dn.lu_plus = 0;
+  HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
 
   // passed list and arenas
   alloc_table(c, 20);
@@ -1404,7 +1654,9 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     if (!init) {
       st->explored += (u64)F;
       ++level;
-      hipLaunchKernelGGL(k_count, dim3(F), dim3(64), 0, c->stream, c->dn, c->fr_dk.p, (int)F, c->rec_cnt.p);
+      const size_t lds_c = (HORA_MAX_PROCS + HORA_MAX_INTS + 2 * kRecipeCap + 3 * kInvCap) * sizeof(int);
+      hipLaunchKernelGGL(k_count, dim3(F), dim3(64), lds_c, c->stream, c->dnp,
+                         c->fr_dk.p, (int)F, c->rec_cnt.p, c->ctr.p);
       R = scan_exclusive(c, c->rec_cnt.p, F, c->pos.p);
       hipLaunchKernelGGL(k_offsets_to_int, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->pos.p, F, c->rec_off.p);
     } else {
@@ -1419,7 +1671,7 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     const size_t lds_e = expand_lds_ints(d2) * sizeof(int);
     HIP_OK(hipEventRecord(c->ev[0], c->stream));
     if (R > 0)
-      hipLaunchKernelGGL(k_expand, dim3(F), dim3(64), lds_e, c->stream, c->dn, c->fr_dk.p, c->fr_zid.p,
+      hipLaunchKernelGGL(k_expand, dim3(F), dim3(64), lds_e, c->stream, c->dnp, c->fr_dk.p, c->fr_zid.p,
                          c->zarena.p, (int)F, c->rec_off.p, eo, c->ctr.p, init ? 1 : 0);
     HIP_OK(hipEventRecord(c->ev[1], c->stream));
     // ---- compaction of non-empty successors
@@ -1460,7 +1712,7 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     hipLaunchKernelGGL(k_find, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->sj.p, c->s_dk.p, G,
                        c->gbk.p, c->ctr.p);
     HIP_OK(hipEventRecord(c->ev[5], c->stream));
-    hipLaunchKernelGGL(k_probe, dim3(blocks_for(G, 4)), dim3(64), 0, c->stream, c->dn, P, c->gbk.p, c->sj.p,
+    hipLaunchKernelGGL(k_probe, dim3(blocks_for(G, 4)), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p,
                        c->s_dk.p, c->s_zone.p, G, c->covered.p, c->ctr.p);
     HIP_OK(hipEventRecord(c->ev[6], c->stream));
     hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p, G,
@@ -1500,10 +1752,10 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     const size_t lds_h = merge_layout(d2, true).total() * sizeof(int);
     HIP_OK(hipEventRecord(c->ev[3], c->stream));
     if (nlh[0] > 0)
-      hipLaunchKernelGGL(k_merge, dim3(nlh[0]), dim3(64), lds_l, c->stream, c->dn, P, c->klight.p, nlh[0],
+      hipLaunchKernelGGL(k_merge, dim3(nlh[0]), dim3(64), lds_l, c->stream, c->dnp, P, c->klight.p, nlh[0],
                          c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 0, zcount);
     if (nlh[1] > 0)
-      hipLaunchKernelGGL(k_merge, dim3(nlh[1]), dim3(64), lds_h, c->stream, c->dn, P, c->kheavy.p, nlh[1],
+      hipLaunchKernelGGL(k_merge, dim3(nlh[1]), dim3(64), lds_h, c->stream, c->dnp, P, c->kheavy.p, nlh[1],
                          c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 1, zcount);
     HIP_OK(hipEventRecord(c->ev[4], c->stream));
     zcount += (u64)W;
@@ -1534,7 +1786,9 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     prev = hc;
     if (hc.err) {
       c->err = hc.err == ERR_HASH_FULL ? "passed-list hash table full"
-               : hc.err == ERR_ZONES_FULL ? "zone arena full" : "chain node arena full";
+               : hc.err == ERR_ZONES_FULL ? "zone arena full"
+               : hc.err == ERR_RECIPES ? "a state has more transitions or invariant constraints than the device holds"
+                                       : "chain node arena full";
       return HORA_HIP_E_MEMORY;
     }
     std::swap(c->fr_dk.p, c->fr_dk2.p); std::swap(c->fr_dk.cap, c->fr_dk2.cap);
@@ -1544,6 +1798,8 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     // adaptive Extra_LU+ switch (engine.cpp:2178-2184)
     if (!c->dn.lu_plus && hc[C_KEYS] && (double)hc[C_LIVE] >= 4.0 * (double)hc[C_KEYS]) {
       c->dn.lu_plus = 1;
+      HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
+      HIP_OK(hipStreamSynchronize(c->stream));
       st->emode_switch_level = level;
     }
     if (F == 0) break;
@@ -1646,51 +1902,80 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
   HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto& ev : c->ev) HIP_OK(hipEventCreate(&ev));
   const int ne = std::max(c->ne, 1);
-  c->locbase.upload(vec(e->locbase, c->np + 1));
-  c->loc_flags.upload(vec(e->loc_flags, c->tl));
-  c->inv_off.upload(vec(e->inv_off, c->tl + 1));
-  c->inv_data.upload(vec(e->inv_data, 3L * e->inv_off[c->tl]));
-  c->le_off.upload(le_off);
-  c->le_edges.upload(le_edges);
-  c->edge_proc.upload(vec(e->edge_proc, ne));
-  c->edge_tgt.upload(vec(e->edge_tgt, ne));
-  c->edge_chan.upload(vec(e->edge_chan, ne));
-  c->edge_dir.upload(vec(e->edge_dir, ne));
-  c->cg_off.upload(vec(e->cg_off, c->ne + 1));
-  c->cg_data.upload(vec(e->cg_data, 3L * e->cg_off[c->ne]));
-  c->rs_off.upload(vec(e->rs_off, c->ne + 1));
-  c->rs_clk.upload(vec(e->rs_clk, (long)e->rs_off[c->ne]));
-  c->ig_off.upload(vec(e->ig_off, c->ne + 1));
-  c->ig_data.upload(vec(e->ig_data, 3L * e->ig_off[c->ne]));
-  c->iu_off.upload(vec(e->iu_off, c->ne + 1));
-  c->iu_data.upload(vec(e->iu_data, 2L * e->iu_off[c->ne]));
-  c->chan_bcast.upload(vec(e->chan_bcast, std::max(e->nchans, 1)));
-  c->chan_urgent.upload(vec(e->chan_urgent, std::max(e->nchans, 1)));
-  c->Lx.upload(Lx);
-  c->Ux.upload(Ux);
+  // all static tables except the per-location bounds go into one blob (u64 tables 8-aligned)
+  std::vector<int> blob;
+  auto put = [&](const std::vector<int>& v) {
+    if (blob.size() & 1) blob.push_back(0);
+    const size_t off = blob.size();
+    blob.insert(blob.end(), v.begin(), v.end());
+    if (v.empty()) blob.push_back(0);
+    return off;
+  };
+  auto put64 = [&](const std::vector<u64>& v) {
+    std::vector<int> w(2 * std::max<size_t>(v.size(), 1), 0);
+    if (!v.empty()) std::memcpy(w.data(), v.data(), v.size() * sizeof(u64));
+    return put(w);
+  };
+  std::vector<int> bc(std::max(e->nchans, 1), 0), ur(std::max(e->nchans, 1), 0);
+  for (int ch = 0; ch < e->nchans; ++ch) {
+    bc[ch] = e->chan_bcast ? e->chan_bcast[ch] : 0;
+    ur[ch] = e->chan_urgent ? e->chan_urgent[ch] : 0;
+  }
+  const size_t o_locbase = put(vec(e->locbase, c->np + 1));
+  const size_t o_loc_flags = put(vec(e->loc_flags, c->tl));
+  const size_t o_inv_off = put(vec(e->inv_off, c->tl + 1));
+  const size_t o_inv_data = put(vec(e->inv_data, 3L * e->inv_off[c->tl]));
+  const size_t o_le_off = put(le_off);
+  const size_t o_le_edges = put(le_edges);
+  const size_t o_edge_proc = put(vec(e->edge_proc, ne));
+  const size_t o_edge_tgt = put(vec(e->edge_tgt, ne));
+  const size_t o_edge_chan = put(vec(e->edge_chan, ne));
+  const size_t o_edge_dir = put(vec(e->edge_dir, ne));
+  const size_t o_cg_off = put(vec(e->cg_off, c->ne + 1));
+  const size_t o_cg_data = put(vec(e->cg_data, 3L * e->cg_off[c->ne]));
+  const size_t o_rs_off = put(vec(e->rs_off, c->ne + 1));
+  const size_t o_rs_clk = put(vec(e->rs_clk, (long)e->rs_off[c->ne]));
+  const size_t o_ig_off = put(vec(e->ig_off, c->ne + 1));
+  const size_t o_ig_data = put(vec(e->ig_data, 3L * e->ig_off[c->ne]));
+  const size_t o_iu_off = put(vec(e->iu_off, c->ne + 1));
+  const size_t o_iu_data = put(vec(e->iu_data, 2L * e->iu_off[c->ne]));
+  const size_t o_bc = put(bc);
+  const size_t o_ur = put(ur);
+  const size_t o_Lx = put(Lx);
+  const size_t o_Ux = put(Ux);
+  const size_t o_Lq = put(Lq);
+  const size_t o_Uq = put(Uq);
+  const size_t o_amask = put64(vec(e->active_mask, c->tl));
+  const size_t o_kshift = put(c->h_kshift);
+  const size_t o_kmask = put64(c->h_kmask);
+  const size_t o_int_lo = put(c->h_int_lo);
+  const size_t o_lr = put(lrv);
+  const size_t o_ic = put(icv);
+  const size_t o_cc = put(ccv);
+  if (blob.size() & 1) blob.push_back(0);
+  c->blob.upload(blob);
   c->lloc.upload(vec(e->lbound_loc, (long)c->tl * d));
   c->uloc.upload(vec(e->ubound_loc, (long)c->tl * d));
-  c->Lq.upload(Lq);
-  c->Uq.upload(Uq);
-  c->amask.upload(vec(e->active_mask, c->tl));
-  c->kshift.upload(c->h_kshift);
-  c->kmask.upload(c->h_kmask);
-  c->int_lo.upload(c->h_int_lo);
-  c->lr.upload(lrv);
-  c->ic.upload(icv);
-  c->cc.upload(ccv);
+  const int* B = c->blob.p;
 
   DevNet& N = c->dn;
   N.d = d; N.d2 = c->d2; N.np = c->np; N.ni = c->ni; N.nclocks = c->nclocks; N.lu_plus = 0;
-  N.locbase = c->locbase.p; N.loc_flags = c->loc_flags.p; N.inv_off = c->inv_off.p; N.inv_data = c->inv_data.p;
-  N.le_off = c->le_off.p; N.le_edges = c->le_edges.p; N.edge_proc = c->edge_proc.p; N.edge_tgt = c->edge_tgt.p;
-  N.edge_chan = c->edge_chan.p; N.edge_dir = c->edge_dir.p;
-  N.cg_off = c->cg_off.p; N.cg_data = c->cg_data.p; N.rs_off = c->rs_off.p; N.rs_clk = c->rs_clk.p;
-  N.ig_off = c->ig_off.p; N.ig_data = c->ig_data.p; N.iu_off = c->iu_off.p; N.iu_data = c->iu_data.p;
-  N.chan_bcast = c->chan_bcast.p; N.chan_urgent = c->chan_urgent.p; N.has_urgent_chan = has_urgent;
-  N.Lx = c->Lx.p; N.Ux = c->Ux.p; N.lloc = c->lloc.p; N.uloc = c->uloc.p; N.Lq = c->Lq.p; N.Uq = c->Uq.p;
-  N.amask = c->amask.p; N.qbits = qbits; N.kshift = c->kshift.p; N.kmask = c->kmask.p; N.int_lo = c->int_lo.p;
-  N.never = never; N.nlr = nlr; N.nic = nic; N.ncc = ncc; N.lr = c->lr.p; N.ic = c->ic.p; N.cc = c->cc.p;
+  N.locbase = B + o_locbase; N.loc_flags = B + o_loc_flags; N.inv_off = B + o_inv_off; N.inv_data = B + o_inv_data;
+  N.le_off = B + o_le_off; N.le_edges = B + o_le_edges; N.edge_proc = B + o_edge_proc; N.edge_tgt = B + o_edge_tgt;
+  N.edge_chan = B + o_edge_chan; N.edge_dir = B + o_edge_dir;
+  N.cg_off = B + o_cg_off; N.cg_data = B + o_cg_data; N.rs_off = B + o_rs_off; N.rs_clk = B + o_rs_clk;
+  N.ig_off = B + o_ig_off; N.ig_data = B + o_ig_data; N.iu_off = B + o_iu_off; N.iu_data = B + o_iu_data;
+  N.chan_bcast = B + o_bc; N.chan_urgent = B + o_ur; N.has_urgent_chan = has_urgent;
+  N.Lx = B + o_Lx; N.Ux = B + o_Ux; N.lloc = c->lloc.p; N.uloc = c->uloc.p; N.Lq = B + o_Lq; N.Uq = B + o_Uq;
+  N.amask = (const u64*)(B + o_amask); N.qbits = qbits; N.kshift = B + o_kshift; N.kmask = (const u64*)(B + o_kmask);
+  N.int_lo = B + o_int_lo;
+  N.lr = B + o_lr; N.ic = B + o_ic; N.cc = B + o_cc;
+  N.blob = B;
+  N.blob_ints = (int)blob.size();
+  N.stage = 0;   // staging the tables in LDS measured slower than the L1/L2-resident originals
+  N.never = never; N.nlr = nlr; N.nic = nic; N.ncc = ncc;
+  HIP_OK(hipMalloc(&c->dnp, sizeof(DevNet)));
+  HIP_OK(hipMemcpy(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice));
 }
 
 }  // namespace
@@ -1733,6 +2018,7 @@ void hora_hip_close(hora_hip_ctx* ctx) {
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->dnp) (void)hipFree(ctx->dnp);
   delete ctx;
 }
 

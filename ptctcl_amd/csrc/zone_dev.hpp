@@ -36,7 +36,20 @@ __device__ __forceinline__ int r_neg(int r) {
   return r_strict(r) ? r_le(-v) : r_lt(-v);
 }
 
-__device__ __forceinline__ void wsync() { __syncthreads(); }
+// Ordering point for LDS inside a one-wavefront workgroup. The wavefront's LDS instructions
+// execute in issue order, so a lane reading what another lane wrote needs only that the compiler
+// keeps the accesses in program order and that earlier LDS operations have completed
+// (lgkmcnt(0)). A workgroup barrier (__syncthreads) would also wait for every outstanding global
+// load and store (vmcnt(0) counts both on CDNA), stalling each zone operation on HBM latency.
+__device__ __forceinline__ void wsync() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0); vmcnt and expcnt left at their maxima
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Full workgroup barrier with memory ordering: for global memory written by one lane and read
+// by another, and for multi-wavefront workgroups.
+__device__ __forceinline__ void bsync() { __syncthreads(); }
 
 __device__ __forceinline__ void z_copy(int* __restrict__ dst, const int* __restrict__ src, int d2) {
   for (int e = threadIdx.x; e < d2; e += kWave) dst[e] = src[e];
