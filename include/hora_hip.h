@@ -44,6 +44,7 @@ extern "C" {
 #define HORA_HIP_E_DEVICE (-3)   /* HIP runtime error / no device */
 #define HORA_HIP_E_MEMORY (-4)   /* device memory exhausted or a capacity exceeded */
 #define HORA_HIP_E_INTERNAL (-5)
+#define HORA_HIP_E_CAPPED (-6)   /* a federation outgrew its capacity: no verdict (GfpStats::capped) */
 
 /* Integer guard operators, same order as enum class IntOp (include/hora/network.hpp:23). */
 enum { HORA_OP_EQ = 0, HORA_OP_NE = 1, HORA_OP_LT = 2, HORA_OP_LE = 3, HORA_OP_GT = 4, HORA_OP_GE = 5 };
@@ -180,11 +181,93 @@ enum {
   HORA_ZOP_ALU_LE = 6,     /* out_flag[z] = (a[z] <=_aLU b[z]) under L/U[z] (dbm_is_alu_le)       */
   HORA_ZOP_UNION = 7,      /* out_flag[z] = (a[z] U b[z] convex); out zone = hull when convex,
                               else a copy of a[z] (dbm_try_union)                                 */
-  HORA_ZOP_EMPTY = 8       /* out_flag[z] = (zone is empty) (dbm_empty)                          */
+  HORA_ZOP_EMPTY = 8,      /* out_flag[z] = (zone is empty) (dbm_empty)                          */
+  HORA_ZOP_DOWN = 9,       /* time predecessors: z[0][j] = min(<=0, min_i z[i][j]) (Dbm::down)     */
+  HORA_ZOP_FREE = 10       /* forget the clocks of the bit mask arg[z] (dbm_free_many)            */
 };
 int hora_hip_zone_op(int32_t op, int32_t d, int64_t n, const int32_t* a, const int32_t* b,
                      const int32_t* arg, const int32_t* L, const int32_t* U, int32_t* out,
                      int32_t* out_flag, void* stream);
+
+
+/* ---- liveness: p --> q, E[] psi, A[] not deadlock (backward greatest fixpoint) ----
+ * Replaces leads_to_otf / exists_invariant_run_otf / deadlock_reachable
+ * (include/hora/liveness.hpp:37-170) on the reference's a_LU lane: one forward zone graph, then
+ * W = W /\ (NoSucc \/ Pre(W)) over the closure the query reads (src/liveness/fixpoint.cpp:48-965).
+ *
+ * A discrete predicate in disjunctive normal form, the discrete part of hora::DiscPred
+ * (include/hora/predicate.hpp:36-80): true iff some term has all its atoms true. An atom is four
+ * int32: kind (0 = location of process `index`, 1 = integer variable `index`), index, op
+ * (HORA_OP_*; locations take EQ or NE), value. nterms = 0 is `false`; a term with no atom is `true`. */
+typedef struct hora_dpred {
+  int32_t nterms;
+  const int32_t* term_off;   /* [nterms+1] atom offsets */
+  const int32_t* atoms;      /* 4 int32 per atom */
+} hora_dpred;
+
+enum {
+  HORA_LIVE_LEADSTO = 0,     /* p --> q: holds iff no reachable p-state has a maximal run avoiding q */
+  HORA_LIVE_EG = 1,          /* E[] psi: holds iff the initial state has a maximal psi-run          */
+  HORA_LIVE_DEADLOCK = 2     /* A[] not deadlock: holds iff no reachable state is deadlocked        */
+};
+
+typedef struct hora_live_query {
+  int32_t kind;
+  int32_t early;             /* LEADSTO / DEADLOCK: settle at the first witness state
+                                (leadsto_witness, src/liveness/construction.cpp:786-813)           */
+  hora_dpred p;              /* LEADSTO: p.  EG: psi's discrete part.  DEADLOCK: unused           */
+  hora_dpred q;              /* LEADSTO: q.  Otherwise unused                                     */
+  /* EG whose psi has a clock conjunction: the context was opened on the network restricted by it
+   * (the conjunction adjoined to every location invariant, analyses recomputed); these are the
+   * unrestricted network's invariant tables, in hora_net_enc's layout, for the maximality test
+   * (stop_set's restricted branch, src/liveness/construction.cpp:1200-1238). NULL otherwise. */
+  const int32_t* orig_inv_off;
+  const int32_t* orig_inv_data;
+} hora_live_query;
+
+/* Mirrors hora::GfpStats (include/hora/liveness.hpp:48-76) plus construction counters. */
+typedef struct hora_live_stats {
+  int64_t outer_nodes;       /* zone-graph nodes stored                                         */
+  int64_t fwd_edges;
+  int64_t disc_keys;
+  int64_t expanded_nodes;
+  int64_t generated;         /* non-empty successors                                            */
+  int64_t covered;           /* successors routed to an existing node                           */
+  int64_t levels;            /* forward BFS levels                                              */
+  int64_t closure;           /* nodes the fixpoint runs over                                    */
+  int64_t nonempty;          /* of those, nodes whose converged value is non-empty              */
+  int64_t iters;             /* fixpoint rounds (device) or value-changing relaxations (oracle) */
+  int64_t pre_calls;         /* exact predecessor computations                                  */
+  int64_t fed_max;           /* widest converged value, in zones                                */
+  int32_t early;             /* 1: settled by a witness state during the construction           */
+  int32_t capped;            /* 1: a federation outgrew its capacity; no verdict                */
+  double seconds;            /* whole query                                                     */
+  double build_seconds;      /* forward construction                                            */
+  double gfp_seconds;        /* stop sets, closure and fixpoint                                 */
+} hora_live_stats;
+
+/* Decide one liveness query on the context's network. Writes 1 to *holds if the property holds.
+ * Returns HORA_HIP_E_SCOPE for what the device liveness path does not cover (d > 16, broadcast
+ * channels), HORA_HIP_E_CAPPED when a federation outgrew the device capacity. */
+int hora_hip_live(hora_hip_ctx* ctx, const hora_live_query* q, int32_t* holds, hora_live_stats* stats);
+
+/* Inspection of the last hora_hip_live's graph and fixpoint (no counterpart in the reference:
+ * for parity tests and tools). Always writes nodes, edges and d; copies each array that is
+ * non-NULL (sized from a first call with NULL arrays). Node ids are creation order. */
+typedef struct hora_live_graph {
+  int64_t nodes, edges;
+  int32_t d, pad;
+  uint64_t* dk;              /* [nodes] packed discrete key                                     */
+  int32_t* zones;            /* [nodes*d*d] stored zones                                        */
+  int64_t* row_off;          /* [nodes+1] edges of node v are [row_off[v], row_off[v+1])        */
+  int32_t* etgt;             /* [edges] target node                                             */
+  int32_t* ese;              /* [edges] first participant edge (sender or local)                */
+  int32_t* ere;              /* [edges] receiver edge or -1                                     */
+  int8_t* expanded;          /* [nodes]                                                         */
+  int8_t* in_s;              /* [nodes] in the fixpoint's closure                               */
+  int8_t* w_nonempty;        /* [nodes] converged value non-empty                               */
+} hora_live_graph;
+int hora_hip_live_graph(const hora_hip_ctx* ctx, hora_live_graph* g);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,7 @@ import subprocess
 
 import numpy as np
 
+from ptctcl_amd import liveness as lv
 from ptctcl_amd.network import NetEnc, TargetEnc, hora_reach_stats
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -36,6 +37,13 @@ def lib():
         _lib.oracle_reach_bounded.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
                                               C.POINTER(C.c_int32), C.POINTER(hora_reach_stats),
                                               C.POINTER(C.c_int32)]
+        _lib.oracle_live.argtypes = [C.c_void_p, C.POINTER(lv.hora_live_query), C.POINTER(C.c_int32),
+                                     C.POINTER(lv.hora_live_stats), C.POINTER(C.c_void_p)]
+        _lib.oracle_live_graph.argtypes = [C.c_void_p, C.POINTER(lv.hora_live_graph)]
+        _lib.oracle_live_free.argtypes = [C.c_void_p]
+        _lib.oracle_live_free.restype = None
+        _lib.oracle_live_last_error.restype = C.c_char_p
+        _lib.oracle_zone_subtract.argtypes = [C.c_int32, C.c_int64] + [C.c_void_p] * 2 + [C.c_int32] + [C.c_void_p] * 2
     return _lib
 
 
@@ -111,3 +119,40 @@ def reach_bounded(net, target, nthreads: int, max_seconds: float):
     if rc != 0:
         raise RuntimeError(f"oracle_reach_bounded: {rc}")
     return bool(done.value), st.as_dict()
+
+
+def live(net, query, graph: bool = False):
+    """One liveness query on the CPU restatement: (holds, LiveStats[, graph arrays])."""
+    enc = net.encode()
+    qe = query.encode()
+    r = C.c_int32(0)
+    st = lv.hora_live_stats()
+    h = C.c_void_p()
+    rc = lib().oracle_live(C.addressof(enc.struct), C.byref(qe.struct), C.byref(r), C.byref(st), C.byref(h))
+    try:
+        if rc != 0:
+            raise RuntimeError(f"oracle_live: {rc} {lib().oracle_live_last_error().decode()}")
+        stats = lv.LiveStats.from_c(st)
+        if not graph:
+            return bool(r.value), stats
+        g = lv.hora_live_graph()
+        lib().oracle_live_graph(h, C.byref(g))
+        n, e, d = g.nodes, g.edges, g.d
+        arrs = lv.graph_arrays(n, e, d)
+        lv.fill_graph_struct(g, arrs)
+        lib().oracle_live_graph(h, C.byref(g))
+        return bool(r.value), stats, lv.trim_graph(arrs, n, e, d)
+    finally:
+        if h:
+            lib().oracle_live_free(h)
+
+
+def zone_subtract(d: int, p, y, cap: int = 256):
+    """p \\ y per pair: list of piece arrays (None where more than cap pieces)."""
+    p = np.ascontiguousarray(p, np.int32)
+    y = np.ascontiguousarray(y, np.int32)
+    n = p.shape[0]
+    out = np.zeros((n, cap, d * d), np.int32)
+    cnt = np.zeros(n, np.int32)
+    lib().oracle_zone_subtract(d, n, _p(p), _p(y), cap, _p(out), _p(cnt))
+    return [None if cnt[k] < 0 else out[k, : cnt[k]] for k in range(n)]

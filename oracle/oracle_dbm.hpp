@@ -179,4 +179,71 @@ inline bool dbm_try_union(raw_t* a, const raw_t* b, int d) {
   return true;
 }
 
+// ---- operations of the backward (liveness) pass -------------------------------------------------
+// Textbook definitions (Bengtsson & Yi 2004, section 4; the UPPAAL DBM library's down / freeClock),
+// which HDBM's Dbm::down and dbm_free_many restate; the reference calls them from
+// src/liveness/operators.cpp:257-306 and src/reachability/deadlock.cpp:74-268.
+
+// Time predecessors: every lower bound x_j >= ... relaxed to what the other clocks force,
+// z[0][j] = min(<=0, min_i z[i][j]). Canonical in, canonical out.
+inline void dbm_down(raw_t* z, int d) {
+  for (int j = 1; j < d; ++j) {
+    raw_t v = kLe0;
+    for (int i = 1; i < d; ++i) v = std::min(v, z[i * d + j]);
+    z[j] = v;
+  }
+}
+
+// Forget the clocks of `mask` (bit c = clock c): row c unbounded, column c takes column 0, i.e.
+// only x_c >= 0 remains. Freeing several clocks at once equals freeing them one by one.
+inline void dbm_free_many(raw_t* z, int d, std::uint64_t mask) {
+  if (!mask) return;
+  std::vector<raw_t> t(z, z + d * d);
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) {
+      raw_t v;
+      if (i == j) v = kLe0;
+      else if ((mask >> i) & 1) v = kInf;
+      else if ((mask >> j) & 1) v = t[i * d];
+      else v = t[i * d + j];
+      z[i * d + j] = v;
+    }
+}
+
+// Definitely disjoint: a two-clock cycle through both zones is negative. A necessary condition
+// for an empty intersection of canonical zones, used only to skip work.
+inline bool dbm_disjoint_quick(const raw_t* a, const raw_t* b, int d) {
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j)
+      if (radd(a[i * d + j], b[j * d + i]) < kLe0) return true;
+  return false;
+}
+
+// out := a /\ b (entrywise min, then closure). Returns false iff the intersection is empty.
+inline bool dbm_intersect(raw_t* out, const raw_t* a, const raw_t* b, int d) {
+  if (dbm_disjoint_quick(a, b, d)) return false;
+  for (int k = 0; k < d * d; ++k) out[k] = std::min(a[k], b[k]);
+  return dbm_close(out, d);
+}
+
+// p \ y as disjoint zones appended to `out` (flat, d*d each): walk y's facets (i, j) in row-major
+// order; where y is tighter than the remainder r, split off r /\ not(x_i - x_j <| y_ij) and keep
+// r /\ (x_i - x_j <| y_ij). When p and y are disjoint the answer is p itself.
+inline void dbm_subtract(const raw_t* p, const raw_t* y, int d, std::vector<raw_t>& out) {
+  const int d2 = d * d;
+  std::vector<raw_t> r(p, p + d2), t(d2);
+  if (!dbm_intersect(t.data(), p, y, d)) { out.insert(out.end(), p, p + d2); return; }
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) {
+      if (i == j) continue;
+      const raw_t yij = y[i * d + j];
+      if (yij >= r[i * d + j]) continue;
+      std::copy(r.begin(), r.end(), t.begin());
+      const int v = bnd(yij);
+      const raw_t neg = is_strict(yij) ? le(-v) : lt(-v);
+      if (dbm_constrain(t.data(), d, j, i, neg)) out.insert(out.end(), t.begin(), t.end());
+      if (!dbm_constrain(r.data(), d, i, j, yij)) return;
+    }
+}
+
 }  // namespace orc

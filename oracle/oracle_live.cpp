@@ -1,0 +1,698 @@
+// oracle_live.cpp — CPU restatement of HORA's liveness pass: forward zone graph with recorded
+// transitions, stop sets, exact predecessor, and the backward greatest fixpoint
+// W = W /\ (NoSucc \/ Pre(W)), for `p --> q`, `E[] psi` and `A[] not deadlock`.
+//
+// TEST INFRASTRUCTURE ONLY: this is the oracle. Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may build, load or call it; the HIP back end never does.
+//
+// What is restated, and from where (paths relative to the HORA source tree):
+//   transitions      tagged_successors, src/liveness/operators.cpp:55-206 (local, then binary
+//                    synchronisation; the order of gen_succ_reuse). No extrapolation on this lane
+//                    (emode = ExNone, src/liveness/engine.cpp:256-276).
+//   graph            LivenessEngine::intern / expand, src/liveness/construction.cpp:863-1086,
+//                    on its append-only branch: a successor goes to the first (oldest) stored node
+//                    of its discrete state that covers it, else it becomes a new node. The
+//                    in-place enlargement and convex-union rewrites of intern() are not taken:
+//                    the reference names append-only storage as the alternative they trade
+//                    against (construction.cpp:855-861); it stores more nodes and decides the same.
+//   covering         a_LU with per-location bounds raised to M = max(L, U) on every clock, i.e.
+//                    the a_M bisimulation the reference uses where a deadlock must be preserved
+//                    (construction.cpp:899-913, Behrmann-Bouyer-Larsen-Pelanek STTT 2006 L1/L3).
+//   order            level-synchronous: the frontier of a level is the nodes the previous level
+//                    created, in creation order; successors are processed in (frontier index,
+//                    transition index) order, so node ids are creation order. The reference's
+//                    worklist is a FIFO over the same graph (src/liveness/api.cpp:37-117).
+//   stop set         stop_set, construction.cpp:1200-1238; time_diverges :1097-1162;
+//                    orig_time_open :1164-1191; state_has_deadlock,
+//                    src/reachability/deadlock.cpp:74-268.
+//   predecessor      exact_predecessor, src/liveness/operators.cpp:257-306 (tag resets
+//                    src/liveness/fixpoint.cpp:638-647, dead mask :599-633).
+//   fixpoint         solve_greatest_fixpoint's relax, src/liveness/fixpoint.cpp:750-864, run to
+//                    convergence by a worklist; the limit is the unique greatest fixpoint, so any
+//                    fair schedule reaches the same sets (the device runs Jacobi rounds).
+//   readouts         certificate_in / accepting_closure (fixpoint.cpp:979-1101); eg_closure /
+//                    eg_verdict (src/liveness/api.cpp:169-188, 290-337); early witness
+//                    leadsto_witness (construction.cpp:786-813), checked after each level.
+//
+// Federations are lists of zones; only the sets they denote matter to the verdict and to the
+// per-node emptiness the tests compare (zone difference splits along the subtrahend's facets in
+// row-major order, oracle_dbm.hpp).
+#include <array>
+#include <chrono>
+#include <deque>
+
+#include "oracle_engine.hpp"
+
+namespace orc {
+namespace live {
+
+static thread_local std::string g_err;
+
+struct DPred {
+  std::vector<std::vector<std::array<int, 4>>> terms;
+  void load(const hora_dpred& p) {
+    terms.clear();
+    for (int t = 0; t < p.nterms; ++t) {
+      std::vector<std::array<int, 4>> term;
+      for (int a = p.term_off[t]; a < p.term_off[t + 1]; ++a)
+        term.push_back({p.atoms[4 * a], p.atoms[4 * a + 1], p.atoms[4 * a + 2], p.atoms[4 * a + 3]});
+      terms.push_back(term);
+    }
+  }
+  bool eval(const int* locs, const int* ints) const {
+    for (const auto& term : terms) {
+      bool all = true;
+      for (const auto& a : term) {
+        const int v = a[0] == 0 ? locs[a[1]] : ints[a[1]];
+        if (!int_ok({IntAtom{0, a[2], a[3]}}, &v)) { all = false; break; }
+      }
+      if (all) return true;
+    }
+    return false;
+  }
+};
+
+using Fed = std::vector<raw_t>;   // zones back to back, d*d each
+
+struct Live {
+  Engine& E;
+  const Net& n;
+  int d, d2, kind;
+  bool early_on, restricted = false;
+  std::vector<std::vector<Cons>> orig_inv;   // unrestricted invariants (E[] with a clock part)
+  DPred p, q;
+  // graph
+  std::vector<std::uint64_t> dk;
+  std::vector<raw_t> zones;
+  std::vector<std::int64_t> row_off;
+  std::vector<std::int32_t> etgt, ese, ere;
+  std::vector<std::int8_t> expanded, in_s, wne;
+  std::unordered_map<std::uint64_t, std::vector<int>> chains;
+  std::vector<std::int64_t> ecnt;
+  hora_live_stats st{};
+  std::size_t fedcap = 4096;   // fixpoint.cpp's federation cap
+
+  Live(Engine& e, const hora_live_query* qq) : E(e), n(e.n), d(e.n.d), d2(e.n.d * e.n.d) {
+    kind = qq->kind;
+    early_on = qq->early != 0;
+    p.load(qq->p);
+    q.load(qq->q);
+    if (qq->orig_inv_off) {
+      restricted = true;
+      orig_inv.assign(n.tl, {});
+      for (int L = 0; L < n.tl; ++L)
+        for (int c = qq->orig_inv_off[L]; c < qq->orig_inv_off[L + 1]; ++c)
+          orig_inv[L].push_back({qq->orig_inv_data[3 * c], qq->orig_inv_data[3 * c + 1], qq->orig_inv_data[3 * c + 2]});
+    }
+  }
+
+  const raw_t* Z(int v) const { return &zones[static_cast<std::size_t>(v) * d2]; }
+  void state(int v, int* locs, int* ints) const { E.decode(dk[v], locs, ints); }
+  bool psi(int v) const {
+    int locs[64], ints[64];
+    state(v, locs, ints);
+    return p.eval(locs, ints);
+  }
+  bool is_p(int v) const { int l[64], i[64]; state(v, l, i); return p.eval(l, i); }
+  bool is_q(int v) const { int l[64], i[64]; state(v, l, i); return q.eval(l, i); }
+  bool frozen(const int* locs, const int* ints) const {
+    bool c, u;
+    E.frozen_flags(locs, ints, &c, &u);
+    return c || u;
+  }
+  std::uint64_t dead_mask(const int* locs) const {
+    std::uint64_t a = 0;
+    for (int pr = 0; pr < n.np; ++pr) a |= n.amask[n.gl(pr, locs[pr])];
+    return ~a & (((std::uint64_t(1) << n.nclocks) - 1) << 1);
+  }
+  const std::vector<Cons>& inv_of(const std::vector<std::vector<Cons>>& table, int pr, int l) const {
+    return table[n.gl(pr, l)];
+  }
+  bool apply_inv_table(const std::vector<std::vector<Cons>>& table, const int* locs, raw_t* z) const {
+    for (int pr = 0; pr < n.np; ++pr)
+      if (!apply_all(z, d, table[n.gl(pr, locs[pr])])) return false;
+    return true;
+  }
+  bool has_upper(const std::vector<std::vector<Cons>>& table, const int* locs) const {
+    for (int pr = 0; pr < n.np; ++pr)
+      for (const auto& c : table[n.gl(pr, locs[pr])])
+        if (c.i > 0 && c.j == 0 && c.b < kInf) return true;
+    return false;
+  }
+
+  // a_M bounds of a location vector: the a_LU state bounds with L and U raised to their maximum
+  void mbounds(const int* locs, std::int32_t* M) const {
+    std::int32_t L2[64], U2[64];
+    E.state_bounds(locs, L2, U2);
+    for (int c = 0; c < d; ++c) M[c] = std::max(L2[c], U2[c]);
+  }
+
+  // successors with their transition (se, re): tagged_successors on the a_LU lane
+  template <class F>
+  void succ_rec(const int* sl, const int* si, const raw_t* sz, F&& emit) const {
+    std::vector<raw_t> z(d2);
+    std::vector<int> nl(n.np), niv(n.ni);
+    const bool src_c = [&] { bool c, u; E.frozen_flags(sl, si, &c, &u); return c; }();
+    auto fire = [&](int se, int re) {
+      std::copy(sl, sl + n.np, nl.begin());
+      std::copy(si, si + n.ni, niv.begin());
+      const Edge& a = n.edges[se];
+      nl[a.proc] = a.tgt;
+      for (const auto& u : a.iu) niv[u.var] = u.val;
+      if (re >= 0) {
+        const Edge& b = n.edges[re];
+        nl[b.proc] = b.tgt;
+        for (const auto& u : b.iu) niv[u.var] = u.val;
+      }
+      std::copy(sz, sz + d2, z.begin());
+      if (!apply_all(z.data(), d, a.guard)) return;
+      if (re >= 0 && !apply_all(z.data(), d, n.edges[re].guard)) return;
+      for (int r : a.resets) dbm_reset(z.data(), d, r);
+      if (re >= 0) for (int r : n.edges[re].resets) dbm_reset(z.data(), d, r);
+      if (!E.close_succ_zone(nl.data(), niv.data(), z.data(), false)) return;
+      if (dbm_empty(z.data(), d)) return;
+      emit(nl.data(), niv.data(), z.data(), se, re);
+    };
+    for (int pr = 0; pr < n.np; ++pr) {
+      if (src_c && !E.loc_committed(pr, sl[pr])) continue;
+      for (int ei : n.edges_of[n.gl(pr, sl[pr])]) {
+        const Edge& e = n.edges[ei];
+        if (e.chan >= 0 || !int_ok(e.ig, si)) continue;
+        fire(ei, -1);
+      }
+    }
+    for (int i = 0; i < n.np; ++i)
+      for (int ei : n.edges_of[n.gl(i, sl[i])]) {
+        const Edge& se = n.edges[ei];
+        if (se.chan < 0 || se.dir != 1 || n.bcast[se.chan]) continue;
+        if (!int_ok(se.ig, si)) continue;
+        const bool i_comm = E.loc_committed(i, sl[i]);
+        for (int j = 0; j < n.np; ++j) {
+          if (j == i) continue;
+          for (int ej : n.edges_of[n.gl(j, sl[j])]) {
+            const Edge& re = n.edges[ej];
+            if (re.chan != se.chan || re.dir != -1) continue;
+            if (src_c && !(i_comm || E.loc_committed(j, sl[j]))) continue;
+            if (!int_ok(re.ig, si)) continue;
+            fire(ei, ej);
+          }
+        }
+      }
+  }
+
+  int new_node(std::uint64_t key, const raw_t* z) {
+    const int id = static_cast<int>(dk.size());
+    dk.push_back(key);
+    zones.insert(zones.end(), z, z + d2);
+    expanded.push_back(0);
+    ecnt.push_back(0);
+    chains[key].push_back(id);
+    return id;
+  }
+
+  // first stored node of the same discrete state that covers z (a_M), else a new node
+  int intern(std::uint64_t key, const int* locs, const raw_t* z, bool* created) {
+    *created = false;
+    auto it = chains.find(key);
+    if (it != chains.end()) {
+      std::int32_t M[64];
+      mbounds(locs, M);
+      for (int m : it->second)
+        if (dbm_alu_le(z, Z(m), d, M, M)) { ++st.covered; return m; }
+    }
+    *created = true;
+    return new_node(key, z);
+  }
+
+  // ---- stop set ------------------------------------------------------------------------------
+  bool time_diverges(int v) const {
+    if (n.nclocks == 0) return false;
+    int locs[64], ints[64];
+    state(v, locs, ints);
+    if (has_upper(n.inv, locs)) return false;
+    if (frozen(locs, ints)) return false;
+    std::vector<raw_t> z(Z(v), Z(v) + d2);
+    dbm_up(z.data(), d);
+    if (!E.apply_inv(locs, z.data())) return false;
+    for (int i = 1; i < d; ++i)
+      if (z[i * d] != kInf) return false;
+    return true;
+  }
+  bool orig_time_open(int v) const {
+    if (n.nclocks == 0) return false;
+    int locs[64], ints[64];
+    state(v, locs, ints);
+    if (has_upper(orig_inv, locs)) return false;
+    if (frozen(locs, ints)) return false;
+    std::vector<raw_t> z(d2, kInf);
+    for (int j = 0; j < d; ++j) z[j] = kLe0;
+    for (int i = 0; i < d; ++i) z[i * d + i] = kLe0;
+    if (!apply_inv_table(orig_inv, locs, z.data())) return false;
+    for (int i = 1; i < d; ++i)
+      if (z[i * d] != kInf) return false;
+    return true;
+  }
+
+  // keep `live` inclusion-minimal (fed_add)
+  void fed_add(Fed& live, const raw_t* g) const {
+    const std::size_t m = live.size() / d2;
+    for (std::size_t k = 0; k < m; ++k)
+      if (dbm_includes(&live[k * d2], g, d)) return;
+    Fed kept;
+    for (std::size_t k = 0; k < m; ++k)
+      if (!dbm_includes(g, &live[k * d2], d)) kept.insert(kept.end(), &live[k * d2], &live[k * d2] + d2);
+    kept.insert(kept.end(), g, g + d2);
+    live.swap(kept);
+  }
+
+  // z minus every zone of `sub`, as a federation
+  Fed fed_subtract(const raw_t* z, const Fed& sub) const {
+    Fed cur(z, z + d2);
+    for (std::size_t c = 0; c < sub.size() / d2 && !cur.empty(); ++c) {
+      Fed nxt;
+      for (std::size_t k = 0; k < cur.size() / d2; ++k) dbm_subtract(&cur[k * d2], &sub[c * d2], d, nxt);
+      cur.swap(nxt);
+    }
+    return cur;
+  }
+  bool fed_covers(const Fed& F, const raw_t* z) const { return fed_subtract(z, F).empty(); }
+
+  // state_has_deadlock (deadlock.cpp:74-268) judged with invariant table `inv`
+  bool state_has_deadlock(int v, const std::vector<std::vector<Cons>>& inv, Fed* stuck) const {
+    int locs[64], ints[64];
+    state(v, locs, ints);
+    const bool src_c = [&] { bool c, u; E.frozen_flags(locs, ints, &c, &u); return c; }();
+    const bool fz = frozen(locs, ints);
+    std::vector<raw_t> Zd(Z(v), Z(v) + d2);
+    if (!fz) dbm_up(Zd.data(), d);
+    if (!apply_inv_table(inv, locs, Zd.data())) return false;
+    Fed live;
+    bool covered = false;
+    std::vector<int> tl(n.np);
+    auto add_src = [&](std::vector<raw_t>& g, int se, int re) {
+      std::uint64_t rm = 0;
+      for (int r : n.edges[se].resets) rm |= std::uint64_t(1) << r;
+      if (re >= 0) for (int r : n.edges[re].resets) rm |= std::uint64_t(1) << r;
+      std::copy(locs, locs + n.np, tl.begin());
+      tl[n.edges[se].proc] = n.edges[se].tgt;
+      if (re >= 0) tl[n.edges[re].proc] = n.edges[re].tgt;
+      for (int pr = 0; pr < n.np; ++pr)
+        for (const auto& c : inv[n.gl(pr, tl[pr])]) {
+          const int i = ((rm >> c.i) & 1) ? 0 : c.i;
+          const int j = ((rm >> c.j) & 1) ? 0 : c.j;
+          if (i == j) { if (c.b < kLe0) return; continue; }
+          if (!dbm_constrain(g.data(), d, i, j, c.b)) return;
+        }
+      if (std::memcmp(g.data(), Zd.data(), sizeof(raw_t) * d2) == 0) { covered = true; return; }
+      fed_add(live, g.data());
+    };
+    std::vector<raw_t> g(d2);
+    for (int pr = 0; pr < n.np && !covered; ++pr) {
+      if (src_c && !E.loc_committed(pr, locs[pr])) continue;
+      for (int ei : n.edges_of[n.gl(pr, locs[pr])]) {
+        const Edge& e = n.edges[ei];
+        if (e.chan >= 0 || !int_ok(e.ig, ints)) continue;
+        std::copy(Zd.begin(), Zd.end(), g.begin());
+        if (!apply_all(g.data(), d, e.guard)) continue;
+        add_src(g, ei, -1);
+        if (covered) break;
+      }
+    }
+    for (int i = 0; i < n.np && !covered; ++i)
+      for (int ei : n.edges_of[n.gl(i, locs[i])]) {
+        if (covered) break;
+        const Edge& se = n.edges[ei];
+        if (se.chan < 0 || se.dir != 1 || n.bcast[se.chan]) continue;
+        if (!int_ok(se.ig, ints)) continue;
+        const bool i_comm = E.loc_committed(i, locs[i]);
+        for (int j = 0; j < n.np && !covered; ++j) {
+          if (j == i) continue;
+          for (int ej : n.edges_of[n.gl(j, locs[j])]) {
+            const Edge& re = n.edges[ej];
+            if (re.chan != se.chan || re.dir != -1) continue;
+            if (src_c && !(i_comm || E.loc_committed(j, locs[j]))) continue;
+            if (!int_ok(re.ig, ints)) continue;
+            std::copy(Zd.begin(), Zd.end(), g.begin());
+            if (!apply_all(g.data(), d, se.guard) || !apply_all(g.data(), d, re.guard)) continue;
+            add_src(g, ei, ej);
+            if (covered) break;
+          }
+        }
+      }
+    if (stuck) stuck->clear();
+    if (covered) return false;
+    const std::size_t m = live.size() / d2;
+    if (!fz)
+      for (std::size_t k = 0; k < m; ++k) dbm_down(&live[k * d2], d);
+    for (std::size_t k = 0; k < m; ++k)
+      if (dbm_includes(&live[k * d2], Zd.data(), d)) return false;
+    Fed dd = fed_subtract(Zd.data(), live);
+    const bool any = !dd.empty();
+    if (stuck) stuck->swap(dd);
+    return any;
+  }
+
+  Fed stop_set(int v) const {
+    Fed f;
+    if (!restricted) {
+      if (row_off[v + 1] == row_off[v] || time_diverges(v)) { f.assign(Z(v), Z(v) + d2); return f; }
+      state_has_deadlock(v, n.inv, &f);
+      return f;
+    }
+    if (time_diverges(v)) { f.assign(Z(v), Z(v) + d2); return f; }
+    if (orig_time_open(v)) return f;
+    state_has_deadlock(v, orig_inv, &f);
+    return f;
+  }
+
+  // ---- exact predecessor (operators.cpp:257-306) ----------------------------------------------
+  bool pre(int v, std::int64_t e, const raw_t* X, raw_t* Y) const {
+    const int t = etgt[e], se = ese[e], re = ere[e];
+    int sl[64], si[64], tl[64], ti[64];
+    state(v, sl, si);
+    state(t, tl, ti);
+    std::copy(X, X + d2, Y);
+    dbm_free_many(Y, d, dead_mask(tl));
+    if (!E.apply_inv(tl, Y)) return false;
+    if (!frozen(tl, ti)) {
+      dbm_down(Y, d);
+      if (!E.apply_inv(tl, Y)) return false;
+    }
+    std::uint64_t rm = 0;
+    for (int k = 0; k < 2; ++k) {
+      const int ed = k == 0 ? se : re;
+      if (ed < 0) continue;
+      for (int r : n.edges[ed].resets) {
+        if (!dbm_constrain(Y, d, r, 0, kLe0)) return false;
+        if (!dbm_constrain(Y, d, 0, r, kLe0)) return false;
+        rm |= std::uint64_t(1) << r;
+      }
+    }
+    dbm_free_many(Y, d, rm);
+    if (!apply_all(Y, d, n.edges[se].guard)) return false;
+    if (re >= 0 && !apply_all(Y, d, n.edges[re].guard)) return false;
+    if (!frozen(sl, si)) dbm_down(Y, d);
+    if (!E.apply_inv(sl, Y)) return false;
+    return true;
+  }
+
+  // drop members contained in another (for equal members keep the first)
+  void compact(Fed& F) {
+    const std::size_t m = F.size() / d2;
+    std::vector<char> drop(m, 0);
+    for (std::size_t k = 0; k < m; ++k)
+      for (std::size_t j = 0; j < m && !drop[k]; ++j) {
+        if (j == k) continue;
+        if (!dbm_includes(&F[j * d2], &F[k * d2], d)) continue;
+        if (!dbm_includes(&F[k * d2], &F[j * d2], d) || j < k) drop[k] = 1;
+      }
+    Fed out;
+    for (std::size_t k = 0; k < m; ++k)
+      if (!drop[k]) out.insert(out.end(), &F[k * d2], &F[k * d2] + d2);
+    F.swap(out);
+    if (F.size() / d2 > fedcap) st.capped = 1;
+  }
+
+  // ---- construction -----------------------------------------------------------------------------
+  // Returns 1/0 when an early witness settled the query (value = holds), -1 otherwise.
+  int build() {
+    std::vector<int> l0(n.np), i0(n.ni);
+    for (int pr = 0; pr < n.np; ++pr) l0[pr] = n.loc_init[pr];
+    for (int v = 0; v < n.ni; ++v) i0[v] = n.int_init[v];
+    std::vector<raw_t> z0(d2);
+    dbm_zero(z0.data(), d);
+    if (!E.close_succ_zone(l0.data(), i0.data(), z0.data(), false) || dbm_empty(z0.data(), d)) return -1;
+    new_node(E.encode(l0.data(), i0.data()), z0.data());
+    std::vector<int> frontier;
+    if (kind != HORA_LIVE_EG || psi(0)) frontier.push_back(0);
+    std::vector<int> sl(n.np), si(n.ni);
+    while (!frontier.empty()) {
+      ++st.levels;
+      std::vector<int> next;
+      for (int v : frontier) {
+        expanded[v] = 1;
+        ++st.expanded_nodes;
+        state(v, sl.data(), si.data());
+        const std::vector<raw_t> zv(Z(v), Z(v) + d2);
+        succ_rec(sl.data(), si.data(), zv.data(), [&](const int* nl, const int* niv, const raw_t* z, int se, int re) {
+          ++st.generated;
+          bool created = false;
+          const int t = intern(E.encode(nl, niv), nl, z, &created);
+          if (created && (kind != HORA_LIVE_EG || psi(t))) next.push_back(t);
+          etgt.push_back(t);
+          ese.push_back(se);
+          ere.push_back(re);
+          ++ecnt[v];
+        });
+      }
+      build_row_off();
+      if (early_on && kind == HORA_LIVE_LEADSTO) {
+        for (int v : frontier)
+          if (is_p(v) && !is_q(v) && !stop_set(v).empty()) { st.early = 1; return 0; }
+      }
+      if (early_on && kind == HORA_LIVE_DEADLOCK) {
+        for (int v : frontier)
+          if (state_has_deadlock(v, n.inv, nullptr)) { st.early = 1; return 0; }
+      }
+      frontier.swap(next);
+    }
+    return -1;
+  }
+  void build_row_off() {
+    row_off.assign(dk.size() + 1, 0);
+    for (std::size_t v = 0; v < dk.size(); ++v) row_off[v + 1] = row_off[v] + ecnt[v];
+  }
+
+  // ---- backward pass ------------------------------------------------------------------------------
+  std::vector<int> closure() {
+    const int N = static_cast<int>(dk.size());
+    in_s.assign(N, 0);
+    std::vector<int> stk, out;
+    auto nq = [&](int v) { return kind == HORA_LIVE_EG ? psi(v) : !is_q(v); };
+    if (kind == HORA_LIVE_LEADSTO) {
+      for (int v = 0; v < N; ++v)
+        if (expanded[v] && nq(v) && is_p(v)) { in_s[v] = 1; stk.push_back(v); }
+    } else if (N > 0 && expanded[0] && nq(0)) {
+      in_s[0] = 1;
+      stk.push_back(0);
+    }
+    while (!stk.empty()) {
+      const int v = stk.back();
+      stk.pop_back();
+      out.push_back(v);
+      for (std::int64_t e = row_off[v]; e < row_off[v + 1]; ++e) {
+        const int t = etgt[e];
+        if (!in_s[t] && expanded[t] && nq(t)) { in_s[t] = 1; stk.push_back(t); }
+      }
+    }
+    std::sort(out.begin(), out.end());
+    return out;
+  }
+
+  std::vector<Fed> W;
+  std::vector<Fed> stops;
+  std::vector<char> stop_ready;
+
+  const Fed& stop_of(int v) {
+    if (!stop_ready[v]) { stops[v] = stop_set(v); stop_ready[v] = 1; }
+    return stops[v];
+  }
+
+  // relax (fixpoint.cpp:750-864): W[v] := W[v] /\ (NoSucc(v) \/ U_e Pre_e(W[t])). True if changed.
+  bool relax(int v) {
+    Fed& wv = W[v];
+    const std::size_t nmem = wv.size() / d2;
+    if (nmem == 0) return false;
+    std::vector<char> cov(nmem, 0);
+    std::size_t ncov = 0;
+    Fed cand;
+    std::vector<raw_t> pz(d2);
+    for (std::int64_t e = row_off[v]; e < row_off[v + 1]; ++e) {
+      const int t = etgt[e];
+      if (!in_s[t]) continue;
+      const Fed& wt = W[t];
+      for (std::size_t y = 0; y < wt.size() / d2; ++y) {
+        ++st.pre_calls;
+        if (!pre(v, e, &wt[y * d2], pz.data())) continue;
+        for (std::size_t k = 0; k < nmem; ++k)
+          if (!cov[k] && dbm_includes(pz.data(), &wv[k * d2], d)) { cov[k] = 1; ++ncov; }
+        if (ncov == nmem) return false;
+        cand.insert(cand.end(), pz.begin(), pz.end());
+      }
+    }
+    compact(cand);
+    auto escapes = [&] {
+      for (std::size_t k = 0; k < nmem; ++k) {
+        bool whole = false;
+        for (std::size_t c = 0; c < cand.size() / d2 && !whole; ++c)
+          if (dbm_includes(&cand[c * d2], &wv[k * d2], d)) whole = true;
+        if (!whole && !fed_covers(cand, &wv[k * d2])) return true;
+      }
+      return false;
+    };
+    if (!escapes()) return false;
+    const Fed& ns = stop_of(v);
+    if (!ns.empty()) {
+      cand.insert(cand.end(), ns.begin(), ns.end());
+      compact(cand);
+      if (!escapes()) return false;
+    }
+    Fed nw;
+    std::vector<raw_t> t(d2);
+    for (std::size_t k = 0; k < nmem; ++k) {
+      const raw_t* z = &wv[k * d2];
+      bool whole = false;
+      for (std::size_t c = 0; c < cand.size() / d2 && !whole; ++c)
+        if (dbm_includes(&cand[c * d2], z, d)) whole = true;
+      if (whole) { nw.insert(nw.end(), z, z + d2); continue; }
+      for (std::size_t c = 0; c < cand.size() / d2; ++c)
+        if (dbm_intersect(t.data(), z, &cand[c * d2], d)) nw.insert(nw.end(), t.begin(), t.end());
+    }
+    compact(nw);
+    wv.swap(nw);
+    return true;
+  }
+
+  void solve(const std::vector<int>& S) {
+    const int N = static_cast<int>(dk.size());
+    W.assign(N, {});
+    stops.assign(N, {});
+    stop_ready.assign(N, 0);
+    std::vector<std::vector<int>> preds(N);
+    for (int v : S) {
+      W[v].assign(Z(v), Z(v) + d2);
+      for (std::int64_t e = row_off[v]; e < row_off[v + 1]; ++e)
+        if (in_s[etgt[e]]) preds[etgt[e]].push_back(v);
+    }
+    std::deque<int> work(S.begin(), S.end());
+    std::vector<char> inq(N, 0);
+    for (int v : S) inq[v] = 1;
+    while (!work.empty() && !st.capped) {
+      const int v = work.front();
+      work.pop_front();
+      inq[v] = 0;
+      if (!relax(v)) continue;
+      ++st.iters;
+      for (int u : preds[v])
+        if (!inq[u]) { inq[u] = 1; work.push_back(u); }
+    }
+    wne.assign(N, 0);
+    for (int v : S) {
+      const std::int64_t m = static_cast<std::int64_t>(W[v].size() / d2);
+      if (m) { wne[v] = 1; ++st.nonempty; st.fed_max = std::max(st.fed_max, m); }
+    }
+  }
+
+  int run(int32_t* holds) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const int er = build();
+    build_row_off();
+    in_s.assign(dk.size(), 0);
+    wne.assign(dk.size(), 0);
+    const auto t1 = std::chrono::steady_clock::now();
+    st.outer_nodes = static_cast<std::int64_t>(dk.size());
+    st.fwd_edges = static_cast<std::int64_t>(etgt.size());
+    st.disc_keys = static_cast<std::int64_t>(chains.size());
+    if (er >= 0) {
+      *holds = er;
+    } else if (kind == HORA_LIVE_DEADLOCK) {
+      bool dl = false;
+      for (std::size_t v = 0; v < dk.size() && !dl; ++v)
+        if (expanded[v] && state_has_deadlock(static_cast<int>(v), n.inv, nullptr)) dl = true;
+      *holds = dl ? 0 : 1;
+    } else {
+      const std::vector<int> S = closure();
+      st.closure = static_cast<std::int64_t>(S.size());
+      solve(S);
+      if (kind == HORA_LIVE_LEADSTO) {
+        bool violated = false;
+        for (int v : S)
+          if (wne[v] && is_p(v)) { violated = true; break; }
+        *holds = violated ? 0 : 1;
+      } else {
+        *holds = (!dk.empty() && in_s[0] && wne[0]) ? 1 : 0;
+      }
+    }
+    const auto t2 = std::chrono::steady_clock::now();
+    st.build_seconds = std::chrono::duration<double>(t1 - t0).count();
+    st.gfp_seconds = std::chrono::duration<double>(t2 - t1).count();
+    st.seconds = std::chrono::duration<double>(t2 - t0).count();
+    return st.capped ? HORA_HIP_E_CAPPED : 0;
+  }
+};
+
+struct Held {
+  std::unique_ptr<Engine> E;
+  std::unique_ptr<Live> L;
+};
+
+}  // namespace live
+}  // namespace orc
+
+extern "C" {
+
+const char* oracle_live_last_error(void) { return orc::live::g_err.c_str(); }
+
+// One liveness query on the CPU. *graph (optional) receives a handle for oracle_live_graph /
+// oracle_live_free.
+int oracle_live(const hora_net_enc* e, const hora_live_query* q, int32_t* holds, hora_live_stats* st,
+                void** graph) {
+  using namespace orc::live;
+  try {
+    if (!e || !q || !holds) return HORA_HIP_E_ARG;
+    auto h = std::make_unique<Held>();
+    hora_target never{};
+    never.never = 1;
+    h->E = std::make_unique<orc::Engine>(e, &never);
+    if (h->E->n.has_diag) { g_err = "diagonal constraints"; return HORA_HIP_E_SCOPE; }
+    for (int c = 0; c < h->E->n.nch; ++c)
+      if (h->E->n.bcast[c]) { g_err = "broadcast channels"; return HORA_HIP_E_SCOPE; }
+    h->L = std::make_unique<Live>(*h->E, q);
+    const int rc = h->L->run(holds);
+    if (st) *st = h->L->st;
+    if (graph) *graph = h.release();
+    return rc;
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return HORA_HIP_E_INTERNAL;
+  }
+}
+
+int oracle_live_graph(void* handle, hora_live_graph* g) {
+  auto* h = static_cast<orc::live::Held*>(handle);
+  if (!h || !g) return HORA_HIP_E_ARG;
+  const auto& L = *h->L;
+  g->nodes = static_cast<int64_t>(L.dk.size());
+  g->edges = static_cast<int64_t>(L.etgt.size());
+  g->d = L.d;
+  auto cp = [](auto* dst, const auto& v) { if (dst) std::copy(v.begin(), v.end(), dst); };
+  cp(g->dk, L.dk);
+  cp(g->zones, L.zones);
+  cp(g->row_off, L.row_off);
+  cp(g->etgt, L.etgt);
+  cp(g->ese, L.ese);
+  cp(g->ere, L.ere);
+  cp(g->expanded, L.expanded);
+  cp(g->in_s, L.in_s);
+  cp(g->w_nonempty, L.wne);
+  return 0;
+}
+
+void oracle_live_free(void* handle) { delete static_cast<orc::live::Held*>(handle); }
+
+// p \ y for n pairs of zones: out receives at most `cap` pieces per pair (cap*d*d int32 each),
+// counts[k] the number of pieces (or -1 if more than cap).
+int oracle_zone_subtract(int32_t d, int64_t n, const int32_t* p, const int32_t* y, int32_t cap, int32_t* out,
+                         int32_t* counts) {
+  const int d2 = d * d;
+  for (int64_t k = 0; k < n; ++k) {
+    std::vector<orc::raw_t> pieces;
+    orc::dbm_subtract(p + k * d2, y + k * d2, d, pieces);
+    const int m = static_cast<int>(pieces.size() / d2);
+    counts[k] = m > cap ? -1 : m;
+    if (m <= cap) std::copy(pieces.begin(), pieces.end(), out + k * static_cast<int64_t>(cap) * d2);
+  }
+  return 0;
+}
+
+}  // extern "C"

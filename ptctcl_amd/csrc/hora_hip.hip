@@ -341,6 +341,8 @@ struct ExpandOut {
   u64* dk;         // [R] packed discrete key
   int* zone;       // [R*d2]
   int* tgt;        // [R] 1 = matches the target
+  int* se;         // [R] transition of the slot (sender or local edge), or null: not recorded
+  int* re;         // [R] receiver edge or -1
 };
 
 // Wavefront-wide helpers. One lane per process (np <= 64): the per-process table lookups of a
@@ -609,6 +611,10 @@ __global__ __launch_bounds__(64) void k_expand(const DevNet* __restrict__ Np, co
       out.tgt[slot] = tg ? 1 : 0;
       out.dk[slot] = ok ? key : 0;
       if (tg) atomicMin(&ctr->first_target, slot);
+      if (out.se) {
+        out.se[slot] = se;
+        out.re[slot] = re;
+      }
     }
     if (ok) {
       int* dst = out.zone + (size_t)slot * d2;
@@ -1388,6 +1394,12 @@ __global__ __launch_bounds__(64) void k_zop_wave(ZopArgs A) {
       case HORA_ZOP_EMPTY:
         flag = z_empty(Z, d) ? 1 : 0;
         break;
+      case HORA_ZOP_DOWN:
+        z_down(Z, d);
+        break;
+      case HORA_ZOP_FREE:
+        z_free_many(Z, d, (uint64_t)(uint32_t)A.arg[z], fa);
+        break;
     }
     wsync();
     if (A.out) {
@@ -1462,6 +1474,9 @@ struct DBuf {
 
 }  // namespace
 
+struct LiveState;             // liveness pass state (liveness.hpp)
+void live_free(LiveState* L);
+
 struct hora_hip_ctx {
   std::string err;
   hipStream_t stream = nullptr;
@@ -1492,6 +1507,7 @@ struct hora_hip_ctx {
   hora_reach_profile prof{};
   bool trace = false;
   hipEvent_t ev[8] = {};
+  LiveState* live = nullptr;
 };
 
 namespace {
@@ -1667,7 +1683,7 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     c->s_tgt.ensure(R);
     c->s_dk.ensure(R);
     c->s_zone.ensure((size_t)R * d2);
-    ExpandOut eo{c->s_valid.p, c->s_dk.p, c->s_zone.p, c->s_tgt.p};
+    ExpandOut eo{c->s_valid.p, c->s_dk.p, c->s_zone.p, c->s_tgt.p, nullptr, nullptr};
     const size_t lds_e = expand_lds_ints(d2) * sizeof(int);
     HIP_OK(hipEventRecord(c->ev[0], c->stream));
     if (R > 0)
@@ -1980,6 +1996,8 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
 
 }  // namespace
 
+#include "liveness.hpp"
+
 extern "C" {
 
 int hora_hip_abi_version(void) { return HORA_HIP_ABI_VERSION; }
@@ -2019,6 +2037,7 @@ void hora_hip_close(hora_hip_ctx* ctx) {
     if (ev) (void)hipEventDestroy(ev);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->dnp) (void)hipFree(ctx->dnp);
+  if (ctx->live) live_free(ctx->live);
   delete ctx;
 }
 
@@ -2046,8 +2065,8 @@ int hora_hip_reach_profile(const hora_hip_ctx* ctx, hora_reach_profile* out) {
 int hora_hip_zone_op(int32_t op, int32_t d, int64_t n, const int32_t* a, const int32_t* b, const int32_t* arg,
                      const int32_t* L, const int32_t* U, int32_t* out, int32_t* out_flag, void* stream) {
   if (d < 1 || d > HORA_MAX_DIM || n < 0 || !a) return HORA_HIP_E_ARG;
-  if (op < HORA_ZOP_CLOSE || op > HORA_ZOP_EMPTY) return HORA_HIP_E_ARG;
-  if ((op == HORA_ZOP_RESET || op == HORA_ZOP_CONSTRAIN) && !arg) return HORA_HIP_E_ARG;
+  if (op < HORA_ZOP_CLOSE || op > HORA_ZOP_FREE) return HORA_HIP_E_ARG;
+  if ((op == HORA_ZOP_RESET || op == HORA_ZOP_CONSTRAIN || op == HORA_ZOP_FREE) && !arg) return HORA_HIP_E_ARG;
   if ((op == HORA_ZOP_EXTRA_LU || op == HORA_ZOP_ALU_LE) && (!L || !U)) return HORA_HIP_E_ARG;
   if ((op == HORA_ZOP_INCLUDES || op == HORA_ZOP_ALU_LE || op == HORA_ZOP_UNION) && !b) return HORA_HIP_E_ARG;
   if (n == 0) return HORA_HIP_OK;
@@ -2063,6 +2082,28 @@ int hora_hip_zone_op(int32_t op, int32_t d, int64_t n, const int32_t* a, const i
     hipLaunchKernelGGL(k_zop_wave, dim3(grid), dim3(64), lds, s, A);
   }
   return hipGetLastError() == hipSuccess ? HORA_HIP_OK : HORA_HIP_E_DEVICE;
+}
+
+int hora_hip_live(hora_hip_ctx* ctx, const hora_live_query* q, int32_t* holds, hora_live_stats* stats) {
+  if (!ctx || !q || !holds || !stats || !ctx->stream) return HORA_HIP_E_ARG;
+  try {
+    return run_live(ctx, q, holds, stats);
+  } catch (const HipFail& ex) {
+    ctx->err = ex.what();
+    return HORA_HIP_E_DEVICE;
+  } catch (const std::exception& ex) {
+    ctx->err = ex.what();
+    return HORA_HIP_E_INTERNAL;
+  }
+}
+
+int hora_hip_live_graph(const hora_hip_ctx* ctx, hora_live_graph* g) {
+  if (!ctx || !g) return HORA_HIP_E_ARG;
+  try {
+    return live_graph(ctx, g);
+  } catch (const std::exception&) {
+    return HORA_HIP_E_DEVICE;
+  }
 }
 
 }  // extern "C"

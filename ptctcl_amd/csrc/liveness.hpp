@@ -1,0 +1,1294 @@
+// liveness.hpp — device liveness pass of the HIP back end: `p --> q`, `E[] psi`,
+// `A[] not deadlock`. Part of the hora_hip.hip translation unit (it uses that file's DevNet,
+// expansion kernels, passed-list hash table and scans).
+//
+// Forward: the zone graph is built level by level on the device with the expansion pipeline of
+// the reachability search, with no extrapolation and with every transition recorded. Each
+// successor goes to the first (oldest) stored node of its discrete state that covers it under a_M
+// (a_LU with per-location bounds raised to M = max(L, U): a bisimulation, so deadlocks survive the
+// quotient; src/liveness/construction.cpp:899-913), else it becomes a new node: the append-only
+// branch of LivenessEngine::intern (construction.cpp:863-1003). Node ids are creation order:
+// frontier index, then transition index, exactly as a sequential pass makes them.
+//
+//   k_gfilter  E[]: only psi nodes are expanded (src/liveness/api.cpp:83-96)
+//   k_count / k_expand / scan     successors in (state, transition) order, transition recorded
+//   k_grows    CSR row offsets of the frontier nodes
+//   k_find     discrete key -> hash slot
+//   k_gprobe   first covering node among the key's nodes stored before this level
+//   k_ticket / scatter            per-key work lists of the successors not covered there
+//   k_gmerge   one wavefront per key: the key's successors in order, each to the first node
+//              created earlier in this level that covers it, else a new node
+//   scan       new-node ranks = node ids
+//   k_gassign / k_gcopy / k_glink edges, node zones, the key's node chain
+//   k_stop     (early witness) stop set of the level's p /\ !q nodes, or deadlock test
+//
+// Backward (src/liveness/fixpoint.cpp:48-965): closure S of the query (BFS kernels), stop sets
+// NoSucc(v) (k_stop: whole zone when no successor or time diverges, else the valuations that enable
+// nothing, src/liveness/construction.cpp:1200-1238, src/reachability/deadlock.cpp:74-268), then
+// Jacobi rounds of relax (k_gfp: W[v] := W[v] /\ (NoSucc(v) \/ U_e Pre_e(W[t])), exact predecessor
+// of src/liveness/operators.cpp:257-306) on two banks of federations until no value changes. The
+// limit is the unique greatest fixpoint, so it equals any sequential schedule's.
+#pragma once
+
+namespace {
+
+constexpr int kLiveMaxDim = 16;
+constexpr int ERR_CAPPED = 5;
+constexpr int kLiveW = 8;       // zones in one converged value W[v]
+constexpr int kStopCap = 16;    // zones in one stop set
+
+struct DevPred {
+  int nterms;
+  const int* off;
+  const int* atoms;
+};
+
+// Truth of a DNF predicate on a packed discrete key (each lane may call it; no memory written).
+__device__ __forceinline__ bool pred_eval(const DevNet& N, const DevPred& P, u64 dk) {
+  for (int t = 0; t < P.nterms; ++t) {
+    bool all = true;
+    for (int a = P.off[t]; a < P.off[t + 1] && all; ++a) {
+      const int kind = P.atoms[4 * a], idx = P.atoms[4 * a + 1], op = P.atoms[4 * a + 2], val = P.atoms[4 * a + 3];
+      int v;
+      if (kind == 0) v = (int)((dk >> N.kshift[idx]) & N.kmask[idx]);
+      else v = (int)((dk >> N.kshift[N.np + idx]) & N.kmask[N.np + idx]) + N.int_lo[idx];
+      all = atom_holds(v, op, val);
+    }
+    if (all) return true;
+  }
+  return false;
+}
+
+// Everything the liveness kernels read besides the network, in device memory.
+struct LiveArgs {
+  DevPred p, q;
+  int kind, restricted;
+  const int* oinv_off;        // unrestricted invariant tables (E[] with a clock part)
+  const int* oinv_data;
+  const u64* gdk;             // [N] node keys
+  const int* gzone;           // [N*d2] node zones
+  const long long* grow;      // [N+1] CSR rows
+  const int* etgt;
+  const int* ese;
+  const int* ere;
+  const signed char* expanded;
+  const int* in_s;            // [N] 1 = in the closure
+  const int* sidx;            // [N] index into the S-indexed arrays, or -1
+  int kl, kp, kc;             // federation capacities (LDS lists), set from the dimension
+};
+
+struct LiveCtr {
+  int hit;                    // early witness / deadlock found
+  int err;
+  int ns;                     // closure list length
+  int changed;                // values changed in the last round
+  unsigned long long pre_calls;
+  long long nonempty;
+  int fed_max;
+  int violated;
+  int pad[6];
+};
+
+// ---------------------------------------------------------------- forward construction
+
+__global__ void k_gfilter(const DevNet* __restrict__ Np, const LiveArgs* __restrict__ A, int use_psi,
+                          const u64* __restrict__ fr_dk, int F, int* __restrict__ cnt,
+                          signed char* __restrict__ expanded, long fbase) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= F) return;
+  const bool keep = !use_psi || pred_eval(*Np, A->p, fr_dk[i]);
+  if (!keep) cnt[i] = 0;
+  expanded[fbase + i] = keep ? 1 : 0;
+}
+
+// rows of the frontier nodes: node fbase+i owns the non-empty successors of its recipe slots
+__global__ void k_grows(const int* __restrict__ rec_off, const long long* __restrict__ vpos, long R, int F,
+                        long fbase, long long ebase, long long G, long long* __restrict__ grow) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > F) return;
+  if (i == F) { grow[fbase + F] = ebase + G; return; }
+  const long s = rec_off[i];
+  grow[fbase + i] = ebase + (s < R ? vpos[s] : G);
+}
+
+__global__ void k_gfill_rows(long long* __restrict__ grow, long from, long to, long long E) {
+  const long v = from + (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v <= to) grow[v] = E;
+}
+
+// a_M bounds of a successor's location vector (state_bounds, engine.cpp:1487-1497, with L and U
+// raised to their maximum); `lg` holds the decoded locations
+__device__ __forceinline__ int m_bound(const DevNet& N, const int* lg, int c) {
+  if (c == 0) return 0;
+  int lb = kAluNegInf, ub = kAluNegInf;
+  for (int p = 0; p < N.np; ++p) {
+    const size_t b = (size_t)(N.locbase[p] + lg[p]) * N.d + c;
+    lb = max(lb, N.lloc[b]);
+    ub = max(ub, N.uloc[b]);
+  }
+  lb = max(lb, N.Lq[c]);
+  ub = max(ub, N.Uq[c]);
+  return max(lb, ub);
+}
+
+// The first node stored before this level that covers the successor (the key's chain, oldest
+// first); four 16-lane groups per wavefront, one successor each.
+__global__ __launch_bounds__(64) void k_gprobe(const DevNet* __restrict__ Np, Passed P, const long* __restrict__ gbk,
+                                               const int* __restrict__ sj, const u64* __restrict__ sdk,
+                                               const int* __restrict__ szone, long G, const u32* __restrict__ gnext,
+                                               const int* __restrict__ gzone, int* __restrict__ covered,
+                                               int* __restrict__ cov_old, Counters* ctr) {
+  __shared__ int Mg[4][HORA_MAX_DIM], lg[4][HORA_MAX_PROCS];
+  const DevNet& N = *Np;
+  const int lane = threadIdx.x, q = lane / kGroup, t = lane % kGroup;
+  const long g = (long)blockIdx.x * 4 + q;
+  const bool active = g < G;
+  const int d = N.d, d2 = N.d2;
+  const long h = active ? gbk[g] : -1;
+  const u32 len = (h >= 0) ? P.len[h] : 0u;
+  if (active && len > 0) {
+    const u64 dk = sdk[sj[g]];
+    for (int p = t; p < N.np; p += kGroup) lg[q][p] = (int)((dk >> N.kshift[p]) & N.kmask[p]);
+  }
+  __syncthreads();
+  if (active && len > 0)
+    for (int c = t; c < d; c += kGroup) Mg[q][c] = m_bound(N, lg[q], c);
+  __syncthreads();
+  const int* S = szone + (size_t)(active ? sj[g] : 0) * d2;
+  const int* M = Mg[q];
+  const u64 gmask = 0xffffull << (q * kGroup);
+  int found = -1, reads = 0;
+  u32 node = len > 0 ? (u32)(P.off[h] >> 32) : 0u;
+  for (u32 m = 0; m < len; ++m) {   // group-uniform loop
+    const int* Z = gzone + (size_t)node * d2;
+    ++reads;
+    bool wit = false;
+    for (int e = t; e < d2; e += kGroup) {
+      const int y = e / d, x = e - y * d;
+      if (x == y) continue;
+      const int ux = M[x], ly = M[y];
+      if (ux == kAluNegInf || ly == kAluNegInf) continue;
+      const int s0x = S[x];
+      if (s0x < r_le(-ux)) continue;
+      const int zp = Z[e];
+      if (zp < S[e] && radd(zp, r_lt(-ly)) < s0x) wit = true;
+    }
+    if ((__ballot(wit) & gmask) == 0) { found = (int)node; break; }
+    node = gnext[node];
+  }
+  if (active && t == 0) {
+    covered[g] = found >= 0 ? 1 : 0;
+    cov_old[g] = found;
+  }
+  const bool lead = active && t == 0;
+  wave_stat(ctr, C_PREADS, lead ? (unsigned long long)reads : 0ull);
+  wave_stat(ctr, C_PSUB, (lead && found >= 0) ? 1ull : 0ull);
+}
+
+// One wavefront per touched key: its remaining successors in (state, transition) order, each to
+// the first node created earlier in this level that covers it, else a new node. The creators'
+// successor indices are kept in order at the front of the key's list, and their number replaces
+// the key's queue length for k_glink.
+__global__ __launch_bounds__(64) void k_gmerge(const DevNet* __restrict__ Np, Passed P, const u32* __restrict__ touched,
+                                               int T, int* __restrict__ wlist, const int* __restrict__ sj,
+                                               const u64* __restrict__ sdk, const int* __restrict__ szone,
+                                               int* __restrict__ tslot, int* __restrict__ fnew, Counters* ctr) {
+  extern __shared__ int sm[];
+  const DevNet& N = *Np;
+  const int d = N.d, d2 = N.d2, l = threadIdx.x;
+  int* S = sm;                       // d2
+  int* M = S + d2;                   // HORA_MAX_DIM
+  int* lg = M + HORA_MAX_DIM;        // HORA_MAX_PROCS
+  int* lst = lg + HORA_MAX_PROCS;    // kSortLds
+  const int t = blockIdx.x;
+  if (t >= T) return;
+  const u32 h = touched[t];
+  const int n = (int)P.ncnt[h];
+  const u64 base = P.noff[h];
+  const bool in_lds = n <= kSortLds;
+  int* list;
+  if (in_lds) {
+    int cap = 1;
+    while (cap < n) cap <<= 1;
+    for (int k = l; k < cap; k += kWave) lst[k] = k < n ? wlist[base + k] : 0x7fffffff;
+    bitonic_sort<false>(lst, cap);
+    list = lst;
+  } else {
+    const int cap = (int)list_cap((u64)n);
+    for (int k = n + l; k < cap; k += kWave) wlist[base + k] = 0x7fffffff;
+    bsync();
+    bitonic_sort<true>(wlist + base, cap);
+    list = wlist + base;
+  }
+  {
+    const u64 dk = sdk[sj[list[0]]];
+    if (l < N.np) lg[l] = (int)((dk >> N.kshift[l]) & N.kmask[l]);
+    wsync();
+    if (l < d) M[l] = m_bound(N, lg, l);
+  }
+  int nc = 0;
+  unsigned long long cov = 0;
+  for (int k = 0; k < n; ++k) {
+    if (in_lds) wsync(); else bsync();
+    const int g = list[k];
+    int first = -1;
+    if (nc > 0) {
+      wsync();
+      const int* src = szone + (size_t)sj[g] * d2;
+      for (int e = l; e < d2; e += kWave) S[e] = src[e];
+      wsync();
+      for (int cb = 0; cb < nc && first < 0; cb += kWave) {
+        const int c = cb + l;
+        bool ok = false;
+        if (c < nc) ok = alu_le_lane(S, szone + (size_t)sj[list[c]] * d2, d, M, M);
+        const u64 b = __ballot(ok);
+        if (b) first = cb + __builtin_ctzll(b);
+      }
+    }
+    if (first >= 0) {
+      if (l == 0) tslot[g] = list[first];
+      ++cov;
+      continue;
+    }
+    if (in_lds) wsync(); else bsync();
+    if (l == 0) {
+      list[nc] = g;          // nc <= k: only consumed entries are overwritten
+      tslot[g] = g;
+      fnew[g] = 1;
+    }
+    ++nc;
+    if (!in_lds) { __threadfence_block(); bsync(); }
+  }
+  wsync();
+  if (in_lds)
+    for (int k = l; k < nc; k += kWave) wlist[base + k] = lst[k];
+  if (l == 0) {
+    P.ncnt[h] = (u32)nc;
+    stat_add(ctr, C_SUB, cov);
+  }
+}
+
+__global__ void k_gassign(long G, long long nbase, long long ebase, int record, const int* __restrict__ cov_old,
+                          const int* __restrict__ tslot, const int* __restrict__ fnew,
+                          const long long* __restrict__ rank, const int* __restrict__ sj,
+                          const int* __restrict__ s_se, const int* __restrict__ s_re, int* __restrict__ etgt,
+                          int* __restrict__ ese, int* __restrict__ ere, int* __restrict__ newlist) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const int co = cov_old[g];
+  const long long tgt = co >= 0 ? (long long)co : nbase + rank[tslot[g]];
+  if (record) {
+    etgt[ebase + g] = (int)tgt;
+    ese[ebase + g] = s_se[sj[g]];
+    ere[ebase + g] = s_re[sj[g]];
+  }
+  if (co < 0 && fnew[g]) newlist[rank[g]] = (int)g;
+}
+
+__global__ __launch_bounds__(64) void k_gcopy(long F2, long long nbase, const int* __restrict__ newlist,
+                                              const int* __restrict__ sj, const u64* __restrict__ sdk,
+                                              const int* __restrict__ szone, int d2, int* __restrict__ gzone,
+                                              u64* __restrict__ gdk, u32* __restrict__ gnext,
+                                              u64* __restrict__ fr_dk, u32* __restrict__ fr_zid) {
+  const long b = blockIdx.x;
+  if (b >= F2) return;
+  const int g = newlist[b];
+  const long long id = nbase + b;
+  const int* src = szone + (size_t)sj[g] * d2;
+  int* dst = gzone + (size_t)id * d2;
+  for (int e = threadIdx.x; e < d2; e += kWave) dst[e] = src[e];
+  if (threadIdx.x == 0) {
+    gdk[id] = sdk[sj[g]];
+    gnext[id] = 0xffffffffu;
+    fr_dk[b] = sdk[sj[g]];
+    fr_zid[b] = (u32)id;
+  }
+}
+
+// append the key's new nodes to its chain, in creation order; chain = (head << 32 | tail) in off
+__global__ void k_glink(Passed P, const u32* __restrict__ touched, int T, const int* __restrict__ wlist,
+                        const long long* __restrict__ rank, long long nbase, u32* __restrict__ gnext) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  const u32 h = touched[t];
+  const u32 nc = P.ncnt[h];
+  const u64 base = P.noff[h];
+  u32 len = P.len[h];
+  u32 head = (u32)(P.off[h] >> 32), tail = (u32)P.off[h];
+  for (u32 c = 0; c < nc; ++c) {
+    const u32 id = (u32)(nbase + rank[wlist[base + c]]);
+    if (len == 0) head = id;
+    else gnext[tail] = id;
+    tail = id;
+    ++len;
+  }
+  P.len[h] = len;
+  P.off[h] = ((u64)head << 32) | tail;
+  P.ncnt[h] = 0;
+}
+
+// ---------------------------------------------------------------- stop sets and deadlock
+
+// Gather the combined invariant of a location vector from the tables (off, data), one lane per
+// process, as (i, j, raw) triples.
+__device__ __forceinline__ int wave_gather_inv(const DevNet& N, const int* off, const int* data, const int* locs,
+                                               int* inv) {
+  const int l = threadIdx.x;
+  int ia = 0, ib = 0;
+  if (l < N.np) {
+    const int g = N.locbase[l] + locs[l];
+    ia = off[g];
+    ib = off[g + 1];
+  }
+  int total = 0;
+  const int o0 = wave_excl_scan(ib - ia, &total);
+  if (total <= kInvCap)
+    for (int c = ia; c < ib; ++c) {
+      const int o = 3 * (o0 + c - ia);
+      inv[o] = data[3 * c];
+      inv[o + 1] = data[3 * c + 1];
+      inv[o + 2] = data[3 * c + 2];
+    }
+  wsync();
+  return total;
+}
+
+__device__ __forceinline__ bool has_upper_list(const int* inv, int n) {
+  for (int c = 0; c < n; ++c)
+    if (inv[3 * c] > 0 && inv[3 * c + 1] == 0 && inv[3 * c + 2] < kInf) return true;
+  return false;
+}
+
+__device__ __forceinline__ void zcopy_w(int* dst, const int* src, int d2) {
+  wsync();
+  for (int e = threadIdx.x; e < d2; e += kWave) dst[e] = src[e];
+}
+
+// Drop members of a list contained in another member (the first of equal members stays).
+// `drop` is cap ints of LDS scratch. Returns the new length.
+__device__ __forceinline__ int fed_compact(int* F, int n, int zs, int d, int* drop) {
+  const int l = threadIdx.x, d2 = d * d;
+  wsync();
+  for (int k = l; k < n; k += kWave) drop[k] = 0;
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j) {
+      if (j == k) continue;
+      wsync();
+      if (drop[k]) break;
+      if (!z_includes(F + j * zs, F + k * zs, d)) continue;
+      if (j < k || !z_includes(F + k * zs, F + j * zs, d)) {
+        wsync();
+        if (l == 0) drop[k] = 1;
+      }
+    }
+  wsync();
+  int m = 0;
+  for (int k = 0; k < n; ++k) {
+    wsync();
+    if (drop[k]) continue;
+    if (m != k)
+      for (int e = l; e < d2; e += kWave) F[m * zs + e] = F[k * zs + e];
+    ++m;
+  }
+  wsync();
+  return m;
+}
+
+// z covered by the union of the list? Subtract every member; `cur`/`nxt` are piece lists of
+// capacity kp. Returns 1 covered, 0 not, -1 capacity exceeded.
+__device__ __forceinline__ int fed_covers(const int* F, int n, int zs, const int* z, int d, int* cur, int* nxt,
+                                          int kp, int* r, int* t) {
+  const int d2 = d * d;
+  for (int c = 0; c < n; ++c)
+    if (z_includes(F + c * zs, z, d)) return 1;
+  int nc = 1;
+  zcopy_w(cur, z, d2);
+  for (int c = 0; c < n && nc > 0; ++c) {
+    int nn = 0;
+    for (int k = 0; k < nc; ++k)
+      if (!z_subtract(cur + k * zs, F + c * zs, d, nxt, zs, &nn, kp, r, t)) return -1;
+    int* sw = cur; cur = nxt; nxt = sw;
+    nc = nn;
+  }
+  return nc == 0 ? 1 : 0;
+}
+
+// Combined layout of k_stop and k_gfp in LDS ints: zone lists at stride zs = d2.
+struct LiveLds {
+  int d2, kl, kp, kc;
+  __host__ __device__ int misc() const { return 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * 3 * kInvCap + 2 * kRecipeCap + 64 + 64; }
+  __host__ __device__ int stop_ints() const { return (6 + kl + 2 * kp) * d2 + misc(); }
+  __host__ __device__ int gfp_ints() const { return (6 + kLiveW + kc + 2 * kp) * d2 + misc(); }
+};
+
+// Stop set of node v (mode 0), early witness test (mode 1: p /\ !q nodes only), deadlock test
+// (mode 2). Writes scnt (mode 0): -1 whole zone, else the number of zones in sz; sets ctr->hit
+// for a non-empty answer in modes 1 and 2.
+__global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, const LiveArgs* __restrict__ Ap,
+                                             const int* __restrict__ list, long n, long long first_id, int mode,
+                                             int* __restrict__ scnt, int* __restrict__ sz, LiveCtr* lc) {
+  extern __shared__ int sm[];
+  const DevNet& N = *Np;
+  const LiveArgs& A = *Ap;
+  const int d = N.d, d2 = N.d2, l = threadIdx.x;
+  const LiveLds LL{d2, A.kl, A.kp, A.kc};
+  int* Zd = sm;
+  int* g = Zd + d2;
+  int* tmp = g + d2;
+  int* r = tmp + d2;
+  int* t = r + d2;
+  int* zu = t + d2;
+  int* live = zu + d2;
+  int* cur = live + LL.kl * d2;
+  int* nxt = cur + LL.kp * d2;
+  int* locs = nxt + LL.kp * d2;
+  int* ints = locs + HORA_MAX_PROCS;
+  int* tl = ints + HORA_MAX_INTS;
+  int* ti = tl + HORA_MAX_PROCS;
+  int* inv = ti + HORA_MAX_INTS;
+  int* tinv = inv + 3 * kInvCap;
+  int* rec = tinv + 3 * kInvCap;
+  int* drop = rec + 2 * kRecipeCap;
+  const long b = blockIdx.x;
+  if (b >= n) return;
+  const long v = list ? list[b] : first_id + b;
+  const u64 dk = A.gdk[v];
+  if (mode == 1 && !(pred_eval(N, A.p, dk) && !pred_eval(N, A.q, dk))) return;
+  decode_key(N, dk, locs, ints);
+  wsync();
+  int fz = 0, ninv = 0;
+  u64 am = 0;
+  wave_loc_info(N, locs, ints, &fz, &am, inv, &ninv);   // the network's (restricted) invariant
+  if (ninv > kInvCap) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); return; }
+  const bool frozen = fz != 0, src_c = (fz & 1) != 0;
+  const int* zv = A.gzone + (size_t)v * d2;
+  const u64 cm = ((1ull << N.nclocks) - 1) << 1;
+  auto out_whole = [&]() __attribute__((always_inline)) {
+    if (mode == 0) { if (l == 0) scnt[A.sidx[v]] = -1; }
+    else if (l == 0) atomicExch(&lc->hit, 1);
+  };
+  auto out_none = [&]() __attribute__((always_inline)) {
+    if (mode == 0 && l == 0) scnt[A.sidx[v]] = 0;
+  };
+  // time_diverges (construction.cpp:1097-1162) on the network's invariant
+  auto time_diverges = [&]() __attribute__((always_inline)) -> bool {
+    if (N.nclocks == 0 || has_upper_list(inv, ninv) || frozen) return false;
+    zcopy_w(t, zv, d2);
+    z_up(t, d);
+    if (!z_apply_list(t, d, inv, ninv)) return false;
+    wsync();
+    bool bounded = false;
+    if (l >= 1 && l < d) bounded = t[l * d] != kInf;
+    return !__any(bounded);
+  };
+  // the table the deadlock is judged in: the unrestricted network's when psi restricted it
+  const int* doff = A.restricted ? A.oinv_off : N.inv_off;
+  const int* ddat = A.restricted ? A.oinv_data : N.inv_data;
+  if (mode == 0) {
+    if (!A.restricted) {
+      if (A.grow[v + 1] == A.grow[v] || time_diverges()) { out_whole(); return; }
+    } else {
+      if (time_diverges()) { out_whole(); return; }
+      // orig_time_open (construction.cpp:1164-1191)
+      const int no = wave_gather_inv(N, doff, ddat, locs, tinv);
+      if (no > kInvCap) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); return; }
+      if (N.nclocks > 0 && !has_upper_list(tinv, no) && !frozen) {
+        wsync();
+        for (int e = l; e < d2; e += kWave) {
+          const int i = e / d, j = e - i * d;
+          t[e] = (i == j || i == 0) ? kLe0 : kInf;
+        }
+        if (z_apply_list(t, d, tinv, no)) {
+          wsync();
+          bool bounded = false;
+          if (l >= 1 && l < d) bounded = t[l * d] != kInf;
+          if (!__any(bounded)) { out_none(); return; }
+        }
+      }
+    }
+  }
+  // state_has_deadlock (deadlock.cpp:74-268)
+  const int nd = wave_gather_inv(N, doff, ddat, locs, tinv);
+  if (nd > kInvCap) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); return; }
+  zcopy_w(Zd, zv, d2);
+  if (!frozen) z_up(Zd, d);
+  if (!z_apply_list(Zd, d, tinv, nd)) { out_none(); return; }
+  const int nr = build_recipes(N, locs, ints, src_c, rec, kRecipeCap);
+  if (nr < 0) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); return; }
+  wsync();
+  int nl = 0;
+  for (int k = 0; k < nr; ++k) {
+    wsync();
+    const int se = rec[2 * k], re = rec[2 * k + 1];
+    zcopy_w(g, Zd, d2);
+    if (!apply_edge_guard(N, se, g)) continue;
+    if (re >= 0 && !apply_edge_guard(N, re, g)) continue;
+    wsync();
+    if (l < N.np) tl[l] = locs[l];
+    wsync();
+    if (l == 0) {
+      tl[N.edge_proc[se]] = N.edge_tgt[se];
+      if (re >= 0) tl[N.edge_proc[re]] = N.edge_tgt[re];
+    }
+    wsync();
+    const int nt = wave_gather_inv(N, doff, ddat, tl, tinv);
+    if (nt > kInvCap) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); return; }
+    const u64 rm = edge_reset_mask(N, se) | (re >= 0 ? edge_reset_mask(N, re) : 0ull);
+    bool ok = true;
+    for (int c = 0; c < nt && ok; ++c) {
+      const int ci = tinv[3 * c], cj = tinv[3 * c + 1], cb = tinv[3 * c + 2];
+      const int i = ((rm >> ci) & 1) ? 0 : ci, j = ((rm >> cj) & 1) ? 0 : cj;
+      if (i == j) { if (cb < kLe0) ok = false; continue; }
+      ok = z_constrain(g, d, i, j, cb);
+    }
+    if (!ok) continue;
+    if (z_equal(g, Zd, d)) { out_none(); return; }   // enabled everywhere: no deadlock
+    // keep `live` inclusion-minimal
+    bool dom = false;
+    for (int m = 0; m < nl && !dom; ++m) dom = z_includes(live + m * d2, g, d);
+    if (dom) continue;
+    int m2 = 0;
+    for (int m = 0; m < nl; ++m) {
+      if (z_includes(g, live + m * d2, d)) continue;
+      if (m2 != m) zcopy_w(live + m2 * d2, live + m * d2, d2);
+      ++m2;
+    }
+    nl = m2;
+    if (nl >= LL.kl) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); return; }
+    zcopy_w(live + nl * d2, g, d2);
+    ++nl;
+  }
+  if (!frozen)
+    for (int m = 0; m < nl; ++m) z_down(live + m * d2, d);
+  for (int m = 0; m < nl; ++m)
+    if (z_includes(live + m * d2, Zd, d)) { out_none(); return; }
+  // Dead = Zd minus every escape
+  int nc = 1;
+  zcopy_w(cur, Zd, d2);
+  int* c0 = cur;
+  int* c1 = nxt;
+  for (int m = 0; m < nl && nc > 0; ++m) {
+    int nn = 0;
+    for (int k = 0; k < nc; ++k)
+      if (!z_subtract(c0 + k * d2, live + m * d2, d, c1, d2, &nn, LL.kp, r, t)) {
+        if (l == 0) atomicExch(&lc->err, ERR_CAPPED);
+        return;
+      }
+    int* sw = c0; c0 = c1; c1 = sw;
+    nc = nn;
+  }
+  if (nc == 0) { out_none(); return; }
+  if (mode != 0) { if (l == 0) atomicExch(&lc->hit, 1); return; }
+  if (nc > kStopCap) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); return; }
+  const int s = A.sidx[v];
+  wsync();
+  for (int k = 0; k < nc; ++k)
+    for (int e = l; e < d2; e += kWave) sz[((size_t)s * kStopCap + k) * d2 + e] = c0[k * d2 + e];
+  if (l == 0) scnt[s] = nc;
+}
+
+// ---------------------------------------------------------------- closure of the backward pass
+
+__global__ void k_sseed(const DevNet* __restrict__ Np, const LiveArgs* __restrict__ Ap, long N, int* __restrict__ in_s,
+                        int* __restrict__ slist, LiveCtr* lc) {
+  const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const LiveArgs& A = *Ap;
+  bool seed = false;
+  if (v < N && A.expanded[v]) {
+    const u64 dk = A.gdk[v];
+    if (A.kind == HORA_LIVE_LEADSTO) seed = pred_eval(*Np, A.p, dk) && !pred_eval(*Np, A.q, dk);
+    else seed = v == 0 && pred_eval(*Np, A.p, dk);
+  }
+  if (v < N) in_s[v] = seed ? 1 : 0;
+  const int idx = wave_append(&lc->ns, seed);
+  if (seed) slist[idx] = (int)v;
+}
+
+__global__ void k_sstep(const DevNet* __restrict__ Np, const LiveArgs* __restrict__ Ap, const int* __restrict__ slist,
+                        int from, int to, int* __restrict__ in_s, int* __restrict__ slist_out, LiveCtr* lc) {
+  const LiveArgs& A = *Ap;
+  const int k = from + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= to) return;
+  const int v = slist[k];
+  for (long long e = A.grow[v]; e < A.grow[v + 1]; ++e) {
+    const int t = A.etgt[e];
+    if (in_s[t] || !A.expanded[t]) continue;
+    const u64 dk = A.gdk[t];
+    const bool nq = A.kind == HORA_LIVE_LEADSTO ? !pred_eval(*Np, A.q, dk) : pred_eval(*Np, A.p, dk);
+    if (!nq) continue;
+    if (atomicCAS(&in_s[t], 0, 1) == 0) slist_out[atomicAdd(&lc->ns, 1)] = t;
+  }
+}
+
+__global__ void k_sindex(const int* __restrict__ slist, int ns, int* __restrict__ sidx) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < ns) sidx[slist[k]] = k;
+}
+
+// ---------------------------------------------------------------- greatest fixpoint
+
+// W[s] := {Z_v} in bank 0
+__global__ __launch_bounds__(64) void k_gseed(const LiveArgs* __restrict__ Ap, const int* __restrict__ slist, int ns,
+                                              int d2, int* __restrict__ wcnt0, int* __restrict__ wz0,
+                                              int* __restrict__ sel, int* __restrict__ chg) {
+  const int s = blockIdx.x;
+  if (s >= ns) return;
+  const int v = slist[s];
+  const int* src = Ap->gzone + (size_t)v * d2;
+  int* dst = wz0 + (size_t)s * kLiveW * d2;
+  for (int e = threadIdx.x; e < d2; e += kWave) dst[e] = src[e];
+  if (threadIdx.x == 0) {
+    wcnt0[s] = 1;
+    sel[s] = 0;
+    chg[s] = 1;
+  }
+}
+
+struct GfpBanks {
+  int* wcnt0;       // [ns] zones in the value, bank 0 / bank 1
+  int* wcnt1;
+  int* wz0;         // [ns*kLiveW*d2]
+  int* wz1;
+  const int* sel_in;
+  int* sel_out;
+  const int* chg_in;
+  int* chg_out;
+  const int* scnt;  // stop sets
+  const int* sz;
+};
+
+// One Jacobi round of relax (fixpoint.cpp:750-864) over the closure, one wavefront per node. A
+// node is revisited only when a successor's value changed in the previous round. Values are read
+// from the bank each node selected at the start of the round and written to the other bank.
+__global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const LiveArgs* __restrict__ Ap,
+                                            const int* __restrict__ slist, int ns, int first, GfpBanks B,
+                                            LiveCtr* lc) {
+  extern __shared__ int sm[];
+  const DevNet& N = *Np;
+  const LiveArgs& A = *Ap;
+  const int d = N.d, d2 = N.d2, l = threadIdx.x;
+  const LiveLds LL{d2, A.kl, A.kp, A.kc};
+  int* pz = sm;
+  int* tmp = pz + d2;
+  int* r = tmp + d2;
+  int* t = r + d2;
+  int* Wv = t + 2 * d2;
+  int* cand = Wv + kLiveW * d2;
+  int* cur = cand + LL.kc * d2;
+  int* nxt = cur + LL.kp * d2;
+  int* locs = nxt + LL.kp * d2;
+  int* ints = locs + HORA_MAX_PROCS;
+  int* tlocs = ints + HORA_MAX_INTS;
+  int* tints = tlocs + HORA_MAX_PROCS;
+  int* inv = tints + HORA_MAX_INTS;
+  int* tinv = inv + 3 * kInvCap;
+  int* drop = tinv + 3 * kInvCap;
+  const int s = blockIdx.x;
+  if (s >= ns) return;
+  const int v = slist[s];
+  const int bi = B.sel_in[s];
+  auto keep = [&]() __attribute__((always_inline)) {
+    if (l == 0) { B.sel_out[s] = bi; B.chg_out[s] = 0; }
+  };
+  const long long e0 = A.grow[v], e1 = A.grow[v + 1];
+  if (!first) {
+    bool dirty = false;
+    for (long long e = e0 + l; e < e1; e += kWave) {
+      const int tt = A.etgt[e];
+      if (A.in_s[tt] && B.chg_in[A.sidx[tt]]) dirty = true;
+    }
+    if (!__any(dirty)) { keep(); return; }
+  }
+  auto wcnt = [&](int b) __attribute__((always_inline)) { return b ? B.wcnt1 : B.wcnt0; };
+  auto wzb = [&](int b) __attribute__((always_inline)) { return b ? B.wz1 : B.wz0; };
+  const int nW = wcnt(bi)[s];
+  if (nW == 0) { keep(); return; }
+  {
+    const int* src = wzb(bi) + (size_t)s * kLiveW * d2;
+    for (int e = l; e < nW * d2; e += kWave) Wv[e] = src[e];
+  }
+  decode_key(N, A.gdk[v], locs, ints);
+  wsync();
+  int fzv = 0, ninv = 0;
+  u64 amv = 0;
+  wave_loc_info(N, locs, ints, &fzv, &amv, inv, &ninv);
+  if (ninv > kInvCap) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); keep(); return; }
+  const u64 cm = ((1ull << N.nclocks) - 1) << 1;
+  u32 covm = 0;
+  const u32 full = (nW >= 32) ? 0xffffffffu : ((1u << nW) - 1);
+  int nc = 0;
+  unsigned long long pcalls = 0;
+  bool capped = false;
+  for (long long e = e0; e < e1 && !capped; ++e) {
+    const int tt = A.etgt[e];
+    if (!A.in_s[tt]) continue;
+    const int st = A.sidx[tt];
+    const int bt = B.sel_in[st];
+    const int nY = wcnt(bt)[st];
+    if (nY == 0) continue;
+    wsync();
+    decode_key(N, A.gdk[tt], tlocs, tints);
+    wsync();
+    int fzt = 0, ntinv = 0;
+    u64 amt = 0;
+    wave_loc_info(N, tlocs, tints, &fzt, &amt, tinv, &ntinv);
+    if (ntinv > kInvCap) { capped = true; break; }
+    const u64 dead_t = ~amt & cm;
+    const int se = A.ese[e], re = A.ere[e];
+    const u64 rm = edge_reset_mask(N, se) | (re >= 0 ? edge_reset_mask(N, re) : 0ull);
+    const int* Ys = wzb(bt) + (size_t)st * kLiveW * d2;
+    for (int y = 0; y < nY; ++y) {
+      ++pcalls;
+      // exact_predecessor (operators.cpp:257-306)
+      zcopy_w(pz, Ys + y * d2, d2);
+      z_free_many(pz, d, dead_t, tmp);
+      if (!z_apply_list(pz, d, tinv, ntinv)) continue;
+      if (!fzt) {
+        z_down(pz, d);
+        if (!z_apply_list(pz, d, tinv, ntinv)) continue;
+      }
+      bool ok = true;
+      for (u64 m = rm; m && ok; m &= m - 1) {
+        const int rc = __builtin_ctzll(m);
+        ok = z_constrain(pz, d, rc, 0, kLe0) && z_constrain(pz, d, 0, rc, kLe0);
+      }
+      if (!ok) continue;
+      z_free_many(pz, d, rm, tmp);
+      if (!apply_edge_guard(N, se, pz)) continue;
+      if (re >= 0 && !apply_edge_guard(N, re, pz)) continue;
+      if (!fzv) z_down(pz, d);
+      if (!z_apply_list(pz, d, inv, ninv)) continue;
+      for (int k = 0; k < nW; ++k)
+        if (!((covm >> k) & 1) && z_includes(pz, Wv + k * d2, d)) covm |= 1u << k;
+      if (covm == full) {   // every member covered: the value cannot move
+        if (l == 0) atomicAdd(&lc->pre_calls, pcalls);
+        keep();
+        return;
+      }
+      if (nc >= LL.kc) nc = fed_compact(cand, nc, d2, d, drop);
+      if (nc >= LL.kc) { capped = true; break; }
+      zcopy_w(cand + nc * d2, pz, d2);
+      ++nc;
+    }
+  }
+  if (l == 0) atomicAdd(&lc->pre_calls, pcalls);
+  if (capped) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); keep(); return; }
+  nc = fed_compact(cand, nc, d2, d, drop);
+  // does cand cover every member of W[v]? -1: capacity exceeded
+  auto escapes = [&]() __attribute__((always_inline)) -> int {
+    for (int k = 0; k < nW; ++k) {
+      const int cv = fed_covers(cand, nc, d2, Wv + k * d2, d, cur, nxt, LL.kp, r, t);
+      if (cv < 0) return -1;
+      if (cv == 0) return 1;
+    }
+    return 0;
+  };
+  int es = escapes();
+  if (es < 0) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); keep(); return; }
+  if (es == 0) { keep(); return; }
+  const int sc = B.scnt[s];
+  if (sc != 0) {
+    const int add = sc < 0 ? 1 : sc;
+    if (nc + add > LL.kc) nc = fed_compact(cand, nc, d2, d, drop);
+    if (nc + add > LL.kc) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); keep(); return; }
+    if (sc < 0) zcopy_w(cand + nc * d2, A.gzone + (size_t)v * d2, d2);
+    else
+      for (int k = 0; k < sc; ++k) zcopy_w(cand + (nc + k) * d2, B.sz + ((size_t)s * kStopCap + k) * d2, d2);
+    nc += add;
+    nc = fed_compact(cand, nc, d2, d, drop);
+    es = escapes();
+    if (es < 0) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); keep(); return; }
+    if (es == 0) { keep(); return; }
+  }
+  // W[v] /\ cand, member by member (products into `cur`, capacity kp)
+  int nn = 0;
+  for (int k = 0; k < nW; ++k) {
+    const int* z = Wv + k * d2;
+    bool whole = false;
+    for (int c = 0; c < nc && !whole; ++c) whole = z_includes(cand + c * d2, z, d);
+    if (whole) {
+      if (nn >= LL.kp) { capped = true; break; }
+      zcopy_w(cur + nn * d2, z, d2);
+      ++nn;
+      continue;
+    }
+    for (int c = 0; c < nc; ++c) {
+      if (!z_intersect(tmp, z, cand + c * d2, d)) continue;
+      if (nn >= LL.kp) nn = fed_compact(cur, nn, d2, d, drop);
+      if (nn >= LL.kp) { capped = true; break; }
+      zcopy_w(cur + nn * d2, tmp, d2);
+      ++nn;
+    }
+    if (capped) break;
+  }
+  if (!capped) nn = fed_compact(cur, nn, d2, d, drop);
+  if (capped || nn > kLiveW) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); keep(); return; }
+  const int bo = 1 - bi;
+  int* dst = wzb(bo) + (size_t)s * kLiveW * d2;
+  wsync();
+  for (int e = l; e < nn * d2; e += kWave) dst[e] = cur[e];
+  if (l == 0) {
+    wcnt(bo)[s] = nn;
+    B.sel_out[s] = bo;
+    B.chg_out[s] = 1;
+    atomicAdd(&lc->changed, 1);
+  }
+}
+
+__global__ void k_gverdict(const DevNet* __restrict__ Np, const LiveArgs* __restrict__ Ap, const int* __restrict__ slist,
+                           int ns, const int* __restrict__ sel, const int* __restrict__ wcnt0,
+                           const int* __restrict__ wcnt1, signed char* __restrict__ wne, LiveCtr* lc) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const LiveArgs& A = *Ap;
+  int cnt = 0;
+  bool viol = false;
+  if (s < ns) {
+    const int v = slist[s];
+    cnt = sel[s] ? wcnt1[s] : wcnt0[s];
+    wne[v] = cnt > 0 ? 1 : 0;
+    if (cnt > 0) {
+      if (A.kind == HORA_LIVE_LEADSTO) viol = pred_eval(*Np, A.p, A.gdk[v]);
+      else viol = v == 0;
+      atomicMax(&lc->fed_max, cnt);
+    }
+  }
+  const u64 b = __ballot(cnt > 0);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd((unsigned long long*)&lc->nonempty, (unsigned long long)__popcll(b));
+  if (__any(viol) && (threadIdx.x & 63) == 0) atomicExch(&lc->violated, 1);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host driver
+
+struct LiveState {
+  // query
+  DBuf<int> pred;          // term offsets and atoms of p and q
+  DBuf<int> oinv_off, oinv_data;
+  LiveArgs args{};
+  LiveArgs* argsp = nullptr;
+  DBuf<LiveCtr> lc;
+  // graph
+  DBuf<u64> gdk;
+  DBuf<u32> gnext;
+  DBuf<long long> grow;
+  DBuf<int> etgt, ese, ere, s_se, s_re, cov_old, tslot, fnew, newlist;
+  DBuf<signed char> expanded, wne;
+  long long nodes = 0, edges = 0;
+  // backward pass
+  DBuf<int> in_s, slist, sidx, scnt, sz, wcnt0, wcnt1, wz0, wz1, sel0, sel1, chg0, chg1;
+  int ns = 0;
+  ~LiveState() {
+    if (argsp) { (void)hipDeviceSynchronize(); (void)hipFree(argsp); }
+  }
+};
+
+void live_free(LiveState* L) { delete L; }
+
+namespace {
+
+template <class T>
+void grow_keep_to(DBuf<T>& b, size_t n, hipStream_t s) {
+  if (n > b.cap || !b.p) b.grow_keep(std::max<size_t>(n, 1024), s);
+}
+
+LiveCtr read_lc(hora_hip_ctx* c, LiveState& L) {
+  LiveCtr h;
+  HIP_OK(hipMemcpyAsync(&h, L.lc.p, sizeof(LiveCtr), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return h;
+}
+
+void upload_args(hora_hip_ctx* c, LiveState& L) {
+  HIP_OK(hipMemcpyAsync(L.argsp, &L.args, sizeof(LiveArgs), hipMemcpyHostToDevice, c->stream));
+}
+
+void sync_graph_args(LiveState& L) {
+  L.args.gdk = L.gdk.p;
+  L.args.gzone = nullptr;
+  L.args.grow = L.grow.p;
+  L.args.etgt = L.etgt.p;
+  L.args.ese = L.ese.p;
+  L.args.ere = L.ere.p;
+  L.args.expanded = L.expanded.p;
+}
+
+// Federation capacities from the LDS budget (64 KiB per workgroup).
+void live_caps(int d2, LiveArgs& a) {
+  const int budget = 64 * 1024 / 4;
+  LiveLds t{d2, 0, 0, 0};
+  const int zones = (budget - t.misc()) / d2;
+  const int k = std::min(64, std::max(4, (zones - 6 - kLiveW) / 3));
+  a.kl = k;
+  a.kp = k;
+  a.kc = k;
+}
+
+int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_live_stats* st) {
+  const auto t0 = std::chrono::steady_clock::now();
+  std::memset(st, 0, sizeof(*st));
+  *holds = 0;
+  if (c->d > kLiveMaxDim) { c->err = "liveness on the device covers zone dimension up to 16"; return HORA_HIP_E_SCOPE; }
+  if (c->dn.ncc > 0) { c->err = "open the context with a target without clock constraints for liveness"; return HORA_HIP_E_ARG; }
+  if (q->kind < HORA_LIVE_LEADSTO || q->kind > HORA_LIVE_DEADLOCK) return HORA_HIP_E_ARG;
+  if (!c->live) c->live = new LiveState();
+  LiveState& L = *c->live;
+  const int d2 = c->d2;
+  // ---- query tables
+  {
+    std::vector<int> blob;
+    auto put_pred = [&](const hora_dpred& p, DevPred& out, size_t* o_off, size_t* o_at) {
+      out.nterms = p.nterms;
+      *o_off = blob.size();
+      for (int t = 0; t <= p.nterms; ++t) blob.push_back(p.nterms > 0 ? p.term_off[t] : 0);
+      *o_at = blob.size();
+      const int na = p.nterms > 0 ? p.term_off[p.nterms] : 0;
+      for (int a = 0; a < 4 * na; ++a) blob.push_back(p.atoms[a]);
+      blob.push_back(0);
+    };
+    size_t po, pa, qo, qa;
+    put_pred(q->p, L.args.p, &po, &pa);
+    put_pred(q->q, L.args.q, &qo, &qa);
+    L.pred.upload(blob);
+    L.args.p.off = L.pred.p + po; L.args.p.atoms = L.pred.p + pa;
+    L.args.q.off = L.pred.p + qo; L.args.q.atoms = L.pred.p + qa;
+  }
+  L.args.kind = q->kind;
+  L.args.restricted = q->orig_inv_off ? 1 : 0;
+  if (L.args.restricted) {
+    L.oinv_off.upload(std::vector<int>(q->orig_inv_off, q->orig_inv_off + c->tl + 1));
+    const int nc3 = 3 * q->orig_inv_off[c->tl];
+    L.oinv_data.upload(nc3 > 0 ? std::vector<int>(q->orig_inv_data, q->orig_inv_data + nc3) : std::vector<int>(3, 0));
+    L.args.oinv_off = L.oinv_off.p;
+    L.args.oinv_data = L.oinv_data.p;
+  }
+  live_caps(d2, L.args);
+  if (!L.argsp) HIP_OK(hipMalloc(&L.argsp, sizeof(LiveArgs)));
+  L.lc.ensure(1);
+  HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
+
+  c->dn.lu_plus = 0;
+  HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
+  alloc_table(c, 16);
+  c->ctr.ensure(1);
+  HIP_OK(hipMemsetAsync(c->ctr.p, 0, sizeof(Counters), c->stream));
+  {
+    const int ft = 0x7fffffff;
+    HIP_OK(hipMemcpyAsync(&c->ctr.p->first_target, &ft, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  }
+  // graph storage
+  size_t ncap = 1 << 16, ecap = 1 << 18;
+  L.gdk.ensure(ncap); L.gnext.ensure(ncap); L.grow.ensure(ncap + 1); L.expanded.ensure(ncap);
+  c->zarena.ensure(ncap * d2);
+  L.etgt.ensure(ecap); L.ese.ensure(ecap); L.ere.ensure(ecap);
+  HIP_OK(hipMemsetAsync(L.expanded.p, 0, L.expanded.cap, c->stream));
+  L.nodes = 0;
+  L.edges = 0;
+  L.ns = 0;
+
+  const bool eg = q->kind == HORA_LIVE_EG;
+  u64 dk0 = 0;
+  for (int p = 0; p < c->np; ++p) dk0 |= (u64)(unsigned)c->h_loc_init[p] << c->h_kshift[p];
+  for (int v = 0; v < c->ni; ++v) dk0 |= (u64)(unsigned)(c->h_int_init[v] - c->h_int_lo[v]) << c->h_kshift[c->np + v];
+  c->fr_dk.ensure(1);
+  c->fr_zid.ensure(1);
+  HIP_OK(hipMemcpyAsync(c->fr_dk.p, &dk0, sizeof(u64), hipMemcpyHostToDevice, c->stream));
+
+  long F = 1;
+  long long fbase = 0, rows_done = -1;   // rows [0, rows_done] written
+  bool init = true;
+  int early = -1;
+  Tally prev;
+  for (;;) {
+    sync_graph_args(L);
+    upload_args(c, L);
+    long R = 1;
+    c->rec_cnt.ensure(F);
+    c->rec_off.ensure(F);
+    c->pos.ensure(F + 1);
+    if (!init) {
+      ++st->levels;
+      const size_t lds_c = (HORA_MAX_PROCS + HORA_MAX_INTS + 2 * kRecipeCap + 3 * kInvCap) * sizeof(int);
+      hipLaunchKernelGGL(k_count, dim3(F), dim3(64), lds_c, c->stream, c->dnp, c->fr_dk.p, (int)F, c->rec_cnt.p, c->ctr.p);
+      hipLaunchKernelGGL(k_gfilter, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, eg ? 1 : 0,
+                         c->fr_dk.p, (int)F, c->rec_cnt.p, L.expanded.p, (long)fbase);
+      R = scan_exclusive(c, c->rec_cnt.p, F, c->pos.p);
+      hipLaunchKernelGGL(k_offsets_to_int, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->pos.p, F, c->rec_off.p);
+    } else {
+      HIP_OK(hipMemsetAsync(c->rec_off.p, 0, sizeof(int), c->stream));
+    }
+    c->s_valid.ensure(R);
+    c->s_tgt.ensure(R);
+    c->s_dk.ensure(R);
+    c->s_zone.ensure((size_t)R * d2);
+    L.s_se.ensure(R);
+    L.s_re.ensure(R);
+    ExpandOut eo{c->s_valid.p, c->s_dk.p, c->s_zone.p, c->s_tgt.p, L.s_se.p, L.s_re.p};
+    if (R > 0)
+      hipLaunchKernelGGL(k_expand, dim3(F), dim3(64), expand_lds_ints(d2) * sizeof(int), c->stream, c->dnp, c->fr_dk.p,
+                         c->fr_zid.p, c->zarena.p, (int)F, c->rec_off.p, eo, c->ctr.p, init ? 1 : 0);
+    c->pos.ensure(R + 1);
+    const long G = R > 0 ? scan_exclusive(c, c->s_valid.p, R, c->pos.p) : 0;
+    if (!init) {
+      st->expanded_nodes += F;
+      st->generated += G;
+      grow_keep_to(L.grow, (size_t)(fbase + F + 1), c->stream);
+      hipLaunchKernelGGL(k_grows, dim3(blocks_for(F + 1, 256)), dim3(256), 0, c->stream, c->rec_off.p, c->pos.p, R,
+                         (int)F, (long)fbase, L.edges, (long long)G, L.grow.p);
+      rows_done = fbase + F;
+    }
+    {
+      const Tally hc = read_counters(c);
+      if (hc.err) { c->err = "a state has more transitions or invariant constraints than the device holds"; return HORA_HIP_E_MEMORY; }
+    }
+    long F2 = 0;
+    if (G > 0) {
+      c->sj.ensure(G);
+      hipLaunchKernelGGL(k_compact_valid, dim3(blocks_for(R, 256)), dim3(256), 0, c->stream, c->s_valid.p, c->pos.p, R, c->sj.p);
+      {
+        const Tally hc = read_counters(c);
+        while ((double)(hc[C_KEYS] + (u64)G) > 0.6 * (double)c->H) grow_table(c);
+      }
+      Passed P = passed_view(c);
+      c->gbk.ensure(G);
+      c->gticket.ensure(G);
+      c->touched.ensure(G);
+      c->covered.ensure(G);
+      L.cov_old.ensure(G);
+      L.tslot.ensure(G);
+      L.fnew.ensure(G);
+      HIP_OK(hipMemsetAsync(L.fnew.p, 0, (size_t)G * sizeof(int), c->stream));
+      HIP_OK(hipMemsetAsync(&c->ctr.p->ntouched, 0, sizeof(int), c->stream));
+      hipLaunchKernelGGL(k_find, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->sj.p, c->s_dk.p, G, c->gbk.p, c->ctr.p);
+      hipLaunchKernelGGL(k_gprobe, dim3(blocks_for(G, 4)), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p, c->s_dk.p,
+                         c->s_zone.p, G, L.gnext.p, c->zarena.p, c->covered.p, L.cov_old.p, c->ctr.p);
+      hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p, G,
+                         c->gticket.p, c->touched.p, c->ctr.p);
+      int T = 0;
+      HIP_OK(hipMemcpyAsync(&T, &c->ctr.p->ntouched, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIP_OK(hipStreamSynchronize(c->stream));
+      if (T > 0) {
+        c->caps.ensure(T);
+        c->segs.ensure(T);
+        hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                           c->caps.p, c->segs.p);
+        c->pos2.ensure(T + 1);
+        const long long Wl = scan_exclusive(c, c->caps.p, T, c->pos2.p);
+        hipLaunchKernelGGL(k_touched_offsets, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                           c->pos2.p, c->pos2.p, (u64)0);
+        c->wlist.ensure(Wl + 1);
+        hipLaunchKernelGGL(k_scatter, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p,
+                           c->gticket.p, G, c->wlist.p);
+        const size_t lds_m = (d2 + HORA_MAX_DIM + HORA_MAX_PROCS + kSortLds) * sizeof(int);
+        hipLaunchKernelGGL(k_gmerge, dim3(T), dim3(64), lds_m, c->stream, c->dnp, P, c->touched.p, T, c->wlist.p,
+                           c->sj.p, c->s_dk.p, c->s_zone.p, L.tslot.p, L.fnew.p, c->ctr.p);
+      }
+      c->pos.ensure(G + 1);
+      F2 = scan_exclusive(c, L.fnew.p, G, c->pos.p);
+      // capacities
+      const long long nbase = L.nodes;
+      const size_t nneed = (size_t)(nbase + F2 + 1);
+      if (nneed > L.gdk.cap) {
+        const size_t nc2 = std::max(nneed, (size_t)L.gdk.cap * 2);
+        grow_keep_to(L.gdk, nc2, c->stream);
+        grow_keep_to(L.gnext, nc2, c->stream);
+        grow_keep_to(L.grow, nc2 + 1, c->stream);
+        const size_t old = L.expanded.cap;
+        grow_keep_to(L.expanded, nc2, c->stream);
+        HIP_OK(hipMemsetAsync(L.expanded.p + old, 0, L.expanded.cap - old, c->stream));
+      }
+      if (nneed * d2 > c->zarena.cap) c->zarena.grow_keep(std::max(nneed * d2, c->zarena.cap * 2), c->stream);
+      if (!init && (size_t)(L.edges + G) > L.etgt.cap) {
+        const size_t ec2 = std::max((size_t)(L.edges + G), L.etgt.cap * 2);
+        grow_keep_to(L.etgt, ec2, c->stream);
+        grow_keep_to(L.ese, ec2, c->stream);
+        grow_keep_to(L.ere, ec2, c->stream);
+      }
+      L.newlist.ensure(std::max<long>(F2, 1));
+      hipLaunchKernelGGL(k_gassign, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, G, nbase, L.edges, init ? 0 : 1,
+                         L.cov_old.p, L.tslot.p, L.fnew.p, c->pos.p, c->sj.p, L.s_se.p, L.s_re.p, L.etgt.p, L.ese.p,
+                         L.ere.p, L.newlist.p);
+      c->fr_dk2.ensure(std::max<long>(F2, 1));
+      c->fr_zid2.ensure(std::max<long>(F2, 1));
+      if (F2 > 0)
+        hipLaunchKernelGGL(k_gcopy, dim3(F2), dim3(64), 0, c->stream, (long)F2, nbase, L.newlist.p, c->sj.p, c->s_dk.p,
+                           c->s_zone.p, d2, c->zarena.p, L.gdk.p, L.gnext.p, c->fr_dk2.p, c->fr_zid2.p);
+      if (T > 0)
+        hipLaunchKernelGGL(k_glink, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T, c->wlist.p,
+                           c->pos.p, nbase, L.gnext.p);
+      if (!init) {
+        L.edges += G;
+        st->fwd_edges = L.edges;
+      }
+      L.nodes += F2;
+    }
+    // early witness on the level just expanded (leadsto_witness / the deadlock scan)
+    if (!init && q->early && (q->kind == HORA_LIVE_LEADSTO || q->kind == HORA_LIVE_DEADLOCK)) {
+      sync_graph_args(L);
+      L.args.gzone = c->zarena.p;
+      upload_args(c, L);
+      const LiveLds LL{d2, L.args.kl, L.args.kp, L.args.kc};
+      hipLaunchKernelGGL(k_stop, dim3(F), dim3(64), LL.stop_ints() * sizeof(int), c->stream, c->dnp, L.argsp,
+                         (const int*)nullptr, (long)F, (long long)fbase, q->kind == HORA_LIVE_LEADSTO ? 1 : 2,
+                         (int*)nullptr, (int*)nullptr, L.lc.p);
+      const LiveCtr h = read_lc(c, L);
+      if (h.err) {
+        c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a stop set" : "invariant or transition table too large";
+        return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
+      }
+      if (h.hit) { early = 0; st->early = 1; }
+    }
+    {
+      const Tally hc = read_counters(c);
+      if (hc.err) { c->err = hc.err == ERR_HASH_FULL ? "hash table full" : "device capacity exceeded"; return HORA_HIP_E_MEMORY; }
+      st->covered = (long long)(hc[C_SUB] + hc[C_PSUB]);
+      st->disc_keys = (long long)hc[C_KEYS];
+      prev = hc;
+    }
+    if (early >= 0) break;
+    std::swap(c->fr_dk.p, c->fr_dk2.p); std::swap(c->fr_dk.cap, c->fr_dk2.cap);
+    std::swap(c->fr_zid.p, c->fr_zid2.p); std::swap(c->fr_zid.cap, c->fr_zid2.cap);
+    fbase = init ? 0 : fbase + F;
+    F = F2;
+    init = false;
+    if (F == 0) break;
+  }
+  // rows of nodes never expanded
+  if (rows_done < L.nodes) {
+    grow_keep_to(L.grow, (size_t)(L.nodes + 1), c->stream);
+    const long from = (long)(rows_done + 1);
+    hipLaunchKernelGGL(k_gfill_rows, dim3(blocks_for(L.nodes - from + 1, 256)), dim3(256), 0, c->stream, L.grow.p, from,
+                       (long)L.nodes, L.edges);
+  }
+  st->outer_nodes = L.nodes;
+  st->fwd_edges = L.edges;
+  const auto t1 = std::chrono::steady_clock::now();
+  sync_graph_args(L);
+  L.args.gzone = c->zarena.p;
+  const long long N = L.nodes;
+  L.in_s.ensure(std::max<long long>(N, 1));
+  L.wne.ensure(std::max<long long>(N, 1));
+  L.sidx.ensure(std::max<long long>(N, 1));
+  HIP_OK(hipMemsetAsync(L.in_s.p, 0, sizeof(int) * std::max<long long>(N, 1), c->stream));
+  HIP_OK(hipMemsetAsync(L.wne.p, 0, std::max<long long>(N, 1), c->stream));
+  HIP_OK(hipMemsetAsync(L.sidx.p, 0xff, sizeof(int) * std::max<long long>(N, 1), c->stream));
+  L.args.in_s = L.in_s.p;
+  L.args.sidx = L.sidx.p;
+  upload_args(c, L);
+  if (early >= 0) {
+    *holds = early;
+  } else if (N == 0) {
+    *holds = q->kind == HORA_LIVE_EG ? 0 : 1;
+  } else if (q->kind == HORA_LIVE_DEADLOCK) {
+    const LiveLds LL{d2, L.args.kl, L.args.kp, L.args.kc};
+    HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
+    hipLaunchKernelGGL(k_stop, dim3(N), dim3(64), LL.stop_ints() * sizeof(int), c->stream, c->dnp, L.argsp,
+                       (const int*)nullptr, (long)N, (long long)0, 2, (int*)nullptr, (int*)nullptr, L.lc.p);
+    const LiveCtr h = read_lc(c, L);
+    if (h.err) { c->err = "federation capacity exceeded in a deadlock test"; return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY; }
+    *holds = h.hit ? 0 : 1;
+  } else {
+    // ---- closure S
+    HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
+    L.slist.ensure(std::max<long long>(N, 1));
+    hipLaunchKernelGGL(k_sseed, dim3(blocks_for(N, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, (long)N, L.in_s.p,
+                       L.slist.p, L.lc.p);
+    int from = 0, to = read_lc(c, L).ns;
+    while (to > from) {
+      hipLaunchKernelGGL(k_sstep, dim3(blocks_for(to - from, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, L.slist.p,
+                         from, to, L.in_s.p, L.slist.p, L.lc.p);
+      from = to;
+      to = read_lc(c, L).ns;
+    }
+    const int ns = to;
+    L.ns = ns;
+    st->closure = ns;
+    if (ns > 0) {
+      hipLaunchKernelGGL(k_sindex, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.slist.p, ns, L.sidx.p);
+      // ---- stop sets
+      L.scnt.ensure(ns);
+      L.sz.ensure((size_t)ns * kStopCap * d2);
+      const LiveLds LL{d2, L.args.kl, L.args.kp, L.args.kc};
+      hipLaunchKernelGGL(k_stop, dim3(ns), dim3(64), LL.stop_ints() * sizeof(int), c->stream, c->dnp, L.argsp,
+                         L.slist.p, (long)ns, (long long)0, 0, L.scnt.p, L.sz.p, L.lc.p);
+      {
+        const LiveCtr h = read_lc(c, L);
+        if (h.err) {
+          c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a stop set" : "invariant or transition table too large";
+          st->capped = h.err == ERR_CAPPED;
+          return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
+        }
+      }
+      // ---- fixpoint
+      L.wcnt0.ensure(ns); L.wcnt1.ensure(ns);
+      L.wz0.ensure((size_t)ns * kLiveW * d2); L.wz1.ensure((size_t)ns * kLiveW * d2);
+      L.sel0.ensure(ns); L.sel1.ensure(ns); L.chg0.ensure(ns); L.chg1.ensure(ns);
+      HIP_OK(hipMemsetAsync(L.wcnt1.p, 0, sizeof(int) * ns, c->stream));
+      hipLaunchKernelGGL(k_gseed, dim3(ns), dim3(64), 0, c->stream, L.argsp, L.slist.p, ns, d2, L.wcnt0.p, L.wz0.p,
+                         L.sel0.p, L.chg0.p);
+      int* sel_in = L.sel0.p; int* sel_out = L.sel1.p;
+      int* chg_in = L.chg0.p; int* chg_out = L.chg1.p;
+      int round = 0;
+      for (;;) {
+        HIP_OK(hipMemsetAsync(&L.lc.p->changed, 0, sizeof(int), c->stream));
+        GfpBanks B{L.wcnt0.p, L.wcnt1.p, L.wz0.p, L.wz1.p, sel_in, sel_out, chg_in, chg_out, L.scnt.p, L.sz.p};
+        hipLaunchKernelGGL(k_gfp, dim3(ns), dim3(64), LL.gfp_ints() * sizeof(int), c->stream, c->dnp, L.argsp, L.slist.p,
+                           ns, round == 0 ? 1 : 0, B, L.lc.p);
+        const LiveCtr h = read_lc(c, L);
+        ++round;
+        if (h.err) {
+          st->capped = h.err == ERR_CAPPED;
+          c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in the fixpoint" : "invariant table too large";
+          return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
+        }
+        std::swap(sel_in, sel_out);
+        std::swap(chg_in, chg_out);
+        if (h.changed == 0) break;
+      }
+      st->iters = round;
+      hipLaunchKernelGGL(k_gverdict, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, L.slist.p, ns,
+                         sel_in, L.wcnt0.p, L.wcnt1.p, L.wne.p, L.lc.p);
+      const LiveCtr h = read_lc(c, L);
+      st->nonempty = h.nonempty;
+      st->fed_max = h.fed_max;
+      st->pre_calls = (long long)h.pre_calls;
+      *holds = h.violated ? 0 : 1;
+      if (q->kind == HORA_LIVE_EG) *holds = h.violated ? 1 : 0;
+    } else {
+      *holds = q->kind == HORA_LIVE_EG ? 0 : 1;
+    }
+  }
+  const auto t2 = std::chrono::steady_clock::now();
+  st->build_seconds = std::chrono::duration<double>(t1 - t0).count();
+  st->gfp_seconds = std::chrono::duration<double>(t2 - t1).count();
+  st->seconds = std::chrono::duration<double>(t2 - t0).count();
+  return HORA_HIP_OK;
+}
+
+int live_graph(const hora_hip_ctx* c, hora_live_graph* g) {
+  const LiveState* L = c->live;
+  g->nodes = L ? L->nodes : 0;
+  g->edges = L ? L->edges : 0;
+  g->d = c->d;
+  if (!L) return HORA_HIP_OK;
+  auto cp = [](void* dst, const void* src, size_t bytes) {
+    if (dst && bytes) HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  };
+  const size_t N = (size_t)L->nodes, E = (size_t)L->edges;
+  cp(g->dk, L->gdk.p, N * sizeof(u64));
+  cp(g->zones, c->zarena.p, N * c->d2 * sizeof(int));
+  cp(g->row_off, L->grow.p, (N + 1) * sizeof(long long));
+  cp(g->etgt, L->etgt.p, E * sizeof(int));
+  cp(g->ese, L->ese.p, E * sizeof(int));
+  cp(g->ere, L->ere.p, E * sizeof(int));
+  cp(g->expanded, L->expanded.p, N);
+  if (g->in_s) {
+    std::vector<int> tmp(N);
+    cp(tmp.data(), L->in_s.p, N * sizeof(int));
+    for (size_t v = 0; v < N; ++v) g->in_s[v] = (int8_t)tmp[v];
+  }
+  cp(g->w_nonempty, L->wne.p, N);
+  return HORA_HIP_OK;
+}
+
+}  // namespace

@@ -234,4 +234,101 @@ __device__ __forceinline__ bool z_try_union(int* a, const int* b, int d, int* fa
   return true;
 }
 
+// ---- backward-pass operations (textbook down / free, Bengtsson & Yi 2004 section 4) -----------
+
+// Time predecessors: z[0][j] = min(<=0, min_{i>=1} z[i][j]). Canonical in, canonical out.
+__device__ __forceinline__ void z_down(int* z, int d) {
+  wsync();
+  const int l = threadIdx.x;
+  int v = kLe0;
+  if (l >= 1 && l < d)
+    for (int i = 1; i < d; ++i) v = min(v, z[i * d + l]);
+  wsync();
+  if (l >= 1 && l < d) z[l] = v;
+}
+
+// Forget the clocks of `mask`: row c unbounded, column c takes column 0. `tmp` is d*d LDS scratch.
+__device__ __forceinline__ void z_free_many(int* z, int d, uint64_t mask, int* tmp) {
+  if (!mask) return;
+  wsync();
+  const int l = threadIdx.x, d2 = d * d;
+  for (int e = l; e < d2; e += kWave) {
+    const int i = e / d, j = e - i * d;
+    int v;
+    if (i == j) v = kLe0;
+    else if ((mask >> i) & 1) v = kInf;
+    else if ((mask >> j) & 1) v = z[i * d];
+    else v = z[e];
+    tmp[e] = v;
+  }
+  wsync();
+  for (int e = l; e < d2; e += kWave) z[e] = tmp[e];
+}
+
+// A two-clock cycle through both zones is negative: the zones are certainly disjoint.
+__device__ __forceinline__ bool z_disjoint_quick(const int* a, const int* b, int d) {
+  wsync();
+  const int l = threadIdx.x, d2 = d * d;
+  bool neg = false;
+  for (int e = l; e < d2; e += kWave) {
+    const int i = e / d, j = e - i * d;
+    if (radd(a[e], b[j * d + i]) < kLe0) neg = true;
+  }
+  return __any(neg);
+}
+
+// out := a /\ b (entrywise min, closure). Returns false iff empty; out is then unspecified.
+__device__ __forceinline__ bool z_intersect(int* out, const int* a, const int* b, int d) {
+  if (z_disjoint_quick(a, b, d)) return false;
+  const int l = threadIdx.x, d2 = d * d;
+  for (int e = l; e < d2; e += kWave) out[e] = min(a[e], b[e]);
+  z_close(out, d);
+  return !z_empty(out, d);
+}
+
+// Equal entrywise.
+__device__ __forceinline__ bool z_equal(const int* a, const int* b, int d) {
+  wsync();
+  const int l = threadIdx.x, d2 = d * d;
+  bool diff = false;
+  for (int e = l; e < d2; e += kWave)
+    if (a[e] != b[e]) diff = true;
+  return !__any(diff);
+}
+
+// p \ y as disjoint zones appended to the list `out` (stride `zs` ints, `*n` entries, at most
+// `cap`): y's facets (i, j) in row-major order; where y is tighter than the remainder r, split off
+// r /\ not(x_i - x_j <| y_ij) and keep r /\ (x_i - x_j <| y_ij). `r`, `t` are d*d LDS scratch.
+// Returns false if the list would exceed `cap`.
+__device__ __forceinline__ bool z_subtract(const int* p, const int* y, int d, int* out, int zs, int* n, int cap,
+                                           int* r, int* t) {
+  const int l = threadIdx.x, d2 = d * d;
+  if (!z_intersect(t, p, y, d)) {
+    if (*n >= cap) return false;
+    wsync();
+    for (int e = l; e < d2; e += kWave) out[*n * zs + e] = p[e];
+    ++*n;
+    return true;
+  }
+  wsync();
+  for (int e = l; e < d2; e += kWave) r[e] = p[e];
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) {
+      if (i == j) continue;
+      wsync();
+      const int yij = y[i * d + j];
+      if (yij >= r[i * d + j]) continue;
+      wsync();
+      for (int e = l; e < d2; e += kWave) t[e] = r[e];
+      if (z_constrain(t, d, j, i, r_neg(yij))) {
+        if (*n >= cap) return false;
+        wsync();
+        for (int e = l; e < d2; e += kWave) out[*n * zs + e] = t[e];
+        ++*n;
+      }
+      if (!z_constrain(r, d, i, j, yij)) return true;
+    }
+  return true;
+}
+
 }  // namespace hh

@@ -18,17 +18,20 @@ import os
 from dataclasses import dataclass
 from typing import Optional
 
+from . import liveness as lv
 from .network import NetEnc, Network, Target, TargetEnc, hora_reach_stats
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhora_hip.so")
 
-E_OK, E_ARG, E_SCOPE, E_DEVICE, E_MEMORY, E_INTERNAL = 0, -1, -2, -3, -4, -5
+E_OK, E_ARG, E_SCOPE, E_DEVICE, E_MEMORY, E_INTERNAL, E_CAPPED = 0, -1, -2, -3, -4, -5, -6
 
-ZOP_CLOSE, ZOP_UP, ZOP_RESET, ZOP_CONSTRAIN, ZOP_EXTRA_LU, ZOP_INCLUDES, ZOP_ALU_LE, ZOP_UNION, ZOP_EMPTY = range(9)
+(ZOP_CLOSE, ZOP_UP, ZOP_RESET, ZOP_CONSTRAIN, ZOP_EXTRA_LU, ZOP_INCLUDES, ZOP_ALU_LE, ZOP_UNION, ZOP_EMPTY,
+ ZOP_DOWN, ZOP_FREE) = range(11)
 
 EXPORTS = ("hora_hip_abi_version", "hora_hip_build_info", "hora_hip_open", "hora_hip_close",
-           "hora_hip_last_error", "hora_hip_reach", "hora_hip_reach_profile", "hora_hip_zone_op")
+           "hora_hip_last_error", "hora_hip_reach", "hora_hip_reach_profile", "hora_hip_zone_op",
+           "hora_hip_live", "hora_hip_live_graph")
 
 
 class Backend(enum.Enum):
@@ -74,6 +77,9 @@ def lib():
         L.hora_hip_reach.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(hora_reach_stats)]
         L.hora_hip_reach_profile.argtypes = [C.c_void_p, C.POINTER(hora_reach_profile)]
         L.hora_hip_zone_op.argtypes = [C.c_int32, C.c_int32, C.c_int64] + [C.c_void_p] * 8
+        L.hora_hip_live.argtypes = [C.c_void_p, C.POINTER(lv.hora_live_query), C.POINTER(C.c_int32),
+                                    C.POINTER(lv.hora_live_stats)]
+        L.hora_hip_live_graph.argtypes = [C.c_void_p, C.POINTER(lv.hora_live_graph)]
         _lib = L
     return _lib
 
@@ -124,6 +130,28 @@ class HipContext:
             raise HipError(rc, lib().hora_hip_last_error(self.h).decode())
         return bool(r.value), ReachStats.from_c(st)
 
+    def live(self, query: "lv.LiveQuery"):
+        """Decide one liveness query (``hora_hip_live``): returns (holds, LiveStats)."""
+        enc = query.encode()
+        r = C.c_int32(0)
+        st = lv.hora_live_stats()
+        rc = lib().hora_hip_live(self.h, C.byref(enc.struct), C.byref(r), C.byref(st))
+        if rc != E_OK:
+            raise (OutOfScope if rc == E_SCOPE else HipError)(rc, lib().hora_hip_last_error(self.h).decode())
+        return bool(r.value), lv.LiveStats.from_c(st)
+
+    def live_graph(self) -> dict:
+        """The last liveness query's zone graph and fixpoint emptiness, copied to host arrays."""
+        g = lv.hora_live_graph()
+        lib().hora_hip_live_graph(self.h, C.byref(g))
+        n, e, d = g.nodes, g.edges, g.d
+        arrs = lv.graph_arrays(n, e, d)
+        lv.fill_graph_struct(g, arrs)
+        rc = lib().hora_hip_live_graph(self.h, C.byref(g))
+        if rc != E_OK:
+            raise HipError(rc, "hora_hip_live_graph")
+        return lv.trim_graph(arrs, n, e, d)
+
     def profile(self) -> dict:
         """Per-kernel device time (ms) and algorithmic bytes of the last search."""
         p = hora_reach_profile()
@@ -157,6 +185,36 @@ def reachable_net(net: Network, target: Target, backend: Backend = Backend.Hip,
         r, st = ctx.reach()
     if stats is not None:
         stats.__dict__.update(st.__dict__)
+    return r
+
+
+def leads_to_otf(net: Network, p: "lv.DPred", q: "lv.DPred", gs: Optional["lv.LiveStats"] = None,
+                 early: bool = True) -> bool:
+    """``p --> q`` on the device (``leads_to_otf``, include/hora/liveness.hpp:96-120)."""
+    with HipContext(net, None) as ctx:
+        r, st = ctx.live(lv.leads_to(p, q, early))
+    if gs is not None:
+        gs.__dict__.update(st.__dict__)
+    return r
+
+
+def exists_invariant_run_otf(net: Network, psi: "lv.DPred", clocks=(), gs: Optional["lv.LiveStats"] = None) -> bool:
+    """``E[] psi`` on the device (``exists_invariant_run_otf``, include/hora/liveness.hpp:121-140)."""
+    rnet, query = lv.exists_always(net, psi, clocks)
+    with HipContext(rnet, None) as ctx:
+        r, st = ctx.live(query)
+    if gs is not None:
+        gs.__dict__.update(st.__dict__)
+    return r
+
+
+def deadlock_free(net: Network, gs: Optional["lv.LiveStats"] = None, early: bool = True) -> bool:
+    """``A[] not deadlock`` on the device (the negation of ``deadlock_reachable``,
+    include/hora/liveness.hpp:144-170)."""
+    with HipContext(net, None) as ctx:
+        r, st = ctx.live(lv.not_deadlock(early))
+    if gs is not None:
+        gs.__dict__.update(st.__dict__)
     return r
 
 

@@ -33,12 +33,16 @@ def test_struct_layout_matches_c(tmp_path):
     import shutil
     import subprocess
     from ptctcl_amd.network import hora_net_enc, hora_reach_stats, hora_target
+    from ptctcl_amd.hip import hora_reach_profile
+    from ptctcl_amd.liveness import hora_dpred, hora_live_graph, hora_live_query, hora_live_stats
+    structs = (hora_net_enc, hora_target, hora_reach_stats, hora_reach_profile, hora_dpred, hora_live_query,
+               hora_live_stats, hora_live_graph)
     cc = shutil.which("gcc") or shutil.which("cc")
     if cc is None:
         import pytest
         pytest.skip("no C compiler")
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HDR}"', "int main(void){"]
-    for S in (hora_net_enc, hora_target, hora_reach_stats):
+    for S in structs:
         lines.append(f'printf("{S.__name__} %zu\\n", sizeof({S.__name__}));')
         for f, _ in S._fields_:
             lines.append(f'printf("{S.__name__}.{f} %zu\\n", offsetof({S.__name__}, {f}));')
@@ -48,7 +52,7 @@ def test_struct_layout_matches_c(tmp_path):
     exe = tmp_path / "layout"
     subprocess.run([cc, "-o", str(exe), str(src)], check=True)
     got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True).stdout.splitlines())
-    for S in (hora_net_enc, hora_target, hora_reach_stats):
+    for S in structs:
         assert int(got[S.__name__]) == C.sizeof(S)
         for f, _ in S._fields_:
             assert int(got[f"{S.__name__}.{f}"]) == getattr(S, f).offset, (S.__name__, f)

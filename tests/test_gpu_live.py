@@ -1,0 +1,111 @@
+"""The device liveness pass against the oracle's restatement: same verdict, same zone graph (node
+keys and zones, CSR rows, edge targets and transitions, expanded flags), same closure and the same
+per-node emptiness of the converged fixpoint.
+
+Cases: the reference's fischer family with its recorded queries (`P0.req --> P0.wait` satisfied,
+`P0.cs --> P0.req` and `P0.wait --> P0.cs` violated, `A[] not deadlock` satisfied, four `E[]` rows
+with a clock part), the seeded token ring (binary synchronisation), and the small automata of
+tests/live_models.py with their definitional verdicts. At a size the oracle needs minutes for
+(fischer 8, 2 million nodes), the node and edge counts the oracle produced once are the pin
+(tests/golden/live_counts.json, written by tests/golden/make_live_counts.py).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import binding
+from ptctcl_amd import hip, liveness as lv, models
+from ptctcl_amd.network import c_le
+
+import live_models
+
+pytestmark = pytest.mark.gpu
+P = lv.DPred
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "live_counts.json")))
+STATS = ("outer_nodes", "fwd_edges", "disc_keys", "generated", "covered", "closure", "nonempty", "early")
+GRAPH = ("dk", "zones", "row_off", "etgt", "ese", "ere", "expanded", "in_s", "w_nonempty")
+
+
+def check(net, query, label=""):
+    print("case", net.name, label, query.kind, query.early, flush=True)
+    r_o, st_o, g_o = binding.live(net, query, graph=True)
+    with hip.HipContext(net, None) as ctx:
+        r_h, st_h = ctx.live(query)
+        g_h = ctx.live_graph()
+    print("  oracle", r_o, {k: getattr(st_o, k) for k in STATS}, flush=True)
+    print("  device", r_h, {k: getattr(st_h, k) for k in STATS}, flush=True)
+    assert r_h == r_o
+    for k in STATS:
+        assert getattr(st_h, k) == getattr(st_o, k), k
+    if query.kind != lv.EG:
+        assert st_h.levels == st_o.levels
+    for k in GRAPH:
+        assert np.array_equal(g_h[k], g_o[k]), k
+    return r_h, st_h
+
+
+def fischer_queries(net):
+    a = lambda loc: P.at(net, "P0", loc)   # noqa: E731
+    return [("req-->wait", lv.leads_to(a("req"), a("wait")), True),
+            ("req-->wait full", lv.leads_to(a("req"), a("wait"), early=False), True),
+            ("cs-->req full", lv.leads_to(a("cs"), a("req"), early=False), False),
+            ("wait-->cs", lv.leads_to(a("wait"), a("cs")), False),
+            ("wait-->cs full", lv.leads_to(a("wait"), a("cs"), early=False), False),
+            ("deadlock", lv.not_deadlock(), True),
+            ("deadlock full", lv.not_deadlock(False), True)]
+
+
+@pytest.mark.parametrize("N", [3, 4, 5, 6])
+def test_fischer_recorded_queries(N):
+    net = models.fischer(N)
+    for label, q, want in fischer_queries(net):
+        r, _ = check(net, q, label)
+        assert r == want, label
+
+
+@pytest.mark.parametrize("N", [4, 5])
+def test_fischer_eg_recorded(N):
+    net = models.fischer(N)
+    rows = [(P.true(), True), (~P.at(net, "P1", "cs"), True),
+            (P.at(net, "P0", "idle") | P.at(net, "P0", "wait"), False),
+            (~(P.at(net, "P1", "cs") & P.at(net, "P2", "cs")), True)]
+    for k, (psi, want) in enumerate(rows):
+        rnet, q = lv.exists_always(net, psi, [c_le(1, 40)])
+        r, _ = check(rnet, q, f"eg row {k}")
+        assert r == want
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("N", [4, 6, 8])
+def test_token_ring(N, seed):
+    net = models.token_ring(N, seed)
+    h0, h1, i0 = P.at(net, "P0", "hold"), P.at(net, "P1", "hold"), P.at(net, "P0", "idle")
+    for q in (lv.leads_to(h0, h1), lv.leads_to(h0, i0, early=False), lv.leads_to(i0, h0, early=False),
+              lv.not_deadlock(False), lv.LiveQuery(lv.EG, ~h1)):
+        check(net, q)
+
+
+@pytest.mark.parametrize("builder", live_models.ALL, ids=lambda b: b.__name__)
+def test_small_models(builder):
+    net, queries = builder()
+    for q, want in queries:
+        r, _ = check(net, q)
+        assert r == want
+
+
+def test_fischer7_full_graph_parity():
+    net = models.fischer(7)
+    r, st = check(net, lv.leads_to(P.at(net, "P0", "req"), P.at(net, "P0", "wait"), early=False))
+    assert r and st.outer_nodes == GOLD["fischer_req_wait"]["7"]["nodes"]
+
+
+def test_fischer8_counts_at_size():
+    net = models.fischer(8)
+    with hip.HipContext(net, None) as ctx:
+        r, st = ctx.live(lv.leads_to(P.at(net, "P0", "req"), P.at(net, "P0", "wait"), early=False))
+    g = GOLD["fischer_req_wait"]["8"]
+    print("fischer8", r, st, flush=True)
+    assert r and st.nonempty == 0
+    assert (st.outer_nodes, st.fwd_edges, st.closure) == (g["nodes"], g["edges"], g["closure"])

@@ -54,6 +54,19 @@ def test_up_reset(d, op):
 
 
 @pytest.mark.parametrize("d", DIMS)
+@pytest.mark.parametrize("op", [hip.ZOP_DOWN, hip.ZOP_FREE])
+def test_down_free(d, op):
+    rng = np.random.default_rng(250 + d)
+    z = zg.hull_zones(N, d, rng)
+    arg = None
+    if op == hip.ZOP_FREE:
+        arg = (rng.integers(1, 1 << min(d - 1, 30), size=N).astype(np.int64) << 1).astype(np.int32)
+    eo, _ = binding.zone_op(op, d, z, arg=arg)
+    go, _ = run(op, d, z, arg=arg)
+    assert np.array_equal(eo, go)
+
+
+@pytest.mark.parametrize("d", DIMS)
 def test_constrain(d):
     rng = np.random.default_rng(300 + d)
     z = zg.hull_zones(N, d, rng)

@@ -1,0 +1,158 @@
+"""The oracle's liveness restatement pinned (CPU only).
+
+* The backward-pass zone operations against set semantics on a point grid (the grid argument of
+  test_oracle_dbm.py): down (time predecessors), free (existential projection of clocks), and
+  zone difference (pieces pairwise disjoint, union = p minus y).
+* Verdicts against the reference's recorded results for its fischer family (gen_fischer.py):
+  `P0.req --> P0.wait` satisfied (benchmarks/results/live/hora_backward/hora_live_t1.csv:67-88),
+  `P0.cs --> P0.req` and `P0.wait --> P0.cs` violated (benchmarks/queries/live_violated/fischer.q,
+  README.md:31-35), `A[] not deadlock` satisfied (benchmarks/results/deadlock/hora/
+  hora_deadlock_t1.csv:69-85), and the four `E[]` rows of benchmarks/queries/eg/results-4-5.txt:2-11.
+* Verdicts on small automata whose answers follow from their definition (tests/live_models.py):
+  timelock, forced progress, a cycle witness that no single state certifies, a partial deadlock.
+"""
+import numpy as np
+import pytest
+
+from oracle import binding
+from ptctcl_amd import hip, liveness as lv, models
+from ptctcl_amd.network import c_le
+
+import live_models
+from test_oracle_dbm import INF, Grid, le, rand_zone
+
+P = lv.DPred
+
+
+def member_pts(pts, z, d, K):
+    z = np.asarray(z, np.int64).reshape(d, d)
+    ok = np.ones(len(pts), bool)
+    for i in range(d):
+        for j in range(d):
+            if i == j or z[i, j] >= INF:
+                continue
+            v, weak = z[i, j] >> 1, z[i, j] & 1
+            diff = pts[:, i] - pts[:, j]
+            ok &= (diff <= K * v) if weak else (diff < K * v)
+    return ok
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_down_matches_grid(n):
+    rng = np.random.default_rng(300 + n)
+    C = 3
+    G = Grid(n, 3 * C + 3)
+    d = n + 1
+    for _ in range(40):
+        z = rand_zone(rng, n, C)
+        out, _ = binding.zone_op(hip.ZOP_DOWN, d, z.reshape(1, -1).astype(np.int32))
+        mask = np.zeros(len(G.P), bool)
+        for s in range(0, (C + 3) * G.K + 1):      # p + s/K for every grid delay that can matter
+            sh = G.P.copy()
+            sh[:, 1:] += s
+            mask |= member_pts(sh, z, d, G.K)
+        assert np.array_equal(out[0], G.canon(mask, C + 1)), z
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_free_matches_grid(n):
+    rng = np.random.default_rng(320 + n)
+    C = 3
+    G = Grid(n, 3 * C + 3)
+    d = n + 1
+    for _ in range(40):
+        z = rand_zone(rng, n, C)
+        mask_bits = int(rng.integers(1, 1 << n)) << 1
+        out, _ = binding.zone_op(hip.ZOP_FREE, d, z.reshape(1, -1).astype(np.int32),
+                                 arg=np.array([mask_bits], np.int32))
+        freed = [c for c in range(1, d) if (mask_bits >> c) & 1]
+        # the grid is a product (itertools order): project out the freed axes and broadcast back
+        A = G.R * G.K + 1
+        m = G.member(z).reshape((A,) * n)
+        proj = np.any(m, axis=tuple(c - 1 for c in freed), keepdims=True)
+        mask = np.broadcast_to(proj, m.shape).reshape(-1)
+        assert np.array_equal(out[0], G.canon(mask, C + 1)), (z, mask_bits)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_subtract_matches_grid(n):
+    rng = np.random.default_rng(340 + n)
+    C = 4
+    G = Grid(n, C + 3)
+    d = n + 1
+    ps = np.stack([rand_zone(rng, n, C) for _ in range(60)])
+    ys = np.stack([rand_zone(rng, n, C) for _ in range(60)])
+    pieces = binding.zone_subtract(d, ps, ys)
+    for p, y, pc in zip(ps, ys, pieces):
+        assert pc is not None
+        want = G.member(p) & ~G.member(y)
+        masks = [G.member(q) for q in pc]
+        got = np.zeros(len(G.P), bool)
+        for m in masks:
+            assert m.any()                       # no empty piece
+            assert not (got & m).any()           # pairwise disjoint
+            got |= m
+        assert np.array_equal(got, want)
+
+
+def test_dpred_negation_is_complement():
+    net = models.fischer(3)
+    a, b = P.at(net, "P0", "cs"), P.at(net, "P1", "req")
+    for pred in (a, a & b, a | b, ~(a & b), P.true(), P.false(), P.var(net, "id", lv.OP_GE, 2)):
+        for locs in np.ndindex(4, 4, 4):
+            for idv in range(4):
+                assert pred.eval(locs, [idv]) != (~pred).eval(locs, [idv])
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 6])
+def test_fischer_leadsto_recorded_verdicts(n):
+    net = models.fischer(n)
+    for early in (True, False):
+        r, st = binding.live(net, lv.leads_to(P.at(net, "P0", "req"), P.at(net, "P0", "wait"), early))
+        assert r and not st.capped and st.nonempty == 0
+        r, st = binding.live(net, lv.leads_to(P.at(net, "P0", "cs"), P.at(net, "P0", "req"), early))
+        assert not r
+        r, st = binding.live(net, lv.leads_to(P.at(net, "P0", "wait"), P.at(net, "P0", "cs"), early))
+        assert not r
+        if early:
+            assert st.early == 1     # a single waiting state already deadlocks or diverges: stop witness
+
+
+@pytest.mark.parametrize("n", [4, 5])
+def test_fischer_deadlock_free_recorded(n):
+    net = models.fischer(n)
+    for early in (True, False):
+        r, st = binding.live(net, lv.not_deadlock(early))
+        assert r
+
+
+@pytest.mark.parametrize("n", [4, 5])
+def test_fischer_eg_recorded_verdicts(n):
+    net = models.fischer(n)
+    x0 = [c_le(1, 40)]
+    rows = [(P.true(), True), (~P.at(net, "P1", "cs"), True),
+            (P.at(net, "P0", "idle") | P.at(net, "P0", "wait"), False),
+            (~(P.at(net, "P1", "cs") & P.at(net, "P2", "cs")), True)]
+    for psi, want in rows:
+        rnet, q = lv.exists_always(net, psi, x0)
+        r, st = binding.live(rnet, q)
+        assert r == want and not st.capped
+
+
+@pytest.mark.parametrize("builder", live_models.ALL, ids=lambda b: b.__name__)
+def test_small_models_known_verdicts(builder):
+    net, queries = builder()
+    for q, want in queries:
+        r, st = binding.live(net, q)
+        assert r == want, (net.name, q.kind, q.early)
+
+
+def test_early_witness_needs_no_fixpoint():
+    net, _ = live_models.cycle()
+    a, b = P.at(net, "A", "a"), P.at(net, "A", "b")
+    r, st = binding.live(net, lv.leads_to(a, b))
+    assert not r and st.early == 0 and st.nonempty >= 1      # cycle: only the fixpoint decides
+    net, _ = live_models.timelock()
+    a, b = P.at(net, "A", "a"), P.at(net, "A", "b")
+    r, st = binding.live(net, lv.leads_to(a, b))
+    assert not r and st.early == 1 and st.closure == 0       # stop witness at the first state
