@@ -533,11 +533,12 @@ __host__ __device__ constexpr int expand_lds_ints(int d2) {
   return 3 * d2 + HORA_MAX_DIM + 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * kRecipeCap + 3 * kInvCap;
 }
 
-// Successors of frontier states [0, F) into slots rec_off[i] + r. With `init` set, the frontier
+// Successors of frontier states [0, F) into slots rec_off[i] + r, r < rec_cnt[i]. With `init` set, the frontier
 // is the single initial state: its zone is the zero zone and its one recipe is "no transition".
 __global__ __launch_bounds__(64) void k_expand(const DevNet* __restrict__ Np, const u64* __restrict__ fr_dk,
                                                const u32* __restrict__ fr_zid, const int* __restrict__ zarena,
-                                               int F, const int* __restrict__ rec_off, ExpandOut out,
+                                               int F, const int* __restrict__ rec_off,
+                                               const int* __restrict__ rec_cnt, ExpandOut out,
                                                Counters* ctr, int init) {
   extern __shared__ int sm[];
   const DevNet& N = *Np;
@@ -629,7 +630,9 @@ __global__ __launch_bounds__(64) void k_expand(const DevNet* __restrict__ Np, co
   u64 am;
   int ninv;
   wave_loc_info(N, locs, ints, &fz, &am, inv, &ninv);
-  const int nr = build_recipes(N, locs, ints, (fz & 1) != 0, rec, kRecipeCap);
+  // the slots reserved for this state bound what is written: a state filtered out after counting
+  // (rec_cnt[i] = 0) writes nothing
+  const int nr = min(build_recipes(N, locs, ints, (fz & 1) != 0, rec, kRecipeCap), rec_cnt[i]);
   wsync();
   const int base = rec_off[i];
   for (int r = 0; r < nr; ++r) emit(base + r, rec[2 * r], rec[2 * r + 1]);
@@ -1688,7 +1691,7 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     HIP_OK(hipEventRecord(c->ev[0], c->stream));
     if (R > 0)
       hipLaunchKernelGGL(k_expand, dim3(F), dim3(64), lds_e, c->stream, c->dnp, c->fr_dk.p, c->fr_zid.p,
-                         c->zarena.p, (int)F, c->rec_off.p, eo, c->ctr.p, init ? 1 : 0);
+                         c->zarena.p, (int)F, c->rec_off.p, c->rec_cnt.p, eo, c->ctr.p, init ? 1 : 0);
     HIP_OK(hipEventRecord(c->ev[1], c->stream));
     // ---- compaction of non-empty successors
     c->pos.ensure(R + 1);

@@ -484,7 +484,7 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
   // the table the deadlock is judged in: the unrestricted network's when psi restricted it
   const int* doff = A.restricted ? A.oinv_off : N.inv_off;
   const int* ddat = A.restricted ? A.oinv_data : N.inv_data;
-  if (mode == 0) {
+  if (mode != 2) {   // the stop set proper (stop_set); mode 2 is the plain deadlock test
     if (!A.restricted) {
       if (A.grow[v + 1] == A.grow[v] || time_diverges()) { out_whole(); return; }
     } else {
@@ -1026,7 +1026,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     ExpandOut eo{c->s_valid.p, c->s_dk.p, c->s_zone.p, c->s_tgt.p, L.s_se.p, L.s_re.p};
     if (R > 0)
       hipLaunchKernelGGL(k_expand, dim3(F), dim3(64), expand_lds_ints(d2) * sizeof(int), c->stream, c->dnp, c->fr_dk.p,
-                         c->fr_zid.p, c->zarena.p, (int)F, c->rec_off.p, eo, c->ctr.p, init ? 1 : 0);
+                         c->fr_zid.p, c->zarena.p, (int)F, c->rec_off.p, c->rec_cnt.p, eo, c->ctr.p, init ? 1 : 0);
     c->pos.ensure(R + 1);
     const long G = R > 0 ? scan_exclusive(c, c->s_valid.p, R, c->pos.p) : 0;
     if (!init) {
