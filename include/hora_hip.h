@@ -239,6 +239,7 @@ typedef struct hora_live_stats {
   int64_t iters;             /* fixpoint rounds (device) or value-changing relaxations (oracle) */
   int64_t pre_calls;         /* exact predecessor computations                                  */
   int64_t fed_max;           /* widest converged value, in zones                                */
+  int64_t products;          /* W /\ cand intersections formed (fixpoint.cpp:842-850)          */
   int32_t early;             /* 1: settled by a witness state during the construction           */
   int32_t capped;            /* 1: a federation outgrew its capacity; no verdict                */
   double seconds;            /* whole query                                                     */

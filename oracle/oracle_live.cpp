@@ -546,7 +546,7 @@ struct Live {
         if (dbm_includes(&cand[c * d2], z, d)) whole = true;
       if (whole) { nw.insert(nw.end(), z, z + d2); continue; }
       for (std::size_t c = 0; c < cand.size() / d2; ++c)
-        if (dbm_intersect(t.data(), z, &cand[c * d2], d)) nw.insert(nw.end(), t.begin(), t.end());
+        if (dbm_intersect(t.data(), z, &cand[c * d2], d)) { ++st.products; nw.insert(nw.end(), t.begin(), t.end()); }
     }
     compact(nw);
     wv.swap(nw);

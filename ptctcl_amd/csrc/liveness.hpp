@@ -83,10 +83,11 @@ struct LiveCtr {
   int ns;                     // closure list length
   int changed;                // values changed in the last round
   unsigned long long pre_calls;
+  unsigned long long products;
   long long nonempty;
   int fed_max;
   int violated;
-  int pad[6];
+  int pad[4];
 };
 
 // ---------------------------------------------------------------- forward construction
@@ -131,58 +132,48 @@ __device__ __forceinline__ int m_bound(const DevNet& N, const int* lg, int c) {
   return max(lb, ub);
 }
 
-// The first node stored before this level that covers the successor (the key's chain, oldest
-// first); four 16-lane groups per wavefront, one successor each.
+// The first node stored before this level that covers the successor: one wavefront per successor,
+// the key's node segment (oldest first) tested 64 nodes at a time, one node per lane; the lowest
+// covering lane is the answer.
 __global__ __launch_bounds__(64) void k_gprobe(const DevNet* __restrict__ Np, Passed P, const long* __restrict__ gbk,
                                                const int* __restrict__ sj, const u64* __restrict__ sdk,
-                                               const int* __restrict__ szone, long G, const u32* __restrict__ gnext,
-                                               const int* __restrict__ gzone, int* __restrict__ covered,
-                                               int* __restrict__ cov_old, Counters* ctr) {
-  __shared__ int Mg[4][HORA_MAX_DIM], lg[4][HORA_MAX_PROCS];
+                                               const int* __restrict__ szone, long G, const int* __restrict__ gzone,
+                                               int* __restrict__ covered, int* __restrict__ cov_old, Counters* ctr) {
+  __shared__ int S[HORA_MAX_DIM * HORA_MAX_DIM], M[HORA_MAX_DIM], lg[HORA_MAX_PROCS];
   const DevNet& N = *Np;
-  const int lane = threadIdx.x, q = lane / kGroup, t = lane % kGroup;
-  const long g = (long)blockIdx.x * 4 + q;
-  const bool active = g < G;
+  const int l = threadIdx.x;
+  const long g = blockIdx.x;
+  if (g >= G) return;
   const int d = N.d, d2 = N.d2;
-  const long h = active ? gbk[g] : -1;
+  const long h = gbk[g];
   const u32 len = (h >= 0) ? P.len[h] : 0u;
-  if (active && len > 0) {
-    const u64 dk = sdk[sj[g]];
-    for (int p = t; p < N.np; p += kGroup) lg[q][p] = (int)((dk >> N.kshift[p]) & N.kmask[p]);
-  }
-  __syncthreads();
-  if (active && len > 0)
-    for (int c = t; c < d; c += kGroup) Mg[q][c] = m_bound(N, lg[q], c);
-  __syncthreads();
-  const int* S = szone + (size_t)(active ? sj[g] : 0) * d2;
-  const int* M = Mg[q];
-  const u64 gmask = 0xffffull << (q * kGroup);
   int found = -1, reads = 0;
-  u32 node = len > 0 ? (u32)(P.off[h] >> 32) : 0u;
-  for (u32 m = 0; m < len; ++m) {   // group-uniform loop
-    const int* Z = gzone + (size_t)node * d2;
-    ++reads;
-    bool wit = false;
-    for (int e = t; e < d2; e += kGroup) {
-      const int y = e / d, x = e - y * d;
-      if (x == y) continue;
-      const int ux = M[x], ly = M[y];
-      if (ux == kAluNegInf || ly == kAluNegInf) continue;
-      const int s0x = S[x];
-      if (s0x < r_le(-ux)) continue;
-      const int zp = Z[e];
-      if (zp < S[e] && radd(zp, r_lt(-ly)) < s0x) wit = true;
+  if (len > 0) {
+    const u64 dk = sdk[sj[g]];
+    if (l < N.np) lg[l] = (int)((dk >> N.kshift[l]) & N.kmask[l]);
+    const int* src = szone + (size_t)sj[g] * d2;
+    for (int e = l; e < d2; e += kWave) S[e] = src[e];
+    wsync();
+    if (l < d) M[l] = m_bound(N, lg, l);
+    wsync();
+    const u32* seg = P.nodes + P.off[h];
+    for (u32 b = 0; b < len && found < 0; b += kWave) {
+      const u32 m = b + (u32)l;
+      bool ok = false;
+      if (m < len) {
+        ok = alu_le_lane(S, gzone + (size_t)seg[m] * d2, d, M, M);
+        ++reads;
+      }
+      const u64 bal = __ballot(ok);
+      if (bal) found = (int)seg[b + __builtin_ctzll(bal)];
     }
-    if ((__ballot(wit) & gmask) == 0) { found = (int)node; break; }
-    node = gnext[node];
   }
-  if (active && t == 0) {
+  if (l == 0) {
     covered[g] = found >= 0 ? 1 : 0;
     cov_old[g] = found;
   }
-  const bool lead = active && t == 0;
-  wave_stat(ctr, C_PREADS, lead ? (unsigned long long)reads : 0ull);
-  wave_stat(ctr, C_PSUB, (lead && found >= 0) ? 1ull : 0ull);
+  wave_stat(ctr, C_PREADS, (unsigned long long)reads);
+  wave_stat(ctr, C_PSUB, (l == 0 && found >= 0) ? 1ull : 0ull);
 }
 
 // One wavefront per touched key: its remaining successors in (state, transition) order, each to
@@ -268,6 +259,148 @@ __global__ __launch_bounds__(64) void k_gmerge(const DevNet* __restrict__ Np, Pa
   }
 }
 
+// ---- heavy keys: the ordered merge decided in parallel
+//
+// The cover test is a preorder (Z <=_aM Z' iff Z is inside the a_M closure of Z', a closure
+// operator), so a successor creates a node iff no EARLIER successor of its key covers it: an
+// earlier successor that covers it is either a node or covered by an earlier node, which then
+// covers it too. Each successor is tested against all earlier ones independently; a successor
+// that does not create a node then goes to the first created node that covers it. The result is
+// the sequential merge's.
+
+constexpr int kLightLive = 64;    // keys with at most this many queued successors merge sequentially
+
+__global__ void k_gsplit(Passed P, const u32* __restrict__ touched, int T, u32* __restrict__ light,
+                         u32* __restrict__ heavy, Counters* ctr) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = t < T;
+  const u32 h = ok ? touched[t] : 0u;
+  const bool lt = ok && P.ncnt[h] <= (u32)kLightLive;
+  const int il = wave_append(&ctr->nlight, lt);
+  const int ih = wave_append(&ctr->nheavy, ok && !lt);
+  if (lt) light[il] = h;
+  if (ok && !lt) heavy[ih] = h;
+}
+
+// one wavefront per heavy key: sort its list; mark each list position with the key's slot
+__global__ __launch_bounds__(64) void k_gsort(Passed P, const u32* __restrict__ heavy, int nh,
+                                              int* __restrict__ wlist, long long* __restrict__ wkey) {
+  extern __shared__ int sm[];
+  const int t = blockIdx.x;
+  if (t >= nh) return;
+  const int l = threadIdx.x;
+  const u32 h = heavy[t];
+  const int n = (int)P.ncnt[h];
+  const u64 base = P.noff[h];
+  if (n <= kSortLds) {
+    int cap = 1;
+    while (cap < n) cap <<= 1;
+    for (int k = l; k < cap; k += kWave) sm[k] = k < n ? wlist[base + k] : 0x7fffffff;
+    bitonic_sort<false>(sm, cap);
+    for (int k = l; k < n; k += kWave) wlist[base + k] = sm[k];
+  } else {
+    const int cap = (int)list_cap((u64)n);
+    for (int k = n + l; k < cap; k += kWave) wlist[base + k] = 0x7fffffff;
+    bsync();
+    bitonic_sort<true>(wlist + base, cap);
+  }
+  for (int k = l; k < n; k += kWave) wkey[base + k] = (long long)h;
+}
+
+// one wavefront per list position of a heavy key: does an earlier successor cover this one?
+__global__ __launch_bounds__(64) void k_gpair(const DevNet* __restrict__ Np, Passed P, const long long* __restrict__ wkey,
+                                              long W, const int* __restrict__ wlist, const int* __restrict__ sj,
+                                              const u64* __restrict__ sdk, const int* __restrict__ szone,
+                                              int* __restrict__ tslot, int* __restrict__ fnew) {
+  __shared__ int S[HORA_MAX_DIM * HORA_MAX_DIM], M[HORA_MAX_DIM], lg[HORA_MAX_PROCS];
+  const long p = blockIdx.x;
+  if (p >= W) return;
+  const long long hh = wkey[p];
+  if (hh < 0) return;
+  const DevNet& N = *Np;
+  const int l = threadIdx.x, d = N.d, d2 = N.d2;
+  const u64 base = P.noff[hh];
+  const int k = (int)(p - (long)base);
+  const int g = wlist[p];
+  const u64 dk = sdk[sj[g]];
+  if (l < N.np) lg[l] = (int)((dk >> N.kshift[l]) & N.kmask[l]);
+  const int* src = szone + (size_t)sj[g] * d2;
+  for (int e = l; e < d2; e += kWave) S[e] = src[e];
+  wsync();
+  if (l < d) M[l] = m_bound(N, lg, l);
+  wsync();
+  bool cov = false;
+  for (int b = 0; b < k && !cov; b += kWave) {
+    const int m = b + l;
+    bool ok = false;
+    if (m < k) ok = alu_le_lane(S, szone + (size_t)sj[wlist[base + m]] * d2, d, M, M);
+    cov = __ballot(ok) != 0;
+  }
+  if (l == 0) {
+    tslot[g] = cov ? -2 : g;
+    fnew[g] = cov ? 0 : 1;
+  }
+}
+
+// one wavefront per heavy key: its creators to the front of its list, in order; their number
+// replaces the queue length for k_glink and k_gfirst
+__global__ __launch_bounds__(64) void k_gcompact(Passed P, const u32* __restrict__ heavy, int nh,
+                                                 int* __restrict__ wlist, const int* __restrict__ fnew) {
+  const int t = blockIdx.x;
+  if (t >= nh) return;
+  const int l = threadIdx.x;
+  const u32 h = heavy[t];
+  const int n = (int)P.ncnt[h];
+  const u64 base = P.noff[h];
+  int nc = 0;
+  for (int b = 0; b < n; b += kWave) {
+    const int k = b + l;
+    const int g = k < n ? wlist[base + k] : 0;
+    const bool cr = k < n && fnew[g];
+    const u64 bal = __ballot(cr);
+    wsync();
+    if (cr) wlist[base + nc + __popcll(bal & ((1ull << l) - 1))] = g;
+    nc += __popcll(bal);
+    __threadfence_block();
+  }
+  if (l == 0) P.ncnt[h] = (u32)nc;
+}
+
+// one wavefront per successor still pending: the first created node of its key that covers it
+__global__ __launch_bounds__(64) void k_gfirst(const DevNet* __restrict__ Np, Passed P, const long* __restrict__ gbk,
+                                               long G, const int* __restrict__ wlist, const int* __restrict__ sj,
+                                               const u64* __restrict__ sdk, const int* __restrict__ szone,
+                                               int* __restrict__ tslot, Counters* ctr) {
+  __shared__ int S[HORA_MAX_DIM * HORA_MAX_DIM], M[HORA_MAX_DIM], lg[HORA_MAX_PROCS];
+  const long g = blockIdx.x;
+  if (g >= G || tslot[g] != -2) return;
+  const DevNet& N = *Np;
+  const int l = threadIdx.x, d = N.d, d2 = N.d2;
+  const long h = gbk[g];
+  const int nc = (int)P.ncnt[h];
+  const u64 base = P.noff[h];
+  const u64 dk = sdk[sj[g]];
+  if (l < N.np) lg[l] = (int)((dk >> N.kshift[l]) & N.kmask[l]);
+  const int* src = szone + (size_t)sj[g] * d2;
+  for (int e = l; e < d2; e += kWave) S[e] = src[e];
+  wsync();
+  if (l < d) M[l] = m_bound(N, lg, l);
+  wsync();
+  int first = -1;
+  for (int b = 0; b < nc && first < 0; b += kWave) {
+    const int m = b + l;
+    bool ok = false;
+    if (m < nc) ok = alu_le_lane(S, szone + (size_t)sj[wlist[base + m]] * d2, d, M, M);
+    const u64 bal = __ballot(ok);
+    if (bal) first = b + __builtin_ctzll(bal);
+  }
+  if (l == 0) {
+    tslot[g] = first >= 0 ? wlist[base + first] : (int)g;
+    if (first < 0) atomicExch(&ctr->err, ERR_CAPPED + 1);   // cannot happen: an earlier cover exists
+    else stat_add(ctr, C_SUB, 1);
+  }
+}
+
 __global__ void k_gassign(long G, long long nbase, long long ebase, int record, const int* __restrict__ cov_old,
                           const int* __restrict__ tslot, const int* __restrict__ fnew,
                           const long long* __restrict__ rank, const int* __restrict__ sj,
@@ -288,8 +421,8 @@ __global__ void k_gassign(long G, long long nbase, long long ebase, int record, 
 __global__ __launch_bounds__(64) void k_gcopy(long F2, long long nbase, const int* __restrict__ newlist,
                                               const int* __restrict__ sj, const u64* __restrict__ sdk,
                                               const int* __restrict__ szone, int d2, int* __restrict__ gzone,
-                                              u64* __restrict__ gdk, u32* __restrict__ gnext,
-                                              u64* __restrict__ fr_dk, u32* __restrict__ fr_zid) {
+                                              u64* __restrict__ gdk, u64* __restrict__ fr_dk,
+                                              u32* __restrict__ fr_zid) {
   const long b = blockIdx.x;
   if (b >= F2) return;
   const int g = newlist[b];
@@ -299,32 +432,36 @@ __global__ __launch_bounds__(64) void k_gcopy(long F2, long long nbase, const in
   for (int e = threadIdx.x; e < d2; e += kWave) dst[e] = src[e];
   if (threadIdx.x == 0) {
     gdk[id] = sdk[sj[g]];
-    gnext[id] = 0xffffffffu;
     fr_dk[b] = sdk[sj[g]];
     fr_zid[b] = (u32)id;
   }
 }
 
-// append the key's new nodes to its chain, in creation order; chain = (head << 32 | tail) in off
-__global__ void k_glink(Passed P, const u32* __restrict__ touched, int T, const int* __restrict__ wlist,
-                        const long long* __restrict__ rank, long long nbase, u32* __restrict__ gnext) {
+// segment sizes of the touched keys for this level: the stored nodes plus the queued successors
+__global__ void k_gsegs(Passed P, const u32* __restrict__ touched, int T, int* __restrict__ segs) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < T) segs[t] = (int)(P.len[touched[t]] + P.ncnt[touched[t]]);
+}
+
+// One wavefront per touched key: its node segment moves to the one reserved for this level and the
+// key's new nodes are appended in creation order.
+__global__ __launch_bounds__(64) void k_glink(Passed P, const u32* __restrict__ touched, int T,
+                                              const int* __restrict__ wlist, const long long* __restrict__ rank,
+                                              long long nbase) {
+  const int t = blockIdx.x;
   if (t >= T) return;
+  const int l = threadIdx.x;
   const u32 h = touched[t];
   const u32 nc = P.ncnt[h];
-  const u64 base = P.noff[h];
-  u32 len = P.len[h];
-  u32 head = (u32)(P.off[h] >> 32), tail = (u32)P.off[h];
-  for (u32 c = 0; c < nc; ++c) {
-    const u32 id = (u32)(nbase + rank[wlist[base + c]]);
-    if (len == 0) head = id;
-    else gnext[tail] = id;
-    tail = id;
-    ++len;
+  const u32 len = P.len[h];
+  const u64 base = P.noff[h], from = P.off[h], to = P.nseg[h];
+  for (u32 k = l; k < len; k += kWave) P.nodes[to + k] = P.nodes[from + k];
+  for (u32 c = l; c < nc; c += kWave) P.nodes[to + len + c] = (u32)(nbase + rank[wlist[base + c]]);
+  if (l == 0) {
+    P.off[h] = to;
+    P.len[h] = len + nc;
+    P.ncnt[h] = 0;
   }
-  P.len[h] = len;
-  P.off[h] = ((u64)head << 32) | tail;
-  P.ncnt[h] = 0;
 }
 
 // ---------------------------------------------------------------- stop sets and deadlock
@@ -802,6 +939,7 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
   }
   // W[v] /\ cand, member by member (products into `cur`, capacity kp)
   int nn = 0;
+  unsigned long long prods = 0;
   for (int k = 0; k < nW; ++k) {
     const int* z = Wv + k * d2;
     bool whole = false;
@@ -814,6 +952,7 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
     }
     for (int c = 0; c < nc; ++c) {
       if (!z_intersect(tmp, z, cand + c * d2, d)) continue;
+      ++prods;
       if (nn >= LL.kp) nn = fed_compact(cur, nn, d2, d, drop);
       if (nn >= LL.kp) { capped = true; break; }
       zcopy_w(cur + nn * d2, tmp, d2);
@@ -821,6 +960,7 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
     }
     if (capped) break;
   }
+  if (l == 0 && prods) atomicAdd(&lc->products, prods);
   if (!capped) nn = fed_compact(cur, nn, d2, d, drop);
   if (capped || nn > kLiveW) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); keep(); return; }
   const int bo = 1 - bi;
@@ -870,8 +1010,7 @@ struct LiveState {
   DBuf<LiveCtr> lc;
   // graph
   DBuf<u64> gdk;
-  DBuf<u32> gnext;
-  DBuf<long long> grow;
+  DBuf<long long> grow, segpos, wkey;
   DBuf<int> etgt, ese, ere, s_se, s_re, cov_old, tslot, fnew, newlist;
   DBuf<signed char> expanded, wne;
   long long nodes = 0, edges = 0;
@@ -978,7 +1117,9 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
   }
   // graph storage
   size_t ncap = 1 << 16, ecap = 1 << 18;
-  L.gdk.ensure(ncap); L.gnext.ensure(ncap); L.grow.ensure(ncap + 1); L.expanded.ensure(ncap);
+  L.gdk.ensure(ncap); L.grow.ensure(ncap + 1); L.expanded.ensure(ncap);
+  c->nodes.ensure(1 << 20);
+  u64 ncount = 0;   // node-segment entries handed out so far
   c->zarena.ensure(ncap * d2);
   L.etgt.ensure(ecap); L.ese.ensure(ecap); L.ere.ensure(ecap);
   HIP_OK(hipMemsetAsync(L.expanded.p, 0, L.expanded.cap, c->stream));
@@ -1058,10 +1199,12 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       L.tslot.ensure(G);
       L.fnew.ensure(G);
       HIP_OK(hipMemsetAsync(L.fnew.p, 0, (size_t)G * sizeof(int), c->stream));
+      HIP_OK(hipMemsetAsync(L.tslot.p, 0xff, (size_t)G * sizeof(int), c->stream));
       HIP_OK(hipMemsetAsync(&c->ctr.p->ntouched, 0, sizeof(int), c->stream));
       hipLaunchKernelGGL(k_find, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->sj.p, c->s_dk.p, G, c->gbk.p, c->ctr.p);
-      hipLaunchKernelGGL(k_gprobe, dim3(blocks_for(G, 4)), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p, c->s_dk.p,
-                         c->s_zone.p, G, L.gnext.p, c->zarena.p, c->covered.p, L.cov_old.p, c->ctr.p);
+      HIP_OK(hipEventRecord(c->ev[1], c->stream));
+      hipLaunchKernelGGL(k_gprobe, dim3(G), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p, c->s_dk.p,
+                         c->s_zone.p, G, c->zarena.p, c->covered.p, L.cov_old.p, c->ctr.p);
       hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p, G,
                          c->gticket.p, c->touched.p, c->ctr.p);
       int T = 0;
@@ -1074,14 +1217,42 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
                            c->caps.p, c->segs.p);
         c->pos2.ensure(T + 1);
         const long long Wl = scan_exclusive(c, c->caps.p, T, c->pos2.p);
+        hipLaunchKernelGGL(k_gsegs, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T, c->segs.p);
+        L.segpos.ensure(T + 1);
+        const long long Sseg = scan_exclusive(c, c->segs.p, T, L.segpos.p);
+        if (ncount + (u64)Sseg > c->nodes.cap) c->nodes.grow_keep((size_t)(ncount + (u64)Sseg) * 2, c->stream);
+        P = passed_view(c);
         hipLaunchKernelGGL(k_touched_offsets, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
-                           c->pos2.p, c->pos2.p, (u64)0);
+                           c->pos2.p, L.segpos.p, ncount);
+        ncount += (u64)Sseg;
         c->wlist.ensure(Wl + 1);
         hipLaunchKernelGGL(k_scatter, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p,
                            c->gticket.p, G, c->wlist.p);
         const size_t lds_m = (d2 + HORA_MAX_DIM + HORA_MAX_PROCS + kSortLds) * sizeof(int);
-        hipLaunchKernelGGL(k_gmerge, dim3(T), dim3(64), lds_m, c->stream, c->dnp, P, c->touched.p, T, c->wlist.p,
-                           c->sj.p, c->s_dk.p, c->s_zone.p, L.tslot.p, L.fnew.p, c->ctr.p);
+        HIP_OK(hipEventRecord(c->ev[2], c->stream));
+        c->klight.ensure(T);
+        c->kheavy.ensure(T);
+        HIP_OK(hipMemsetAsync(&c->ctr.p->nlight, 0, 2 * sizeof(int), c->stream));
+        hipLaunchKernelGGL(k_gsplit, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                           c->klight.p, c->kheavy.p, c->ctr.p);
+        int nlh[2] = {0, 0};
+        HIP_OK(hipMemcpyAsync(nlh, &c->ctr.p->nlight, sizeof(nlh), hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        if (nlh[0] > 0)
+          hipLaunchKernelGGL(k_gmerge, dim3(nlh[0]), dim3(64), lds_m, c->stream, c->dnp, P, c->klight.p, nlh[0],
+                             c->wlist.p, c->sj.p, c->s_dk.p, c->s_zone.p, L.tslot.p, L.fnew.p, c->ctr.p);
+        if (nlh[1] > 0) {
+          L.wkey.ensure(Wl + 1);
+          HIP_OK(hipMemsetAsync(L.wkey.p, 0xff, (size_t)(Wl + 1) * sizeof(long long), c->stream));
+          hipLaunchKernelGGL(k_gsort, dim3(nlh[1]), dim3(64), kSortLds * sizeof(int), c->stream, P, c->kheavy.p,
+                             nlh[1], c->wlist.p, L.wkey.p);
+          hipLaunchKernelGGL(k_gpair, dim3(Wl), dim3(64), 0, c->stream, c->dnp, P, L.wkey.p, (long)Wl, c->wlist.p,
+                             c->sj.p, c->s_dk.p, c->s_zone.p, L.tslot.p, L.fnew.p);
+          hipLaunchKernelGGL(k_gcompact, dim3(nlh[1]), dim3(64), 0, c->stream, P, c->kheavy.p, nlh[1], c->wlist.p,
+                             L.fnew.p);
+          hipLaunchKernelGGL(k_gfirst, dim3(G), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, (long)G, c->wlist.p,
+                             c->sj.p, c->s_dk.p, c->s_zone.p, L.tslot.p, c->ctr.p);
+        }
       }
       c->pos.ensure(G + 1);
       F2 = scan_exclusive(c, L.fnew.p, G, c->pos.p);
@@ -1091,7 +1262,6 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       if (nneed > L.gdk.cap) {
         const size_t nc2 = std::max(nneed, (size_t)L.gdk.cap * 2);
         grow_keep_to(L.gdk, nc2, c->stream);
-        grow_keep_to(L.gnext, nc2, c->stream);
         grow_keep_to(L.grow, nc2 + 1, c->stream);
         const size_t old = L.expanded.cap;
         grow_keep_to(L.expanded, nc2, c->stream);
@@ -1105,6 +1275,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
         grow_keep_to(L.ere, ec2, c->stream);
       }
       L.newlist.ensure(std::max<long>(F2, 1));
+      HIP_OK(hipEventRecord(c->ev[3], c->stream));
       hipLaunchKernelGGL(k_gassign, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, G, nbase, L.edges, init ? 0 : 1,
                          L.cov_old.p, L.tslot.p, L.fnew.p, c->pos.p, c->sj.p, L.s_se.p, L.s_re.p, L.etgt.p, L.ese.p,
                          L.ere.p, L.newlist.p);
@@ -1112,10 +1283,17 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       c->fr_zid2.ensure(std::max<long>(F2, 1));
       if (F2 > 0)
         hipLaunchKernelGGL(k_gcopy, dim3(F2), dim3(64), 0, c->stream, (long)F2, nbase, L.newlist.p, c->sj.p, c->s_dk.p,
-                           c->s_zone.p, d2, c->zarena.p, L.gdk.p, L.gnext.p, c->fr_dk2.p, c->fr_zid2.p);
+                           c->s_zone.p, d2, c->zarena.p, L.gdk.p, c->fr_dk2.p, c->fr_zid2.p);
       if (T > 0)
-        hipLaunchKernelGGL(k_glink, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T, c->wlist.p,
-                           c->pos.p, nbase, L.gnext.p);
+        hipLaunchKernelGGL(k_glink, dim3(T), dim3(64), 0, c->stream, P, c->touched.p, T, c->wlist.p, c->pos.p, nbase);
+      HIP_OK(hipEventRecord(c->ev[4], c->stream));
+      if (c->trace) {
+        HIP_OK(hipStreamSynchronize(c->stream));
+        const Tally ht = read_counters(c);
+        std::fprintf(stderr, "live level %lld F=%ld R=%ld G=%ld T=%d F2=%ld | probe %.3f (covered %llu, reads %llu) merge %.3f assign+link %.3f ms\n",
+                     (long long)st->levels, F, R, G, T, F2, ev_ms(c->ev[1], c->ev[2]), ht[C_PSUB], ht[C_PREADS],
+                     T > 0 ? ev_ms(c->ev[2], c->ev[3]) : 0.0f, ev_ms(c->ev[3], c->ev[4]));
+      }
       if (!init) {
         L.edges += G;
         st->fwd_edges = L.edges;
@@ -1252,6 +1430,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       st->nonempty = h.nonempty;
       st->fed_max = h.fed_max;
       st->pre_calls = (long long)h.pre_calls;
+      st->products = (long long)h.products;
       *holds = h.violated ? 0 : 1;
       if (q->kind == HORA_LIVE_EG) *holds = h.violated ? 1 : 0;
     } else {
