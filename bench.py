@@ -28,6 +28,17 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 
+def live_workload(name: str):
+    """live-fischer<N>: P0.req --> P0.wait on the reference's fischer family (k = 2), the liveness
+    query of benchmarks/queries/live/fischer.q, decided by the full graph and fixpoint."""
+    from ptctcl_amd import liveness as lv, models
+    n = int(name[len("live-fischer"):])
+    net = models.fischer(n)
+    q = lv.leads_to(lv.DPred.at(net, "P0", "req"), lv.DPred.at(net, "P0", "wait"))
+    return net, q, {"workload": f"fischer N={n}, k=2, P0.req --> P0.wait (forward graph + greatest fixpoint)",
+                    "processes": n, "zone_dim": n + 1, "k": 2}
+
+
 def workload(name: str):
     from ptctcl_amd import models
     if name.startswith("fischerv"):        # fischerv<N>: the plain-reachability variant, k = N
@@ -77,6 +88,124 @@ def cpu_baseline(name: str, seconds: float) -> dict:
     }
 
 
+def live_cpu_baseline(name: str, seconds: float) -> dict:
+    """The oracle's single-threaded liveness restatement on the largest instance of the family it
+    finishes within `seconds` (sizes tried upward from 5)."""
+    from oracle import binding
+    n_top = int(name[len("live-fischer"):])
+    best = None
+    for n in range(5, n_top + 1):
+        net, q, _ = live_workload(f"live-fischer{n}")
+        t0 = time.perf_counter()
+        _, st = binding.live(net, q)
+        dt = time.perf_counter() - t0
+        best = (n, st.outer_nodes, dt)
+        if dt * 12 > seconds:      # the next size costs roughly ten times more
+            break
+    n, nodes, dt = best
+    return {"value": round(nodes / dt, 1), "unit": "nodes/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle liveness restatement, fischer N={n} (same query and graph rules): "
+                       f"{nodes} nodes in {dt:.2f} s, one thread")}
+
+
+def live_main(a) -> int:
+    import torch
+    from ptctcl_amd import hip
+    net, q, cfg = live_workload(a.workload)
+    ctx = hip.HipContext(net, None)
+    for _ in range(a.warmup):
+        ctx.live(q)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        holds, st = ctx.live(q)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / a.steps
+    ctx.close()
+    d2 = (net.nclocks + 1) ** 2
+    # construction phase, algorithmic bytes (int32 entries): frontier zones read, successors written
+    # and read back, stored zones read by the cover tests, new nodes written
+    b_build = 4.0 * d2 * (st.expanded_nodes + 2 * st.generated + st.chain_reads + st.outer_nodes)
+    achieved = b_build / st.build_seconds / 1e9 if st.build_seconds > 0 else 0.0
+    line = {
+        "metric": "symbolic states/sec (liveness: zone-graph nodes per second, construction + fixpoint)",
+        "value": round(st.outer_nodes / (ms / 1000.0), 1),
+        "unit": "nodes/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (seeded generated model; no external data)",
+        "config": dict(cfg, holds=holds, outer_nodes=st.outer_nodes, fwd_edges=st.fwd_edges, closure=st.closure,
+                       nonempty=st.nonempty, fixpoint_rounds=st.iters, pre_calls=st.pre_calls,
+                       build_ms=round(st.build_seconds * 1000, 2), fixpoint_ms=round(st.gfp_seconds * 1000, 2)),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "construction phase (k_expand, k_gprobe, k_gmerge/k_gpair, k_gassign)"},
+    }
+    if not a.no_cpu:
+        cb = live_cpu_baseline(a.workload, a.cpu_seconds)
+        line["cpu_baseline"] = cb
+        line["speedup_vs_cpu"] = round(line["value"] / cb["value"], 2) if cb["value"] > 0 else None
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def serve_main(a) -> int:
+    """Serving: 512 seeded mixed jobs (ptctcl_amd.serve.job_mix) arriving as a Poisson process.
+    The arrival rate is set from the measured one-job-at-a-time capacity (all jobs queued at once on
+    one slot): 0.8 of it. The same schedule then runs on one slot (plain one-job-at-a-time GPU) and
+    on `--slots` slots (continuous batching); latency is arrival -> verdict."""
+    from ptctcl_amd import serve
+    import numpy as np
+    n = 512
+    jobs = serve.job_mix(n, seed=2026)
+    # warm-up: every slot allocates its buffers once
+    with serve.Server(slots=a.slots) as srv:
+        for spec in jobs[:64]:
+            spec.submit(srv)
+        got = 0
+        while got < 64:
+            got += len(srv.wait(256, 500))
+    with serve.Server(slots=1) as srv:
+        t0 = time.perf_counter()
+        for spec in jobs:
+            spec.submit(srv)
+        res = []
+        while len(res) < n:
+            res.extend(srv.wait(256, 500))
+        closed = time.perf_counter() - t0
+    capacity = n / closed
+    rate = 0.8 * capacity
+    arrivals = serve.poisson_arrivals(n, rate, seed=2027)
+    summary = {}
+    verdicts = {}
+    for slots in (1, a.slots):
+        with serve.Server(slots=slots) as srv:
+            res = serve.run_schedule(srv, jobs, arrivals)
+        summary[slots] = serve.latency_summary(res)
+        verdicts[slots] = sorted((r.job - min(x.job for x in res), r.verdict, r.status) for r in res)
+    same = [v[1:] for v in verdicts[1]] == [v[1:] for v in verdicts[a.slots]]
+    served, single = summary[a.slots], summary[1]
+    line = {
+        "metric": "served verdicts/sec (512 mixed jobs, Poisson arrivals at 0.8x one-at-a-time capacity)",
+        "value": served["verdicts_per_s"],
+        "unit": "verdicts/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(served["span_s"] * 1000, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (seeded job mix; no external data)",
+        "config": {"workload": "serving: 512 jobs, types variant reach N~U[3,7], fischer reach N~U[3,10] k=N, "
+                               "fischer A[] not deadlock N~U[3,6], token-ring P0.hold --> P1.hold N~U[3,10]",
+                   "slots": a.slots, "arrival_rate_per_s": round(rate, 2),
+                   "one_at_a_time_capacity_per_s": round(capacity, 2),
+                   "served": served, "one_at_a_time": single,
+                   "p99_ratio_served_over_single": round(served["p99_ms"] / single["p99_ms"], 4),
+                   "verdicts_identical": same},
+        "roofline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,11 +214,17 @@ def main() -> int:
     ap.add_argument("--workload", default="fischer12")
     ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--slots", type=int, default=8, help="serving workload: device slots")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this back end runs on one GPU; multi-GPU runs are not part of this build "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         return 2
+
+    if a.workload.startswith("live-"):
+        return live_main(a)
+    if a.workload == "serve":
+        return serve_main(a)
 
     import torch
     from ptctcl_amd import hip

@@ -240,6 +240,7 @@ typedef struct hora_live_stats {
   int64_t pre_calls;         /* exact predecessor computations                                  */
   int64_t fed_max;           /* widest converged value, in zones                                */
   int64_t products;          /* W /\ cand intersections formed (fixpoint.cpp:842-850)          */
+  int64_t chain_reads;       /* stored or queued zones read by the cover tests of the construction */
   int32_t early;             /* 1: settled by a witness state during the construction           */
   int32_t capped;            /* 1: a federation outgrew its capacity; no verdict                */
   double seconds;            /* whole query                                                     */
@@ -248,7 +249,7 @@ typedef struct hora_live_stats {
 } hora_live_stats;
 
 /* Decide one liveness query on the context's network. Writes 1 to *holds if the property holds.
- * Returns HORA_HIP_E_SCOPE for what the device liveness path does not cover (d > 16, broadcast
+ * Returns HORA_HIP_E_SCOPE for what the device liveness path does not cover (d > 23, broadcast
  * channels), HORA_HIP_E_CAPPED when a federation outgrew the device capacity. */
 int hora_hip_live(hora_hip_ctx* ctx, const hora_live_query* q, int32_t* holds, hora_live_stats* stats);
 
@@ -269,6 +270,40 @@ typedef struct hora_live_graph {
   int8_t* w_nonempty;        /* [nodes] converged value non-empty                               */
 } hora_live_graph;
 int hora_hip_live_graph(const hora_hip_ctx* ctx, hora_live_graph* g);
+
+/* ---- serving: verification jobs of different networks, continuous batching on one GPU ----
+ * A server owns `slots` device slots (a HIP stream, device-resident passed list, frontier and
+ * successor buffers each, kept across jobs). Submitted jobs wait in arrival order; a job is
+ * admitted when a slot is free and the declared HBM footprints of the running jobs plus its own
+ * stay within the budget (always when nothing runs). Admitted jobs run concurrently, level by
+ * level on their own streams. No counterpart in the reference (its CLI runs one query per
+ * process); a job's verdict and counters are those of hora_hip_reach / hora_hip_live alone. */
+typedef struct hora_hip_server hora_hip_server;
+
+typedef struct hora_serve_job_result {
+  int64_t job;               /* id returned at submission                                       */
+  int32_t kind;              /* 0 reachability (hora_hip_reach), 1 liveness (hora_hip_live)       */
+  int32_t status;            /* HORA_HIP_OK or the error code of the job                          */
+  int32_t verdict;           /* reachability: 1 reachable; liveness: 1 holds                     */
+  int32_t slot;
+  hora_reach_stats reach;
+  hora_live_stats live;
+  double t_submit, t_start, t_done;   /* seconds on the server's clock (hora_hip_server_now)  */
+} hora_serve_job_result;
+
+/* hbm_budget_bytes <= 0: 90% of the device memory free at open. */
+int hora_hip_server_open(int32_t slots, int64_t hbm_budget_bytes, hora_hip_server** out);
+/* The job's arrays are copied; est_bytes <= 0 takes a default footprint from the zone dimension. */
+int hora_hip_server_submit_reach(hora_hip_server* s, const hora_net_enc* net, const hora_target* target,
+                                 int64_t est_bytes, int64_t* job);
+int hora_hip_server_submit_live(hora_hip_server* s, const hora_net_enc* net, const hora_live_query* q,
+                                int64_t est_bytes, int64_t* job);
+/* Completed jobs, up to max; waits up to timeout_ms for at least one (0: do not wait). */
+int hora_hip_server_wait(hora_hip_server* s, hora_serve_job_result* out, int32_t max, int32_t* n,
+                         int32_t timeout_ms);
+double hora_hip_server_now(const hora_hip_server* s);
+/* Finishes the queued jobs, then releases the slots. */
+void hora_hip_server_close(hora_hip_server* s);
 
 #ifdef __cplusplus
 }

@@ -217,8 +217,10 @@ struct Live {
     if (it != chains.end()) {
       std::int32_t M[64];
       mbounds(locs, M);
-      for (int m : it->second)
+      for (int m : it->second) {
+        ++st.chain_reads;
         if (dbm_alu_le(z, Z(m), d, M, M)) { ++st.covered; return m; }
+      }
     }
     *created = true;
     return new_node(key, z);

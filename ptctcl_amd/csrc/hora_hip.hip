@@ -1469,9 +1469,13 @@ struct DBuf {
     p = q;
     cap = c;
   }
-  void upload(const std::vector<T>& v) {
+  // synchronous copy on `s` (the owner's stream, so that other streams keep running)
+  void upload(const std::vector<T>& v, hipStream_t s) {
     ensure(v.size());
-    if (!v.empty()) HIP_OK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!v.empty()) {
+      HIP_OK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+      HIP_OK(hipStreamSynchronize(s));
+    }
   }
 };
 
@@ -1708,7 +1712,8 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
       // stops; generated counts those plus the target itself (engine.cpp:2085-2095).
       long long before = 0;
       if (hc.first_target > 0)
-        HIP_OK(hipMemcpy(&before, c->pos.p + hc.first_target, sizeof(before), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpyAsync(&before, c->pos.p + hc.first_target, sizeof(before), hipMemcpyDeviceToHost, c->stream));
+      HIP_OK(hipStreamSynchronize(c->stream));
       if (!init) st->generated += (u64)before + 1;
       st->found = 1;
       *reachable = 1;
@@ -1841,8 +1846,12 @@ std::vector<T> vec(const T* p, long n) {
   return p && n > 0 ? std::vector<T>(p, p + n) : std::vector<T>(std::max<long>(n, 1), T{});
 }
 
+// Bind a context to a network and query. Callable again on a bound context (a serving slot moving
+// to its next job): the stream, events and every device buffer are kept, only the tables change.
 void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
   if (!e) throw ScopeFail("null network");
+  c->h_kshift.clear();
+  c->h_kmask.clear();
   c->nclocks = e->nclocks;
   c->d = e->nclocks + 1;
   c->d2 = c->d * c->d;
@@ -1918,8 +1927,10 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
     const char* tr = std::getenv("HORA_HIP_TRACE");
     c->trace = tr && *tr == '1';
   }
-  HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  for (auto& ev : c->ev) HIP_OK(hipEventCreate(&ev));
+  if (!c->stream) {
+    HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& ev : c->ev) HIP_OK(hipEventCreate(&ev));
+  }
   const int ne = std::max(c->ne, 1);
   // all static tables except the per-location bounds go into one blob (u64 tables 8-aligned)
   std::vector<int> blob;
@@ -1972,9 +1983,9 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
   const size_t o_ic = put(icv);
   const size_t o_cc = put(ccv);
   if (blob.size() & 1) blob.push_back(0);
-  c->blob.upload(blob);
-  c->lloc.upload(vec(e->lbound_loc, (long)c->tl * d));
-  c->uloc.upload(vec(e->ubound_loc, (long)c->tl * d));
+  c->blob.upload(blob, c->stream);
+  c->lloc.upload(vec(e->lbound_loc, (long)c->tl * d), c->stream);
+  c->uloc.upload(vec(e->ubound_loc, (long)c->tl * d), c->stream);
   const int* B = c->blob.p;
 
   DevNet& N = c->dn;
@@ -1993,13 +2004,15 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
   N.blob_ints = (int)blob.size();
   N.stage = 0;   // staging the tables in LDS measured slower than the L1/L2-resident originals
   N.never = never; N.nlr = nlr; N.nic = nic; N.ncc = ncc;
-  HIP_OK(hipMalloc(&c->dnp, sizeof(DevNet)));
-  HIP_OK(hipMemcpy(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice));
+  if (!c->dnp) HIP_OK(hipMalloc(&c->dnp, sizeof(DevNet)));
+  HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
 }
 
 }  // namespace
 
 #include "liveness.hpp"
+#include "serve.hpp"
 
 extern "C" {
 
@@ -2107,6 +2120,76 @@ int hora_hip_live_graph(const hora_hip_ctx* ctx, hora_live_graph* g) {
   } catch (const std::exception&) {
     return HORA_HIP_E_DEVICE;
   }
+}
+
+int hora_hip_server_open(int32_t slots, int64_t hbm_budget_bytes, hora_hip_server** out) {
+  if (!out || slots < 1 || slots > 64) return HORA_HIP_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return HORA_HIP_E_DEVICE;
+  auto* S = new (std::nothrow) hora_hip_server();
+  if (!S) return HORA_HIP_E_MEMORY;
+  S->slots = slots;
+  if (hbm_budget_bytes > 0) {
+    S->budget = hbm_budget_bytes;
+  } else {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete S; return HORA_HIP_E_DEVICE; }
+    S->budget = (int64_t)(0.9 * (double)fr);
+  }
+  for (int k = 0; k < slots; ++k) S->ctxs.push_back(new hora_hip_ctx());
+  for (int k = 0; k < slots; ++k) S->workers.emplace_back(worker, S, k);
+  *out = S;
+  return HORA_HIP_OK;
+}
+
+int hora_hip_server_submit_reach(hora_hip_server* s, const hora_net_enc* net, const hora_target* target,
+                                 int64_t est_bytes, int64_t* job) {
+  if (!s || !net) return HORA_HIP_E_ARG;
+  Job* j = new Job();
+  j->kind = 0;
+  j->data.copy_net(net);
+  j->data.copy_target(target);
+  j->est_bytes = est_bytes > 0 ? est_bytes : default_footprint(net);
+  return submit(s, j, job);
+}
+
+int hora_hip_server_submit_live(hora_hip_server* s, const hora_net_enc* net, const hora_live_query* q,
+                                int64_t est_bytes, int64_t* job) {
+  if (!s || !net || !q) return HORA_HIP_E_ARG;
+  Job* j = new Job();
+  j->kind = 1;
+  j->data.copy_net(net);
+  j->data.copy_query(q);
+  j->est_bytes = est_bytes > 0 ? est_bytes : default_footprint(net);
+  return submit(s, j, job);
+}
+
+int hora_hip_server_wait(hora_hip_server* s, hora_serve_job_result* out, int32_t max, int32_t* n,
+                         int32_t timeout_ms) {
+  if (!s || !out || !n || max < 1) return HORA_HIP_E_ARG;
+  std::unique_lock<std::mutex> lk(s->m);
+  if (s->done.empty() && timeout_ms > 0)
+    s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return !s->done.empty(); });
+  const int k = std::min<int>(max, (int)s->done.size());
+  for (int i = 0; i < k; ++i) out[i] = s->done[i];
+  s->done.erase(s->done.begin(), s->done.begin() + k);
+  *n = k;
+  return HORA_HIP_OK;
+}
+
+double hora_hip_server_now(const hora_hip_server* s) { return s ? s->now() : 0.0; }
+
+void hora_hip_server_close(hora_hip_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->m);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  for (auto& t : s->workers) t.join();
+  for (auto* c : s->ctxs) hora_hip_close(c);
+  delete s;
 }
 
 }  // extern "C"

@@ -32,7 +32,7 @@
 
 namespace {
 
-constexpr int kLiveMaxDim = 16;
+constexpr int kLiveMaxDim = 23;
 constexpr int ERR_CAPPED = 5;
 constexpr int kLiveW = 8;       // zones in one converged value W[v]
 constexpr int kStopCap = 16;    // zones in one stop set
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(64) void k_gmerge(const DevNet* __restrict__ Np, Pa
     if (l < d) M[l] = m_bound(N, lg, l);
   }
   int nc = 0;
-  unsigned long long cov = 0;
+  unsigned long long cov = 0, reads = 0;
   for (int k = 0; k < n; ++k) {
     if (in_lds) wsync(); else bsync();
     const int g = list[k];
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(64) void k_gmerge(const DevNet* __restrict__ Np, Pa
       for (int cb = 0; cb < nc && first < 0; cb += kWave) {
         const int c = cb + l;
         bool ok = false;
-        if (c < nc) ok = alu_le_lane(S, szone + (size_t)sj[list[c]] * d2, d, M, M);
+        if (c < nc) { ok = alu_le_lane(S, szone + (size_t)sj[list[c]] * d2, d, M, M); ++reads; }
         const u64 b = __ballot(ok);
         if (b) first = cb + __builtin_ctzll(b);
       }
@@ -257,6 +257,7 @@ __global__ __launch_bounds__(64) void k_gmerge(const DevNet* __restrict__ Np, Pa
     P.ncnt[h] = (u32)nc;
     stat_add(ctr, C_SUB, cov);
   }
+  wave_stat(ctr, C_READS, reads);
 }
 
 // ---- heavy keys: the ordered merge decided in parallel
@@ -311,7 +312,7 @@ __global__ __launch_bounds__(64) void k_gsort(Passed P, const u32* __restrict__ 
 __global__ __launch_bounds__(64) void k_gpair(const DevNet* __restrict__ Np, Passed P, const long long* __restrict__ wkey,
                                               long W, const int* __restrict__ wlist, const int* __restrict__ sj,
                                               const u64* __restrict__ sdk, const int* __restrict__ szone,
-                                              int* __restrict__ tslot, int* __restrict__ fnew) {
+                                              int* __restrict__ tslot, int* __restrict__ fnew, Counters* ctr) {
   __shared__ int S[HORA_MAX_DIM * HORA_MAX_DIM], M[HORA_MAX_DIM], lg[HORA_MAX_PROCS];
   const long p = blockIdx.x;
   if (p >= W) return;
@@ -330,12 +331,14 @@ __global__ __launch_bounds__(64) void k_gpair(const DevNet* __restrict__ Np, Pas
   if (l < d) M[l] = m_bound(N, lg, l);
   wsync();
   bool cov = false;
+  unsigned long long reads = 0;
   for (int b = 0; b < k && !cov; b += kWave) {
     const int m = b + l;
     bool ok = false;
-    if (m < k) ok = alu_le_lane(S, szone + (size_t)sj[wlist[base + m]] * d2, d, M, M);
+    if (m < k) { ok = alu_le_lane(S, szone + (size_t)sj[wlist[base + m]] * d2, d, M, M); ++reads; }
     cov = __ballot(ok) != 0;
   }
+  wave_stat(ctr, C_READS, reads);
   if (l == 0) {
     tslot[g] = cov ? -2 : g;
     fnew[g] = cov ? 0 : 1;
@@ -387,13 +390,15 @@ __global__ __launch_bounds__(64) void k_gfirst(const DevNet* __restrict__ Np, Pa
   if (l < d) M[l] = m_bound(N, lg, l);
   wsync();
   int first = -1;
+  unsigned long long reads = 0;
   for (int b = 0; b < nc && first < 0; b += kWave) {
     const int m = b + l;
     bool ok = false;
-    if (m < nc) ok = alu_le_lane(S, szone + (size_t)sj[wlist[base + m]] * d2, d, M, M);
+    if (m < nc) { ok = alu_le_lane(S, szone + (size_t)sj[wlist[base + m]] * d2, d, M, M); ++reads; }
     const u64 bal = __ballot(ok);
     if (bal) first = b + __builtin_ctzll(bal);
   }
+  wave_stat(ctr, C_READS, reads);
   if (l == 0) {
     tslot[g] = first >= 0 ? wlist[base + first] : (int)g;
     if (first < 0) atomicExch(&ctr->err, ERR_CAPPED + 1);   // cannot happen: an earlier cover exists
@@ -1052,22 +1057,30 @@ void sync_graph_args(LiveState& L) {
   L.args.expanded = L.expanded.p;
 }
 
-// Federation capacities from the LDS budget (64 KiB per workgroup).
-void live_caps(int d2, LiveArgs& a) {
+// Federation capacities from the LDS budget (64 KiB per workgroup). False when fewer than four
+// zones per list fit (zone dimension above 23).
+bool live_caps(int d2, LiveArgs& a) {
   const int budget = 64 * 1024 / 4;
   LiveLds t{d2, 0, 0, 0};
   const int zones = (budget - t.misc()) / d2;
-  const int k = std::min(64, std::max(4, (zones - 6 - kLiveW) / 3));
+  const int k = std::min(64, (zones - 6 - kLiveW) / 3);
   a.kl = k;
   a.kp = k;
   a.kc = k;
+  return k >= 4;
 }
 
 int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_live_stats* st) {
   const auto t0 = std::chrono::steady_clock::now();
   std::memset(st, 0, sizeof(*st));
   *holds = 0;
-  if (c->d > kLiveMaxDim) { c->err = "liveness on the device covers zone dimension up to 16"; return HORA_HIP_E_SCOPE; }
+  {
+    LiveArgs probe{};
+    if (c->d > kLiveMaxDim || !live_caps(c->d2, probe)) {
+      c->err = "liveness on the device covers zone dimension up to 23";
+      return HORA_HIP_E_SCOPE;
+    }
+  }
   if (c->dn.ncc > 0) { c->err = "open the context with a target without clock constraints for liveness"; return HORA_HIP_E_ARG; }
   if (q->kind < HORA_LIVE_LEADSTO || q->kind > HORA_LIVE_DEADLOCK) return HORA_HIP_E_ARG;
   if (!c->live) c->live = new LiveState();
@@ -1088,16 +1101,17 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     size_t po, pa, qo, qa;
     put_pred(q->p, L.args.p, &po, &pa);
     put_pred(q->q, L.args.q, &qo, &qa);
-    L.pred.upload(blob);
+    L.pred.upload(blob, c->stream);
     L.args.p.off = L.pred.p + po; L.args.p.atoms = L.pred.p + pa;
     L.args.q.off = L.pred.p + qo; L.args.q.atoms = L.pred.p + qa;
   }
   L.args.kind = q->kind;
   L.args.restricted = q->orig_inv_off ? 1 : 0;
   if (L.args.restricted) {
-    L.oinv_off.upload(std::vector<int>(q->orig_inv_off, q->orig_inv_off + c->tl + 1));
+    L.oinv_off.upload(std::vector<int>(q->orig_inv_off, q->orig_inv_off + c->tl + 1), c->stream);
     const int nc3 = 3 * q->orig_inv_off[c->tl];
-    L.oinv_data.upload(nc3 > 0 ? std::vector<int>(q->orig_inv_data, q->orig_inv_data + nc3) : std::vector<int>(3, 0));
+    L.oinv_data.upload(nc3 > 0 ? std::vector<int>(q->orig_inv_data, q->orig_inv_data + nc3) : std::vector<int>(3, 0),
+                       c->stream);
     L.args.oinv_off = L.oinv_off.p;
     L.args.oinv_data = L.oinv_data.p;
   }
@@ -1247,7 +1261,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
           hipLaunchKernelGGL(k_gsort, dim3(nlh[1]), dim3(64), kSortLds * sizeof(int), c->stream, P, c->kheavy.p,
                              nlh[1], c->wlist.p, L.wkey.p);
           hipLaunchKernelGGL(k_gpair, dim3(Wl), dim3(64), 0, c->stream, c->dnp, P, L.wkey.p, (long)Wl, c->wlist.p,
-                             c->sj.p, c->s_dk.p, c->s_zone.p, L.tslot.p, L.fnew.p);
+                             c->sj.p, c->s_dk.p, c->s_zone.p, L.tslot.p, L.fnew.p, c->ctr.p);
           hipLaunchKernelGGL(k_gcompact, dim3(nlh[1]), dim3(64), 0, c->stream, P, c->kheavy.p, nlh[1], c->wlist.p,
                              L.fnew.p);
           hipLaunchKernelGGL(k_gfirst, dim3(G), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, (long)G, c->wlist.p,
@@ -1320,6 +1334,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       const Tally hc = read_counters(c);
       if (hc.err) { c->err = hc.err == ERR_HASH_FULL ? "hash table full" : "device capacity exceeded"; return HORA_HIP_E_MEMORY; }
       st->covered = (long long)(hc[C_SUB] + hc[C_PSUB]);
+      st->chain_reads = (long long)(hc[C_READS] + hc[C_PREADS]);
       st->disc_keys = (long long)hc[C_KEYS];
       prev = hc;
     }

@@ -31,7 +31,8 @@ E_OK, E_ARG, E_SCOPE, E_DEVICE, E_MEMORY, E_INTERNAL, E_CAPPED = 0, -1, -2, -3, 
 
 EXPORTS = ("hora_hip_abi_version", "hora_hip_build_info", "hora_hip_open", "hora_hip_close",
            "hora_hip_last_error", "hora_hip_reach", "hora_hip_reach_profile", "hora_hip_zone_op",
-           "hora_hip_live", "hora_hip_live_graph")
+           "hora_hip_live", "hora_hip_live_graph", "hora_hip_server_open", "hora_hip_server_submit_reach",
+           "hora_hip_server_submit_live", "hora_hip_server_wait", "hora_hip_server_now", "hora_hip_server_close")
 
 
 class Backend(enum.Enum):

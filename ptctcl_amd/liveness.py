@@ -79,7 +79,7 @@ class hora_live_query(C.Structure):
 
 
 _STAT_I64 = ("outer_nodes", "fwd_edges", "disc_keys", "expanded_nodes", "generated", "covered", "levels",
-             "closure", "nonempty", "iters", "pre_calls", "fed_max", "products")
+             "closure", "nonempty", "iters", "pre_calls", "fed_max", "products", "chain_reads")
 
 
 class hora_live_stats(C.Structure):
@@ -111,6 +111,7 @@ class LiveStats:
     pre_calls: int = 0
     fed_max: int = 0
     products: int = 0
+    chain_reads: int = 0
     early: int = 0
     capped: int = 0
     seconds: float = 0.0

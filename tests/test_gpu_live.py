@@ -78,7 +78,7 @@ def test_fischer_eg_recorded(N):
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3])
-@pytest.mark.parametrize("N", [4, 6, 8])
+@pytest.mark.parametrize("N", [4, 6, 8, 10])
 def test_token_ring(N, seed):
     net = models.token_ring(N, seed)
     h0, h1, i0 = P.at(net, "P0", "hold"), P.at(net, "P1", "hold"), P.at(net, "P0", "idle")
