@@ -56,6 +56,20 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
+def measured_traffic(workload_name: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload_name`, from the FETCH_SIZE / WRITE_SIZE counter
+    runs summarised in profiles/<kernel>_<workload>_traffic.md (scaled by the copy calibration
+    recorded there), or None when no such measurement exists."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload_name}_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch {kernel} = "):
+                return int(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def physical_cores() -> int:
     try:
         cores = set()
@@ -254,7 +268,7 @@ def main() -> int:
     kms, kbytes = prof[f + "_ms"], prof[f + "_bytes"]
     achieved = (kbytes / launches) / ((kms / launches) / 1000.0) / 1e9 if kms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kern,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(a.workload, kern), "kernel": kern,
                 "avg_launch_ms": round(kms / launches, 4), "bytes_per_launch": round(kbytes / launches)}
     line = {
         "metric": "symbolic states/sec (full forward zone-graph search)",
