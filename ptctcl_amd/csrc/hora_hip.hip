@@ -194,7 +194,7 @@ __device__ __forceinline__ bool committed_at(const DevNet& N, int p, int l) {
 
 // Lanes decode the packed key (DiscKeyCodecT::decode, engine.cpp:1018-1025) into LDS.
 __device__ __forceinline__ void decode_key(const DevNet& N, u64 dk, int* locs, int* ints) {
-  const int l = threadIdx.x;
+  const int l = lane_id();
   if (l < N.np) locs[l] = (int)((dk >> N.kshift[l]) & N.kmask[l]);
   if (l < N.ni) ints[l] = (int)((dk >> N.kshift[N.np + l]) & N.kmask[N.np + l]) + N.int_lo[l];
 }
@@ -354,7 +354,7 @@ __device__ __forceinline__ u64 wave_or64(u64 v) {
 }
 
 __device__ __forceinline__ int wave_excl_scan(int v, int* total) {
-  const int l = threadIdx.x;
+  const int l = lane_id();
   int x = v;
   for (int o = 1; o < kWave; o <<= 1) {
     const int y = __shfl_up(x, o);
@@ -366,7 +366,7 @@ __device__ __forceinline__ int wave_excl_scan(int v, int* total) {
 
 // Packed key of a location vector and valuation (DiscKeyCodecT::encode, engine.cpp:1006-1013).
 __device__ __forceinline__ u64 wave_encode_key(const DevNet& N, const int* locs, const int* ints) {
-  const int l = threadIdx.x;
+  const int l = lane_id();
   u64 k = 0;
   if (l < N.np) k |= (u64)(unsigned)locs[l] << N.kshift[l];
   if (l < N.ni) k |= (u64)(unsigned)(ints[l] - N.int_lo[l]) << N.kshift[N.np + l];
@@ -378,7 +378,7 @@ __device__ __forceinline__ u64 wave_encode_key(const DevNet& N, const int* locs,
 // gathered into `inv` as (i, j, raw) triples (all_invariants, semantics.hpp:82-86).
 __device__ __forceinline__ void wave_loc_info(const DevNet& N, const int* locs, const int* ints, int* frozen,
                                               u64* amask, int* inv, int* ninv) {
-  const int l = threadIdx.x;
+  const int l = lane_id();
   int f = 0, ia = 0, ib = 0;
   u64 am = 0;
   if (l < N.np) {
@@ -409,7 +409,7 @@ __device__ __forceinline__ void wave_loc_info(const DevNet& N, const int* locs, 
 __device__ __forceinline__ bool z_apply_list(int* z, int d, const int* cs, int n) {
   for (int base = 0; base < n; base += kWave) {
     wsync();
-    const int k = base + (int)threadIdx.x;
+    const int k = base + lane_id();
     bool cand = false;
     if (k < n) cand = cs[3 * k + 2] < z[cs[3 * k] * d + cs[3 * k + 1]];
     for (u64 b = __ballot(cand); b; b &= b - 1) {
@@ -426,7 +426,7 @@ __device__ __forceinline__ bool z_apply_list(int* z, int d, const int* cs, int n
 __device__ __forceinline__ void z_reset_many(int* z, int d, u64 mask, int* tmp) {
   if (!mask) return;
   wsync();
-  const int l = threadIdx.x, d2 = d * d;
+  const int l = lane_id(), d2 = d * d;
   for (int e = l; e < d2; e += kWave) {
     const int i = e / d, j = e - i * d;
     const int si = ((mask >> i) & 1) ? 0 : i, sj = ((mask >> j) & 1) ? 0 : j;
@@ -449,7 +449,7 @@ __device__ __forceinline__ u64 edge_reset_mask(const DevNet& N, int e) {
 // Returns the number of recipes, or -1 if more than `cap`.
 __device__ __forceinline__ int build_recipes(const DevNet& N, const int* locs, const int* ints, bool src_c,
                                              int* rec, int cap) {
-  const int l = threadIdx.x;
+  const int l = lane_id();
   int cnt = 0, g = 0;
   bool comm = false;
   if (l < N.np) {
@@ -528,20 +528,73 @@ __global__ __launch_bounds__(64) void k_count(const DevNet* __restrict__ Np, con
   }
 }
 
-// LDS layout of k_expand: 3 zones + row-0 scratch + 2 discrete states + recipes + invariant list.
+// LDS layout of k_expand: 3 zones + row-0 scratch + 2 discrete states + recipes + the source
+// state's invariant triples with their process + per-process flags and active-clock masks.
 __host__ __device__ constexpr int expand_lds_ints(int d2) {
-  return 3 * d2 + HORA_MAX_DIM + 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * kRecipeCap + 3 * kInvCap;
+  return 3 * d2 + HORA_MAX_DIM + 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * kRecipeCap + 4 * kInvCap +
+         3 * HORA_MAX_PROCS + 2;
+}
+
+// z_apply_list over the source state's invariant triples, skipping those of processes p1 and p2
+// (which moved): `tag[k]` is the process of triple k.
+__device__ __forceinline__ bool z_apply_list_skip(int* z, int d, const int* cs, const int* tag, int n, int p1,
+                                                  int p2) {
+  for (int base = 0; base < n; base += kWave) {
+    wsync();
+    const int k = base + lane_id();
+    bool cand = false;
+    if (k < n && tag[k] != p1 && tag[k] != p2) cand = cs[3 * k + 2] < z[cs[3 * k] * d + cs[3 * k + 1]];
+    for (u64 b = __ballot(cand); b; b &= b - 1) {
+      const int c = base + __builtin_ctzll(b);
+      if (!z_constrain(z, d, cs[3 * c], cs[3 * c + 1], cs[3 * c + 2])) return false;
+    }
+  }
+  return true;
+}
+
+// The invariant of global location g (wave-uniform), conjoined.
+__device__ __forceinline__ bool apply_loc_inv(const DevNet& N, int g, int* z) {
+  for (int c = N.inv_off[g]; c < N.inv_off[g + 1]; ++c)
+    if (!z_constrain(z, N.d, N.inv_data[3 * c], N.inv_data[3 * c + 1], N.inv_data[3 * c + 2])) return false;
+  return true;
+}
+
+// The combined invariant of a successor: the source state's, minus the moved processes' old
+// locations, plus their new locations' (g1, g2 global locations; p = -1 for none).
+__device__ __forceinline__ bool apply_succ_inv(const DevNet& N, int* z, const int* sinv, const int* stag, int ns,
+                                               int p1, int g1, int p2, int g2) {
+  if (!z_apply_list_skip(z, N.d, sinv, stag, ns, p1, p2)) return false;
+  if (p1 >= 0 && !apply_loc_inv(N, g1, z)) return false;
+  if (p2 >= 0 && !apply_loc_inv(N, g2, z)) return false;
+  return true;
+}
+
+__device__ __forceinline__ int bcast(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ u64 bcast64(u64 v, int lane) {
+  const u32 lo = (u32)__builtin_amdgcn_readlane((int)(u32)v, lane);
+  const u32 hi = (u32)__builtin_amdgcn_readlane((int)(u32)(v >> 32), lane);
+  return ((u64)hi << 32) | lo;
 }
 
 // Successors of frontier states [0, F) into slots rec_off[i] + r, r < rec_cnt[i]. With `init` set, the frontier
 // is the single initial state: its zone is the zero zone and its one recipe is "no transition".
-__global__ __launch_bounds__(64) void k_expand(const DevNet* __restrict__ Np, const u64* __restrict__ fr_dk,
-                                               const u32* __restrict__ fr_zid, const int* __restrict__ zarena,
-                                               int F, const int* __restrict__ rec_off,
-                                               const int* __restrict__ rec_cnt, ExpandOut out,
-                                               Counters* ctr, int init) {
-  extern __shared__ int sm[];
+// A workgroup holds blockDim.x / 64 wavefronts, one frontier state each, with LDS regions of their own.
+//
+// The discrete side of each successor (target key, frozen flags, active-clock mask, which processes
+// moved and where, the target test's discrete part, the reset mask) is computed for up to 64
+// transitions at once, one lane per transition, from the source state's per-process tables held in
+// LDS; the zone side then runs transition by transition over the whole wavefront, reading those
+// values with readlane. The combined invariant of a successor is the source state's triples
+// without the moved processes', plus the moved processes' new locations'.
+__global__ __launch_bounds__(256) void k_expand(const DevNet* __restrict__ Np, const u64* __restrict__ fr_dk,
+                                                const u32* __restrict__ fr_zid, const int* __restrict__ zarena,
+                                                int F, const int* __restrict__ rec_off,
+                                                const int* __restrict__ rec_cnt, ExpandOut out,
+                                                Counters* ctr, int init) {
+  extern __shared__ int smem[];
   const DevNet& N = *Np;
+  const int wid = (int)(threadIdx.x >> 6);
+  int* sm = smem + wid * expand_lds_ints(N.d2);
   int* src = sm;
   int* z = src + N.d2;
   int* tmp = z + N.d2;
@@ -551,11 +604,15 @@ __global__ __launch_bounds__(64) void k_expand(const DevNet* __restrict__ Np, co
   int* nl = ints + HORA_MAX_INTS;
   int* niv = nl + HORA_MAX_PROCS;
   int* rec = niv + HORA_MAX_INTS;
-  int* inv = rec + 2 * kRecipeCap;
-  const int i = blockIdx.x;
+  int* sinv = rec + 2 * kRecipeCap;                 // 3 * kInvCap
+  int* stag = sinv + 3 * kInvCap;                   // kInvCap
+  int* sflags = stag + kInvCap;                     // HORA_MAX_PROCS
+  u64* samask = (u64*)(((uintptr_t)(sflags + HORA_MAX_PROCS) + 7) & ~(uintptr_t)7);   // HORA_MAX_PROCS
+  const int i = (int)blockIdx.x * (int)(blockDim.x >> 6) + wid;
   if (i >= F) return;
-  const int l = threadIdx.x, d = N.d, d2 = N.d2;
-  decode_key(N, fr_dk[i], locs, ints);
+  const int l = lane_id(), d = N.d, d2 = N.d2;
+  const u64 dk = fr_dk[i];
+  decode_key(N, dk, locs, ints);
   if (init) {
     for (int e = l; e < d2; e += kWave) src[e] = kLe0;
   } else {
@@ -563,49 +620,66 @@ __global__ __launch_bounds__(64) void k_expand(const DevNet* __restrict__ Np, co
     for (int e = l; e < d2; e += kWave) src[e] = zs[e];
   }
   wsync();
-  auto emit = [&](int slot, int se, int re) __attribute__((always_inline)) {
-    wsync();
-    if (l < N.np) nl[l] = locs[l];
-    if (l < N.ni) niv[l] = ints[l];
-    wsync();
-    if (l == 0 && se >= 0) {
-      nl[N.edge_proc[se]] = N.edge_tgt[se];
-      if (re >= 0) nl[N.edge_proc[re]] = N.edge_tgt[re];
-      for (int c = N.iu_off[se]; c < N.iu_off[se + 1]; ++c) niv[N.iu_data[2 * c]] = N.iu_data[2 * c + 1];
-      if (re >= 0)
-        for (int c = N.iu_off[re]; c < N.iu_off[re + 1]; ++c) niv[N.iu_data[2 * c]] = N.iu_data[2 * c + 1];
+  // source state: per-process flags, active clocks and invariant triples (lane p = process p)
+  int fz_src = 0, ns = 0;
+  {
+    int f = 0, ia = 0, ib = 0;
+    u64 am = 0;
+    if (l < N.np) {
+      const int g = N.locbase[l] + locs[l];
+      f = N.loc_flags[g] & 3;
+      am = N.amask[g];
+      ia = N.inv_off[g];
+      ib = N.inv_off[g + 1];
+      sflags[l] = f;
+      samask[l] = am;
     }
+    fz_src = (__ballot(f & 1) ? 1 : 0) | (__ballot(f & 2) ? 2 : 0);
+    if (!(fz_src & 2) && N.has_urgent_chan && urgent_sync_enabled(N, locs, ints)) fz_src |= 2;
+    const int o0 = wave_excl_scan(ib - ia, &ns);
+    if (ns > kInvCap) {
+      if (l == 0) atomicExch(&ctr->err, ERR_RECIPES);
+      return;
+    }
+    for (int c = ia; c < ib; ++c) {
+      const int o = o0 + c - ia;
+      sinv[3 * o] = N.inv_data[3 * c];
+      sinv[3 * o + 1] = N.inv_data[3 * c + 1];
+      sinv[3 * o + 2] = N.inv_data[3 * c + 2];
+      stag[o] = l;
+    }
+  }
+  const u64 clocks = ((1ull << N.nclocks) - 1) << 1;
+  // the zone side of one successor; `fz`, `am` describe the target location vector
+  auto finish = [&](int slot, int se, int re, int p1, int g1, int p2, int g2, int fz, u64 am, u64 rm, u64 key,
+                    int tdisc) __attribute__((always_inline)) {
+    wsync();
     z_copy(z, src, d2);
     bool ok = true;
     if (se >= 0) {
       // guards (sender then receiver), then every reset of both edges at once
       ok = apply_edge_guard(N, se, z);
       if (ok && re >= 0) ok = apply_edge_guard(N, re, z);
-      if (ok) z_reset_many(z, d, edge_reset_mask(N, se) | (re >= 0 ? edge_reset_mask(N, re) : 0ull), tmp);
-    }
-    wsync();
-    int fz = 0, ninv = 0;
-    u64 am = 0;
-    wave_loc_info(N, nl, niv, &fz, &am, inv, &ninv);
-    if (ninv > kInvCap) {
-      if (l == 0) atomicExch(&ctr->err, ERR_RECIPES);
-      ok = false;
+      if (ok) z_reset_many(z, d, rm, tmp);
     }
     // close_succ_zone (semantics.hpp:522-568): invariant, delay unless frozen, invariant,
-    // inactive clocks reset, Extra_LU+ when on; then the caller's emptiness test
-    if (ok) ok = z_apply_list(z, d, inv, ninv);
+    // inactive clocks reset, Extra_LU+ when on. A canonical zone that survives the invariant stays
+    // non-empty through the delay, the second invariant, the resets and the extrapolation.
+    if (ok) ok = apply_succ_inv(N, z, sinv, stag, ns, p1, g1, p2, g2);
     if (ok) {
       if (!fz) {
         z_up(z, d);
-        z_apply_list(z, d, inv, ninv);
+        apply_succ_inv(N, z, sinv, stag, ns, p1, g1, p2, g2);
       }
-      const u64 dead = ~(am | N.qbits) & (((1ull << N.nclocks) - 1) << 1);
-      z_reset_many(z, d, dead, tmp);
+      z_reset_many(z, d, ~(am | N.qbits) & clocks, tmp);
       if (N.lu_plus) z_extra_lu_plus(z, d, N.Lx, N.Ux, row0);
-      ok = !z_empty(z, d);
     }
-    const bool tg = ok && is_target(N, nl, niv, z, tmp);
-    const u64 key = wave_encode_key(N, nl, niv);
+    bool tg = ok && tdisc && !N.never;
+    if (tg && N.ncc > 0) {
+      wsync();
+      z_copy(tmp, z, d2);
+      for (int k = 0; k < N.ncc && tg; ++k) tg = z_constrain(tmp, d, N.cc[3 * k], N.cc[3 * k + 1], N.cc[3 * k + 2]);
+    }
     wsync();
     if (l == 0) {
       out.valid[slot] = ok ? 1 : 0;
@@ -622,20 +696,94 @@ __global__ __launch_bounds__(64) void k_expand(const DevNet* __restrict__ Np, co
       for (int e = l; e < d2; e += kWave) dst[e] = z[e];
     }
   };
+  // the discrete side of transition (se, re) from the source state, in the calling lane
+  auto describe = [&](int se, int re, int* p1, int* g1, int* p2, int* g2, int* fz, u64* am, u64* rm, u64* key,
+                      int* tdisc) __attribute__((always_inline)) {
+    const int q1 = N.edge_proc[se], t1 = N.edge_tgt[se];
+    const int q2 = re >= 0 ? N.edge_proc[re] : -1, t2 = re >= 0 ? N.edge_tgt[re] : 0;
+    const int h1 = N.locbase[q1] + t1, h2 = q2 >= 0 ? N.locbase[q2] + t2 : 0;
+    u64 k = dk;
+    k = (k & ~(N.kmask[q1] << N.kshift[q1])) | ((u64)(unsigned)t1 << N.kshift[q1]);
+    if (q2 >= 0) k = (k & ~(N.kmask[q2] << N.kshift[q2])) | ((u64)(unsigned)t2 << N.kshift[q2]);
+    u64 upd = 0;   // integer variables the transition assigns (bit v), in order sender then receiver
+    for (int pass = 0; pass < 2; ++pass) {
+      const int e = pass == 0 ? se : re;
+      if (e < 0) continue;
+      for (int c = N.iu_off[e]; c < N.iu_off[e + 1]; ++c) {
+        const int v = N.iu_data[2 * c], val = N.iu_data[2 * c + 1];
+        const int f = N.np + v;
+        k = (k & ~(N.kmask[f] << N.kshift[f])) | ((u64)(unsigned)(val - N.int_lo[v]) << N.kshift[f]);
+        upd |= 1ull << v;
+      }
+    }
+    int fl = (N.loc_flags[h1] & 3) | (q2 >= 0 ? (N.loc_flags[h2] & 3) : 0);
+    u64 a = N.amask[h1] | (q2 >= 0 ? N.amask[h2] : 0ull);
+    for (int p = 0; p < N.np; ++p)
+      if (p != q1 && p != q2) {
+        fl |= sflags[p];
+        a |= samask[p];
+      }
+    u64 r = 0;
+    for (int c = N.rs_off[se]; c < N.rs_off[se + 1]; ++c) r |= 1ull << N.rs_clk[c];
+    if (re >= 0)
+      for (int c = N.rs_off[re]; c < N.rs_off[re + 1]; ++c) r |= 1ull << N.rs_clk[c];
+    // discrete part of the target test (reachable_net's match, engine.cpp:2275-2287)
+    int td = 1;
+    for (int t = 0; t < N.nlr && td; ++t) {
+      const int p = N.lr[2 * t], want = N.lr[2 * t + 1];
+      const int lp = (int)((k >> N.kshift[p]) & N.kmask[p]);
+      td = lp == want;
+    }
+    for (int t = 0; t < N.nic && td; ++t) {
+      const int v = N.ic[3 * t];
+      const int x = (int)((k >> N.kshift[N.np + v]) & N.kmask[N.np + v]) + N.int_lo[v];
+      td = atom_holds(x, N.ic[3 * t + 1], N.ic[3 * t + 2]);
+    }
+    *p1 = q1; *g1 = h1; *p2 = q2; *g2 = h2; *fz = fl; *am = a; *rm = r; *key = k; *tdisc = td;
+  };
   if (init) {
-    emit(0, -1, -1);
+    // no transition: the initial location vector, its invariant, delay unless frozen, ...
+    int td = 1;
+    for (int t = 0; t < N.nlr && td; ++t) td = locs[N.lr[2 * t]] == N.lr[2 * t + 1];
+    for (int t = 0; t < N.nic && td; ++t) td = atom_holds(ints[N.ic[3 * t]], N.ic[3 * t + 1], N.ic[3 * t + 2]);
+    u64 am0 = 0;
+    for (int p = 0; p < N.np; ++p) am0 |= samask[p];
+    finish(0, -1, -1, -1, 0, -1, 0, fz_src, am0, 0ull, dk, td);
     return;
   }
-  int fz;
-  u64 am;
-  int ninv;
-  wave_loc_info(N, locs, ints, &fz, &am, inv, &ninv);
   // the slots reserved for this state bound what is written: a state filtered out after counting
   // (rec_cnt[i] = 0) writes nothing
-  const int nr = min(build_recipes(N, locs, ints, (fz & 1) != 0, rec, kRecipeCap), rec_cnt[i]);
+  const int nr = min(build_recipes(N, locs, ints, (fz_src & 1) != 0, rec, kRecipeCap), rec_cnt[i]);
   wsync();
   const int base = rec_off[i];
-  for (int r = 0; r < nr; ++r) emit(base + r, rec[2 * r], rec[2 * r + 1]);
+  for (int b0 = 0; b0 < nr; b0 += kWave) {
+    int p1 = -1, g1 = 0, p2 = -1, g2 = 0, fz = 0, td = 0;
+    u64 am = 0, rm = 0, key = 0;
+    if (b0 + l < nr) describe(rec[2 * (b0 + l)], rec[2 * (b0 + l) + 1], &p1, &g1, &p2, &g2, &fz, &am, &rm, &key, &td);
+    const int nb = min(kWave, nr - b0);
+    for (int r = 0; r < nb; ++r) {
+      const int se = rec[2 * (b0 + r)], re = rec[2 * (b0 + r) + 1];
+      int fzr = bcast(fz, r);
+      if (!(fzr & 2) && N.has_urgent_chan) {
+        // urgent synchronisations read the whole target state: materialise it
+        wsync();
+        if (l < N.np) nl[l] = locs[l];
+        if (l < N.ni) niv[l] = ints[l];
+        wsync();
+        if (l == 0) {
+          nl[N.edge_proc[se]] = N.edge_tgt[se];
+          if (re >= 0) nl[N.edge_proc[re]] = N.edge_tgt[re];
+          for (int c = N.iu_off[se]; c < N.iu_off[se + 1]; ++c) niv[N.iu_data[2 * c]] = N.iu_data[2 * c + 1];
+          if (re >= 0)
+            for (int c = N.iu_off[re]; c < N.iu_off[re + 1]; ++c) niv[N.iu_data[2 * c]] = N.iu_data[2 * c + 1];
+        }
+        wsync();
+        if (urgent_sync_enabled(N, nl, niv)) fzr |= 2;
+      }
+      finish(base + b0 + r, se, re, bcast(p1, r), bcast(g1, r), bcast(p2, r), bcast(g2, r), fzr,
+             bcast64(am, r), bcast64(rm, r), bcast64(key, r), bcast(td, r));
+    }
+  }
 }
 
 // ---------------------------------------------------------------- scans
@@ -1515,6 +1663,8 @@ struct hora_hip_ctx {
   bool trace = false;
   hipEvent_t ev[8] = {};
   LiveState* live = nullptr;
+  int expand_wpb = 0;         // wavefronts per k_expand workgroup, picked by expand_tuned()
+  int expand_wpb_d = 0;       // the zone dimension it was picked for
 };
 
 namespace {
@@ -1634,6 +1784,41 @@ void account(hora_hip_ctx* c, const Tally& hc, const Tally& prev, long G, int T,
   c->prof.other_ms += ev_ms(c->ev[2], c->ev[5]) + ev_ms(c->ev[6], c->ev[3]);
 }
 
+void launch_expand(hora_hip_ctx* c, int wpb, long F, ExpandOut eo, bool init) {
+  const size_t lds = (size_t)wpb * expand_lds_ints(c->d2) * sizeof(int);
+  hipLaunchKernelGGL(k_expand, dim3(blocks_for(F, wpb)), dim3(64 * wpb), lds, c->stream, c->dnp, c->fr_dk.p,
+                     c->fr_zid.p, c->zarena.p, (int)F, c->rec_off.p, c->rec_cnt.p, eo, c->ctr.p, init ? 1 : 0);
+}
+
+// Wavefronts per k_expand workgroup. More wavefronts per workgroup lift the per-CU workgroup limit on
+// resident wavefronts; fewer keep LDS allocation granular. The first level with enough states runs
+// the expansion once per candidate (the kernel is idempotent: same slots, same values) and keeps
+// the fastest for the context and zone dimension. HORA_HIP_EXPAND_WPB=1|2|4 fixes the choice.
+int expand_tuned(hora_hip_ctx* c, long F, ExpandOut eo, bool init) {
+  if (c->expand_wpb > 0 && c->expand_wpb_d == c->d) return c->expand_wpb;
+  if (const char* e = std::getenv("HORA_HIP_EXPAND_WPB")) {
+    const int w = std::atoi(e);
+    if (w == 1 || w == 2 || w == 4) { c->expand_wpb = w; c->expand_wpb_d = c->d; return w; }
+  }
+  if (init || F < 65536) return 4;
+  const int cand[3] = {1, 2, 4};
+  float best = 1e30f;
+  int pick = 4;
+  for (int rep = 0; rep < 2; ++rep)
+    for (int w : cand) {
+      HIP_OK(hipEventRecord(c->ev[6], c->stream));
+      launch_expand(c, w, F, eo, init);
+      HIP_OK(hipEventRecord(c->ev[7], c->stream));
+      HIP_OK(hipEventSynchronize(c->ev[7]));
+      const float ms = ev_ms(c->ev[6], c->ev[7]);
+      if (ms < best) { best = ms; pick = w; }
+    }
+  c->expand_wpb = pick;
+  c->expand_wpb_d = c->d;
+  if (c->trace) std::fprintf(stderr, "k_expand: %d wavefronts per workgroup (%.3f ms at F=%ld)\n", pick, best, F);
+  return pick;
+}
+
 int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
   const auto t0 = std::chrono::steady_clock::now();
   std::memset(st, 0, sizeof(*st));
@@ -1691,11 +1876,9 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     c->s_dk.ensure(R);
     c->s_zone.ensure((size_t)R * d2);
     ExpandOut eo{c->s_valid.p, c->s_dk.p, c->s_zone.p, c->s_tgt.p, nullptr, nullptr};
-    const size_t lds_e = expand_lds_ints(d2) * sizeof(int);
+    const int wpb = R > 0 ? expand_tuned(c, F, eo, init) : 1;
     HIP_OK(hipEventRecord(c->ev[0], c->stream));
-    if (R > 0)
-      hipLaunchKernelGGL(k_expand, dim3(F), dim3(64), lds_e, c->stream, c->dnp, c->fr_dk.p, c->fr_zid.p,
-                         c->zarena.p, (int)F, c->rec_off.p, c->rec_cnt.p, eo, c->ctr.p, init ? 1 : 0);
+    if (R > 0) launch_expand(c, wpb, F, eo, init);
     HIP_OK(hipEventRecord(c->ev[1], c->stream));
     // ---- compaction of non-empty successors
     c->pos.ensure(R + 1);

@@ -1179,9 +1179,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     L.s_se.ensure(R);
     L.s_re.ensure(R);
     ExpandOut eo{c->s_valid.p, c->s_dk.p, c->s_zone.p, c->s_tgt.p, L.s_se.p, L.s_re.p};
-    if (R > 0)
-      hipLaunchKernelGGL(k_expand, dim3(F), dim3(64), expand_lds_ints(d2) * sizeof(int), c->stream, c->dnp, c->fr_dk.p,
-                         c->fr_zid.p, c->zarena.p, (int)F, c->rec_off.p, c->rec_cnt.p, eo, c->ctr.p, init ? 1 : 0);
+    if (R > 0) launch_expand(c, expand_tuned(c, F, eo, init), F, eo, init);
     c->pos.ensure(R + 1);
     const long G = R > 0 ? scan_exclusive(c, c->s_valid.p, R, c->pos.p) : 0;
     if (!init) {

@@ -20,6 +20,10 @@ constexpr int kLe0 = 1;
 constexpr int kAluNegInf = -(1 << 29);
 constexpr int kWave = 64;
 
+// Lane of the calling thread in its wavefront. Zone operations run on one wavefront; a workgroup
+// may hold several wavefronts, each on a zone of its own.
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
 __device__ __forceinline__ int r_le(int v) { return v * 2 + 1; }
 __device__ __forceinline__ int r_lt(int v) { return v * 2; }
 __device__ __forceinline__ int r_bnd(int r) { return r >> 1; }
@@ -52,12 +56,12 @@ __device__ __forceinline__ void wsync() {
 __device__ __forceinline__ void bsync() { __syncthreads(); }
 
 __device__ __forceinline__ void z_copy(int* __restrict__ dst, const int* __restrict__ src, int d2) {
-  for (int e = threadIdx.x; e < d2; e += kWave) dst[e] = src[e];
+  for (int e = lane_id(); e < d2; e += kWave) dst[e] = src[e];
 }
 
 __device__ __forceinline__ bool z_empty(const int* z, int d) {
   wsync();
-  const int l = threadIdx.x;
+  const int l = lane_id();
   const bool neg = l < d && z[l * d + l] < kLe0;
   return __any(neg);
 }
@@ -65,14 +69,14 @@ __device__ __forceinline__ bool z_empty(const int* z, int d) {
 // Delay: every upper bound x_i - 0 dropped.
 __device__ __forceinline__ void z_up(int* z, int d) {
   wsync();
-  const int l = threadIdx.x;
+  const int l = lane_id();
   if (l >= 1 && l < d) z[l * d] = kInf;
 }
 
 // x_k := 0: row k takes row 0, column k takes column 0, z[k][k] = (<=, 0).
 __device__ __forceinline__ void z_reset(int* z, int d, int k) {
   wsync();
-  const int l = threadIdx.x;
+  const int l = lane_id();
   int r0 = 0, c0 = 0;
   if (l < d) { r0 = z[l]; c0 = z[l * d]; }
   wsync();
@@ -92,7 +96,7 @@ __device__ __forceinline__ bool z_constrain(int* z, int d, int a, int b, int r) 
   const int zab = z[a * d + b];
   if (r >= zab) return true;
   if (radd(z[b * d + a], r) < kLe0) return false;
-  const int l = threadIdx.x;
+  const int l = lane_id();
   int col = 0;
   if (l < d) col = radd(z[l * d + a], r);
   wsync();
@@ -112,7 +116,7 @@ __device__ __forceinline__ bool z_constrain(int* z, int d, int a, int b, int r) 
 // Floyd-Warshall closure, one pivot per step. Within a step, row k and column k do not change on
 // a zone without negative cycles, so the parallel update equals the sequential one.
 __device__ __forceinline__ void z_close(int* z, int d) {
-  const int d2 = d * d, l = threadIdx.x;
+  const int d2 = d * d, l = lane_id();
   for (int k = 0; k < d; ++k) {
     wsync();
     for (int e = l; e < d2; e += kWave) {
@@ -129,7 +133,7 @@ __device__ __forceinline__ void z_close(int* z, int d) {
 // then closure. `row0` is d ints of LDS scratch.
 __device__ __forceinline__ void z_extra_lu_plus(int* z, int d, const int* L, const int* U, int* row0) {
   wsync();
-  const int l = threadIdx.x, d2 = d * d;
+  const int l = lane_id(), d2 = d * d;
   if (l < d) row0[l] = z[l];
   wsync();
   for (int e = l; e < d2; e += kWave) {
@@ -153,7 +157,7 @@ __device__ __forceinline__ void z_extra_lu_plus(int* z, int d, const int* L, con
 // operands are read with unit stride; either operand may be in LDS or global memory.
 __device__ __forceinline__ bool z_alu_le(const int* Z, const int* Zp, int d, const int* L, const int* U) {
   wsync();
-  const int l = threadIdx.x, d2 = d * d;
+  const int l = lane_id(), d2 = d * d;
   bool wit = false;
   for (int e = l; e < d2; e += kWave) {
     const int y = e / d, x = e - y * d;
@@ -171,7 +175,7 @@ __device__ __forceinline__ bool z_alu_le(const int* Z, const int* Zp, int d, con
 // Entrywise b <= a.
 __device__ __forceinline__ bool z_includes(const int* a, const int* b, int d) {
   wsync();
-  const int l = threadIdx.x, d2 = d * d;
+  const int l = lane_id(), d2 = d * d;
   bool bad = false;
   for (int e = l; e < d2; e += kWave)
     if (b[e] > a[e]) bad = true;
@@ -189,7 +193,7 @@ __device__ __forceinline__ bool z_includes(const int* a, const int* b, int d) {
 // `cnt` is 2 ints of LDS scratch. Both a and b must be canonical, non-empty and in LDS.
 __device__ __forceinline__ bool z_try_union(int* a, const int* b, int d, int* fa, int* pa, int* cnt) {
   wsync();
-  const int l = threadIdx.x, d2 = d * d;
+  const int l = lane_id(), d2 = d * d;
   if (l == 0) { cnt[0] = 0; cnt[1] = 0; }
   wsync();
   // ballot compaction of the differing entries, in index order
@@ -239,7 +243,7 @@ __device__ __forceinline__ bool z_try_union(int* a, const int* b, int d, int* fa
 // Time predecessors: z[0][j] = min(<=0, min_{i>=1} z[i][j]). Canonical in, canonical out.
 __device__ __forceinline__ void z_down(int* z, int d) {
   wsync();
-  const int l = threadIdx.x;
+  const int l = lane_id();
   int v = kLe0;
   if (l >= 1 && l < d)
     for (int i = 1; i < d; ++i) v = min(v, z[i * d + l]);
@@ -251,7 +255,7 @@ __device__ __forceinline__ void z_down(int* z, int d) {
 __device__ __forceinline__ void z_free_many(int* z, int d, uint64_t mask, int* tmp) {
   if (!mask) return;
   wsync();
-  const int l = threadIdx.x, d2 = d * d;
+  const int l = lane_id(), d2 = d * d;
   for (int e = l; e < d2; e += kWave) {
     const int i = e / d, j = e - i * d;
     int v;
@@ -268,7 +272,7 @@ __device__ __forceinline__ void z_free_many(int* z, int d, uint64_t mask, int* t
 // A two-clock cycle through both zones is negative: the zones are certainly disjoint.
 __device__ __forceinline__ bool z_disjoint_quick(const int* a, const int* b, int d) {
   wsync();
-  const int l = threadIdx.x, d2 = d * d;
+  const int l = lane_id(), d2 = d * d;
   bool neg = false;
   for (int e = l; e < d2; e += kWave) {
     const int i = e / d, j = e - i * d;
@@ -280,7 +284,7 @@ __device__ __forceinline__ bool z_disjoint_quick(const int* a, const int* b, int
 // out := a /\ b (entrywise min, closure). Returns false iff empty; out is then unspecified.
 __device__ __forceinline__ bool z_intersect(int* out, const int* a, const int* b, int d) {
   if (z_disjoint_quick(a, b, d)) return false;
-  const int l = threadIdx.x, d2 = d * d;
+  const int l = lane_id(), d2 = d * d;
   for (int e = l; e < d2; e += kWave) out[e] = min(a[e], b[e]);
   z_close(out, d);
   return !z_empty(out, d);
@@ -289,7 +293,7 @@ __device__ __forceinline__ bool z_intersect(int* out, const int* a, const int* b
 // Equal entrywise.
 __device__ __forceinline__ bool z_equal(const int* a, const int* b, int d) {
   wsync();
-  const int l = threadIdx.x, d2 = d * d;
+  const int l = lane_id(), d2 = d * d;
   bool diff = false;
   for (int e = l; e < d2; e += kWave)
     if (a[e] != b[e]) diff = true;
@@ -302,7 +306,7 @@ __device__ __forceinline__ bool z_equal(const int* a, const int* b, int d) {
 // Returns false if the list would exceed `cap`.
 __device__ __forceinline__ bool z_subtract(const int* p, const int* y, int d, int* out, int zs, int* n, int cap,
                                            int* r, int* t) {
-  const int l = threadIdx.x, d2 = d * d;
+  const int l = lane_id(), d2 = d * d;
   if (!z_intersect(t, p, y, d)) {
     if (*n >= cap) return false;
     wsync();
