@@ -220,6 +220,80 @@ def serve_main(a) -> int:
     return 0
 
 
+def zoneops_main(a) -> int:
+    """Zone-kernel microbenchmark: each HDBM operation on 2^20 zones per dimension d in
+    {4, 8, 16, 32}, device-resident int32 zones. The zones are 2^16 seeded canonical zones
+    (ptctcl_amd.zonegen: point-cloud hulls, loosened; 2^16 distinct for d <= 8, 2^14 for d = 16,
+    2^13 for d = 32, the host-side generation being cubic in d) tiled to 2^20; L, U ~ U[0, 1000]; inclusion
+    pairs half built as B inside A. Algorithmic bytes per zone-op: 8*d*d for the unary operations
+    (zone read + written), 8*d*d + 4 for the pair tests (two zones read, one flag written), 4*d + 4
+    for the emptiness test (diagonal read)."""
+    import numpy as np
+    import torch
+    from ptctcl_amd import hip, zonegen as zg
+    n = 1 << 20
+    rows, tot_ops, tot_s = [], 0, 0.0
+    for d in (4, 8, 16, 32):
+        pool = {4: 1 << 16, 8: 1 << 16, 16: 1 << 14, 32: 1 << 13}[d]
+        rng = np.random.default_rng(1000 + d)
+        zp = zg.hull_zones(pool, d, rng)
+        up = zg.unclose(zp, d, rng)
+        _, bp = zg.pairs_included(zp, d, rng)
+        Lp, Up = zg.bounds(pool, d, rng)
+        cons = zg.constraints(zp, d, rng)
+        rep = n // pool
+        dev = lambda x: torch.from_numpy(np.ascontiguousarray(np.tile(x, (rep,) + (1,) * (x.ndim - 1)))).to("cuda:0")
+        Z, UZ, B, L, U = dev(zp), dev(up), dev(bp), dev(Lp), dev(Up)
+        C = dev(cons).reshape(-1)
+        Rk = torch.randint(1, d, (n,), dtype=torch.int32, device="cuda:0")
+        Fm = (torch.randint(1, 1 << min(d - 1, 30), (n,), dtype=torch.int64, device="cuda:0") << 1).to(torch.int32)
+        out = torch.empty_like(Z)
+        flag = torch.empty(n, dtype=torch.int32, device="cuda:0")
+        z2 = 4 * d * d
+        ops = [("close", hip.ZOP_CLOSE, UZ, None, None, 2 * z2), ("up", hip.ZOP_UP, Z, None, None, 2 * z2),
+               ("reset", hip.ZOP_RESET, Z, None, Rk, 2 * z2), ("constrain", hip.ZOP_CONSTRAIN, Z, None, C, 2 * z2),
+               ("extra_lu", hip.ZOP_EXTRA_LU, Z, None, None, 2 * z2), ("includes", hip.ZOP_INCLUDES, Z, B, None, 2 * z2 + 4),
+               ("alu_le", hip.ZOP_ALU_LE, B, Z, None, 2 * z2 + 4), ("union", hip.ZOP_UNION, Z, B, None, 2 * z2 + 4),
+               ("empty", hip.ZOP_EMPTY, Z, None, None, 4 * d + 4), ("down", hip.ZOP_DOWN, Z, None, None, 2 * z2),
+               ("free", hip.ZOP_FREE, Z, None, Fm, 2 * z2)]
+        for name, op, A, Bx, arg, bytes_per in ops:
+            def go():
+                hip.zone_op(op, d, A, Bx, arg, L, U, out, flag)
+            go()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = 5
+            e0.record()
+            for _ in range(reps):
+                go()
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            gbs = bytes_per * n / (ms / 1000.0) / 1e9
+            rows.append({"d": d, "op": name, "ms": round(ms, 4), "zone_ops_per_s": round(n / (ms / 1000.0), 1),
+                         "GB_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)})
+            tot_ops += n
+            tot_s += ms / 1000.0
+        del Z, UZ, B, L, U, C, Rk, Fm, out, flag
+        torch.cuda.empty_cache()
+    dom = max(rows, key=lambda r: r["ms"])
+    stream = [r for r in rows if r["op"] in ("up", "reset", "down", "free", "includes")]
+    best = max(stream, key=lambda r: r["GB_s"])
+    line = {
+        "metric": "zone operations/sec (HDBM ops on 2^20 device-resident zones, d in {4,8,16,32})",
+        "value": round(tot_ops / tot_s, 1), "unit": "zone-ops/s", "n_gpus": 1, "steps": 5, "warmup": 1,
+        "ms_per_step": round(tot_s * 1000, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic (seeded canonical zones, 2^13-2^16 distinct tiled to 2^20)",
+        "config": {"workload": "zone-kernel microbenchmark", "zones_per_op": n, "ops": rows,
+                   "best_streaming_op": best},
+        "roofline": {"bound": "hbm", "achieved": dom["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dom["GB_s"] / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": f"{dom['op']} d={dom['d']} (the slowest op)"},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,6 +313,8 @@ def main() -> int:
         return live_main(a)
     if a.workload == "serve":
         return serve_main(a)
+    if a.workload == "zoneops":
+        return zoneops_main(a)
 
     import torch
     from ptctcl_amd import hip

@@ -205,6 +205,8 @@ typedef struct hora_dpred {
   const int32_t* atoms;      /* 4 int32 per atom */
 } hora_dpred;
 
+enum { HORA_COVER_AUTO = 0, HORA_COVER_AM = 1 };
+
 enum {
   HORA_LIVE_LEADSTO = 0,     /* p --> q: holds iff no reachable p-state has a maximal run avoiding q */
   HORA_LIVE_EG = 1,          /* E[] psi: holds iff the initial state has a maximal psi-run          */
@@ -215,6 +217,10 @@ typedef struct hora_live_query {
   int32_t kind;
   int32_t early;             /* LEADSTO / DEADLOCK: settle at the first witness state
                                 (leadsto_witness, src/liveness/construction.cpp:786-813)           */
+  int32_t cover;             /* HORA_COVER_AUTO: a_LU covers unless a reachable discrete state can
+                                seed a deadlock atom (online_seed_key, src/liveness/refinement.cpp:
+                                176-282), then the a_M graph; HORA_COVER_AM: the a_M graph always */
+  int32_t pad;
   hora_dpred p;              /* LEADSTO: p.  EG: psi's discrete part.  DEADLOCK: unused           */
   hora_dpred q;              /* LEADSTO: q.  Otherwise unused                                     */
   /* EG whose psi has a clock conjunction: the context was opened on the network restricted by it
@@ -243,6 +249,8 @@ typedef struct hora_live_stats {
   int64_t chain_reads;       /* stored or queued zones read by the cover tests of the construction */
   int32_t early;             /* 1: settled by a witness state during the construction           */
   int32_t capped;            /* 1: a federation outgrew its capacity; no verdict                */
+  int32_t route;             /* covering of the graph decided on: 0 a_LU, 1 a_M                 */
+  int32_t seed_keys;         /* discrete states that could seed a deadlock atom (0 on a_LU)    */
   double seconds;            /* whole query                                                     */
   double build_seconds;      /* forward construction                                            */
   double gfp_seconds;        /* stop sets, closure and fixpoint                                 */

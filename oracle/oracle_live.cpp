@@ -210,16 +210,23 @@ struct Live {
     return id;
   }
 
-  // first stored node of the same discrete state that covers z (a_M), else a new node
+  bool cover_am = true;   // covering relation of the graph being built: a_M, else a_LU
+
+  // first stored node of the same discrete state that covers z, else a new node
   int intern(std::uint64_t key, const int* locs, const raw_t* z, bool* created) {
     *created = false;
     auto it = chains.find(key);
     if (it != chains.end()) {
-      std::int32_t M[64];
-      mbounds(locs, M);
+      std::int32_t L2[64], U2[64];
+      if (cover_am) {
+        mbounds(locs, L2);
+        std::copy(L2, L2 + d, U2);
+      } else {
+        E.state_bounds(locs, L2, U2);
+      }
       for (int m : it->second) {
         ++st.chain_reads;
-        if (dbm_alu_le(z, Z(m), d, M, M)) { ++st.covered; return m; }
+        if (dbm_alu_le(z, Z(m), d, L2, U2)) { ++st.covered; return m; }
       }
     }
     *created = true;
@@ -279,13 +286,20 @@ struct Live {
   }
   bool fed_covers(const Fed& F, const raw_t* z) const { return fed_subtract(z, F).empty(); }
 
-  // state_has_deadlock (deadlock.cpp:74-268) judged with invariant table `inv`
   bool state_has_deadlock(int v, const std::vector<std::vector<Cons>>& inv, Fed* stuck) const {
     int locs[64], ints[64];
     state(v, locs, ints);
+    return state_has_deadlock(locs, ints, Z(v), inv, stuck, nullptr);
+  }
+
+  // state_has_deadlock (deadlock.cpp:74-268) of the state (locs, ints, zone) judged with invariant
+  // table `inv`; `escapes` receives the enabling regions after the time predecessor, when asked
+  bool state_has_deadlock(const int* locs, const int* ints, const raw_t* zone,
+                          const std::vector<std::vector<Cons>>& inv, Fed* stuck, Fed* escapes) const {
     const bool src_c = [&] { bool c, u; E.frozen_flags(locs, ints, &c, &u); return c; }();
     const bool fz = frozen(locs, ints);
-    std::vector<raw_t> Zd(Z(v), Z(v) + d2);
+    if (escapes) escapes->clear();
+    std::vector<raw_t> Zd(zone, zone + d2);
     if (!fz) dbm_up(Zd.data(), d);
     if (!apply_inv_table(inv, locs, Zd.data())) return false;
     Fed live;
@@ -346,6 +360,7 @@ struct Live {
     const std::size_t m = live.size() / d2;
     if (!fz)
       for (std::size_t k = 0; k < m; ++k) dbm_down(&live[k * d2], d);
+    if (escapes) *escapes = live;
     for (std::size_t k = 0; k < m; ++k)
       if (dbm_includes(&live[k * d2], Zd.data(), d)) return false;
     Fed dd = fed_subtract(Zd.data(), live);
@@ -365,6 +380,38 @@ struct Live {
     if (orig_time_open(v)) return f;
     state_has_deadlock(v, orig_inv, &f);
     return f;
+  }
+
+  // ---- route: can a discrete state seed a deadlock atom? (online_seed_key,
+  // src/liveness/refinement.cpp:176-282) -----------------------------------------------------------
+  // D refines a_LU only at discrete states whose deadlock region is partial: a deadlocked valuation
+  // exists and some escape meets the invariant. Judged on the universe zone in the unrestricted
+  // network; a non-frozen state without an upper invariant, or whose invariant lets every clock
+  // grow, cannot seed one except for `A[] not deadlock`.
+  bool key_seeds(std::uint64_t key) const {
+    int locs[64], ints[64];
+    E.decode(key, locs, ints);
+    const auto& inv = restricted ? orig_inv : n.inv;
+    const bool fz = frozen(locs, ints);
+    const bool dl = kind == HORA_LIVE_DEADLOCK;
+    if (!fz && !dl && !has_upper(inv, locs)) return false;
+    std::vector<raw_t> u(d2, kInf);
+    for (int j = 0; j < d; ++j) u[j] = kLe0;
+    for (int i = 0; i < d; ++i) u[i * d + i] = kLe0;
+    std::vector<raw_t> iz(u);
+    if (!apply_inv_table(inv, locs, iz.data())) return false;
+    if (!fz && !dl) {
+      bool open = true;
+      for (int i = 1; i < d; ++i)
+        if (iz[i * d] != kInf) open = false;
+      if (open) return false;
+    }
+    Fed stuck, esc;
+    if (!state_has_deadlock(locs, ints, u.data(), inv, &stuck, &esc)) return false;
+    std::vector<raw_t> t(d2);
+    for (std::size_t k = 0; k < esc.size() / d2; ++k)
+      if (dbm_intersect(t.data(), &esc[k * d2], iz.data(), d)) return true;
+    return false;                              // Dead == Inv: no distinguishing atom
   }
 
   // ---- exact predecessor (operators.cpp:257-306) ----------------------------------------------
@@ -585,9 +632,30 @@ struct Live {
     }
   }
 
-  int run(int32_t* holds) {
+  void reset_graph() {
+    dk.clear(); zones.clear(); row_off.clear(); etgt.clear(); ese.clear(); ere.clear();
+    expanded.clear(); in_s.clear(); wne.clear(); chains.clear(); ecnt.clear();
+    const std::int32_t seeds = st.seed_keys;
+    st = hora_live_stats{};
+    st.seed_keys = seeds;
+  }
+
+  int run(int32_t* holds, int cover) {
     const auto t0 = std::chrono::steady_clock::now();
-    const int er = build();
+    cover_am = cover == HORA_COVER_AM;
+    int er = build();
+    if (er < 0 && !cover_am) {
+      // the a_LU graph stands unless one of its discrete states can seed a deadlock atom
+      std::int32_t seeds = 0;
+      for (const auto& kv : chains) seeds += key_seeds(kv.first) ? 1 : 0;
+      st.seed_keys = seeds;
+      if (seeds > 0) {
+        reset_graph();
+        cover_am = true;
+        er = build();
+      }
+    }
+    st.route = cover_am ? 1 : 0;
     build_row_off();
     in_s.assign(dk.size(), 0);
     wne.assign(dk.size(), 0);
@@ -650,7 +718,7 @@ int oracle_live(const hora_net_enc* e, const hora_live_query* q, int32_t* holds,
     for (int c = 0; c < h->E->n.nch; ++c)
       if (h->E->n.bcast[c]) { g_err = "broadcast channels"; return HORA_HIP_E_SCOPE; }
     h->L = std::make_unique<Live>(*h->E, q);
-    const int rc = h->L->run(holds);
+    const int rc = h->L->run(holds, q->cover);
     if (st) *st = h->L->st;
     if (graph) *graph = h.release();
     return rc;

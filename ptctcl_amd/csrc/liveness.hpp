@@ -117,10 +117,10 @@ __global__ void k_gfill_rows(long long* __restrict__ grow, long from, long to, l
   if (v <= to) grow[v] = E;
 }
 
-// a_M bounds of a successor's location vector (state_bounds, engine.cpp:1487-1497, with L and U
-// raised to their maximum); `lg` holds the decoded locations
-__device__ __forceinline__ int m_bound(const DevNet& N, const int* lg, int c) {
-  if (c == 0) return 0;
+// Covering bounds of a location vector for clock c (`lg` holds the decoded locations): the a_LU
+// state bounds (state_bounds, engine.cpp:1487-1497), or with `alu` = 0 both raised to M = max(L, U).
+__device__ __forceinline__ void cover_bounds(const DevNet& N, const int* lg, int c, int alu, int* Lc, int* Uc) {
+  if (c == 0) { *Lc = 0; *Uc = 0; return; }
   int lb = kAluNegInf, ub = kAluNegInf;
   for (int p = 0; p < N.np; ++p) {
     const size_t b = (size_t)(N.locbase[p] + lg[p]) * N.d + c;
@@ -129,7 +129,9 @@ __device__ __forceinline__ int m_bound(const DevNet& N, const int* lg, int c) {
   }
   lb = max(lb, N.Lq[c]);
   ub = max(ub, N.Uq[c]);
-  return max(lb, ub);
+  if (!alu) lb = ub = max(lb, ub);
+  *Lc = lb;
+  *Uc = ub;
 }
 
 // The first node stored before this level that covers the successor: one wavefront per successor,
@@ -138,8 +140,9 @@ __device__ __forceinline__ int m_bound(const DevNet& N, const int* lg, int c) {
 __global__ __launch_bounds__(64) void k_gprobe(const DevNet* __restrict__ Np, Passed P, const long* __restrict__ gbk,
                                                const int* __restrict__ sj, const u64* __restrict__ sdk,
                                                const int* __restrict__ szone, long G, const int* __restrict__ gzone,
-                                               int* __restrict__ covered, int* __restrict__ cov_old, Counters* ctr) {
-  __shared__ int S[HORA_MAX_DIM * HORA_MAX_DIM], M[HORA_MAX_DIM], lg[HORA_MAX_PROCS];
+                                               int* __restrict__ covered, int* __restrict__ cov_old, Counters* ctr,
+                                               int alu) {
+  __shared__ int S[HORA_MAX_DIM * HORA_MAX_DIM], Lb[HORA_MAX_DIM], Ub[HORA_MAX_DIM], lg[HORA_MAX_PROCS];
   const DevNet& N = *Np;
   const int l = threadIdx.x;
   const long g = blockIdx.x;
@@ -154,14 +157,14 @@ __global__ __launch_bounds__(64) void k_gprobe(const DevNet* __restrict__ Np, Pa
     const int* src = szone + (size_t)sj[g] * d2;
     for (int e = l; e < d2; e += kWave) S[e] = src[e];
     wsync();
-    if (l < d) M[l] = m_bound(N, lg, l);
+    if (l < d) cover_bounds(N, lg, l, alu, &Lb[l], &Ub[l]);
     wsync();
     const u32* seg = P.nodes + P.off[h];
     for (u32 b = 0; b < len && found < 0; b += kWave) {
       const u32 m = b + (u32)l;
       bool ok = false;
       if (m < len) {
-        ok = alu_le_lane(S, gzone + (size_t)seg[m] * d2, d, M, M);
+        ok = alu_le_lane(S, gzone + (size_t)seg[m] * d2, d, Lb, Ub);
         ++reads;
       }
       const u64 bal = __ballot(ok);
@@ -183,13 +186,15 @@ __global__ __launch_bounds__(64) void k_gprobe(const DevNet* __restrict__ Np, Pa
 __global__ __launch_bounds__(64) void k_gmerge(const DevNet* __restrict__ Np, Passed P, const u32* __restrict__ touched,
                                                int T, int* __restrict__ wlist, const int* __restrict__ sj,
                                                const u64* __restrict__ sdk, const int* __restrict__ szone,
-                                               int* __restrict__ tslot, int* __restrict__ fnew, Counters* ctr) {
+                                               int* __restrict__ tslot, int* __restrict__ fnew, Counters* ctr,
+                                               int alu) {
   extern __shared__ int sm[];
   const DevNet& N = *Np;
   const int d = N.d, d2 = N.d2, l = threadIdx.x;
   int* S = sm;                       // d2
-  int* M = S + d2;                   // HORA_MAX_DIM
-  int* lg = M + HORA_MAX_DIM;        // HORA_MAX_PROCS
+  int* Lb = S + d2;                  // HORA_MAX_DIM
+  int* Ub = Lb + HORA_MAX_DIM;       // HORA_MAX_DIM
+  int* lg = Ub + HORA_MAX_DIM;       // HORA_MAX_PROCS
   int* lst = lg + HORA_MAX_PROCS;    // kSortLds
   const int t = blockIdx.x;
   if (t >= T) return;
@@ -215,7 +220,7 @@ __global__ __launch_bounds__(64) void k_gmerge(const DevNet* __restrict__ Np, Pa
     const u64 dk = sdk[sj[list[0]]];
     if (l < N.np) lg[l] = (int)((dk >> N.kshift[l]) & N.kmask[l]);
     wsync();
-    if (l < d) M[l] = m_bound(N, lg, l);
+    if (l < d) cover_bounds(N, lg, l, alu, &Lb[l], &Ub[l]);
   }
   int nc = 0;
   unsigned long long cov = 0, reads = 0;
@@ -231,7 +236,7 @@ __global__ __launch_bounds__(64) void k_gmerge(const DevNet* __restrict__ Np, Pa
       for (int cb = 0; cb < nc && first < 0; cb += kWave) {
         const int c = cb + l;
         bool ok = false;
-        if (c < nc) { ok = alu_le_lane(S, szone + (size_t)sj[list[c]] * d2, d, M, M); ++reads; }
+        if (c < nc) { ok = alu_le_lane(S, szone + (size_t)sj[list[c]] * d2, d, Lb, Ub); ++reads; }
         const u64 b = __ballot(ok);
         if (b) first = cb + __builtin_ctzll(b);
       }
@@ -312,8 +317,9 @@ __global__ __launch_bounds__(64) void k_gsort(Passed P, const u32* __restrict__ 
 __global__ __launch_bounds__(64) void k_gpair(const DevNet* __restrict__ Np, Passed P, const long long* __restrict__ wkey,
                                               long W, const int* __restrict__ wlist, const int* __restrict__ sj,
                                               const u64* __restrict__ sdk, const int* __restrict__ szone,
-                                              int* __restrict__ tslot, int* __restrict__ fnew, Counters* ctr) {
-  __shared__ int S[HORA_MAX_DIM * HORA_MAX_DIM], M[HORA_MAX_DIM], lg[HORA_MAX_PROCS];
+                                              int* __restrict__ tslot, int* __restrict__ fnew, Counters* ctr,
+                                              int alu) {
+  __shared__ int S[HORA_MAX_DIM * HORA_MAX_DIM], Lb[HORA_MAX_DIM], Ub[HORA_MAX_DIM], lg[HORA_MAX_PROCS];
   const long p = blockIdx.x;
   if (p >= W) return;
   const long long hh = wkey[p];
@@ -328,14 +334,14 @@ __global__ __launch_bounds__(64) void k_gpair(const DevNet* __restrict__ Np, Pas
   const int* src = szone + (size_t)sj[g] * d2;
   for (int e = l; e < d2; e += kWave) S[e] = src[e];
   wsync();
-  if (l < d) M[l] = m_bound(N, lg, l);
+  if (l < d) cover_bounds(N, lg, l, alu, &Lb[l], &Ub[l]);
   wsync();
   bool cov = false;
   unsigned long long reads = 0;
   for (int b = 0; b < k && !cov; b += kWave) {
     const int m = b + l;
     bool ok = false;
-    if (m < k) { ok = alu_le_lane(S, szone + (size_t)sj[wlist[base + m]] * d2, d, M, M); ++reads; }
+    if (m < k) { ok = alu_le_lane(S, szone + (size_t)sj[wlist[base + m]] * d2, d, Lb, Ub); ++reads; }
     cov = __ballot(ok) != 0;
   }
   wave_stat(ctr, C_READS, reads);
@@ -373,8 +379,8 @@ __global__ __launch_bounds__(64) void k_gcompact(Passed P, const u32* __restrict
 __global__ __launch_bounds__(64) void k_gfirst(const DevNet* __restrict__ Np, Passed P, const long* __restrict__ gbk,
                                                long G, const int* __restrict__ wlist, const int* __restrict__ sj,
                                                const u64* __restrict__ sdk, const int* __restrict__ szone,
-                                               int* __restrict__ tslot, Counters* ctr) {
-  __shared__ int S[HORA_MAX_DIM * HORA_MAX_DIM], M[HORA_MAX_DIM], lg[HORA_MAX_PROCS];
+                                               int* __restrict__ tslot, Counters* ctr, int alu) {
+  __shared__ int S[HORA_MAX_DIM * HORA_MAX_DIM], Lb[HORA_MAX_DIM], Ub[HORA_MAX_DIM], lg[HORA_MAX_PROCS];
   const long g = blockIdx.x;
   if (g >= G || tslot[g] != -2) return;
   const DevNet& N = *Np;
@@ -387,14 +393,14 @@ __global__ __launch_bounds__(64) void k_gfirst(const DevNet* __restrict__ Np, Pa
   const int* src = szone + (size_t)sj[g] * d2;
   for (int e = l; e < d2; e += kWave) S[e] = src[e];
   wsync();
-  if (l < d) M[l] = m_bound(N, lg, l);
+  if (l < d) cover_bounds(N, lg, l, alu, &Lb[l], &Ub[l]);
   wsync();
   int first = -1;
   unsigned long long reads = 0;
   for (int b = 0; b < nc && first < 0; b += kWave) {
     const int m = b + l;
     bool ok = false;
-    if (m < nc) { ok = alu_le_lane(S, szone + (size_t)sj[wlist[base + m]] * d2, d, M, M); ++reads; }
+    if (m < nc) { ok = alu_le_lane(S, szone + (size_t)sj[wlist[base + m]] * d2, d, Lb, Ub); ++reads; }
     const u64 bal = __ballot(ok);
     if (bal) first = b + __builtin_ctzll(bal);
   }
@@ -467,6 +473,14 @@ __global__ __launch_bounds__(64) void k_glink(Passed P, const u32* __restrict__ 
     P.len[h] = len + nc;
     P.ncnt[h] = 0;
   }
+}
+
+// the first node of every discrete state of the graph
+__global__ void k_heads(Passed P, long H, int* __restrict__ heads, LiveCtr* lc) {
+  const long h = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = h < H && P.key[h] != kEmptyKey && P.len[h] > 0;
+  const int idx = wave_append(&lc->ns, ok);
+  if (ok) heads[idx] = (int)P.nodes[P.off[h]];
 }
 
 // ---------------------------------------------------------------- stop sets and deadlock
@@ -564,8 +578,9 @@ struct LiveLds {
 };
 
 // Stop set of node v (mode 0), early witness test (mode 1: p /\ !q nodes only), deadlock test
-// (mode 2). Writes scnt (mode 0): -1 whole zone, else the number of zones in sz; sets ctr->hit
-// for a non-empty answer in modes 1 and 2.
+// (mode 2), seed test of v's discrete state (mode 3). Writes scnt (mode 0): -1 whole zone, else the
+// number of zones in sz; sets ctr->hit for a non-empty answer in modes 1 and 2; counts the seeds
+// in ctr->hit in mode 3.
 __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, const LiveArgs* __restrict__ Ap,
                                              const int* __restrict__ list, long n, long long first_id, int mode,
                                              int* __restrict__ scnt, int* __restrict__ sz, LiveCtr* lc) {
@@ -604,7 +619,6 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
   if (ninv > kInvCap) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); return; }
   const bool frozen = fz != 0, src_c = (fz & 1) != 0;
   const int* zv = A.gzone + (size_t)v * d2;
-  const u64 cm = ((1ull << N.nclocks) - 1) << 1;
   auto out_whole = [&]() __attribute__((always_inline)) {
     if (mode == 0) { if (l == 0) scnt[A.sidx[v]] = -1; }
     else if (l == 0) atomicExch(&lc->hit, 1);
@@ -626,7 +640,20 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
   // the table the deadlock is judged in: the unrestricted network's when psi restricted it
   const int* doff = A.restricted ? A.oinv_off : N.inv_off;
   const int* ddat = A.restricted ? A.oinv_data : N.inv_data;
-  if (mode != 2) {   // the stop set proper (stop_set); mode 2 is the plain deadlock test
+  if (mode == 3) {
+    // seed test of the discrete state (online_seed_key, refinement.cpp:176-282): the universe
+    // zone, judged in the unrestricted network
+    const int no = wave_gather_inv(N, doff, ddat, locs, tinv);
+    if (no > kInvCap) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); return; }
+    if (!frozen && A.kind != HORA_LIVE_DEADLOCK && !has_upper_list(tinv, no)) return;
+    wsync();
+    for (int e = l; e < d2; e += kWave) {
+      const int i = e / d, j = e - i * d;
+      zu[e] = (i == j || i == 0) ? kLe0 : kInf;
+    }
+    zv = zu;
+  }
+  if (mode == 0 || mode == 1) {   // the stop set proper (stop_set); modes 2, 3 test deadlock only
     if (!A.restricted) {
       if (A.grow[v + 1] == A.grow[v] || time_diverges()) { out_whole(); return; }
     } else {
@@ -655,6 +682,13 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
   zcopy_w(Zd, zv, d2);
   if (!frozen) z_up(Zd, d);
   if (!z_apply_list(Zd, d, tinv, nd)) { out_none(); return; }
+  if (mode == 3 && !frozen && A.kind != HORA_LIVE_DEADLOCK) {
+    // a time-open state: every valuation delays for ever, so no deadlock completion to refine
+    wsync();
+    bool bounded = false;
+    if (l >= 1 && l < d) bounded = Zd[l * d] != kInf;
+    if (!__any(bounded)) return;
+  }
   const int nr = build_recipes(N, locs, ints, src_c, rec, kRecipeCap);
   if (nr < 0) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); return; }
   wsync();
@@ -720,6 +754,13 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
     nc = nn;
   }
   if (nc == 0) { out_none(); return; }
+  if (mode == 3) {
+    // a seed iff some escape meets the invariant region: otherwise Dead = Inv, nothing to refine
+    bool meets = false;
+    for (int m = 0; m < nl && !meets; ++m) meets = z_intersect(tmp, live + m * d2, Zd, d);
+    if (meets && l == 0) atomicAdd(&lc->hit, 1);
+    return;
+  }
   if (mode != 0) { if (l == 0) atomicExch(&lc->hit, 1); return; }
   if (nc > kStopCap) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); return; }
   const int s = A.sidx[v];
@@ -1020,7 +1061,7 @@ struct LiveState {
   DBuf<signed char> expanded, wne;
   long long nodes = 0, edges = 0;
   // backward pass
-  DBuf<int> in_s, slist, sidx, scnt, sz, wcnt0, wcnt1, wz0, wz1, sel0, sel1, chg0, chg1;
+  DBuf<int> in_s, slist, sidx, scnt, sz, wcnt0, wcnt1, wz0, wz1, sel0, sel1, chg0, chg1, heads;
   int ns = 0;
   ~LiveState() {
     if (argsp) { (void)hipDeviceSynchronize(); (void)hipFree(argsp); }
@@ -1120,239 +1161,292 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
   L.lc.ensure(1);
   HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
 
-  c->dn.lu_plus = 0;
-  HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
-  alloc_table(c, 16);
-  c->ctr.ensure(1);
-  HIP_OK(hipMemsetAsync(c->ctr.p, 0, sizeof(Counters), c->stream));
-  {
-    const int ft = 0x7fffffff;
-    HIP_OK(hipMemcpyAsync(&c->ctr.p->first_target, &ft, sizeof(int), hipMemcpyHostToDevice, c->stream));
-  }
-  // graph storage
-  size_t ncap = 1 << 16, ecap = 1 << 18;
-  L.gdk.ensure(ncap); L.grow.ensure(ncap + 1); L.expanded.ensure(ncap);
-  c->nodes.ensure(1 << 20);
-  u64 ncount = 0;   // node-segment entries handed out so far
-  c->zarena.ensure(ncap * d2);
-  L.etgt.ensure(ecap); L.ese.ensure(ecap); L.ere.ensure(ecap);
-  HIP_OK(hipMemsetAsync(L.expanded.p, 0, L.expanded.cap, c->stream));
-  L.nodes = 0;
-  L.edges = 0;
-  L.ns = 0;
-
-  const bool eg = q->kind == HORA_LIVE_EG;
-  u64 dk0 = 0;
-  for (int p = 0; p < c->np; ++p) dk0 |= (u64)(unsigned)c->h_loc_init[p] << c->h_kshift[p];
-  for (int v = 0; v < c->ni; ++v) dk0 |= (u64)(unsigned)(c->h_int_init[v] - c->h_int_lo[v]) << c->h_kshift[c->np + v];
-  c->fr_dk.ensure(1);
-  c->fr_zid.ensure(1);
-  HIP_OK(hipMemcpyAsync(c->fr_dk.p, &dk0, sizeof(u64), hipMemcpyHostToDevice, c->stream));
-
-  long F = 1;
-  long long fbase = 0, rows_done = -1;   // rows [0, rows_done] written
-  bool init = true;
+  // One construction of the graph under a_M covers (am) or a_LU covers. Returns HORA_HIP_OK or an
+  // error code; `early` receives the verdict of an early witness, or -1.
   int early = -1;
-  Tally prev;
-  for (;;) {
-    sync_graph_args(L);
-    upload_args(c, L);
-    long R = 1;
-    c->rec_cnt.ensure(F);
-    c->rec_off.ensure(F);
-    c->pos.ensure(F + 1);
-    if (!init) {
-      ++st->levels;
-      const size_t lds_c = (HORA_MAX_PROCS + HORA_MAX_INTS + 2 * kRecipeCap + 3 * kInvCap) * sizeof(int);
-      hipLaunchKernelGGL(k_count, dim3(F), dim3(64), lds_c, c->stream, c->dnp, c->fr_dk.p, (int)F, c->rec_cnt.p, c->ctr.p);
-      hipLaunchKernelGGL(k_gfilter, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, eg ? 1 : 0,
-                         c->fr_dk.p, (int)F, c->rec_cnt.p, L.expanded.p, (long)fbase);
-      R = scan_exclusive(c, c->rec_cnt.p, F, c->pos.p);
-      hipLaunchKernelGGL(k_offsets_to_int, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->pos.p, F, c->rec_off.p);
-    } else {
-      HIP_OK(hipMemsetAsync(c->rec_off.p, 0, sizeof(int), c->stream));
-    }
-    c->s_valid.ensure(R);
-    c->s_tgt.ensure(R);
-    c->s_dk.ensure(R);
-    c->s_zone.ensure((size_t)R * d2);
-    L.s_se.ensure(R);
-    L.s_re.ensure(R);
-    ExpandOut eo{c->s_valid.p, c->s_dk.p, c->s_zone.p, c->s_tgt.p, L.s_se.p, L.s_re.p};
-    if (R > 0) launch_expand(c, expand_tuned(c, F, eo, init), F, eo, init);
-    c->pos.ensure(R + 1);
-    const long G = R > 0 ? scan_exclusive(c, c->s_valid.p, R, c->pos.p) : 0;
-    if (!init) {
-      st->expanded_nodes += F;
-      st->generated += G;
-      grow_keep_to(L.grow, (size_t)(fbase + F + 1), c->stream);
-      hipLaunchKernelGGL(k_grows, dim3(blocks_for(F + 1, 256)), dim3(256), 0, c->stream, c->rec_off.p, c->pos.p, R,
-                         (int)F, (long)fbase, L.edges, (long long)G, L.grow.p);
-      rows_done = fbase + F;
-    }
+  long long rows_done = -1;
+  auto construct = [&](bool am) -> int {
+    const int alu = am ? 0 : 1;
+    early = -1;
+    rows_done = -1;
+    HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));   // a seed test may have counted hits
     {
-      const Tally hc = read_counters(c);
-      if (hc.err) { c->err = "a state has more transitions or invariant constraints than the device holds"; return HORA_HIP_E_MEMORY; }
+      const int32_t seeds = st->seed_keys;
+      std::memset(st, 0, sizeof(*st));
+      st->seed_keys = seeds;
     }
-    long F2 = 0;
-    if (G > 0) {
-      c->sj.ensure(G);
-      hipLaunchKernelGGL(k_compact_valid, dim3(blocks_for(R, 256)), dim3(256), 0, c->stream, c->s_valid.p, c->pos.p, R, c->sj.p);
+    c->dn.lu_plus = 0;
+    HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
+    alloc_table(c, 16);
+    c->ctr.ensure(1);
+    HIP_OK(hipMemsetAsync(c->ctr.p, 0, sizeof(Counters), c->stream));
+    {
+      const int ft = 0x7fffffff;
+      HIP_OK(hipMemcpyAsync(&c->ctr.p->first_target, &ft, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    }
+    // graph storage
+    size_t ncap = 1 << 16, ecap = 1 << 18;
+    L.gdk.ensure(ncap); L.grow.ensure(ncap + 1); L.expanded.ensure(ncap);
+    c->nodes.ensure(1 << 20);
+    u64 ncount = 0;   // node-segment entries handed out so far
+    c->zarena.ensure(ncap * d2);
+    L.etgt.ensure(ecap); L.ese.ensure(ecap); L.ere.ensure(ecap);
+    HIP_OK(hipMemsetAsync(L.expanded.p, 0, L.expanded.cap, c->stream));
+    L.nodes = 0;
+    L.edges = 0;
+    L.ns = 0;
+  
+    const bool eg = q->kind == HORA_LIVE_EG;
+    u64 dk0 = 0;
+    for (int p = 0; p < c->np; ++p) dk0 |= (u64)(unsigned)c->h_loc_init[p] << c->h_kshift[p];
+    for (int v = 0; v < c->ni; ++v) dk0 |= (u64)(unsigned)(c->h_int_init[v] - c->h_int_lo[v]) << c->h_kshift[c->np + v];
+    c->fr_dk.ensure(1);
+    c->fr_zid.ensure(1);
+    HIP_OK(hipMemcpyAsync(c->fr_dk.p, &dk0, sizeof(u64), hipMemcpyHostToDevice, c->stream));
+  
+    long F = 1;
+    long long fbase = 0;   // rows [0, rows_done] written
+    bool init = true;
+    Tally prev;
+    for (;;) {
+      sync_graph_args(L);
+      upload_args(c, L);
+      long R = 1;
+      c->rec_cnt.ensure(F);
+      c->rec_off.ensure(F);
+      c->pos.ensure(F + 1);
+      if (!init) {
+        ++st->levels;
+        const size_t lds_c = (HORA_MAX_PROCS + HORA_MAX_INTS + 2 * kRecipeCap + 3 * kInvCap) * sizeof(int);
+        hipLaunchKernelGGL(k_count, dim3(F), dim3(64), lds_c, c->stream, c->dnp, c->fr_dk.p, (int)F, c->rec_cnt.p, c->ctr.p);
+        hipLaunchKernelGGL(k_gfilter, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, eg ? 1 : 0,
+                           c->fr_dk.p, (int)F, c->rec_cnt.p, L.expanded.p, (long)fbase);
+        R = scan_exclusive(c, c->rec_cnt.p, F, c->pos.p);
+        hipLaunchKernelGGL(k_offsets_to_int, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->pos.p, F, c->rec_off.p);
+      } else {
+        HIP_OK(hipMemsetAsync(c->rec_off.p, 0, sizeof(int), c->stream));
+      }
+      c->s_valid.ensure(R);
+      c->s_tgt.ensure(R);
+      c->s_dk.ensure(R);
+      c->s_zone.ensure((size_t)R * d2);
+      L.s_se.ensure(R);
+      L.s_re.ensure(R);
+      ExpandOut eo{c->s_valid.p, c->s_dk.p, c->s_zone.p, c->s_tgt.p, L.s_se.p, L.s_re.p};
+      if (R > 0) launch_expand(c, expand_tuned(c, F, eo, init), F, eo, init);
+      c->pos.ensure(R + 1);
+      const long G = R > 0 ? scan_exclusive(c, c->s_valid.p, R, c->pos.p) : 0;
+      if (!init) {
+        st->expanded_nodes += F;
+        st->generated += G;
+        grow_keep_to(L.grow, (size_t)(fbase + F + 1), c->stream);
+        hipLaunchKernelGGL(k_grows, dim3(blocks_for(F + 1, 256)), dim3(256), 0, c->stream, c->rec_off.p, c->pos.p, R,
+                           (int)F, (long)fbase, L.edges, (long long)G, L.grow.p);
+        rows_done = fbase + F;
+      }
       {
         const Tally hc = read_counters(c);
-        while ((double)(hc[C_KEYS] + (u64)G) > 0.6 * (double)c->H) grow_table(c);
+        if (hc.err) { c->err = "a state has more transitions or invariant constraints than the device holds"; return HORA_HIP_E_MEMORY; }
       }
-      Passed P = passed_view(c);
-      c->gbk.ensure(G);
-      c->gticket.ensure(G);
-      c->touched.ensure(G);
-      c->covered.ensure(G);
-      L.cov_old.ensure(G);
-      L.tslot.ensure(G);
-      L.fnew.ensure(G);
-      HIP_OK(hipMemsetAsync(L.fnew.p, 0, (size_t)G * sizeof(int), c->stream));
-      HIP_OK(hipMemsetAsync(L.tslot.p, 0xff, (size_t)G * sizeof(int), c->stream));
-      HIP_OK(hipMemsetAsync(&c->ctr.p->ntouched, 0, sizeof(int), c->stream));
-      hipLaunchKernelGGL(k_find, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->sj.p, c->s_dk.p, G, c->gbk.p, c->ctr.p);
-      HIP_OK(hipEventRecord(c->ev[1], c->stream));
-      hipLaunchKernelGGL(k_gprobe, dim3(G), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p, c->s_dk.p,
-                         c->s_zone.p, G, c->zarena.p, c->covered.p, L.cov_old.p, c->ctr.p);
-      hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p, G,
-                         c->gticket.p, c->touched.p, c->ctr.p);
-      int T = 0;
-      HIP_OK(hipMemcpyAsync(&T, &c->ctr.p->ntouched, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      HIP_OK(hipStreamSynchronize(c->stream));
-      if (T > 0) {
-        c->caps.ensure(T);
-        c->segs.ensure(T);
-        hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
-                           c->caps.p, c->segs.p);
-        c->pos2.ensure(T + 1);
-        const long long Wl = scan_exclusive(c, c->caps.p, T, c->pos2.p);
-        hipLaunchKernelGGL(k_gsegs, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T, c->segs.p);
-        L.segpos.ensure(T + 1);
-        const long long Sseg = scan_exclusive(c, c->segs.p, T, L.segpos.p);
-        if (ncount + (u64)Sseg > c->nodes.cap) c->nodes.grow_keep((size_t)(ncount + (u64)Sseg) * 2, c->stream);
-        P = passed_view(c);
-        hipLaunchKernelGGL(k_touched_offsets, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
-                           c->pos2.p, L.segpos.p, ncount);
-        ncount += (u64)Sseg;
-        c->wlist.ensure(Wl + 1);
-        hipLaunchKernelGGL(k_scatter, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p,
-                           c->gticket.p, G, c->wlist.p);
-        const size_t lds_m = (d2 + HORA_MAX_DIM + HORA_MAX_PROCS + kSortLds) * sizeof(int);
-        HIP_OK(hipEventRecord(c->ev[2], c->stream));
-        c->klight.ensure(T);
-        c->kheavy.ensure(T);
-        HIP_OK(hipMemsetAsync(&c->ctr.p->nlight, 0, 2 * sizeof(int), c->stream));
-        hipLaunchKernelGGL(k_gsplit, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
-                           c->klight.p, c->kheavy.p, c->ctr.p);
-        int nlh[2] = {0, 0};
-        HIP_OK(hipMemcpyAsync(nlh, &c->ctr.p->nlight, sizeof(nlh), hipMemcpyDeviceToHost, c->stream));
-        HIP_OK(hipStreamSynchronize(c->stream));
-        if (nlh[0] > 0)
-          hipLaunchKernelGGL(k_gmerge, dim3(nlh[0]), dim3(64), lds_m, c->stream, c->dnp, P, c->klight.p, nlh[0],
-                             c->wlist.p, c->sj.p, c->s_dk.p, c->s_zone.p, L.tslot.p, L.fnew.p, c->ctr.p);
-        if (nlh[1] > 0) {
-          L.wkey.ensure(Wl + 1);
-          HIP_OK(hipMemsetAsync(L.wkey.p, 0xff, (size_t)(Wl + 1) * sizeof(long long), c->stream));
-          hipLaunchKernelGGL(k_gsort, dim3(nlh[1]), dim3(64), kSortLds * sizeof(int), c->stream, P, c->kheavy.p,
-                             nlh[1], c->wlist.p, L.wkey.p);
-          hipLaunchKernelGGL(k_gpair, dim3(Wl), dim3(64), 0, c->stream, c->dnp, P, L.wkey.p, (long)Wl, c->wlist.p,
-                             c->sj.p, c->s_dk.p, c->s_zone.p, L.tslot.p, L.fnew.p, c->ctr.p);
-          hipLaunchKernelGGL(k_gcompact, dim3(nlh[1]), dim3(64), 0, c->stream, P, c->kheavy.p, nlh[1], c->wlist.p,
-                             L.fnew.p);
-          hipLaunchKernelGGL(k_gfirst, dim3(G), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, (long)G, c->wlist.p,
-                             c->sj.p, c->s_dk.p, c->s_zone.p, L.tslot.p, c->ctr.p);
+      long F2 = 0;
+      if (G > 0) {
+        c->sj.ensure(G);
+        hipLaunchKernelGGL(k_compact_valid, dim3(blocks_for(R, 256)), dim3(256), 0, c->stream, c->s_valid.p, c->pos.p, R, c->sj.p);
+        {
+          const Tally hc = read_counters(c);
+          while ((double)(hc[C_KEYS] + (u64)G) > 0.6 * (double)c->H) grow_table(c);
         }
-      }
-      c->pos.ensure(G + 1);
-      F2 = scan_exclusive(c, L.fnew.p, G, c->pos.p);
-      // capacities
-      const long long nbase = L.nodes;
-      const size_t nneed = (size_t)(nbase + F2 + 1);
-      if (nneed > L.gdk.cap) {
-        const size_t nc2 = std::max(nneed, (size_t)L.gdk.cap * 2);
-        grow_keep_to(L.gdk, nc2, c->stream);
-        grow_keep_to(L.grow, nc2 + 1, c->stream);
-        const size_t old = L.expanded.cap;
-        grow_keep_to(L.expanded, nc2, c->stream);
-        HIP_OK(hipMemsetAsync(L.expanded.p + old, 0, L.expanded.cap - old, c->stream));
-      }
-      if (nneed * d2 > c->zarena.cap) c->zarena.grow_keep(std::max(nneed * d2, c->zarena.cap * 2), c->stream);
-      if (!init && (size_t)(L.edges + G) > L.etgt.cap) {
-        const size_t ec2 = std::max((size_t)(L.edges + G), L.etgt.cap * 2);
-        grow_keep_to(L.etgt, ec2, c->stream);
-        grow_keep_to(L.ese, ec2, c->stream);
-        grow_keep_to(L.ere, ec2, c->stream);
-      }
-      L.newlist.ensure(std::max<long>(F2, 1));
-      HIP_OK(hipEventRecord(c->ev[3], c->stream));
-      hipLaunchKernelGGL(k_gassign, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, G, nbase, L.edges, init ? 0 : 1,
-                         L.cov_old.p, L.tslot.p, L.fnew.p, c->pos.p, c->sj.p, L.s_se.p, L.s_re.p, L.etgt.p, L.ese.p,
-                         L.ere.p, L.newlist.p);
-      c->fr_dk2.ensure(std::max<long>(F2, 1));
-      c->fr_zid2.ensure(std::max<long>(F2, 1));
-      if (F2 > 0)
-        hipLaunchKernelGGL(k_gcopy, dim3(F2), dim3(64), 0, c->stream, (long)F2, nbase, L.newlist.p, c->sj.p, c->s_dk.p,
-                           c->s_zone.p, d2, c->zarena.p, L.gdk.p, c->fr_dk2.p, c->fr_zid2.p);
-      if (T > 0)
-        hipLaunchKernelGGL(k_glink, dim3(T), dim3(64), 0, c->stream, P, c->touched.p, T, c->wlist.p, c->pos.p, nbase);
-      HIP_OK(hipEventRecord(c->ev[4], c->stream));
-      if (c->trace) {
+        Passed P = passed_view(c);
+        c->gbk.ensure(G);
+        c->gticket.ensure(G);
+        c->touched.ensure(G);
+        c->covered.ensure(G);
+        L.cov_old.ensure(G);
+        L.tslot.ensure(G);
+        L.fnew.ensure(G);
+        HIP_OK(hipMemsetAsync(L.fnew.p, 0, (size_t)G * sizeof(int), c->stream));
+        HIP_OK(hipMemsetAsync(L.tslot.p, 0xff, (size_t)G * sizeof(int), c->stream));
+        HIP_OK(hipMemsetAsync(&c->ctr.p->ntouched, 0, sizeof(int), c->stream));
+        hipLaunchKernelGGL(k_find, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->sj.p, c->s_dk.p, G, c->gbk.p, c->ctr.p);
+        HIP_OK(hipEventRecord(c->ev[1], c->stream));
+        hipLaunchKernelGGL(k_gprobe, dim3(G), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p, c->s_dk.p,
+                           c->s_zone.p, G, c->zarena.p, c->covered.p, L.cov_old.p, c->ctr.p, alu);
+        hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p, G,
+                           c->gticket.p, c->touched.p, c->ctr.p);
+        int T = 0;
+        HIP_OK(hipMemcpyAsync(&T, &c->ctr.p->ntouched, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));
-        const Tally ht = read_counters(c);
-        std::fprintf(stderr, "live level %lld F=%ld R=%ld G=%ld T=%d F2=%ld | probe %.3f (covered %llu, reads %llu) merge %.3f assign+link %.3f ms\n",
-                     (long long)st->levels, F, R, G, T, F2, ev_ms(c->ev[1], c->ev[2]), ht[C_PSUB], ht[C_PREADS],
-                     T > 0 ? ev_ms(c->ev[2], c->ev[3]) : 0.0f, ev_ms(c->ev[3], c->ev[4]));
+        if (T > 0) {
+          c->caps.ensure(T);
+          c->segs.ensure(T);
+          hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                             c->caps.p, c->segs.p);
+          c->pos2.ensure(T + 1);
+          const long long Wl = scan_exclusive(c, c->caps.p, T, c->pos2.p);
+          hipLaunchKernelGGL(k_gsegs, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T, c->segs.p);
+          L.segpos.ensure(T + 1);
+          const long long Sseg = scan_exclusive(c, c->segs.p, T, L.segpos.p);
+          if (ncount + (u64)Sseg > c->nodes.cap) c->nodes.grow_keep((size_t)(ncount + (u64)Sseg) * 2, c->stream);
+          P = passed_view(c);
+          hipLaunchKernelGGL(k_touched_offsets, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                             c->pos2.p, L.segpos.p, ncount);
+          ncount += (u64)Sseg;
+          c->wlist.ensure(Wl + 1);
+          hipLaunchKernelGGL(k_scatter, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p,
+                             c->gticket.p, G, c->wlist.p);
+          const size_t lds_m = (d2 + 2 * HORA_MAX_DIM + HORA_MAX_PROCS + kSortLds) * sizeof(int);
+          HIP_OK(hipEventRecord(c->ev[2], c->stream));
+          c->klight.ensure(T);
+          c->kheavy.ensure(T);
+          HIP_OK(hipMemsetAsync(&c->ctr.p->nlight, 0, 2 * sizeof(int), c->stream));
+          hipLaunchKernelGGL(k_gsplit, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                             c->klight.p, c->kheavy.p, c->ctr.p);
+          int nlh[2] = {0, 0};
+          HIP_OK(hipMemcpyAsync(nlh, &c->ctr.p->nlight, sizeof(nlh), hipMemcpyDeviceToHost, c->stream));
+          HIP_OK(hipStreamSynchronize(c->stream));
+          if (nlh[0] > 0)
+            hipLaunchKernelGGL(k_gmerge, dim3(nlh[0]), dim3(64), lds_m, c->stream, c->dnp, P, c->klight.p, nlh[0],
+                               c->wlist.p, c->sj.p, c->s_dk.p, c->s_zone.p, L.tslot.p, L.fnew.p, c->ctr.p, alu);
+          if (nlh[1] > 0) {
+            L.wkey.ensure(Wl + 1);
+            HIP_OK(hipMemsetAsync(L.wkey.p, 0xff, (size_t)(Wl + 1) * sizeof(long long), c->stream));
+            hipLaunchKernelGGL(k_gsort, dim3(nlh[1]), dim3(64), kSortLds * sizeof(int), c->stream, P, c->kheavy.p,
+                               nlh[1], c->wlist.p, L.wkey.p);
+            hipLaunchKernelGGL(k_gpair, dim3(Wl), dim3(64), 0, c->stream, c->dnp, P, L.wkey.p, (long)Wl, c->wlist.p,
+                               c->sj.p, c->s_dk.p, c->s_zone.p, L.tslot.p, L.fnew.p, c->ctr.p, alu);
+            hipLaunchKernelGGL(k_gcompact, dim3(nlh[1]), dim3(64), 0, c->stream, P, c->kheavy.p, nlh[1], c->wlist.p,
+                               L.fnew.p);
+            hipLaunchKernelGGL(k_gfirst, dim3(G), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, (long)G, c->wlist.p,
+                               c->sj.p, c->s_dk.p, c->s_zone.p, L.tslot.p, c->ctr.p, alu);
+          }
+        }
+        c->pos.ensure(G + 1);
+        F2 = scan_exclusive(c, L.fnew.p, G, c->pos.p);
+        // capacities
+        const long long nbase = L.nodes;
+        const size_t nneed = (size_t)(nbase + F2 + 1);
+        if (nneed > L.gdk.cap) {
+          const size_t nc2 = std::max(nneed, (size_t)L.gdk.cap * 2);
+          grow_keep_to(L.gdk, nc2, c->stream);
+          grow_keep_to(L.grow, nc2 + 1, c->stream);
+          const size_t old = L.expanded.cap;
+          grow_keep_to(L.expanded, nc2, c->stream);
+          HIP_OK(hipMemsetAsync(L.expanded.p + old, 0, L.expanded.cap - old, c->stream));
+        }
+        if (nneed * d2 > c->zarena.cap) c->zarena.grow_keep(std::max(nneed * d2, c->zarena.cap * 2), c->stream);
+        if (!init && (size_t)(L.edges + G) > L.etgt.cap) {
+          const size_t ec2 = std::max((size_t)(L.edges + G), L.etgt.cap * 2);
+          grow_keep_to(L.etgt, ec2, c->stream);
+          grow_keep_to(L.ese, ec2, c->stream);
+          grow_keep_to(L.ere, ec2, c->stream);
+        }
+        L.newlist.ensure(std::max<long>(F2, 1));
+        HIP_OK(hipEventRecord(c->ev[3], c->stream));
+        hipLaunchKernelGGL(k_gassign, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, G, nbase, L.edges, init ? 0 : 1,
+                           L.cov_old.p, L.tslot.p, L.fnew.p, c->pos.p, c->sj.p, L.s_se.p, L.s_re.p, L.etgt.p, L.ese.p,
+                           L.ere.p, L.newlist.p);
+        c->fr_dk2.ensure(std::max<long>(F2, 1));
+        c->fr_zid2.ensure(std::max<long>(F2, 1));
+        if (F2 > 0)
+          hipLaunchKernelGGL(k_gcopy, dim3(F2), dim3(64), 0, c->stream, (long)F2, nbase, L.newlist.p, c->sj.p, c->s_dk.p,
+                             c->s_zone.p, d2, c->zarena.p, L.gdk.p, c->fr_dk2.p, c->fr_zid2.p);
+        if (T > 0)
+          hipLaunchKernelGGL(k_glink, dim3(T), dim3(64), 0, c->stream, P, c->touched.p, T, c->wlist.p, c->pos.p, nbase);
+        HIP_OK(hipEventRecord(c->ev[4], c->stream));
+        if (c->trace) {
+          HIP_OK(hipStreamSynchronize(c->stream));
+          const Tally ht = read_counters(c);
+          std::fprintf(stderr, "live level %lld F=%ld R=%ld G=%ld T=%d F2=%ld | probe %.3f (covered %llu, reads %llu) merge %.3f assign+link %.3f ms\n",
+                       (long long)st->levels, F, R, G, T, F2, ev_ms(c->ev[1], c->ev[2]), ht[C_PSUB], ht[C_PREADS],
+                       T > 0 ? ev_ms(c->ev[2], c->ev[3]) : 0.0f, ev_ms(c->ev[3], c->ev[4]));
+        }
+        if (!init) {
+          L.edges += G;
+          st->fwd_edges = L.edges;
+        }
+        L.nodes += F2;
       }
-      if (!init) {
-        L.edges += G;
-        st->fwd_edges = L.edges;
+      // early witness on the level just expanded (leadsto_witness / the deadlock scan)
+      if (!init && q->early && (q->kind == HORA_LIVE_LEADSTO || q->kind == HORA_LIVE_DEADLOCK)) {
+        sync_graph_args(L);
+        L.args.gzone = c->zarena.p;
+        upload_args(c, L);
+        const LiveLds LL{d2, L.args.kl, L.args.kp, L.args.kc};
+        hipLaunchKernelGGL(k_stop, dim3(F), dim3(64), LL.stop_ints() * sizeof(int), c->stream, c->dnp, L.argsp,
+                           (const int*)nullptr, (long)F, (long long)fbase, q->kind == HORA_LIVE_LEADSTO ? 1 : 2,
+                           (int*)nullptr, (int*)nullptr, L.lc.p);
+        const LiveCtr h = read_lc(c, L);
+        if (h.err) {
+          c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a stop set" : "invariant or transition table too large";
+          return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
+        }
+        if (h.hit) { early = 0; st->early = 1; }
       }
-      L.nodes += F2;
+      {
+        const Tally hc = read_counters(c);
+        if (hc.err) { c->err = hc.err == ERR_HASH_FULL ? "hash table full" : "device capacity exceeded"; return HORA_HIP_E_MEMORY; }
+        st->covered = (long long)(hc[C_SUB] + hc[C_PSUB]);
+        st->chain_reads = (long long)(hc[C_READS] + hc[C_PREADS]);
+        st->disc_keys = (long long)hc[C_KEYS];
+        prev = hc;
+      }
+      if (early >= 0) break;
+      std::swap(c->fr_dk.p, c->fr_dk2.p); std::swap(c->fr_dk.cap, c->fr_dk2.cap);
+      std::swap(c->fr_zid.p, c->fr_zid2.p); std::swap(c->fr_zid.cap, c->fr_zid2.cap);
+      fbase = init ? 0 : fbase + F;
+      F = F2;
+      init = false;
+      if (F == 0) break;
     }
-    // early witness on the level just expanded (leadsto_witness / the deadlock scan)
-    if (!init && q->early && (q->kind == HORA_LIVE_LEADSTO || q->kind == HORA_LIVE_DEADLOCK)) {
-      sync_graph_args(L);
-      L.args.gzone = c->zarena.p;
-      upload_args(c, L);
+    // rows of nodes never expanded
+    if (rows_done < L.nodes) {
+      grow_keep_to(L.grow, (size_t)(L.nodes + 1), c->stream);
+      const long from = (long)(rows_done + 1);
+      hipLaunchKernelGGL(k_gfill_rows, dim3(blocks_for(L.nodes - from + 1, 256)), dim3(256), 0, c->stream, L.grow.p, from,
+                         (long)L.nodes, L.edges);
+    }
+    st->outer_nodes = L.nodes;
+    st->fwd_edges = L.edges;
+    return HORA_HIP_OK;
+  };
+  const bool auto_route = q->cover == HORA_COVER_AUTO;
+  {
+    const int rc = construct(!auto_route);
+    if (rc != HORA_HIP_OK) return rc;
+  }
+  if (auto_route && early < 0 && L.nodes > 0) {
+    // the a_LU graph stands unless one of its discrete states can seed a deadlock atom
+    sync_graph_args(L);
+    L.args.gzone = c->zarena.p;
+    upload_args(c, L);
+    HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
+    L.heads.ensure((size_t)std::max<u64>(c->H, 1));
+    Passed P = passed_view(c);
+    hipLaunchKernelGGL(k_heads, dim3(blocks_for((long)c->H, 256)), dim3(256), 0, c->stream, P, (long)c->H, L.heads.p,
+                       L.lc.p);
+    const int nh = read_lc(c, L).ns;
+    int seeds = 0;
+    if (nh > 0) {
+      HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
       const LiveLds LL{d2, L.args.kl, L.args.kp, L.args.kc};
-      hipLaunchKernelGGL(k_stop, dim3(F), dim3(64), LL.stop_ints() * sizeof(int), c->stream, c->dnp, L.argsp,
-                         (const int*)nullptr, (long)F, (long long)fbase, q->kind == HORA_LIVE_LEADSTO ? 1 : 2,
-                         (int*)nullptr, (int*)nullptr, L.lc.p);
+      hipLaunchKernelGGL(k_stop, dim3(nh), dim3(64), LL.stop_ints() * sizeof(int), c->stream, c->dnp, L.argsp,
+                         L.heads.p, (long)nh, (long long)0, 3, (int*)nullptr, (int*)nullptr, L.lc.p);
       const LiveCtr h = read_lc(c, L);
       if (h.err) {
-        c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a stop set" : "invariant or transition table too large";
+        c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a seed test" : "invariant or transition table too large";
         return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
       }
-      if (h.hit) { early = 0; st->early = 1; }
+      seeds = h.hit;
     }
-    {
-      const Tally hc = read_counters(c);
-      if (hc.err) { c->err = hc.err == ERR_HASH_FULL ? "hash table full" : "device capacity exceeded"; return HORA_HIP_E_MEMORY; }
-      st->covered = (long long)(hc[C_SUB] + hc[C_PSUB]);
-      st->chain_reads = (long long)(hc[C_READS] + hc[C_PREADS]);
-      st->disc_keys = (long long)hc[C_KEYS];
-      prev = hc;
+    st->seed_keys = seeds;
+    if (seeds > 0) {
+      const int rc = construct(true);
+      if (rc != HORA_HIP_OK) return rc;
+      st->route = 1;
     }
-    if (early >= 0) break;
-    std::swap(c->fr_dk.p, c->fr_dk2.p); std::swap(c->fr_dk.cap, c->fr_dk2.cap);
-    std::swap(c->fr_zid.p, c->fr_zid2.p); std::swap(c->fr_zid.cap, c->fr_zid2.cap);
-    fbase = init ? 0 : fbase + F;
-    F = F2;
-    init = false;
-    if (F == 0) break;
+  } else {
+    st->route = auto_route ? 0 : 1;
   }
-  // rows of nodes never expanded
-  if (rows_done < L.nodes) {
-    grow_keep_to(L.grow, (size_t)(L.nodes + 1), c->stream);
-    const long from = (long)(rows_done + 1);
-    hipLaunchKernelGGL(k_gfill_rows, dim3(blocks_for(L.nodes - from + 1, 256)), dim3(256), 0, c->stream, L.grow.p, from,
-                       (long)L.nodes, L.edges);
-  }
-  st->outer_nodes = L.nodes;
-  st->fwd_edges = L.edges;
   const auto t1 = std::chrono::steady_clock::now();
   sync_graph_args(L);
   L.args.gzone = c->zarena.p;
@@ -1426,6 +1520,12 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
         hipLaunchKernelGGL(k_gfp, dim3(ns), dim3(64), LL.gfp_ints() * sizeof(int), c->stream, c->dnp, L.argsp, L.slist.p,
                            ns, round == 0 ? 1 : 0, B, L.lc.p);
         const LiveCtr h = read_lc(c, L);
+        if (c->trace) {
+          HIP_OK(hipEventRecord(c->ev[5], c->stream));
+          HIP_OK(hipEventSynchronize(c->ev[5]));
+          std::fprintf(stderr, "gfp round %d: %d values changed, pre_calls so far %llu\n", round, h.changed,
+                       (unsigned long long)h.pre_calls);
+        }
         ++round;
         if (h.err) {
           st->capped = h.err == ERR_CAPPED;

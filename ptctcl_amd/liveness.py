@@ -73,8 +73,12 @@ class hora_dpred(C.Structure):
     _fields_ = [("nterms", C.c_int32), ("term_off", C.c_void_p), ("atoms", C.c_void_p)]
 
 
+COVER_AUTO, COVER_AM = 0, 1
+
+
 class hora_live_query(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("early", C.c_int32), ("p", hora_dpred), ("q", hora_dpred),
+    _fields_ = [("kind", C.c_int32), ("early", C.c_int32), ("cover", C.c_int32), ("pad", C.c_int32),
+                ("p", hora_dpred), ("q", hora_dpred),
                 ("orig_inv_off", C.c_void_p), ("orig_inv_data", C.c_void_p)]
 
 
@@ -83,7 +87,8 @@ _STAT_I64 = ("outer_nodes", "fwd_edges", "disc_keys", "expanded_nodes", "generat
 
 
 class hora_live_stats(C.Structure):
-    _fields_ = [(k, C.c_int64) for k in _STAT_I64] + [("early", C.c_int32), ("capped", C.c_int32)] + \
+    _fields_ = [(k, C.c_int64) for k in _STAT_I64] + [("early", C.c_int32), ("capped", C.c_int32),
+                                                      ("route", C.c_int32), ("seed_keys", C.c_int32)] + \
                [(k, C.c_double) for k in ("seconds", "build_seconds", "gfp_seconds")]
 
 
@@ -114,6 +119,8 @@ class LiveStats:
     chain_reads: int = 0
     early: int = 0
     capped: int = 0
+    route: int = 0
+    seed_keys: int = 0
     seconds: float = 0.0
     build_seconds: float = 0.0
     gfp_seconds: float = 0.0
@@ -143,6 +150,7 @@ class LiveQuery:
     q: DPred = field(default_factory=DPred.false)
     early: bool = True
     orig: Optional[Network] = None
+    cover: int = COVER_AUTO
 
     def encode(self) -> "LiveQueryEnc":
         return LiveQueryEnc(self)
@@ -154,6 +162,7 @@ class LiveQueryEnc:
         s = hora_live_query()
         s.kind = q.kind
         s.early = 1 if q.early else 0
+        s.cover = q.cover
         for name, pred in (("p", q.p), ("q", q.q)):
             off, atoms = _pred_arrays(pred)
             self.keep += [off, atoms]
@@ -169,12 +178,12 @@ class LiveQueryEnc:
         self.struct = s
 
 
-def leads_to(p: DPred, q: DPred, early: bool = True) -> LiveQuery:
-    return LiveQuery(LEADSTO, p, q, early)
+def leads_to(p: DPred, q: DPred, early: bool = True, cover: int = COVER_AUTO) -> LiveQuery:
+    return LiveQuery(LEADSTO, p, q, early, cover=cover)
 
 
-def not_deadlock(early: bool = True) -> LiveQuery:
-    return LiveQuery(DEADLOCK, early=early)
+def not_deadlock(early: bool = True, cover: int = COVER_AUTO) -> LiveQuery:
+    return LiveQuery(DEADLOCK, early=early, cover=cover)
 
 
 def restrict(net: Network, clocks: Sequence[Tuple[int, int, int]]) -> Network:

@@ -6,8 +6,10 @@ Cases: the reference's fischer family with its recorded queries (`P0.req --> P0.
 `P0.cs --> P0.req` and `P0.wait --> P0.cs` violated, `A[] not deadlock` satisfied, four `E[]` rows
 with a clock part), the seeded token ring (binary synchronisation), and the small automata of
 tests/live_models.py with their definitional verdicts. At a size the oracle needs minutes for
-(fischer 8, 2 million nodes), the node and edge counts the oracle produced once are the pin
-(tests/golden/live_counts.json, written by tests/golden/make_live_counts.py).
+(fischer 10), the node and edge counts the oracle produced once are the pin
+(tests/golden/live_counts.json, written by tests/golden/make_live_counts.py). Both covering routes
+are checked: the automatic one (a_LU unless a discrete state seeds a deadlock atom) and the a_M
+graph.
 """
 import json
 import os
@@ -24,7 +26,8 @@ import live_models
 pytestmark = pytest.mark.gpu
 P = lv.DPred
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "live_counts.json")))
-STATS = ("outer_nodes", "fwd_edges", "disc_keys", "generated", "covered", "closure", "nonempty", "early")
+STATS = ("outer_nodes", "fwd_edges", "disc_keys", "generated", "covered", "closure", "nonempty", "early", "route",
+         "seed_keys")
 GRAPH = ("dk", "zones", "row_off", "etgt", "ese", "ere", "expanded", "in_s", "w_nonempty")
 
 
@@ -57,10 +60,12 @@ def fischer_queries(net):
             ("deadlock full", lv.not_deadlock(False), True)]
 
 
+@pytest.mark.parametrize("cover", [lv.COVER_AUTO, lv.COVER_AM])
 @pytest.mark.parametrize("N", [3, 4, 5, 6])
-def test_fischer_recorded_queries(N):
+def test_fischer_recorded_queries(N, cover):
     net = models.fischer(N)
     for label, q, want in fischer_queries(net):
+        q.cover = cover
         r, _ = check(net, q, label)
         assert r == want, label
 
@@ -90,22 +95,34 @@ def test_token_ring(N, seed):
 @pytest.mark.parametrize("builder", live_models.ALL, ids=lambda b: b.__name__)
 def test_small_models(builder):
     net, queries = builder()
-    for q, want in queries:
-        r, _ = check(net, q)
-        assert r == want
+    for cover in (lv.COVER_AUTO, lv.COVER_AM):
+        for q, want in queries:
+            q.cover = cover
+            r, _ = check(net, q)
+            assert r == want
 
 
 def test_fischer7_full_graph_parity():
     net = models.fischer(7)
-    r, st = check(net, lv.leads_to(P.at(net, "P0", "req"), P.at(net, "P0", "wait"), early=False))
-    assert r and st.outer_nodes == GOLD["fischer_req_wait"]["7"]["nodes"]
+    for cover, gold in ((lv.COVER_AUTO, "fischer_req_wait"), (lv.COVER_AM, "fischer_req_wait_am")):
+        q = lv.leads_to(P.at(net, "P0", "req"), P.at(net, "P0", "wait"), early=False, cover=cover)
+        r, st = check(net, q, f"cover {cover}")
+        assert r and st.outer_nodes == GOLD[gold]["7"]["nodes"]
 
 
-def test_fischer8_counts_at_size():
-    net = models.fischer(8)
+def test_partial_deadlock_takes_the_am_route():
+    net, _ = live_models.partial_deadlock()
+    q = lv.leads_to(P.at(net, "A", "a"), P.at(net, "A", "c"), early=False)
+    r, st = check(net, q)
+    assert not r and st.route == 1 and st.seed_keys >= 1
+
+
+@pytest.mark.parametrize("N", [8, 10])
+def test_fischer_counts_at_size(N):
+    net = models.fischer(N)
     with hip.HipContext(net, None) as ctx:
         r, st = ctx.live(lv.leads_to(P.at(net, "P0", "req"), P.at(net, "P0", "wait"), early=False))
-    g = GOLD["fischer_req_wait"]["8"]
-    print("fischer8", r, st, flush=True)
-    assert r and st.nonempty == 0
+    g = GOLD["fischer_req_wait"][str(N)]
+    print(f"fischer{N}", r, st, flush=True)
+    assert r and st.nonempty == 0 and st.route == 0
     assert (st.outer_nodes, st.fwd_edges, st.closure) == (g["nodes"], g["edges"], g["closure"])

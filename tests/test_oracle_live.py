@@ -10,6 +10,8 @@
   hora_deadlock_t1.csv:69-85), and the four `E[]` rows of benchmarks/queries/eg/results-4-5.txt:2-11.
 * Verdicts on small automata whose answers follow from their definition (tests/live_models.py):
   timelock, forced progress, a cycle witness that no single state certifies, a partial deadlock.
+* Both covering routes give every verdict; the route itself follows the seed condition of
+  online_seed_key (src/liveness/refinement.cpp:176-282).
 """
 import numpy as np
 import pytest
@@ -104,15 +106,16 @@ def test_dpred_negation_is_complement():
                 assert pred.eval(locs, [idv]) != (~pred).eval(locs, [idv])
 
 
+@pytest.mark.parametrize("cover", [lv.COVER_AUTO, lv.COVER_AM])
 @pytest.mark.parametrize("n", [3, 4, 5, 6])
-def test_fischer_leadsto_recorded_verdicts(n):
+def test_fischer_leadsto_recorded_verdicts(n, cover):
     net = models.fischer(n)
     for early in (True, False):
-        r, st = binding.live(net, lv.leads_to(P.at(net, "P0", "req"), P.at(net, "P0", "wait"), early))
+        r, st = binding.live(net, lv.leads_to(P.at(net, "P0", "req"), P.at(net, "P0", "wait"), early, cover))
         assert r and not st.capped and st.nonempty == 0
-        r, st = binding.live(net, lv.leads_to(P.at(net, "P0", "cs"), P.at(net, "P0", "req"), early))
+        r, st = binding.live(net, lv.leads_to(P.at(net, "P0", "cs"), P.at(net, "P0", "req"), early, cover))
         assert not r
-        r, st = binding.live(net, lv.leads_to(P.at(net, "P0", "wait"), P.at(net, "P0", "cs"), early))
+        r, st = binding.live(net, lv.leads_to(P.at(net, "P0", "wait"), P.at(net, "P0", "cs"), early, cover))
         assert not r
         if early:
             assert st.early == 1     # a single waiting state already deadlocks or diverges: stop witness
@@ -142,9 +145,26 @@ def test_fischer_eg_recorded_verdicts(n):
 @pytest.mark.parametrize("builder", live_models.ALL, ids=lambda b: b.__name__)
 def test_small_models_known_verdicts(builder):
     net, queries = builder()
-    for q, want in queries:
-        r, st = binding.live(net, q)
-        assert r == want, (net.name, q.kind, q.early)
+    for cover in (lv.COVER_AUTO, lv.COVER_AM):
+        for q, want in queries:
+            q.cover = cover
+            r, st = binding.live(net, q)
+            assert r == want, (net.name, q.kind, q.early, cover)
+
+
+def test_route_choice():
+    """fischer seeds no deadlock atom (each state with an upper invariant, `req`, can always leave
+    it), so its graph is built under a_LU; the partial-deadlock automaton's first location seeds one
+    (a valuation past x = 4 is stuck while the edge is enabled below it), so it takes the a_M graph."""
+    net = models.fischer(5)
+    r, st = binding.live(net, lv.leads_to(P.at(net, "P0", "req"), P.at(net, "P0", "wait"), early=False))
+    assert r and st.route == 0 and st.seed_keys == 0
+    r_am, st_am = binding.live(net, lv.leads_to(P.at(net, "P0", "req"), P.at(net, "P0", "wait"), early=False,
+                                                cover=lv.COVER_AM))
+    assert r_am and st_am.route == 1 and st_am.outer_nodes > st.outer_nodes
+    net, _ = live_models.partial_deadlock()
+    r, st = binding.live(net, lv.leads_to(P.at(net, "A", "a"), P.at(net, "A", "c"), early=False))
+    assert not r and st.route == 1 and st.seed_keys == 1
 
 
 def test_early_witness_needs_no_fixpoint():
