@@ -166,6 +166,7 @@ int hora_hip_reach_profile(const hora_hip_ctx* ctx, hora_reach_profile* out);
 
 /* ---- batched zone operations (zone-kernel microbenchmark; R1-R7, R11, R16-R18) ----
  * One call applies one operation to n zones of dimension d, stored back to back (n*d*d int32).
+ * The flag-only operations (INCLUDES, ALU_LE, EMPTY) leave `out` untouched.
  * Input and output pointers are DEVICE pointers; `stream` is a hipStream_t or NULL for the
  * default stream. The call is asynchronous on that stream. Operation semantics are those of the
  * HDBM calls the reference makes (restated in src/metal/netengine_metal.mm:101-335 and

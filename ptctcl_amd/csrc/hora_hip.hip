@@ -1503,6 +1503,89 @@ __global__ void k_zop_stream(ZopArgs A) {
   if (k == 0 && A.flag) A.flag[z] = 1;
 }
 
+// Entry-parallel ops for d in {4, 8, 16, 32}: one thread per matrix entry, a workgroup holds whole
+// zones (256 threads, or 1024 for d = 32), per-zone flags reduced in LDS. Every entry is read once
+// from HBM; the few other entries of the same zone an entry needs (row 0, column b, ...) come from
+// the caches its neighbours just filled. `ld` = log2(d).
+template <int OP>
+__global__ __launch_bounds__(1024) void k_zop_flat(ZopArgs A, int ld) {
+  __shared__ int zflag[256];
+  __shared__ int stage[1024];
+  const int d = A.d, d2 = d * d;
+  const int zpb = (int)blockDim.x >> (2 * ld);
+  const int lz = (int)threadIdx.x >> (2 * ld);
+  const int e = (int)threadIdx.x & (d2 - 1);
+  const long z = (long)blockIdx.x * zpb + lz;
+  const bool act = z < A.n;
+  const int i = e >> ld, j = e & (d - 1);
+  if ((int)threadIdx.x < zpb) zflag[threadIdx.x] = 1;
+  // the workgroup's zones, one coalesced load; entries other than a thread's own come from LDS
+  const bool staged = OP != HORA_ZOP_INCLUDES;
+  if (staged) stage[threadIdx.x] = act ? A.a[z * d2 + e] : 0;
+  __syncthreads();
+  const int* a = staged ? stage + lz * d2 : A.a + (act ? z : 0) * d2;
+  if (act) {
+    if (OP == HORA_ZOP_INCLUDES) {
+      if (A.b[z * d2 + e] > a[e]) zflag[lz] = 0;
+    } else if (OP == HORA_ZOP_ALU_LE) {
+      // Z = a, Zp = b: a witness (y, x) = (i, j) refutes Z <=_aLU Zp
+      const int* L = A.L + z * d;
+      const int* U = A.U + z * d;
+      if (i != j) {
+        const int ux = U[j], ly = L[i];
+        if (ux != kAluNegInf && ly != kAluNegInf) {
+          const int z0x = a[j];
+          const int zp = A.b[z * d2 + e];
+          if (z0x >= r_le(-ux) && zp < a[e] && radd(zp, r_lt(-ly)) < z0x) zflag[lz] = 0;
+        }
+      }
+    } else if (OP == HORA_ZOP_DOWN) {
+      int v = a[e];
+      if (i == 0 && j >= 1) {
+        v = kLe0;
+        for (int k = 1; k < d; ++k) v = min(v, a[k * d + j]);
+      }
+      A.out[z * d2 + e] = v;
+    } else if (OP == HORA_ZOP_FREE) {
+      const unsigned m = (unsigned)A.arg[z];
+      int v;
+      if (i == j) v = kLe0;
+      else if ((m >> i) & 1) v = kInf;
+      else if ((m >> j) & 1) v = a[i * d];
+      else v = a[e];
+      A.out[z * d2 + e] = v;
+    } else if (OP == HORA_ZOP_CONSTRAIN) {
+      const int ca = A.arg[3 * z], cb = A.arg[3 * z + 1], r = A.arg[3 * z + 2];
+      int v = a[e];
+      if (r < a[ca * d + cb]) {
+        if (radd(a[cb * d + ca], r) < kLe0) {
+          if (e == 0) zflag[lz] = 0;   // empty: the matrix is left as it was
+        } else {
+          // the incremental closure of one tightened edge: column b first, then every entry via b
+          const int col = min(a[i * d + cb], radd(a[i * d + ca], r));
+          v = min(v, radd(col, a[cb * d + j]));
+        }
+      }
+      if (A.out) A.out[z * d2 + e] = v;
+    }
+  }
+  __syncthreads();
+  if (act && e == 0 && A.flag) A.flag[z] = zflag[lz];
+}
+
+// Emptiness for d in {4, 8, 16, 32}: one thread per diagonal entry, 256 threads per workgroup.
+__global__ __launch_bounds__(256) void k_zop_empty(ZopArgs A, int ld) {
+  __shared__ int zflag[64];
+  const int d = A.d, zpb = 256 >> ld;
+  const int lz = (int)threadIdx.x >> ld, i = (int)threadIdx.x & (d - 1);
+  const long z = (long)blockIdx.x * zpb + lz;
+  if ((int)threadIdx.x < zpb) zflag[threadIdx.x] = 0;
+  __syncthreads();
+  if (z < A.n && A.a[z * d * d + i * d + i] < kLe0) zflag[lz] = 1;
+  __syncthreads();
+  if (z < A.n && i == 0 && A.flag) A.flag[z] = zflag[lz];
+}
+
 // Compute ops: one wavefront per zone (zone staged in LDS).
 __global__ __launch_bounds__(64) void k_zop_wave(ZopArgs A) {
   extern __shared__ int sm[];
@@ -1553,7 +1636,8 @@ __global__ __launch_bounds__(64) void k_zop_wave(ZopArgs A) {
         break;
     }
     wsync();
-    if (A.out) {
+    const bool flag_only = A.op == HORA_ZOP_INCLUDES || A.op == HORA_ZOP_ALU_LE || A.op == HORA_ZOP_EMPTY;
+    if (A.out && !flag_only) {
       int* dst = A.out + z * d2;
       for (int e = l; e < d2; e += kWave) dst[e] = Z[e];
     }
@@ -2271,10 +2355,27 @@ int hora_hip_zone_op(int32_t op, int32_t d, int64_t n, const int32_t* a, const i
   if (n == 0) return HORA_HIP_OK;
   ZopArgs A{op, d, n, a, b, arg, L, U, out, out_flag};
   hipStream_t s = (hipStream_t)stream;
+  const int ld = d == 4 ? 2 : d == 8 ? 3 : d == 16 ? 4 : d == 32 ? 5 : 0;
+  const bool flat = ld > 0 && (op == HORA_ZOP_INCLUDES || op == HORA_ZOP_ALU_LE || op == HORA_ZOP_DOWN ||
+                               op == HORA_ZOP_FREE || op == HORA_ZOP_CONSTRAIN);
+  if ((op == HORA_ZOP_DOWN || op == HORA_ZOP_FREE) && !out) return HORA_HIP_E_ARG;
   if (op == HORA_ZOP_UP || op == HORA_ZOP_RESET) {
     if (!out) return HORA_HIP_E_ARG;
     const long tot = n * (long)d * d;
     hipLaunchKernelGGL(k_zop_stream, dim3(blocks_for(tot, 256)), dim3(256), 0, s, A);
+  } else if (flat) {
+    const int bs = d == 32 ? 1024 : 256;
+    const long zpb = bs / (d * d);
+    const dim3 grid(blocks_for(n, (int)zpb)), blk(bs);
+    switch (op) {
+      case HORA_ZOP_INCLUDES: hipLaunchKernelGGL(k_zop_flat<HORA_ZOP_INCLUDES>, grid, blk, 0, s, A, ld); break;
+      case HORA_ZOP_ALU_LE: hipLaunchKernelGGL(k_zop_flat<HORA_ZOP_ALU_LE>, grid, blk, 0, s, A, ld); break;
+      case HORA_ZOP_DOWN: hipLaunchKernelGGL(k_zop_flat<HORA_ZOP_DOWN>, grid, blk, 0, s, A, ld); break;
+      case HORA_ZOP_FREE: hipLaunchKernelGGL(k_zop_flat<HORA_ZOP_FREE>, grid, blk, 0, s, A, ld); break;
+      default: hipLaunchKernelGGL(k_zop_flat<HORA_ZOP_CONSTRAIN>, grid, blk, 0, s, A, ld); break;
+    }
+  } else if (op == HORA_ZOP_EMPTY && ld > 0) {
+    hipLaunchKernelGGL(k_zop_empty, dim3(blocks_for(n, 256 >> ld)), dim3(256), 0, s, A, ld);
   } else {
     const long grid = std::min<long>(n, 1L << 20);
     const size_t lds = (4 * d * d + HORA_MAX_DIM + 8) * sizeof(int);
