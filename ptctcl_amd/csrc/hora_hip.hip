@@ -375,9 +375,10 @@ __device__ __forceinline__ u64 wave_encode_key(const DevNet& N, const int* locs,
 
 // The target side of close_succ_zone for a location vector: frozen flags (semantics.hpp:160-169),
 // the active-clock mask union (reset_inactive, semantics.hpp:195-205) and the combined invariant
-// gathered into `inv` as (i, j, raw) triples (all_invariants, semantics.hpp:82-86).
+// gathered into `inv` as (i, j, raw) triples (all_invariants, semantics.hpp:82-86). The list is
+// written only when it fits `cap` triples; `ninv` is its length either way.
 __device__ __forceinline__ void wave_loc_info(const DevNet& N, const int* locs, const int* ints, int* frozen,
-                                              u64* amask, int* inv, int* ninv) {
+                                              u64* amask, int* inv, int* ninv, int cap) {
   const int l = lane_id();
   int f = 0, ia = 0, ib = 0;
   u64 am = 0;
@@ -394,12 +395,13 @@ __device__ __forceinline__ void wave_loc_info(const DevNet& N, const int* locs, 
   *amask = wave_or64(am);
   int total = 0;
   const int off = wave_excl_scan(ib - ia, &total);
-  for (int c = ia; c < ib; ++c) {
-    const int o = 3 * (off + c - ia);
-    inv[o] = N.inv_data[3 * c];
-    inv[o + 1] = N.inv_data[3 * c + 1];
-    inv[o + 2] = N.inv_data[3 * c + 2];
-  }
+  if (total <= cap)
+    for (int c = ia; c < ib; ++c) {
+      const int o = 3 * (off + c - ia);
+      inv[o] = N.inv_data[3 * c];
+      inv[o + 1] = N.inv_data[3 * c + 1];
+      inv[o + 2] = N.inv_data[3 * c + 2];
+    }
   *ninv = total;
 }
 
@@ -520,7 +522,7 @@ __global__ __launch_bounds__(64) void k_count(const DevNet* __restrict__ Np, con
   u64 am;
   int ninv;
   int* inv = rec + 2 * kRecipeCap;
-  wave_loc_info(N, locs, ints, &fz, &am, inv, &ninv);
+  wave_loc_info(N, locs, ints, &fz, &am, inv, &ninv, 0);   // flags and mask only
   const int r = build_recipes(N, locs, ints, (fz & 1) != 0, rec, kRecipeCap);
   if (threadIdx.x == 0) {
     cnt[i] = r < 0 ? 0 : r;

@@ -87,7 +87,9 @@ struct LiveCtr {
   long long nonempty;
   int fed_max;
   int violated;
-  int pad[4];
+  int dcnt;                   // nodes a fixpoint round visits
+  int rcnt;                   // of those, nodes sent to the full-capacity launch
+  int pad[2];
 };
 
 // ---------------------------------------------------------------- forward construction
@@ -569,12 +571,13 @@ __device__ __forceinline__ int fed_covers(const int* F, int n, int zs, const int
   return nc == 0 ? 1 : 0;
 }
 
-// Combined layout of k_stop and k_gfp in LDS ints: zone lists at stride zs = d2.
-struct LiveLds {
-  int d2, kl, kp, kc;
-  __host__ __device__ int misc() const { return 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * 3 * kInvCap + 2 * kRecipeCap + 64 + 64; }
-  __host__ __device__ int stop_ints() const { return (6 + kl + 2 * kp) * d2 + misc(); }
-  __host__ __device__ int gfp_ints() const { return (6 + kLiveW + kc + 2 * kp) * d2 + misc(); }
+// LDS capacities of one k_stop launch: zones of the escape list (kl) and of each subtraction piece
+// list (kp), invariant constraints of a location vector (ic), transitions of a state (rc).
+struct StopCaps {
+  int kl, kp, ic, rc;
+  __host__ __device__ int ints(int d2) const {
+    return (6 + kl + 2 * kp) * d2 + 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * 3 * ic + 2 * rc;
+  }
 };
 
 // Stop set of node v (mode 0), early witness test (mode 1: p /\ !q nodes only), deadlock test
@@ -583,12 +586,12 @@ struct LiveLds {
 // in ctr->hit in mode 3.
 __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, const LiveArgs* __restrict__ Ap,
                                              const int* __restrict__ list, long n, long long first_id, int mode,
-                                             int* __restrict__ scnt, int* __restrict__ sz, LiveCtr* lc) {
+                                             StopCaps K, int* __restrict__ retry, int* __restrict__ scnt,
+                                             int* __restrict__ sz, LiveCtr* lc) {
   extern __shared__ int sm[];
   const DevNet& N = *Np;
   const LiveArgs& A = *Ap;
   const int d = N.d, d2 = N.d2, l = threadIdx.x;
-  const LiveLds LL{d2, A.kl, A.kp, A.kc};
   int* Zd = sm;
   int* g = Zd + d2;
   int* tmp = g + d2;
@@ -596,27 +599,31 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
   int* t = r + d2;
   int* zu = t + d2;
   int* live = zu + d2;
-  int* cur = live + LL.kl * d2;
-  int* nxt = cur + LL.kp * d2;
-  int* locs = nxt + LL.kp * d2;
+  int* cur = live + K.kl * d2;
+  int* nxt = cur + K.kp * d2;
+  int* locs = nxt + K.kp * d2;
   int* ints = locs + HORA_MAX_PROCS;
   int* tl = ints + HORA_MAX_INTS;
   int* ti = tl + HORA_MAX_PROCS;
   int* inv = ti + HORA_MAX_INTS;
-  int* tinv = inv + 3 * kInvCap;
-  int* rec = tinv + 3 * kInvCap;
-  int* drop = rec + 2 * kRecipeCap;
+  int* tinv = inv + 3 * K.ic;
+  int* rec = tinv + 3 * K.ic;
   const long b = blockIdx.x;
   if (b >= n) return;
   const long v = list ? list[b] : first_id + b;
   const u64 dk = A.gdk[v];
   if (mode == 1 && !(pred_eval(N, A.p, dk) && !pred_eval(N, A.q, dk))) return;
+  // a capacity of K exceeded: rerun the node with the full capacities, or fail without them
+  auto overflow = [&](int err) __attribute__((always_inline)) {
+    if (retry) { if (l == 0) retry[atomicAdd(&lc->rcnt, 1)] = (int)v; }
+    else if (l == 0) atomicExch(&lc->err, err);
+  };
   decode_key(N, dk, locs, ints);
   wsync();
   int fz = 0, ninv = 0;
   u64 am = 0;
-  wave_loc_info(N, locs, ints, &fz, &am, inv, &ninv);   // the network's (restricted) invariant
-  if (ninv > kInvCap) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); return; }
+  wave_loc_info(N, locs, ints, &fz, &am, inv, &ninv, K.ic);   // the network's (restricted) invariant
+  if (ninv > K.ic) { overflow(ERR_RECIPES); return; }
   const bool frozen = fz != 0, src_c = (fz & 1) != 0;
   const int* zv = A.gzone + (size_t)v * d2;
   auto out_whole = [&]() __attribute__((always_inline)) {
@@ -644,7 +651,7 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
     // seed test of the discrete state (online_seed_key, refinement.cpp:176-282): the universe
     // zone, judged in the unrestricted network
     const int no = wave_gather_inv(N, doff, ddat, locs, tinv);
-    if (no > kInvCap) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); return; }
+    if (no > K.ic) { overflow(ERR_RECIPES); return; }
     if (!frozen && A.kind != HORA_LIVE_DEADLOCK && !has_upper_list(tinv, no)) return;
     wsync();
     for (int e = l; e < d2; e += kWave) {
@@ -660,7 +667,7 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
       if (time_diverges()) { out_whole(); return; }
       // orig_time_open (construction.cpp:1164-1191)
       const int no = wave_gather_inv(N, doff, ddat, locs, tinv);
-      if (no > kInvCap) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); return; }
+      if (no > K.ic) { overflow(ERR_RECIPES); return; }
       if (N.nclocks > 0 && !has_upper_list(tinv, no) && !frozen) {
         wsync();
         for (int e = l; e < d2; e += kWave) {
@@ -678,7 +685,7 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
   }
   // state_has_deadlock (deadlock.cpp:74-268)
   const int nd = wave_gather_inv(N, doff, ddat, locs, tinv);
-  if (nd > kInvCap) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); return; }
+  if (nd > K.ic) { overflow(ERR_RECIPES); return; }
   zcopy_w(Zd, zv, d2);
   if (!frozen) z_up(Zd, d);
   if (!z_apply_list(Zd, d, tinv, nd)) { out_none(); return; }
@@ -689,8 +696,8 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
     if (l >= 1 && l < d) bounded = Zd[l * d] != kInf;
     if (!__any(bounded)) return;
   }
-  const int nr = build_recipes(N, locs, ints, src_c, rec, kRecipeCap);
-  if (nr < 0) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); return; }
+  const int nr = build_recipes(N, locs, ints, src_c, rec, K.rc);
+  if (nr < 0) { overflow(ERR_RECIPES); return; }
   wsync();
   int nl = 0;
   for (int k = 0; k < nr; ++k) {
@@ -708,7 +715,7 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
     }
     wsync();
     const int nt = wave_gather_inv(N, doff, ddat, tl, tinv);
-    if (nt > kInvCap) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); return; }
+    if (nt > K.ic) { overflow(ERR_RECIPES); return; }
     const u64 rm = edge_reset_mask(N, se) | (re >= 0 ? edge_reset_mask(N, re) : 0ull);
     bool ok = true;
     for (int c = 0; c < nt && ok; ++c) {
@@ -730,7 +737,7 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
       ++m2;
     }
     nl = m2;
-    if (nl >= LL.kl) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); return; }
+    if (nl >= K.kl) { overflow(ERR_CAPPED); return; }
     zcopy_w(live + nl * d2, g, d2);
     ++nl;
   }
@@ -746,8 +753,8 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
   for (int m = 0; m < nl && nc > 0; ++m) {
     int nn = 0;
     for (int k = 0; k < nc; ++k)
-      if (!z_subtract(c0 + k * d2, live + m * d2, d, c1, d2, &nn, LL.kp, r, t)) {
-        if (l == 0) atomicExch(&lc->err, ERR_CAPPED);
+      if (!z_subtract(c0 + k * d2, live + m * d2, d, c1, d2, &nn, K.kp, r, t)) {
+        overflow(ERR_CAPPED);
         return;
       }
     int* sw = c0; c0 = c1; c1 = sw;
@@ -840,52 +847,95 @@ struct GfpBanks {
   const int* sz;
 };
 
-// One Jacobi round of relax (fixpoint.cpp:750-864) over the closure, one wavefront per node. A
-// node is revisited only when a successor's value changed in the previous round. Values are read
-// from the bank each node selected at the start of the round and written to the other bank.
+// LDS capacities of one k_gfp launch: zones of the value W[v] (kw), of the candidate federation
+// (kc), of each subtraction piece list (kp), and invariant constraints (ic) of each location vector.
+struct GfpCaps {
+  int kw, kc, kp, ic;
+  __host__ __device__ int drop() const { return kc > kp ? kc : kp; }
+  __host__ __device__ int ints(int d2) const {
+    return (5 + kw + kc + 2 * kp) * d2 + 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * 3 * ic + drop();
+  }
+};
+
+// The nodes a Jacobi round must visit: a non-empty value with a successor whose value changed in
+// the previous round (every non-empty value in the first round). The others keep their bank here.
+__global__ void k_gdirty(const LiveArgs* __restrict__ Ap, const int* __restrict__ slist, int ns, int first,
+                         GfpBanks B, int* __restrict__ dlist, LiveCtr* lc) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const LiveArgs& A = *Ap;
+  bool dirty = false;
+  int bi = 0;
+  if (s < ns) {
+    bi = B.sel_in[s];
+    dirty = (bi ? B.wcnt1 : B.wcnt0)[s] > 0;
+    if (dirty && !first) {
+      dirty = false;
+      const int v = slist[s];
+      for (long long e = A.grow[v]; e < A.grow[v + 1] && !dirty; ++e) {
+        const int tt = A.etgt[e];
+        dirty = A.in_s[tt] && B.chg_in[A.sidx[tt]];
+      }
+    }
+  }
+  const int idx = wave_append(&lc->dcnt, dirty);
+  if (s >= ns) return;
+  if (dirty) dlist[idx] = s;
+  else { B.sel_out[s] = bi; B.chg_out[s] = 0; }
+}
+
+// One Jacobi round of relax (fixpoint.cpp:750-864), one wavefront per listed node of the closure.
+// Values are read from the bank each node selected at the start of the round and written to the
+// other bank, so the order of the nodes does not matter. With `retry` set, a node that outgrows a
+// capacity of K is appended to `retry` untouched (for a launch with the full capacities, which
+// reads the same banks); without it, that is the federation cap of the reference (no verdict).
 __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const LiveArgs* __restrict__ Ap,
-                                            const int* __restrict__ slist, int ns, int first, GfpBanks B,
-                                            LiveCtr* lc) {
+                                            const int* __restrict__ slist, const int* __restrict__ list,
+                                            GfpCaps K, GfpBanks B, int* __restrict__ retry, LiveCtr* lc) {
   extern __shared__ int sm[];
   const DevNet& N = *Np;
   const LiveArgs& A = *Ap;
   const int d = N.d, d2 = N.d2, l = threadIdx.x;
-  const LiveLds LL{d2, A.kl, A.kp, A.kc};
   int* pz = sm;
   int* tmp = pz + d2;
   int* r = tmp + d2;
   int* t = r + d2;
   int* Wv = t + 2 * d2;
-  int* cand = Wv + kLiveW * d2;
-  int* cur = cand + LL.kc * d2;
-  int* nxt = cur + LL.kp * d2;
-  int* locs = nxt + LL.kp * d2;
+  int* cand = Wv + K.kw * d2;
+  int* cur = cand + K.kc * d2;
+  int* nxt = cur + K.kp * d2;
+  int* locs = nxt + K.kp * d2;
   int* ints = locs + HORA_MAX_PROCS;
   int* tlocs = ints + HORA_MAX_INTS;
   int* tints = tlocs + HORA_MAX_PROCS;
   int* inv = tints + HORA_MAX_INTS;
-  int* tinv = inv + 3 * kInvCap;
-  int* drop = tinv + 3 * kInvCap;
-  const int s = blockIdx.x;
-  if (s >= ns) return;
+  int* tinv = inv + 3 * K.ic;
+  int* drop = tinv + 3 * K.ic;
+  const int s = list[blockIdx.x];
   const int v = slist[s];
   const int bi = B.sel_in[s];
+  unsigned long long pcalls = 0, prods = 0;
+  auto stats = [&]() __attribute__((always_inline)) {
+    if (l == 0 && pcalls) atomicAdd(&lc->pre_calls, pcalls);
+    if (l == 0 && prods) atomicAdd(&lc->products, prods);
+  };
   auto keep = [&]() __attribute__((always_inline)) {
+    stats();
     if (l == 0) { B.sel_out[s] = bi; B.chg_out[s] = 0; }
   };
-  const long long e0 = A.grow[v], e1 = A.grow[v + 1];
-  if (!first) {
-    bool dirty = false;
-    for (long long e = e0 + l; e < e1; e += kWave) {
-      const int tt = A.etgt[e];
-      if (A.in_s[tt] && B.chg_in[A.sidx[tt]]) dirty = true;
+  auto overflow = [&](int err) __attribute__((always_inline)) {
+    if (retry) {
+      if (l == 0) retry[atomicAdd(&lc->rcnt, 1)] = s;
+    } else {
+      if (l == 0) atomicExch(&lc->err, err);
+      keep();
     }
-    if (!__any(dirty)) { keep(); return; }
-  }
+  };
   auto wcnt = [&](int b) __attribute__((always_inline)) { return b ? B.wcnt1 : B.wcnt0; };
   auto wzb = [&](int b) __attribute__((always_inline)) { return b ? B.wz1 : B.wz0; };
+  const long long e0 = A.grow[v], e1 = A.grow[v + 1];
   const int nW = wcnt(bi)[s];
   if (nW == 0) { keep(); return; }
+  if (nW > K.kw) { overflow(ERR_CAPPED); return; }
   {
     const int* src = wzb(bi) + (size_t)s * kLiveW * d2;
     for (int e = l; e < nW * d2; e += kWave) Wv[e] = src[e];
@@ -894,15 +944,13 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
   wsync();
   int fzv = 0, ninv = 0;
   u64 amv = 0;
-  wave_loc_info(N, locs, ints, &fzv, &amv, inv, &ninv);
-  if (ninv > kInvCap) { if (l == 0) atomicExch(&lc->err, ERR_RECIPES); keep(); return; }
+  wave_loc_info(N, locs, ints, &fzv, &amv, inv, &ninv, K.ic);
+  if (ninv > K.ic) { overflow(ERR_RECIPES); return; }
   const u64 cm = ((1ull << N.nclocks) - 1) << 1;
   u32 covm = 0;
   const u32 full = (nW >= 32) ? 0xffffffffu : ((1u << nW) - 1);
   int nc = 0;
-  unsigned long long pcalls = 0;
-  bool capped = false;
-  for (long long e = e0; e < e1 && !capped; ++e) {
+  for (long long e = e0; e < e1; ++e) {
     const int tt = A.etgt[e];
     if (!A.in_s[tt]) continue;
     const int st = A.sidx[tt];
@@ -914,8 +962,8 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
     wsync();
     int fzt = 0, ntinv = 0;
     u64 amt = 0;
-    wave_loc_info(N, tlocs, tints, &fzt, &amt, tinv, &ntinv);
-    if (ntinv > kInvCap) { capped = true; break; }
+    wave_loc_info(N, tlocs, tints, &fzt, &amt, tinv, &ntinv, K.ic);
+    if (ntinv > K.ic) { overflow(ERR_RECIPES); return; }
     const u64 dead_t = ~amt & cm;
     const int se = A.ese[e], re = A.ere[e];
     const u64 rm = edge_reset_mask(N, se) | (re >= 0 ? edge_reset_mask(N, re) : 0ull);
@@ -943,55 +991,48 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
       if (!z_apply_list(pz, d, inv, ninv)) continue;
       for (int k = 0; k < nW; ++k)
         if (!((covm >> k) & 1) && z_includes(pz, Wv + k * d2, d)) covm |= 1u << k;
-      if (covm == full) {   // every member covered: the value cannot move
-        if (l == 0) atomicAdd(&lc->pre_calls, pcalls);
-        keep();
-        return;
-      }
-      if (nc >= LL.kc) nc = fed_compact(cand, nc, d2, d, drop);
-      if (nc >= LL.kc) { capped = true; break; }
+      if (covm == full) { keep(); return; }   // every member covered: the value cannot move
+      if (nc >= K.kc) nc = fed_compact(cand, nc, d2, d, drop);
+      if (nc >= K.kc) { overflow(ERR_CAPPED); return; }
       zcopy_w(cand + nc * d2, pz, d2);
       ++nc;
     }
   }
-  if (l == 0) atomicAdd(&lc->pre_calls, pcalls);
-  if (capped) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); keep(); return; }
   nc = fed_compact(cand, nc, d2, d, drop);
   // does cand cover every member of W[v]? -1: capacity exceeded
   auto escapes = [&]() __attribute__((always_inline)) -> int {
     for (int k = 0; k < nW; ++k) {
-      const int cv = fed_covers(cand, nc, d2, Wv + k * d2, d, cur, nxt, LL.kp, r, t);
+      const int cv = fed_covers(cand, nc, d2, Wv + k * d2, d, cur, nxt, K.kp, r, t);
       if (cv < 0) return -1;
       if (cv == 0) return 1;
     }
     return 0;
   };
   int es = escapes();
-  if (es < 0) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); keep(); return; }
+  if (es < 0) { overflow(ERR_CAPPED); return; }
   if (es == 0) { keep(); return; }
   const int sc = B.scnt[s];
   if (sc != 0) {
     const int add = sc < 0 ? 1 : sc;
-    if (nc + add > LL.kc) nc = fed_compact(cand, nc, d2, d, drop);
-    if (nc + add > LL.kc) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); keep(); return; }
+    if (nc + add > K.kc) nc = fed_compact(cand, nc, d2, d, drop);
+    if (nc + add > K.kc) { overflow(ERR_CAPPED); return; }
     if (sc < 0) zcopy_w(cand + nc * d2, A.gzone + (size_t)v * d2, d2);
     else
       for (int k = 0; k < sc; ++k) zcopy_w(cand + (nc + k) * d2, B.sz + ((size_t)s * kStopCap + k) * d2, d2);
     nc += add;
     nc = fed_compact(cand, nc, d2, d, drop);
     es = escapes();
-    if (es < 0) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); keep(); return; }
+    if (es < 0) { overflow(ERR_CAPPED); return; }
     if (es == 0) { keep(); return; }
   }
   // W[v] /\ cand, member by member (products into `cur`, capacity kp)
   int nn = 0;
-  unsigned long long prods = 0;
   for (int k = 0; k < nW; ++k) {
     const int* z = Wv + k * d2;
     bool whole = false;
     for (int c = 0; c < nc && !whole; ++c) whole = z_includes(cand + c * d2, z, d);
     if (whole) {
-      if (nn >= LL.kp) { capped = true; break; }
+      if (nn >= K.kp) { overflow(ERR_CAPPED); return; }
       zcopy_w(cur + nn * d2, z, d2);
       ++nn;
       continue;
@@ -999,16 +1040,19 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
     for (int c = 0; c < nc; ++c) {
       if (!z_intersect(tmp, z, cand + c * d2, d)) continue;
       ++prods;
-      if (nn >= LL.kp) nn = fed_compact(cur, nn, d2, d, drop);
-      if (nn >= LL.kp) { capped = true; break; }
+      if (nn >= K.kp) nn = fed_compact(cur, nn, d2, d, drop);
+      if (nn >= K.kp) { overflow(ERR_CAPPED); return; }
       zcopy_w(cur + nn * d2, tmp, d2);
       ++nn;
     }
-    if (capped) break;
   }
-  if (l == 0 && prods) atomicAdd(&lc->products, prods);
-  if (!capped) nn = fed_compact(cur, nn, d2, d, drop);
-  if (capped || nn > kLiveW) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); keep(); return; }
+  nn = fed_compact(cur, nn, d2, d, drop);
+  if (nn > kLiveW) {                  // the bank's capacity, the same in every launch
+    if (l == 0) atomicExch(&lc->err, ERR_CAPPED);
+    keep();
+    return;
+  }
+  stats();
   const int bo = 1 - bi;
   int* dst = wzb(bo) + (size_t)s * kLiveW * d2;
   wsync();
@@ -1061,7 +1105,7 @@ struct LiveState {
   DBuf<signed char> expanded, wne;
   long long nodes = 0, edges = 0;
   // backward pass
-  DBuf<int> in_s, slist, sidx, scnt, sz, wcnt0, wcnt1, wz0, wz1, sel0, sel1, chg0, chg1, heads;
+  DBuf<int> in_s, slist, sidx, scnt, sz, wcnt0, wcnt1, wz0, wz1, sel0, sel1, chg0, chg1, heads, dlist, rlist;
   int ns = 0;
   ~LiveState() {
     if (argsp) { (void)hipDeviceSynchronize(); (void)hipFree(argsp); }
@@ -1098,18 +1142,68 @@ void sync_graph_args(LiveState& L) {
   L.args.expanded = L.expanded.p;
 }
 
-// Federation capacities from the LDS budget (64 KiB per workgroup). False when fewer than four
-// zones per list fit (zone dimension above 23).
+// Full federation capacities from the LDS budget (64 KiB per workgroup), the same for every list,
+// sized so that both StopCaps and GfpCaps at these capacities fit. False when fewer than four zones
+// per list fit (zone dimension above 23).
 bool live_caps(int d2, LiveArgs& a) {
   const int budget = 64 * 1024 / 4;
-  LiveLds t{d2, 0, 0, 0};
-  const int zones = (budget - t.misc()) / d2;
+  const int misc = 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * 3 * kInvCap + 2 * kRecipeCap + 128;
+  const int zones = (budget - misc) / d2;
   const int k = std::min(64, (zones - 6 - kLiveW) / 3);
   a.kl = k;
   a.kp = k;
   a.kc = k;
   return k >= 4;
 }
+
+// Stop-set launches, like the fixpoint's, run first with small lists and rerun the nodes that outgrow
+// them with the full ones. HORA_HIP_STOP_CAPS=kl,kp,ic,rc sets the small lists (escape zones, piece
+// zones, invariant constraints, transitions).
+StopCaps stop_caps_small(int d2) {
+  StopCaps k{4, 4, 32, 64};
+  if (const char* e = std::getenv("HORA_HIP_STOP_CAPS")) {
+    StopCaps o{};
+    if (std::sscanf(e, "%d,%d,%d,%d", &o.kl, &o.kp, &o.ic, &o.rc) == 4 && o.kl >= 1 && o.kp >= 1 && o.ic >= 1 &&
+        o.ic <= kInvCap && o.rc >= 1 && o.rc <= kRecipeCap && o.ints(d2) <= 64 * 1024 / 4)
+      k = o;
+  }
+  return k;
+}
+
+StopCaps stop_caps_full(const LiveArgs& a) { return StopCaps{a.kl, a.kp, kInvCap, kRecipeCap}; }
+
+// k_stop over `n` nodes (list[b], or first_id + b without a list) in `mode`; the caller reads the
+// counters afterwards.
+void launch_stop(hora_hip_ctx* c, LiveState& L, const int* list, long n, long long first_id, int mode, int* scnt,
+                 int* sz) {
+  if (n <= 0) return;
+  const int d2 = c->d2;
+  L.rlist.ensure(n);
+  HIP_OK(hipMemsetAsync(&L.lc.p->rcnt, 0, sizeof(int), c->stream));
+  const StopCaps KA = stop_caps_small(d2), KB = stop_caps_full(L.args);
+  hipLaunchKernelGGL(k_stop, dim3(n), dim3(64), KA.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp, list, n,
+                     first_id, mode, KA, L.rlist.p, scnt, sz, L.lc.p);
+  const LiveCtr h = read_lc(c, L);
+  if (h.rcnt > 0 && !h.err)
+    hipLaunchKernelGGL(k_stop, dim3(h.rcnt), dim3(64), KB.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+                       (const int*)L.rlist.p, (long)h.rcnt, (long long)0, mode, KB, (int*)nullptr, scnt, sz, L.lc.p);
+}
+
+// Fixpoint launches run first with small LDS lists, so that many wavefronts share a CU; the few
+// nodes that outgrow them rerun with the full lists of live_caps. HORA_HIP_GFP_CAPS=kw,kc,kp,ic sets
+// the small lists (zones of the value, of the candidates, of each piece list; invariant constraints).
+GfpCaps gfp_caps_small(int d2) {
+  GfpCaps k{2, 4, 4, 32};
+  if (const char* e = std::getenv("HORA_HIP_GFP_CAPS")) {
+    GfpCaps o{};
+    if (std::sscanf(e, "%d,%d,%d,%d", &o.kw, &o.kc, &o.kp, &o.ic) == 4 && o.kw >= 1 && o.kw <= kLiveW &&
+        o.kc >= 1 && o.kp >= 2 && o.ic >= 1 && o.ic <= kInvCap && o.ints(d2) <= 64 * 1024 / 4)
+      k = o;
+  }
+  return k;
+}
+
+GfpCaps gfp_caps_full(const LiveArgs& a) { return GfpCaps{kLiveW, a.kc, a.kp, kInvCap}; }
 
 int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_live_stats* st) {
   const auto t0 = std::chrono::steady_clock::now();
@@ -1371,10 +1465,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
         sync_graph_args(L);
         L.args.gzone = c->zarena.p;
         upload_args(c, L);
-        const LiveLds LL{d2, L.args.kl, L.args.kp, L.args.kc};
-        hipLaunchKernelGGL(k_stop, dim3(F), dim3(64), LL.stop_ints() * sizeof(int), c->stream, c->dnp, L.argsp,
-                           (const int*)nullptr, (long)F, (long long)fbase, q->kind == HORA_LIVE_LEADSTO ? 1 : 2,
-                           (int*)nullptr, (int*)nullptr, L.lc.p);
+        launch_stop(c, L, nullptr, (long)F, (long long)fbase, q->kind == HORA_LIVE_LEADSTO ? 1 : 2, nullptr, nullptr);
         const LiveCtr h = read_lc(c, L);
         if (h.err) {
           c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a stop set" : "invariant or transition table too large";
@@ -1428,9 +1519,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     int seeds = 0;
     if (nh > 0) {
       HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
-      const LiveLds LL{d2, L.args.kl, L.args.kp, L.args.kc};
-      hipLaunchKernelGGL(k_stop, dim3(nh), dim3(64), LL.stop_ints() * sizeof(int), c->stream, c->dnp, L.argsp,
-                         L.heads.p, (long)nh, (long long)0, 3, (int*)nullptr, (int*)nullptr, L.lc.p);
+      launch_stop(c, L, L.heads.p, (long)nh, 0, 3, nullptr, nullptr);
       const LiveCtr h = read_lc(c, L);
       if (h.err) {
         c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a seed test" : "invariant or transition table too large";
@@ -1465,10 +1554,8 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
   } else if (N == 0) {
     *holds = q->kind == HORA_LIVE_EG ? 0 : 1;
   } else if (q->kind == HORA_LIVE_DEADLOCK) {
-    const LiveLds LL{d2, L.args.kl, L.args.kp, L.args.kc};
     HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
-    hipLaunchKernelGGL(k_stop, dim3(N), dim3(64), LL.stop_ints() * sizeof(int), c->stream, c->dnp, L.argsp,
-                       (const int*)nullptr, (long)N, (long long)0, 2, (int*)nullptr, (int*)nullptr, L.lc.p);
+    launch_stop(c, L, nullptr, (long)N, 0, 2, nullptr, nullptr);
     const LiveCtr h = read_lc(c, L);
     if (h.err) { c->err = "federation capacity exceeded in a deadlock test"; return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY; }
     *holds = h.hit ? 0 : 1;
@@ -1493,9 +1580,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       // ---- stop sets
       L.scnt.ensure(ns);
       L.sz.ensure((size_t)ns * kStopCap * d2);
-      const LiveLds LL{d2, L.args.kl, L.args.kp, L.args.kc};
-      hipLaunchKernelGGL(k_stop, dim3(ns), dim3(64), LL.stop_ints() * sizeof(int), c->stream, c->dnp, L.argsp,
-                         L.slist.p, (long)ns, (long long)0, 0, L.scnt.p, L.sz.p, L.lc.p);
+      launch_stop(c, L, L.slist.p, (long)ns, 0, 0, L.scnt.p, L.sz.p);
       {
         const LiveCtr h = read_lc(c, L);
         if (h.err) {
@@ -1513,19 +1598,33 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
                          L.sel0.p, L.chg0.p);
       int* sel_in = L.sel0.p; int* sel_out = L.sel1.p;
       int* chg_in = L.chg0.p; int* chg_out = L.chg1.p;
+      L.dlist.ensure(ns); L.rlist.ensure(ns);
+      const GfpCaps KA = gfp_caps_small(d2), KB = gfp_caps_full(L.args);
       int round = 0;
+      long long retried = 0;
       for (;;) {
+        const auto tr0 = std::chrono::steady_clock::now();
+        // changed, dcnt and rcnt are adjacent in LiveCtr
         HIP_OK(hipMemsetAsync(&L.lc.p->changed, 0, sizeof(int), c->stream));
+        HIP_OK(hipMemsetAsync(&L.lc.p->dcnt, 0, 2 * sizeof(int), c->stream));
         GfpBanks B{L.wcnt0.p, L.wcnt1.p, L.wz0.p, L.wz1.p, sel_in, sel_out, chg_in, chg_out, L.scnt.p, L.sz.p};
-        hipLaunchKernelGGL(k_gfp, dim3(ns), dim3(64), LL.gfp_ints() * sizeof(int), c->stream, c->dnp, L.argsp, L.slist.p,
-                           ns, round == 0 ? 1 : 0, B, L.lc.p);
-        const LiveCtr h = read_lc(c, L);
-        if (c->trace) {
-          HIP_OK(hipEventRecord(c->ev[5], c->stream));
-          HIP_OK(hipEventSynchronize(c->ev[5]));
-          std::fprintf(stderr, "gfp round %d: %d values changed, pre_calls so far %llu\n", round, h.changed,
-                       (unsigned long long)h.pre_calls);
+        hipLaunchKernelGGL(k_gdirty, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.argsp, L.slist.p, ns,
+                           round == 0 ? 1 : 0, B, L.dlist.p, L.lc.p);
+        const int nd = read_lc(c, L).dcnt;
+        if (nd > 0)
+          hipLaunchKernelGGL(k_gfp, dim3(nd), dim3(64), KA.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+                             L.slist.p, L.dlist.p, KA, B, L.rlist.p, L.lc.p);
+        LiveCtr h = read_lc(c, L);
+        if (h.rcnt > 0 && !h.err) {
+          hipLaunchKernelGGL(k_gfp, dim3(h.rcnt), dim3(64), KB.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+                             L.slist.p, L.rlist.p, KB, B, (int*)nullptr, L.lc.p);
+          retried += h.rcnt;
+          h = read_lc(c, L);
         }
+        if (c->trace)
+          std::fprintf(stderr, "gfp round %d: %d visited, %d at full capacity, %d values changed, pre_calls so far %llu,"
+                       " %.3f ms\n", round, h.dcnt, h.rcnt, h.changed, (unsigned long long)h.pre_calls,
+                       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tr0).count());
         ++round;
         if (h.err) {
           st->capped = h.err == ERR_CAPPED;
@@ -1537,6 +1636,9 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
         if (h.changed == 0) break;
       }
       st->iters = round;
+      if (c->trace)
+        std::fprintf(stderr, "gfp: %d rounds, %lld node visits at full capacity (small lists %d,%d,%d,%d)\n", round,
+                     retried, KA.kw, KA.kc, KA.kp, KA.ic);
       hipLaunchKernelGGL(k_gverdict, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, L.slist.p, ns,
                          sel_in, L.wcnt0.p, L.wcnt1.p, L.wne.p, L.lc.p);
       const LiveCtr h = read_lc(c, L);

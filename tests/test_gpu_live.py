@@ -102,6 +102,23 @@ def test_small_models(builder):
             assert r == want
 
 
+def test_rerun_at_full_capacity(monkeypatch):
+    """Stop sets and fixpoint rounds first run every node with small LDS lists and rerun the nodes
+    that outgrow them with the full lists. With lists of one or two zones, one constraint and one
+    transition nearly every node reruns; graphs, stop sets and converged values must not change."""
+    monkeypatch.setenv("HORA_HIP_GFP_CAPS", "1,1,2,1")
+    monkeypatch.setenv("HORA_HIP_STOP_CAPS", "1,1,1,1")
+    net = models.fischer(5)
+    for label, q, want in fischer_queries(net):
+        r, _ = check(net, q, label)
+        assert r == want, label
+    for builder in live_models.ALL:
+        net, queries = builder()
+        for q, want in queries:
+            r, _ = check(net, q)
+            assert r == want
+
+
 def test_fischer7_full_graph_parity():
     net = models.fischer(7)
     for cover, gold in ((lv.COVER_AUTO, "fischer_req_wait"), (lv.COVER_AM, "fischer_req_wait_am")):
