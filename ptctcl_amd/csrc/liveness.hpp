@@ -1160,7 +1160,7 @@ bool live_caps(int d2, LiveArgs& a) {
 // them with the full ones. HORA_HIP_STOP_CAPS=kl,kp,ic,rc sets the small lists (escape zones, piece
 // zones, invariant constraints, transitions).
 StopCaps stop_caps_small(int d2) {
-  StopCaps k{4, 4, 32, 64};
+  StopCaps k{2, 2, 32, 64};
   if (const char* e = std::getenv("HORA_HIP_STOP_CAPS")) {
     StopCaps o{};
     if (std::sscanf(e, "%d,%d,%d,%d", &o.kl, &o.kp, &o.ic, &o.rc) == 4 && o.kl >= 1 && o.kp >= 1 && o.ic >= 1 &&
@@ -1193,7 +1193,7 @@ void launch_stop(hora_hip_ctx* c, LiveState& L, const int* list, long n, long lo
 // nodes that outgrow them rerun with the full lists of live_caps. HORA_HIP_GFP_CAPS=kw,kc,kp,ic sets
 // the small lists (zones of the value, of the candidates, of each piece list; invariant constraints).
 GfpCaps gfp_caps_small(int d2) {
-  GfpCaps k{2, 4, 4, 32};
+  GfpCaps k{2, 8, 4, 32};
   if (const char* e = std::getenv("HORA_HIP_GFP_CAPS")) {
     GfpCaps o{};
     if (std::sscanf(e, "%d,%d,%d,%d", &o.kw, &o.kc, &o.kp, &o.ic) == 4 && o.kw >= 1 && o.kw <= kLiveW &&
@@ -1580,9 +1580,13 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       // ---- stop sets
       L.scnt.ensure(ns);
       L.sz.ensure((size_t)ns * kStopCap * d2);
+      const auto ts0 = std::chrono::steady_clock::now();
       launch_stop(c, L, L.slist.p, (long)ns, 0, 0, L.scnt.p, L.sz.p);
       {
         const LiveCtr h = read_lc(c, L);
+        if (c->trace)
+          std::fprintf(stderr, "stop sets: %d nodes, %d at full capacity, %.3f ms\n", ns, h.rcnt,
+                       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts0).count());
         if (h.err) {
           c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a stop set" : "invariant or transition table too large";
           st->capped = h.err == ERR_CAPPED;
