@@ -34,8 +34,7 @@ namespace {
 
 constexpr int kLiveMaxDim = 23;
 constexpr int ERR_CAPPED = 5;
-constexpr int kLiveW = 8;       // zones in one converged value W[v]
-constexpr int kStopCap = 16;    // zones in one stop set
+constexpr int kLiveW = 8;       // zones in one value W[v]
 
 struct DevPred {
   int nterms;
@@ -89,7 +88,10 @@ struct LiveCtr {
   int violated;
   int dcnt;                   // nodes a fixpoint round visits
   int rcnt;                   // of those, nodes sent to the full-capacity launch
-  int pad[2];
+  int spool;                  // zones taken from the stop-set pool
+  int sfull;                  // a stop set did not fit the pool
+  int ptop;                   // zones taken from the value pool
+  int pfull;                  // a value did not fit the pool
 };
 
 // ---------------------------------------------------------------- forward construction
@@ -587,7 +589,7 @@ struct StopCaps {
 __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, const LiveArgs* __restrict__ Ap,
                                              const int* __restrict__ list, long n, long long first_id, int mode,
                                              StopCaps K, int* __restrict__ retry, int* __restrict__ scnt,
-                                             int* __restrict__ sz, LiveCtr* lc) {
+                                             int* __restrict__ soff, int* __restrict__ sz, int scap, LiveCtr* lc) {
   extern __shared__ int sm[];
   const DevNet& N = *Np;
   const LiveArgs& A = *Ap;
@@ -769,12 +771,16 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
     return;
   }
   if (mode != 0) { if (l == 0) atomicExch(&lc->hit, 1); return; }
-  if (nc > kStopCap) { if (l == 0) atomicExch(&lc->err, ERR_CAPPED); return; }
+  // the zones go to the stop-set pool; when it is full the caller grows it and runs the phase again
   const int s = A.sidx[v];
+  int off = 0;
+  if (l == 0) off = atomicAdd(&lc->spool, nc);
+  off = __shfl(off, 0);
+  if ((long long)off + nc > scap) { if (l == 0) atomicExch(&lc->sfull, 1); return; }
   wsync();
   for (int k = 0; k < nc; ++k)
-    for (int e = l; e < d2; e += kWave) sz[((size_t)s * kStopCap + k) * d2 + e] = c0[k * d2 + e];
-  if (l == 0) scnt[s] = nc;
+    for (int e = l; e < d2; e += kWave) sz[((size_t)off + k) * d2 + e] = c0[k * d2 + e];
+  if (l == 0) { scnt[s] = nc; soff[s] = off; }
 }
 
 // ---------------------------------------------------------------- closure of the backward pass
@@ -819,31 +825,39 @@ __global__ void k_sindex(const int* __restrict__ slist, int ns, int* __restrict_
 
 // W[s] := {Z_v} in bank 0
 __global__ __launch_bounds__(64) void k_gseed(const LiveArgs* __restrict__ Ap, const int* __restrict__ slist, int ns,
-                                              int d2, int* __restrict__ wcnt0, int* __restrict__ wz0,
-                                              int* __restrict__ sel, int* __restrict__ chg) {
+                                              int d2, int* __restrict__ wcnt0, int* __restrict__ woff0,
+                                              int* __restrict__ wz, int* __restrict__ sel, int* __restrict__ chg) {
   const int s = blockIdx.x;
   if (s >= ns) return;
   const int v = slist[s];
   const int* src = Ap->gzone + (size_t)v * d2;
-  int* dst = wz0 + (size_t)s * kLiveW * d2;
+  int* dst = wz + (size_t)s * d2;
   for (int e = threadIdx.x; e < d2; e += kWave) dst[e] = src[e];
   if (threadIdx.x == 0) {
     wcnt0[s] = 1;
+    woff0[s] = s;
     sel[s] = 0;
     chg[s] = 1;
   }
 }
 
+// Values: node s's value in bank b is wcnt_b[s] zones at zone offset woff_b[s] of the value pool
+// `wz` (capacity wcap zones), bump-allocated: a value written in a round takes fresh zones, so the
+// values the round reads stay intact. Stop sets: scnt[s] zones at soff[s] of the pool `sz`
+// (scnt -1: the whole zone).
 struct GfpBanks {
-  int* wcnt0;       // [ns] zones in the value, bank 0 / bank 1
+  int* wcnt0;
   int* wcnt1;
-  int* wz0;         // [ns*kLiveW*d2]
-  int* wz1;
+  int* woff0;
+  int* woff1;
+  int* wz;
+  int wcap;
   const int* sel_in;
   int* sel_out;
   const int* chg_in;
   int* chg_out;
-  const int* scnt;  // stop sets
+  const int* scnt;
+  const int* soff;
   const int* sz;
 };
 
@@ -931,13 +945,13 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
     }
   };
   auto wcnt = [&](int b) __attribute__((always_inline)) { return b ? B.wcnt1 : B.wcnt0; };
-  auto wzb = [&](int b) __attribute__((always_inline)) { return b ? B.wz1 : B.wz0; };
+  auto woff = [&](int b) __attribute__((always_inline)) { return b ? B.woff1 : B.woff0; };
   const long long e0 = A.grow[v], e1 = A.grow[v + 1];
   const int nW = wcnt(bi)[s];
   if (nW == 0) { keep(); return; }
   if (nW > K.kw) { overflow(ERR_CAPPED); return; }
   {
-    const int* src = wzb(bi) + (size_t)s * kLiveW * d2;
+    const int* src = B.wz + (size_t)woff(bi)[s] * d2;
     for (int e = l; e < nW * d2; e += kWave) Wv[e] = src[e];
   }
   decode_key(N, A.gdk[v], locs, ints);
@@ -967,7 +981,7 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
     const u64 dead_t = ~amt & cm;
     const int se = A.ese[e], re = A.ere[e];
     const u64 rm = edge_reset_mask(N, se) | (re >= 0 ? edge_reset_mask(N, re) : 0ull);
-    const int* Ys = wzb(bt) + (size_t)st * kLiveW * d2;
+    const int* Ys = B.wz + (size_t)woff(bt)[st] * d2;
     for (int y = 0; y < nY; ++y) {
       ++pcalls;
       // exact_predecessor (operators.cpp:257-306)
@@ -1018,7 +1032,7 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
     if (nc + add > K.kc) { overflow(ERR_CAPPED); return; }
     if (sc < 0) zcopy_w(cand + nc * d2, A.gzone + (size_t)v * d2, d2);
     else
-      for (int k = 0; k < sc; ++k) zcopy_w(cand + (nc + k) * d2, B.sz + ((size_t)s * kStopCap + k) * d2, d2);
+      for (int k = 0; k < sc; ++k) zcopy_w(cand + (nc + k) * d2, B.sz + ((size_t)B.soff[s] + k) * d2, d2);
     nc += add;
     nc = fed_compact(cand, nc, d2, d, drop);
     es = escapes();
@@ -1052,17 +1066,46 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
     keep();
     return;
   }
+  // fresh zones from the value pool; when it is full the caller compacts it and runs the round again
+  int off = 0;
+  if (l == 0) off = atomicAdd(&lc->ptop, nn);
+  off = __shfl(off, 0);
+  if ((long long)off + nn > B.wcap) { if (l == 0) atomicExch(&lc->pfull, 1); return; }
   stats();
   const int bo = 1 - bi;
-  int* dst = wzb(bo) + (size_t)s * kLiveW * d2;
+  int* dst = B.wz + (size_t)off * d2;
   wsync();
   for (int e = l; e < nn * d2; e += kWave) dst[e] = cur[e];
   if (l == 0) {
     wcnt(bo)[s] = nn;
+    woff(bo)[s] = off;
     B.sel_out[s] = bo;
     B.chg_out[s] = 1;
     atomicAdd(&lc->changed, 1);
   }
+}
+
+// Value-pool compaction: each node's current value (the bank `sel` names) moves to `noff[s]` of the
+// other pool, in node order.
+__global__ void k_wcount(int ns, const int* __restrict__ sel, const int* __restrict__ wcnt0,
+                         const int* __restrict__ wcnt1, int* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < ns) out[s] = (sel[s] ? wcnt1 : wcnt0)[s];
+}
+
+__global__ __launch_bounds__(64) void k_wmove(int ns, int d2, const int* __restrict__ sel, const int* __restrict__ wcnt0,
+                                              const int* __restrict__ wcnt1, int* __restrict__ woff0,
+                                              int* __restrict__ woff1, const long long* __restrict__ noff,
+                                              const int* __restrict__ src, int* __restrict__ dst) {
+  const int s = blockIdx.x;
+  if (s >= ns) return;
+  const int b = sel[s];
+  const int cnt = (b ? wcnt1 : wcnt0)[s];
+  int* wo = b ? woff1 : woff0;
+  const int* a = src + (size_t)wo[s] * d2;
+  int* o = dst + (size_t)noff[s] * d2;
+  for (int e = threadIdx.x; e < cnt * d2; e += kWave) o[e] = a[e];
+  if (threadIdx.x == 0) wo[s] = (int)noff[s];
 }
 
 __global__ void k_gverdict(const DevNet* __restrict__ Np, const LiveArgs* __restrict__ Ap, const int* __restrict__ slist,
@@ -1105,7 +1148,11 @@ struct LiveState {
   DBuf<signed char> expanded, wne;
   long long nodes = 0, edges = 0;
   // backward pass
-  DBuf<int> in_s, slist, sidx, scnt, sz, wcnt0, wcnt1, wz0, wz1, sel0, sel1, chg0, chg1, heads, dlist, rlist;
+  DBuf<int> in_s, slist, sidx, scnt, soff, sz, wcnt0, wcnt1, woff0, woff1, sel0, sel1, chg0, chg1, heads, dlist, rlist;
+  DBuf<int> wpool[2], wtmp;   // value pools (current, compaction target); per-node counts
+  DBuf<long long> wnoff;      // compaction offsets
+  int wcur = 0;
+  long long scap = 0, wcap = 0;
   int ns = 0;
   ~LiveState() {
     if (argsp) { (void)hipDeviceSynchronize(); (void)hipFree(argsp); }
@@ -1173,20 +1220,53 @@ StopCaps stop_caps_small(int d2) {
 StopCaps stop_caps_full(const LiveArgs& a) { return StopCaps{a.kl, a.kp, kInvCap, kRecipeCap}; }
 
 // k_stop over `n` nodes (list[b], or first_id + b without a list) in `mode`; the caller reads the
-// counters afterwards.
-void launch_stop(hora_hip_ctx* c, LiveState& L, const int* list, long n, long long first_id, int mode, int* scnt,
-                 int* sz) {
+// counters afterwards. Mode 0 writes the stop sets of the closure into L's stop-set pool.
+void launch_stop(hora_hip_ctx* c, LiveState& L, const int* list, long n, long long first_id, int mode) {
   if (n <= 0) return;
   const int d2 = c->d2;
   L.rlist.ensure(n);
   HIP_OK(hipMemsetAsync(&L.lc.p->rcnt, 0, sizeof(int), c->stream));
   const StopCaps KA = stop_caps_small(d2), KB = stop_caps_full(L.args);
+  int* scnt = mode == 0 ? L.scnt.p : nullptr;
+  int* soff = mode == 0 ? L.soff.p : nullptr;
+  int* sz = mode == 0 ? L.sz.p : nullptr;
+  const int scap = mode == 0 ? (int)L.scap : 0;
   hipLaunchKernelGGL(k_stop, dim3(n), dim3(64), KA.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp, list, n,
-                     first_id, mode, KA, L.rlist.p, scnt, sz, L.lc.p);
+                     first_id, mode, KA, L.rlist.p, scnt, soff, sz, scap, L.lc.p);
   const LiveCtr h = read_lc(c, L);
   if (h.rcnt > 0 && !h.err)
     hipLaunchKernelGGL(k_stop, dim3(h.rcnt), dim3(64), KB.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
-                       (const int*)L.rlist.p, (long)h.rcnt, (long long)0, mode, KB, (int*)nullptr, scnt, sz, L.lc.p);
+                       (const int*)L.rlist.p, (long)h.rcnt, (long long)0, mode, KB, (int*)nullptr, scnt, soff, sz,
+                       scap, L.lc.p);
+}
+
+// HORA_HIP_LIVE_POOL=tight: the stop-set and value pools start at the least they can hold and grow
+// by exactly what is asked, so that growth, compaction and the rerun of a round all happen on small
+// inputs (for tests).
+bool pool_tight() {
+  const char* e = std::getenv("HORA_HIP_LIVE_POOL");
+  return e && std::strcmp(e, "tight") == 0;
+}
+
+// Moves every node's current value to the front of the other value pool, sized for `room` more
+// zones, and makes that pool current.
+void compact_values(hora_hip_ctx* c, LiveState& L, int ns, const int* sel, long long room) {
+  L.wtmp.ensure(ns);
+  L.wnoff.ensure(ns);
+  hipLaunchKernelGGL(k_wcount, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, ns, sel, L.wcnt0.p, L.wcnt1.p,
+                     L.wtmp.p);
+  const long long live = scan_exclusive(c, L.wtmp.p, ns, L.wnoff.p);
+  const long long need = live + room;
+  if (need > INT32_MAX) throw HipFail("value pool above 2^31 zones");
+  DBuf<int>& dst = L.wpool[1 - L.wcur];
+  dst.ensure((size_t)need * c->d2);
+  hipLaunchKernelGGL(k_wmove, dim3(ns), dim3(64), 0, c->stream, ns, c->d2, sel, L.wcnt0.p, L.wcnt1.p, L.woff0.p,
+                     L.woff1.p, L.wnoff.p, L.wpool[L.wcur].p, dst.p);
+  L.wcur = 1 - L.wcur;
+  L.wcap = pool_tight() ? need : std::min<long long>((long long)(dst.cap / c->d2), INT32_MAX);
+  const int top = (int)live;
+  HIP_OK(hipMemcpyAsync(&L.lc.p->ptop, &top, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
 }
 
 // Fixpoint launches run first with small LDS lists, so that many wavefronts share a CU; the few
@@ -1465,7 +1545,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
         sync_graph_args(L);
         L.args.gzone = c->zarena.p;
         upload_args(c, L);
-        launch_stop(c, L, nullptr, (long)F, (long long)fbase, q->kind == HORA_LIVE_LEADSTO ? 1 : 2, nullptr, nullptr);
+        launch_stop(c, L, nullptr, (long)F, (long long)fbase, q->kind == HORA_LIVE_LEADSTO ? 1 : 2);
         const LiveCtr h = read_lc(c, L);
         if (h.err) {
           c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a stop set" : "invariant or transition table too large";
@@ -1519,7 +1599,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     int seeds = 0;
     if (nh > 0) {
       HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
-      launch_stop(c, L, L.heads.p, (long)nh, 0, 3, nullptr, nullptr);
+      launch_stop(c, L, L.heads.p, (long)nh, 0, 3);
       const LiveCtr h = read_lc(c, L);
       if (h.err) {
         c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a seed test" : "invariant or transition table too large";
@@ -1555,7 +1635,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     *holds = q->kind == HORA_LIVE_EG ? 0 : 1;
   } else if (q->kind == HORA_LIVE_DEADLOCK) {
     HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
-    launch_stop(c, L, nullptr, (long)N, 0, 2, nullptr, nullptr);
+    launch_stop(c, L, nullptr, (long)N, 0, 2);
     const LiveCtr h = read_lc(c, L);
     if (h.err) { c->err = "federation capacity exceeded in a deadlock test"; return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY; }
     *holds = h.hit ? 0 : 1;
@@ -1577,57 +1657,103 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     st->closure = ns;
     if (ns > 0) {
       hipLaunchKernelGGL(k_sindex, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.slist.p, ns, L.sidx.p);
-      // ---- stop sets
+      // ---- stop sets (into a pool: most nodes have none; grown and run again when it overflows)
       L.scnt.ensure(ns);
-      L.sz.ensure((size_t)ns * kStopCap * d2);
+      L.soff.ensure(ns);
+      const bool tight = pool_tight();
+      if (tight) {
+        L.sz.ensure(1);
+        L.scap = 1;
+      } else if (L.scap < std::max<long long>(ns / 4, 4096)) {
+        L.sz.ensure((size_t)std::max<long long>(ns / 4, 4096) * d2);
+        L.scap = std::min<long long>((long long)(L.sz.cap / d2), INT32_MAX);
+      }
       const auto ts0 = std::chrono::steady_clock::now();
-      launch_stop(c, L, L.slist.p, (long)ns, 0, 0, L.scnt.p, L.sz.p);
-      {
+      for (;;) {
+        HIP_OK(hipMemsetAsync(&L.lc.p->spool, 0, 2 * sizeof(int), c->stream));   // spool, sfull
+        launch_stop(c, L, L.slist.p, (long)ns, 0, 0);
         const LiveCtr h = read_lc(c, L);
-        if (c->trace)
-          std::fprintf(stderr, "stop sets: %d nodes, %d at full capacity, %.3f ms\n", ns, h.rcnt,
-                       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts0).count());
         if (h.err) {
           c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a stop set" : "invariant or transition table too large";
           st->capped = h.err == ERR_CAPPED;
           return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
         }
+        if (c->trace)
+          std::fprintf(stderr, "stop sets: %d nodes, %d at full capacity, %d pool zones of %lld, %.3f ms\n", ns, h.rcnt,
+                       h.spool, L.scap,
+                       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts0).count());
+        if (!h.sfull) break;
+        L.sz.ensure((size_t)h.spool * d2);
+        L.scap = tight ? h.spool : std::min<long long>((long long)(L.sz.cap / d2), INT32_MAX);
       }
       // ---- fixpoint
-      L.wcnt0.ensure(ns); L.wcnt1.ensure(ns);
-      L.wz0.ensure((size_t)ns * kLiveW * d2); L.wz1.ensure((size_t)ns * kLiveW * d2);
+      L.wcnt0.ensure(ns); L.wcnt1.ensure(ns); L.woff0.ensure(ns); L.woff1.ensure(ns);
       L.sel0.ensure(ns); L.sel1.ensure(ns); L.chg0.ensure(ns); L.chg1.ensure(ns);
+      L.wcur = 0;
+      // the seeds (one zone per node) and, mostly, room for two zones per node changed in a round
+      L.wpool[0].ensure((size_t)(3 * (size_t)ns + ns / 2) * d2);
+      L.wcap = tight ? ns : std::min<long long>((long long)(L.wpool[0].cap / d2), INT32_MAX);
       HIP_OK(hipMemsetAsync(L.wcnt1.p, 0, sizeof(int) * ns, c->stream));
-      hipLaunchKernelGGL(k_gseed, dim3(ns), dim3(64), 0, c->stream, L.argsp, L.slist.p, ns, d2, L.wcnt0.p, L.wz0.p,
-                         L.sel0.p, L.chg0.p);
+      hipLaunchKernelGGL(k_gseed, dim3(ns), dim3(64), 0, c->stream, L.argsp, L.slist.p, ns, d2, L.wcnt0.p, L.woff0.p,
+                         L.wpool[0].p, L.sel0.p, L.chg0.p);
+      {
+        const int top = ns;
+        HIP_OK(hipMemcpyAsync(&L.lc.p->ptop, &top, sizeof(int), hipMemcpyHostToDevice, c->stream));
+      }
       int* sel_in = L.sel0.p; int* sel_out = L.sel1.p;
       int* chg_in = L.chg0.p; int* chg_out = L.chg1.p;
       L.dlist.ensure(ns); L.rlist.ensure(ns);
       const GfpCaps KA = gfp_caps_small(d2), KB = gfp_caps_full(L.args);
       int round = 0;
       long long retried = 0;
+      int compactions = 0;
       for (;;) {
         const auto tr0 = std::chrono::steady_clock::now();
-        // changed, dcnt and rcnt are adjacent in LiveCtr
+        // changed; dcnt and rcnt are adjacent in LiveCtr
         HIP_OK(hipMemsetAsync(&L.lc.p->changed, 0, sizeof(int), c->stream));
         HIP_OK(hipMemsetAsync(&L.lc.p->dcnt, 0, 2 * sizeof(int), c->stream));
-        GfpBanks B{L.wcnt0.p, L.wcnt1.p, L.wz0.p, L.wz1.p, sel_in, sel_out, chg_in, chg_out, L.scnt.p, L.sz.p};
+        auto banks = [&]() {
+          return GfpBanks{L.wcnt0.p, L.wcnt1.p, L.woff0.p, L.woff1.p, L.wpool[L.wcur].p, (int)L.wcap,
+                          sel_in, sel_out, chg_in, chg_out, L.scnt.p, L.soff.p, L.sz.p};
+        };
         hipLaunchKernelGGL(k_gdirty, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.argsp, L.slist.p, ns,
-                           round == 0 ? 1 : 0, B, L.dlist.p, L.lc.p);
-        const int nd = read_lc(c, L).dcnt;
-        if (nd > 0)
-          hipLaunchKernelGGL(k_gfp, dim3(nd), dim3(64), KA.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
-                             L.slist.p, L.dlist.p, KA, B, L.rlist.p, L.lc.p);
-        LiveCtr h = read_lc(c, L);
-        if (h.rcnt > 0 && !h.err) {
-          hipLaunchKernelGGL(k_gfp, dim3(h.rcnt), dim3(64), KB.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
-                             L.slist.p, L.rlist.p, KB, B, (int*)nullptr, L.lc.p);
-          retried += h.rcnt;
-          h = read_lc(c, L);
+                           round == 0 ? 1 : 0, banks(), L.dlist.p, L.lc.p);
+        const LiveCtr h0 = read_lc(c, L);
+        const int nd = h0.dcnt;
+        // most values hold one or two zones: room for two per visited node, else compact first
+        if (!tight && (long long)h0.ptop + 2LL * nd > L.wcap) {
+          compact_values(c, L, ns, sel_in, 2LL * nd + ns / 2);
+          ++compactions;
         }
+        LiveCtr h{};
+        for (;;) {
+          if (nd > 0)
+            hipLaunchKernelGGL(k_gfp, dim3(nd), dim3(64), KA.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+                               L.slist.p, L.dlist.p, KA, banks(), L.rlist.p, L.lc.p);
+          h = read_lc(c, L);
+          if (h.rcnt > 0 && !h.err && !h.pfull) {
+            hipLaunchKernelGGL(k_gfp, dim3(h.rcnt), dim3(64), KB.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+                               L.slist.p, L.rlist.p, KB, banks(), (int*)nullptr, L.lc.p);
+            h = read_lc(c, L);
+          }
+          if (!h.pfull || h.err) break;
+          // the pool ran out: drop the round's writes, compact with room for every value at its
+          // largest, and run the round again from the same values
+          LiveCtr r = h;
+          r.pre_calls = h0.pre_calls;
+          r.products = h0.products;
+          r.changed = 0;
+          r.rcnt = 0;
+          r.pfull = 0;
+          HIP_OK(hipMemcpyAsync(L.lc.p, &r, sizeof(LiveCtr), hipMemcpyHostToDevice, c->stream));
+          compact_values(c, L, ns, sel_in, (long long)kLiveW * nd + (tight ? 0 : ns / 2));
+          ++compactions;
+        }
+        retried += h.rcnt;
         if (c->trace)
           std::fprintf(stderr, "gfp round %d: %d visited, %d at full capacity, %d values changed, pre_calls so far %llu,"
-                       " %.3f ms\n", round, h.dcnt, h.rcnt, h.changed, (unsigned long long)h.pre_calls,
+                       " pool %d of %lld zones, %.3f ms\n", round, h.dcnt, h.rcnt, h.changed,
+                       (unsigned long long)h.pre_calls, h.ptop, L.wcap,
                        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tr0).count());
         ++round;
         if (h.err) {
@@ -1641,8 +1767,8 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       }
       st->iters = round;
       if (c->trace)
-        std::fprintf(stderr, "gfp: %d rounds, %lld node visits at full capacity (small lists %d,%d,%d,%d)\n", round,
-                     retried, KA.kw, KA.kc, KA.kp, KA.ic);
+        std::fprintf(stderr, "gfp: %d rounds, %lld node visits at full capacity (small lists %d,%d,%d,%d), %d pool"
+                     " compactions\n", round, retried, KA.kw, KA.kc, KA.kp, KA.ic, compactions);
       hipLaunchKernelGGL(k_gverdict, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, L.slist.p, ns,
                          sel_in, L.wcnt0.p, L.wcnt1.p, L.wne.p, L.lc.p);
       const LiveCtr h = read_lc(c, L);

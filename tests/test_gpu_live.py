@@ -119,6 +119,24 @@ def test_rerun_at_full_capacity(monkeypatch):
             assert r == want
 
 
+def test_pools_grow_and_compact(monkeypatch):
+    """Stop sets and fixpoint values live in pools. With pools that start at the least they can
+    hold and grow by exactly what is asked, the stop-set phase runs again after growing, and rounds
+    compact the value pool and run again; graphs, stop sets and converged values must not change."""
+    monkeypatch.setenv("HORA_HIP_LIVE_POOL", "tight")
+    net = models.fischer(5)
+    for label, q, want in fischer_queries(net):
+        r, _ = check(net, q, label)
+        assert r == want, label
+    net = models.token_ring(6, 2)
+    check(net, lv.leads_to(P.at(net, "P0", "hold"), P.at(net, "P1", "hold"), early=False))
+    for builder in live_models.ALL:
+        net, queries = builder()
+        for q, want in queries:
+            r, _ = check(net, q)
+            assert r == want
+
+
 def test_fischer7_full_graph_parity():
     net = models.fischer(7)
     for cover, gold in ((lv.COVER_AUTO, "fischer_req_wait"), (lv.COVER_AM, "fischer_req_wait_am")):
