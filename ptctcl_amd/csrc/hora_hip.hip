@@ -90,6 +90,7 @@ struct DevNet {
   const u64* kmask;       // key field masks
   const int* int_lo;
   int never, nlr, nic, ncc;
+  int rcap, icap;         // bounds on one state's transitions and one location vector's invariant
   const int* lr;
   const int* ic;
   const int* cc;
@@ -521,21 +522,23 @@ __global__ __launch_bounds__(64) void k_count(const DevNet* __restrict__ Np, con
   int fz;
   u64 am;
   int ninv;
-  int* inv = rec + 2 * kRecipeCap;
-  wave_loc_info(N, locs, ints, &fz, &am, inv, &ninv, 0);   // flags and mask only
-  const int r = build_recipes(N, locs, ints, (fz & 1) != 0, rec, kRecipeCap);
+  wave_loc_info(N, locs, ints, &fz, &am, nullptr, &ninv, 0);   // flags and mask only
+  const int r = build_recipes(N, locs, ints, (fz & 1) != 0, rec, N.rcap);
   if (threadIdx.x == 0) {
     cnt[i] = r < 0 ? 0 : r;
     if (r < 0) atomicExch(&ctr->err, ERR_RECIPES);
   }
 }
 
-// LDS layout of k_expand: 3 zones + row-0 scratch + 2 discrete states + recipes + the source
-// state's invariant triples with their process + per-process flags and active-clock masks.
-__host__ __device__ constexpr int expand_lds_ints(int d2) {
-  return 3 * d2 + HORA_MAX_DIM + 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * kRecipeCap + 4 * kInvCap +
+// LDS layout of k_expand: 3 zones + row-0 scratch + 2 discrete states + recipes (rcap) + the source
+// state's invariant triples with their process (icap) + per-process flags and active-clock masks.
+__host__ __device__ constexpr int expand_lds_ints(int d2, int rcap, int icap) {
+  return 3 * d2 + HORA_MAX_DIM + 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * rcap + 4 * icap +
          3 * HORA_MAX_PROCS + 2;
 }
+
+// LDS of k_count: one discrete state and the recipes.
+inline size_t count_lds_bytes(int rcap) { return (size_t)(HORA_MAX_PROCS + HORA_MAX_INTS + 2 * rcap) * sizeof(int); }
 
 // z_apply_list over the source state's invariant triples, skipping those of processes p1 and p2
 // (which moved): `tag[k]` is the process of triple k.
@@ -596,7 +599,7 @@ __global__ __launch_bounds__(256) void k_expand(const DevNet* __restrict__ Np, c
   extern __shared__ int smem[];
   const DevNet& N = *Np;
   const int wid = (int)(threadIdx.x >> 6);
-  int* sm = smem + wid * expand_lds_ints(N.d2);
+  int* sm = smem + wid * expand_lds_ints(N.d2, N.rcap, N.icap);
   int* src = sm;
   int* z = src + N.d2;
   int* tmp = z + N.d2;
@@ -606,9 +609,9 @@ __global__ __launch_bounds__(256) void k_expand(const DevNet* __restrict__ Np, c
   int* nl = ints + HORA_MAX_INTS;
   int* niv = nl + HORA_MAX_PROCS;
   int* rec = niv + HORA_MAX_INTS;
-  int* sinv = rec + 2 * kRecipeCap;                 // 3 * kInvCap
-  int* stag = sinv + 3 * kInvCap;                   // kInvCap
-  int* sflags = stag + kInvCap;                     // HORA_MAX_PROCS
+  int* sinv = rec + 2 * N.rcap;                     // 3 * icap
+  int* stag = sinv + 3 * N.icap;                    // icap
+  int* sflags = stag + N.icap;                      // HORA_MAX_PROCS
   u64* samask = (u64*)(((uintptr_t)(sflags + HORA_MAX_PROCS) + 7) & ~(uintptr_t)7);   // HORA_MAX_PROCS
   const int i = (int)blockIdx.x * (int)(blockDim.x >> 6) + wid;
   if (i >= F) return;
@@ -639,7 +642,7 @@ __global__ __launch_bounds__(256) void k_expand(const DevNet* __restrict__ Np, c
     fz_src = (__ballot(f & 1) ? 1 : 0) | (__ballot(f & 2) ? 2 : 0);
     if (!(fz_src & 2) && N.has_urgent_chan && urgent_sync_enabled(N, locs, ints)) fz_src |= 2;
     const int o0 = wave_excl_scan(ib - ia, &ns);
-    if (ns > kInvCap) {
+    if (ns > N.icap) {
       if (l == 0) atomicExch(&ctr->err, ERR_RECIPES);
       return;
     }
@@ -755,7 +758,7 @@ __global__ __launch_bounds__(256) void k_expand(const DevNet* __restrict__ Np, c
   }
   // the slots reserved for this state bound what is written: a state filtered out after counting
   // (rec_cnt[i] = 0) writes nothing
-  const int nr = min(build_recipes(N, locs, ints, (fz_src & 1) != 0, rec, kRecipeCap), rec_cnt[i]);
+  const int nr = min(build_recipes(N, locs, ints, (fz_src & 1) != 0, rec, N.rcap), rec_cnt[i]);
   wsync();
   const int base = rec_off[i];
   for (int b0 = 0; b0 < nr; b0 += kWave) {
@@ -1871,7 +1874,7 @@ void account(hora_hip_ctx* c, const Tally& hc, const Tally& prev, long G, int T,
 }
 
 void launch_expand(hora_hip_ctx* c, int wpb, long F, ExpandOut eo, bool init) {
-  const size_t lds = (size_t)wpb * expand_lds_ints(c->d2) * sizeof(int);
+  const size_t lds = (size_t)wpb * expand_lds_ints(c->d2, c->dn.rcap, c->dn.icap) * sizeof(int);
   hipLaunchKernelGGL(k_expand, dim3(blocks_for(F, wpb)), dim3(64 * wpb), lds, c->stream, c->dnp, c->fr_dk.p,
                      c->fr_zid.p, c->zarena.p, (int)F, c->rec_off.p, c->rec_cnt.p, eo, c->ctr.p, init ? 1 : 0);
 }
@@ -1948,8 +1951,7 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     if (!init) {
       st->explored += (u64)F;
       ++level;
-      const size_t lds_c = (HORA_MAX_PROCS + HORA_MAX_INTS + 2 * kRecipeCap + 3 * kInvCap) * sizeof(int);
-      hipLaunchKernelGGL(k_count, dim3(F), dim3(64), lds_c, c->stream, c->dnp,
+      hipLaunchKernelGGL(k_count, dim3(F), dim3(64), count_lds_bytes(c->dn.rcap), c->stream, c->dnp,
                          c->fr_dk.p, (int)F, c->rec_cnt.p, c->ctr.p);
       R = scan_exclusive(c, c->rec_cnt.p, F, c->pos.p);
       hipLaunchKernelGGL(k_offsets_to_int, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->pos.p, F, c->rec_off.p);
@@ -2117,6 +2119,46 @@ std::vector<T> vec(const T* p, long n) {
 
 // Bind a context to a network and query. Callable again on a bound context (a serving slot moving
 // to its next job): the stream, events and every device buffer are kept, only the tables change.
+// Upper bounds, from the network's tables, on the transitions of one discrete state (local edges
+// plus binary sender/receiver pairs, each process at its busiest location) and on the combined
+// invariant of one location vector (each process at its location with the most constraints),
+// capped at kRecipeCap and kInvCap: k_count's and k_expand's LDS is sized by them, so a small
+// network leaves room for more resident wavefronts. A state beyond the caps is an error as before.
+void expand_caps(const hora_net_enc* e, int* rcap, int* icap) {
+  const int np = e->nprocs, nch = e->nchans, tl = e->total_locs;
+  std::vector<int> lcnt(tl, 0);
+  std::vector<int> scnt((size_t)tl * std::max(nch, 1), 0), rcnt((size_t)tl * std::max(nch, 1), 0);
+  for (int k = 0; k < e->nedges; ++k) {
+    const int g = e->locbase[e->edge_proc[k]] + e->edge_src[k];
+    const int ch = e->edge_chan[k];
+    if (ch < 0) ++lcnt[g];
+    else if (e->edge_dir[k] == 1) ++scnt[(size_t)g * nch + ch];
+    else if (e->edge_dir[k] == -1) ++rcnt[(size_t)g * nch + ch];
+  }
+  long long local = 0, inv = 0, bin = 0;
+  std::vector<long long> smax((size_t)np * std::max(nch, 1), 0), rmax((size_t)np * std::max(nch, 1), 0);
+  for (int p = 0; p < np; ++p) {
+    int lm = 0, im = 0;
+    for (int g = e->locbase[p]; g < e->locbase[p + 1]; ++g) {
+      lm = std::max(lm, lcnt[g]);
+      im = std::max(im, e->inv_off[g + 1] - e->inv_off[g]);
+      for (int ch = 0; ch < nch; ++ch) {
+        smax[(size_t)p * nch + ch] = std::max<long long>(smax[(size_t)p * nch + ch], scnt[(size_t)g * nch + ch]);
+        rmax[(size_t)p * nch + ch] = std::max<long long>(rmax[(size_t)p * nch + ch], rcnt[(size_t)g * nch + ch]);
+      }
+    }
+    local += lm;
+    inv += im;
+  }
+  for (int ch = 0; ch < nch; ++ch) {
+    long long rtot = 0;
+    for (int p = 0; p < np; ++p) rtot += rmax[(size_t)p * nch + ch];
+    for (int p = 0; p < np; ++p) bin += smax[(size_t)p * nch + ch] * (rtot - rmax[(size_t)p * nch + ch]);
+  }
+  *rcap = (int)std::max<long long>(1, std::min<long long>(kRecipeCap, local + bin));
+  *icap = (int)std::max<long long>(1, std::min<long long>(kInvCap, inv));
+}
+
 void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
   if (!e) throw ScopeFail("null network");
   c->h_kshift.clear();
@@ -2273,6 +2315,7 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
   N.blob_ints = (int)blob.size();
   N.stage = 0;   // staging the tables in LDS measured slower than the L1/L2-resident originals
   N.never = never; N.nlr = nlr; N.nic = nic; N.ncc = ncc;
+  expand_caps(e, &N.rcap, &N.icap);
   if (!c->dnp) HIP_OK(hipMalloc(&c->dnp, sizeof(DevNet)));
   HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));

@@ -1391,8 +1391,8 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       c->pos.ensure(F + 1);
       if (!init) {
         ++st->levels;
-        const size_t lds_c = (HORA_MAX_PROCS + HORA_MAX_INTS + 2 * kRecipeCap + 3 * kInvCap) * sizeof(int);
-        hipLaunchKernelGGL(k_count, dim3(F), dim3(64), lds_c, c->stream, c->dnp, c->fr_dk.p, (int)F, c->rec_cnt.p, c->ctr.p);
+        hipLaunchKernelGGL(k_count, dim3(F), dim3(64), count_lds_bytes(c->dn.rcap), c->stream, c->dnp, c->fr_dk.p, (int)F,
+                           c->rec_cnt.p, c->ctr.p);
         hipLaunchKernelGGL(k_gfilter, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, eg ? 1 : 0,
                            c->fr_dk.p, (int)F, c->rec_cnt.p, L.expanded.p, (long)fbase);
         R = scan_exclusive(c, c->rec_cnt.p, F, c->pos.p);
