@@ -591,7 +591,9 @@ __device__ __forceinline__ u64 bcast64(u64 v, int lane) {
 // LDS; the zone side then runs transition by transition over the whole wavefront, reading those
 // values with readlane. The combined invariant of a successor is the source state's triples
 // without the moved processes', plus the moved processes' new locations'.
-__global__ __launch_bounds__(256) void k_expand(const DevNet* __restrict__ Np, const u64* __restrict__ fr_dk,
+// Eight wavefronts per SIMD (64 VGPRs, no spills to memory) measured ~5 % faster than the seven
+// the compiler picks by itself.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_expand(const DevNet* __restrict__ Np, const u64* __restrict__ fr_dk,
                                                 const u32* __restrict__ fr_zid, const int* __restrict__ zarena,
                                                 int F, const int* __restrict__ rec_off,
                                                 const int* __restrict__ rec_cnt, ExpandOut out,
