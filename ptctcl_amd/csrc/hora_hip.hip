@@ -1756,7 +1756,24 @@ struct hora_hip_ctx {
   LiveState* live = nullptr;
   int expand_wpb = 0;         // wavefronts per k_expand workgroup, picked by expand_tuned()
   int expand_wpb_d = 0;       // the zone dimension it was picked for
+  void* hpin = nullptr;       // pinned host staging for the small counter reads of every level
 };
+
+namespace {
+
+constexpr size_t kPinBytes = 64 * 1024;
+
+// A small device-to-host read through the context's pinned staging buffer: a direct DMA into pinned
+// memory, not a copy through the runtime's staging of pageable memory.
+void read_dev(hora_hip_ctx* c, void* dst, const void* src, size_t n) {
+  if (!c->hpin) HIP_OK(hipHostMalloc(&c->hpin, kPinBytes, hipHostMallocDefault));
+  if (n > kPinBytes) throw HipFail("read_dev: request larger than the staging buffer");
+  HIP_OK(hipMemcpyAsync(c->hpin, src, n, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  std::memcpy(dst, c->hpin, n);
+}
+
+}  // namespace
 
 namespace {
 
@@ -1774,8 +1791,7 @@ long long scan_exclusive(hora_hip_ctx* c, const int* in, long n, long long* out)
   if (n > 0)
     hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanBlock), 0, c->stream, in, n, c->bsum.p, out);
   long long total = 0;
-  HIP_OK(hipMemcpyAsync(&total, c->scan_total.p, sizeof(total), hipMemcpyDeviceToHost, c->stream));
-  HIP_OK(hipStreamSynchronize(c->stream));
+  read_dev(c, &total, c->scan_total.p, sizeof(total));
   return total;
 }
 
@@ -1846,8 +1862,7 @@ struct Tally {
 
 Tally read_counters(hora_hip_ctx* c) {
   Counters hc;
-  HIP_OK(hipMemcpyAsync(&hc, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
-  HIP_OK(hipStreamSynchronize(c->stream));
+  read_dev(c, &hc, c->ctr.p, sizeof(Counters));
   Tally t;
   for (int s = 0; s < kStripes; ++s)
     for (int k = 0; k < C_NSTAT; ++k) t.v[k] += hc.st[s][k];
@@ -1921,8 +1936,9 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
   c->dn.lu_plus = 0;
   HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
 
-  // passed list and arenas
-  alloc_table(c, 20);
+  // passed list and arenas; a context searched before starts at the table size it grew to (slot
+  // positions do not affect any result: work lists, frontiers and counts follow successor order)
+  alloc_table(c, std::max(20, c->hbits));
   c->zarena.ensure((size_t)d2 << 20);
   c->nodes.ensure(1 << 22);
   c->ctr.ensure(1);
@@ -1985,8 +2001,7 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
       // stops; generated counts those plus the target itself (engine.cpp:2085-2095).
       long long before = 0;
       if (hc.first_target > 0)
-        HIP_OK(hipMemcpyAsync(&before, c->pos.p + hc.first_target, sizeof(before), hipMemcpyDeviceToHost, c->stream));
-      HIP_OK(hipStreamSynchronize(c->stream));
+        read_dev(c, &before, c->pos.p + hc.first_target, sizeof(before));
       if (!init) st->generated += (u64)before + 1;
       st->found = 1;
       *reachable = 1;
@@ -2015,8 +2030,7 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p, G,
                        c->gticket.p, c->touched.p, c->ctr.p);
     int T = 0;
-    HIP_OK(hipMemcpyAsync(&T, &c->ctr.p->ntouched, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
+    read_dev(c, &T, &c->ctr.p->ntouched, sizeof(int));
     c->caps.ensure(T);
     c->segs.ensure(T);
     hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
@@ -2043,8 +2057,7 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     hipLaunchKernelGGL(k_classify, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
                        c->klight.p, c->kheavy.p, c->ctr.p, kLightMax);
     int nlh[2] = {0, 0};
-    HIP_OK(hipMemcpyAsync(nlh, &c->ctr.p->nlight, sizeof(nlh), hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
+    read_dev(c, nlh, &c->ctr.p->nlight, sizeof(nlh));
     const size_t lds_l = merge_layout(d2, false).total() * sizeof(int);
     const size_t lds_h = merge_layout(d2, true).total() * sizeof(int);
     HIP_OK(hipEventRecord(c->ev[3], c->stream));
@@ -2368,6 +2381,7 @@ void hora_hip_close(hora_hip_ctx* ctx) {
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->dnp) (void)hipFree(ctx->dnp);
   if (ctx->live) live_free(ctx->live);
+  if (ctx->hpin) (void)hipHostFree(ctx->hpin);
   delete ctx;
 }
 

@@ -1170,8 +1170,7 @@ void grow_keep_to(DBuf<T>& b, size_t n, hipStream_t s) {
 
 LiveCtr read_lc(hora_hip_ctx* c, LiveState& L) {
   LiveCtr h;
-  HIP_OK(hipMemcpyAsync(&h, L.lc.p, sizeof(LiveCtr), hipMemcpyDeviceToHost, c->stream));
-  HIP_OK(hipStreamSynchronize(c->stream));
+  read_dev(c, &h, L.lc.p, sizeof(LiveCtr));
   return h;
 }
 
@@ -1351,7 +1350,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     }
     c->dn.lu_plus = 0;
     HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
-    alloc_table(c, 16);
+    alloc_table(c, std::max(16, c->hbits));
     c->ctr.ensure(1);
     HIP_OK(hipMemsetAsync(c->ctr.p, 0, sizeof(Counters), c->stream));
     {
@@ -1448,8 +1447,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
         hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p, G,
                            c->gticket.p, c->touched.p, c->ctr.p);
         int T = 0;
-        HIP_OK(hipMemcpyAsync(&T, &c->ctr.p->ntouched, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIP_OK(hipStreamSynchronize(c->stream));
+        read_dev(c, &T, &c->ctr.p->ntouched, sizeof(int));
         if (T > 0) {
           c->caps.ensure(T);
           c->segs.ensure(T);
@@ -1476,8 +1474,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
           hipLaunchKernelGGL(k_gsplit, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
                              c->klight.p, c->kheavy.p, c->ctr.p);
           int nlh[2] = {0, 0};
-          HIP_OK(hipMemcpyAsync(nlh, &c->ctr.p->nlight, sizeof(nlh), hipMemcpyDeviceToHost, c->stream));
-          HIP_OK(hipStreamSynchronize(c->stream));
+          read_dev(c, nlh, &c->ctr.p->nlight, sizeof(nlh));
           if (nlh[0] > 0)
             hipLaunchKernelGGL(k_gmerge, dim3(nlh[0]), dim3(64), lds_m, c->stream, c->dnp, P, c->klight.p, nlh[0],
                                c->wlist.p, c->sj.p, c->s_dk.p, c->s_zone.p, L.tslot.p, L.fnew.p, c->ctr.p, alu);
