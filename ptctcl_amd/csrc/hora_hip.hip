@@ -1154,7 +1154,9 @@ __host__ __device__ inline MergeLayout merge_layout(int d2, bool heavy) {
   m.zs = d2 | 1;   // odd stride: lanes reading different cached zones hit different banks
   m.sort_cap = heavy ? kSortLds : 64;
   m.chain_cap = heavy ? 2048 : 64;
-  const int budget = heavy ? 60 * 1024 / 4 : 12 * 1024 / 4;   // ints
+  // LDS per workgroup (ints): the light budget leaves the chain cache at its two-zone minimum and
+  // lets registers, not LDS, bound the resident wavefronts; both picked by timing fischer 12
+  const int budget = heavy ? 30 * 1024 / 4 : 6 * 1024 / 4;
   const int fixed = 4 * d2 + 2 * HORA_MAX_DIM + HORA_MAX_PROCS + HORA_MAX_INTS + 16 + 32 + m.sort_cap + 2 * m.chain_cap;
   m.cz = max(2, min(heavy ? 1024 : 16, (budget - fixed) / m.zs));
   return m;
@@ -1168,7 +1170,8 @@ __host__ __device__ inline MergeLayout merge_layout(int d2, bool heavy) {
 // lane, and evaluate the 32-zone union window in one round, taking the first success in
 // newest-first order as the sequential fold does. At the end the key's live new zones get zone
 // ids in one allocation, move to the arena, and are reported as this level's survivors.
-__global__ __launch_bounds__(64) void k_merge(const DevNet* __restrict__ Np, Passed P, const u32* __restrict__ list, int nlist,
+// Six wavefronts per SIMD (77 VGPRs, no spills) measured faster than the compiler's five.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_merge(const DevNet* __restrict__ Np, Passed P, const u32* __restrict__ list, int nlist,
                                               int* __restrict__ wlist, const int* __restrict__ sj,
                                               int* __restrict__ szone, int* __restrict__ survived,
                                               Counters* ctr, int heavy, u64 zbase) {
