@@ -35,6 +35,7 @@ namespace {
 constexpr int kLiveMaxDim = 23;
 constexpr int ERR_CAPPED = 5;
 constexpr int kLiveW = 8;       // zones in one value W[v]
+constexpr u32 kProbeSerial = 16;  // node segments up to this length are probed one node at a time
 
 struct DevPred {
   int nterms;
@@ -139,8 +140,9 @@ __device__ __forceinline__ void cover_bounds(const DevNet& N, const int* lg, int
 }
 
 // The first node stored before this level that covers the successor: one wavefront per successor,
-// the key's node segment (oldest first) tested 64 nodes at a time, one node per lane; the lowest
-// covering lane is the answer.
+// the key's node segment oldest first. Short segments test one node at a time with the lanes over
+// the matrix entries (coalesced reads, stop at the first cover); long ones 64 nodes at a time, one
+// node per lane, the lowest covering lane being the answer.
 __global__ __launch_bounds__(64) void k_gprobe(const DevNet* __restrict__ Np, Passed P, const long* __restrict__ gbk,
                                                const int* __restrict__ sj, const u64* __restrict__ sdk,
                                                const int* __restrict__ szone, long G, const int* __restrict__ gzone,
@@ -164,15 +166,22 @@ __global__ __launch_bounds__(64) void k_gprobe(const DevNet* __restrict__ Np, Pa
     if (l < d) cover_bounds(N, lg, l, alu, &Lb[l], &Ub[l]);
     wsync();
     const u32* seg = P.nodes + P.off[h];
-    for (u32 b = 0; b < len && found < 0; b += kWave) {
-      const u32 m = b + (u32)l;
-      bool ok = false;
-      if (m < len) {
-        ok = alu_le_lane(S, gzone + (size_t)seg[m] * d2, d, Lb, Ub);
-        ++reads;
+    if (len <= kProbeSerial) {
+      for (u32 m = 0; m < len && found < 0; ++m) {
+        if (l == 0) ++reads;
+        if (z_alu_le(S, gzone + (size_t)seg[m] * d2, d, Lb, Ub)) found = (int)seg[m];
       }
-      const u64 bal = __ballot(ok);
-      if (bal) found = (int)seg[b + __builtin_ctzll(bal)];
+    } else {
+      for (u32 b = 0; b < len && found < 0; b += kWave) {
+        const u32 m = b + (u32)l;
+        bool ok = false;
+        if (m < len) {
+          ok = alu_le_lane(S, gzone + (size_t)seg[m] * d2, d, Lb, Ub);
+          ++reads;
+        }
+        const u64 bal = __ballot(ok);
+        if (bal) found = (int)seg[b + __builtin_ctzll(bal)];
+      }
     }
   }
   if (l == 0) {
