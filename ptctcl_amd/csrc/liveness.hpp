@@ -139,57 +139,59 @@ __device__ __forceinline__ void cover_bounds(const DevNet& N, const int* lg, int
   *Uc = ub;
 }
 
-// The first node stored before this level that covers the successor: one wavefront per successor,
-// the key's node segment oldest first. Short segments test one node at a time with the lanes over
-// the matrix entries (coalesced reads, stop at the first cover); long ones 64 nodes at a time, one
-// node per lane, the lowest covering lane being the answer.
+// The first node stored before this level that covers the successor. Four 16-lane groups per
+// wavefront, one successor per group, lanes over the matrix entries (coalesced reads of both
+// zones); the group walks the key's node segment oldest first and stops at the first cover.
 __global__ __launch_bounds__(64) void k_gprobe(const DevNet* __restrict__ Np, Passed P, const long* __restrict__ gbk,
                                                const int* __restrict__ sj, const u64* __restrict__ sdk,
                                                const int* __restrict__ szone, long G, const int* __restrict__ gzone,
                                                int* __restrict__ covered, int* __restrict__ cov_old, Counters* ctr,
                                                int alu) {
-  __shared__ int S[HORA_MAX_DIM * HORA_MAX_DIM], Lb[HORA_MAX_DIM], Ub[HORA_MAX_DIM], lg[HORA_MAX_PROCS];
+  __shared__ int Lg[4][HORA_MAX_DIM], Ug[4][HORA_MAX_DIM], lg[4][HORA_MAX_PROCS];
   const DevNet& N = *Np;
-  const int l = threadIdx.x;
-  const long g = blockIdx.x;
-  if (g >= G) return;
+  const int lane = threadIdx.x, q = lane / kGroup, t = lane % kGroup;
+  const long g = (long)blockIdx.x * 4 + q;
+  const bool active = g < G;
   const int d = N.d, d2 = N.d2;
-  const long h = gbk[g];
+  const long h = active ? gbk[g] : -1;
   const u32 len = (h >= 0) ? P.len[h] : 0u;
-  int found = -1, reads = 0;
   if (len > 0) {
     const u64 dk = sdk[sj[g]];
-    if (l < N.np) lg[l] = (int)((dk >> N.kshift[l]) & N.kmask[l]);
-    const int* src = szone + (size_t)sj[g] * d2;
-    for (int e = l; e < d2; e += kWave) S[e] = src[e];
-    wsync();
-    if (l < d) cover_bounds(N, lg, l, alu, &Lb[l], &Ub[l]);
-    wsync();
-    const u32* seg = P.nodes + P.off[h];
-    if (len <= kProbeSerial) {
-      for (u32 m = 0; m < len && found < 0; ++m) {
-        if (l == 0) ++reads;
-        if (z_alu_le(S, gzone + (size_t)seg[m] * d2, d, Lb, Ub)) found = (int)seg[m];
-      }
-    } else {
-      for (u32 b = 0; b < len && found < 0; b += kWave) {
-        const u32 m = b + (u32)l;
-        bool ok = false;
-        if (m < len) {
-          ok = alu_le_lane(S, gzone + (size_t)seg[m] * d2, d, Lb, Ub);
-          ++reads;
-        }
-        const u64 bal = __ballot(ok);
-        if (bal) found = (int)seg[b + __builtin_ctzll(bal)];
-      }
-    }
+    for (int p = t; p < N.np; p += kGroup) lg[q][p] = (int)((dk >> N.kshift[p]) & N.kmask[p]);
   }
-  if (l == 0) {
+  __syncthreads();
+  if (len > 0)
+    for (int c = t; c < d; c += kGroup) cover_bounds(N, lg[q], c, alu, &Lg[q][c], &Ug[q][c]);
+  __syncthreads();
+  const int* S = szone + (size_t)sj[active ? g : 0] * d2;
+  const u64 gmask = 0xffffull << (q * kGroup);
+  const int* L = Lg[q];
+  const int* U = Ug[q];
+  const u32* seg = P.nodes + (len > 0 ? P.off[h] : 0);
+  int found = -1, reads = 0;
+  for (u32 m = 0; m < len; ++m) {   // group-uniform loop
+    const int* M = gzone + (size_t)seg[m] * d2;
+    ++reads;
+    bool wit = false;   // S <=_aLU M fails (the predicate of z_alu_le)
+    for (int e = t; e < d2; e += kGroup) {
+      const int y = e / d, x = e - y * d;
+      if (x == y) continue;
+      const int ux = U[x], ly = L[y];
+      if (ux == kAluNegInf || ly == kAluNegInf) continue;
+      const int s0x = S[x];
+      if (s0x < r_le(-ux)) continue;
+      const int mp = M[e];
+      if (mp < S[e] && radd(mp, r_lt(-ly)) < s0x) wit = true;
+    }
+    if ((__ballot(wit) & gmask) == 0) { found = (int)seg[m]; break; }
+  }
+  const bool lead = active && t == 0;
+  if (lead) {
     covered[g] = found >= 0 ? 1 : 0;
     cov_old[g] = found;
   }
-  wave_stat(ctr, C_PREADS, (unsigned long long)reads);
-  wave_stat(ctr, C_PSUB, (l == 0 && found >= 0) ? 1ull : 0ull);
+  wave_stat(ctr, C_PREADS, lead ? (unsigned long long)reads : 0ull);
+  wave_stat(ctr, C_PSUB, (lead && found >= 0) ? 1ull : 0ull);
 }
 
 // One wavefront per touched key: its remaining successors in (state, transition) order, each to
@@ -246,12 +248,19 @@ __global__ __launch_bounds__(64) void k_gmerge(const DevNet* __restrict__ Np, Pa
       const int* src = szone + (size_t)sj[g] * d2;
       for (int e = l; e < d2; e += kWave) S[e] = src[e];
       wsync();
-      for (int cb = 0; cb < nc && first < 0; cb += kWave) {
-        const int c = cb + l;
-        bool ok = false;
-        if (c < nc) { ok = alu_le_lane(S, szone + (size_t)sj[list[c]] * d2, d, Lb, Ub); ++reads; }
-        const u64 b = __ballot(ok);
-        if (b) first = cb + __builtin_ctzll(b);
+      if (nc <= (int)kProbeSerial) {   // few creators: one at a time, lanes over entries
+        for (int c = 0; c < nc && first < 0; ++c) {
+          if (l == 0) ++reads;
+          if (z_alu_le(S, szone + (size_t)sj[list[c]] * d2, d, Lb, Ub)) first = c;
+        }
+      } else {
+        for (int cb = 0; cb < nc && first < 0; cb += kWave) {
+          const int c = cb + l;
+          bool ok = false;
+          if (c < nc) { ok = alu_le_lane(S, szone + (size_t)sj[list[c]] * d2, d, Lb, Ub); ++reads; }
+          const u64 b = __ballot(ok);
+          if (b) first = cb + __builtin_ctzll(b);
+        }
       }
     }
     if (first >= 0) {
@@ -1451,7 +1460,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
         HIP_OK(hipMemsetAsync(&c->ctr.p->ntouched, 0, sizeof(int), c->stream));
         hipLaunchKernelGGL(k_find, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->sj.p, c->s_dk.p, G, c->gbk.p, c->ctr.p);
         HIP_OK(hipEventRecord(c->ev[1], c->stream));
-        hipLaunchKernelGGL(k_gprobe, dim3(G), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p, c->s_dk.p,
+        hipLaunchKernelGGL(k_gprobe, dim3(blocks_for(G, 4)), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p, c->s_dk.p,
                            c->s_zone.p, G, c->zarena.p, c->covered.p, L.cov_old.p, c->ctr.p, alu);
         hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p, G,
                            c->gticket.p, c->touched.p, c->ctr.p);
