@@ -431,7 +431,7 @@ __device__ __forceinline__ void z_reset_many(int* z, int d, u64 mask, int* tmp) 
   wsync();
   const int l = lane_id(), d2 = d * d;
   for (int e = l; e < d2; e += kWave) {
-    const int i = e / d, j = e - i * d;
+    const int i = ediv(e, d), j = e - i * d;
     const int si = ((mask >> i) & 1) ? 0 : i, sj = ((mask >> j) & 1) ? 0 : j;
     tmp[e] = z[si * d + sj];
   }
@@ -977,7 +977,7 @@ __global__ __launch_bounds__(64) void k_probe(const DevNet* __restrict__ Np, Pas
     ++reads;
     bool wit = false;
     for (int e = t; e < d2; e += kGroup) {
-      const int y = e / d, x = e - y * d;
+      const int y = ediv(e, d), x = e - y * d;
       if (x == y) continue;
       const int ux = U[x], ly = L[y];
       if (ux == kAluNegInf || ly == kAluNegInf) continue;
@@ -1123,13 +1123,13 @@ __device__ __forceinline__ bool union_convex_lane(const int* a, const int* b, in
     for (int wf = 0; wf < 4; ++wf)
       for (u64 mf = pass == 0 ? fa[wf] : pa[wf]; mf; mf &= mf - 1) {
         const int f = wf * 64 + __builtin_ctzll(mf);
-        const int i = f / d, j = f - i * d;
+        const int i = ediv(f, d), j = f - i * d;
         const int neg = r_neg(z[f]);
 #pragma unroll
         for (int wp = 0; wp < 4; ++wp)
           for (u64 mp = pass == 0 ? pa[wp] : fa[wp]; mp; mp &= mp - 1) {
             const int e = wp * 64 + __builtin_ctzll(mp);
-            const int p = e / d, q = e - p * d;
+            const int p = ediv(e, d), q = e - p * d;
             const int hpj = max(a[p * d + j], b[p * d + j]);
             const int hiq = max(a[i * d + q], b[i * d + q]);
             if (radd(radd(hpj, neg), hiq) > o[e]) return false;
@@ -1498,7 +1498,7 @@ __global__ void k_zop_stream(ZopArgs A) {
   if (e >= A.n * d2) return;
   const long z = e / d2;
   const int k = (int)(e - z * d2);
-  const int i = k / d, j = k - i * d;
+  const int i = ediv(k, d), j = k - i * d;
   const int* src = A.a + z * d2;
   int v = src[k];
   if (A.op == HORA_ZOP_UP) {

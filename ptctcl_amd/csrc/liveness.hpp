@@ -174,7 +174,7 @@ __global__ __launch_bounds__(64) void k_gprobe(const DevNet* __restrict__ Np, Pa
     ++reads;
     bool wit = false;   // S <=_aLU M fails (the predicate of z_alu_le)
     for (int e = t; e < d2; e += kGroup) {
-      const int y = e / d, x = e - y * d;
+      const int y = ediv(e, d), x = e - y * d;
       if (x == y) continue;
       const int ux = U[x], ly = L[y];
       if (ux == kAluNegInf || ly == kAluNegInf) continue;
@@ -675,7 +675,7 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
     if (!frozen && A.kind != HORA_LIVE_DEADLOCK && !has_upper_list(tinv, no)) return;
     wsync();
     for (int e = l; e < d2; e += kWave) {
-      const int i = e / d, j = e - i * d;
+      const int i = ediv(e, d), j = e - i * d;
       zu[e] = (i == j || i == 0) ? kLe0 : kInf;
     }
     zv = zu;
@@ -691,7 +691,7 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
       if (N.nclocks > 0 && !has_upper_list(tinv, no) && !frozen) {
         wsync();
         for (int e = l; e < d2; e += kWave) {
-          const int i = e / d, j = e - i * d;
+          const int i = ediv(e, d), j = e - i * d;
           t[e] = (i == j || i == 0) ? kLe0 : kInf;
         }
         if (z_apply_list(t, d, tinv, no)) {

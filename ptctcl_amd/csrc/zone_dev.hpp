@@ -24,6 +24,14 @@ constexpr int kWave = 64;
 // may hold several wavefronts, each on a zone of its own.
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
+// Row of entry e of a d x d matrix, e < d*d, d <= 32, by a multiply and a shift: with
+// M = ceil(2^16 / d) = (2^16 + r) / d, r < d, floor(e*M / 2^16) = floor(e / d) since e*r < 2^16.
+// (A division by a run-time d costs a dozen vector instructions per entry.)
+__device__ __forceinline__ int ediv(int e, int d) {
+  const unsigned m = (65536u + (unsigned)d - 1u) / (unsigned)d;
+  return (int)(((unsigned)e * m) >> 16);
+}
+
 __device__ __forceinline__ int r_le(int v) { return v * 2 + 1; }
 __device__ __forceinline__ int r_lt(int v) { return v * 2; }
 __device__ __forceinline__ int r_bnd(int r) { return r >> 1; }
@@ -104,7 +112,7 @@ __device__ __forceinline__ bool z_constrain(int* z, int d, int a, int b, int r) 
   wsync();
   const int d2 = d * d;
   for (int e = l; e < d2; e += kWave) {
-    const int i = e / d, k = e - i * d;
+    const int i = ediv(e, d), k = e - i * d;
     const int ib = z[i * d + b];
     if (ib >= kInf) continue;
     const int v = radd(ib, z[b * d + k]);
@@ -120,7 +128,7 @@ __device__ __forceinline__ void z_close(int* z, int d) {
   for (int k = 0; k < d; ++k) {
     wsync();
     for (int e = l; e < d2; e += kWave) {
-      const int i = e / d, j = e - i * d;
+      const int i = ediv(e, d), j = e - i * d;
       const int ik = z[i * d + k];
       if (ik >= kInf) continue;
       const int v = radd(ik, z[k * d + j]);
@@ -137,7 +145,7 @@ __device__ __forceinline__ void z_extra_lu_plus(int* z, int d, const int* L, con
   if (l < d) row0[l] = z[l];
   wsync();
   for (int e = l; e < d2; e += kWave) {
-    const int i = e / d, j = e - i * d;
+    const int i = ediv(e, d), j = e - i * d;
     if (i == j) continue;
     const int v = z[e];
     if (v >= kInf) continue;
@@ -160,7 +168,7 @@ __device__ __forceinline__ bool z_alu_le(const int* Z, const int* Zp, int d, con
   const int l = lane_id(), d2 = d * d;
   bool wit = false;
   for (int e = l; e < d2; e += kWave) {
-    const int y = e / d, x = e - y * d;
+    const int y = ediv(e, d), x = e - y * d;
     if (x == y) continue;
     const int ux = U[x], ly = L[y];
     if (ux == kAluNegInf || ly == kAluNegInf) continue;
@@ -216,7 +224,7 @@ __device__ __forceinline__ bool z_try_union(int* a, const int* b, int d, int* fa
     for (int t = l; t < pairs && !bad; t += kWave) {
       // pass 1: facet of a from FA, entry from PA, compared against b
       const int f = fa[t / np], e = pa[t % np];
-      const int i = f / d, j = f - i * d, p = e / d, q = e - p * d;
+      const int i = ediv(f, d), j = f - i * d, p = ediv(e, d), q = e - p * d;
       const int neg = r_neg(a[f]);
       const int hpj = max(a[p * d + j], b[p * d + j]);
       const int hiq = max(a[i * d + q], b[i * d + q]);
@@ -225,7 +233,7 @@ __device__ __forceinline__ bool z_try_union(int* a, const int* b, int d, int* fa
     for (int t = l; t < pairs && !bad; t += kWave) {
       // pass 2: facet of b from PA, entry from FA, compared against a
       const int f = pa[t / nf], e = fa[t % nf];
-      const int i = f / d, j = f - i * d, p = e / d, q = e - p * d;
+      const int i = ediv(f, d), j = f - i * d, p = ediv(e, d), q = e - p * d;
       const int neg = r_neg(b[f]);
       const int hpj = max(a[p * d + j], b[p * d + j]);
       const int hiq = max(a[i * d + q], b[i * d + q]);
@@ -257,7 +265,7 @@ __device__ __forceinline__ void z_free_many(int* z, int d, uint64_t mask, int* t
   wsync();
   const int l = lane_id(), d2 = d * d;
   for (int e = l; e < d2; e += kWave) {
-    const int i = e / d, j = e - i * d;
+    const int i = ediv(e, d), j = e - i * d;
     int v;
     if (i == j) v = kLe0;
     else if ((mask >> i) & 1) v = kInf;
@@ -275,7 +283,7 @@ __device__ __forceinline__ bool z_disjoint_quick(const int* a, const int* b, int
   const int l = lane_id(), d2 = d * d;
   bool neg = false;
   for (int e = l; e < d2; e += kWave) {
-    const int i = e / d, j = e - i * d;
+    const int i = ediv(e, d), j = e - i * d;
     if (radd(a[e], b[j * d + i]) < kLe0) neg = true;
   }
   return __any(neg);
