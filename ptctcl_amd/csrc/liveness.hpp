@@ -593,11 +593,10 @@ __device__ __forceinline__ int fed_covers(const int* F, int n, int zs, const int
 
 // LDS capacities of one k_stop launch: zones of the escape list (kl) and of each subtraction piece
 // list (kp), invariant constraints of a location vector (ic), transitions of a state (rc).
+// np, ni: the network's processes and integer variables (two discrete states are held).
 struct StopCaps {
-  int kl, kp, ic, rc;
-  __host__ __device__ int ints(int d2) const {
-    return (6 + kl + 2 * kp) * d2 + 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * 3 * ic + 2 * rc;
-  }
+  int kl, kp, ic, rc, np, ni;
+  __host__ __device__ int ints(int d2) const { return (6 + kl + 2 * kp) * d2 + 2 * (np + ni) + 2 * 3 * ic + 2 * rc; }
 };
 
 // Stop set of node v (mode 0), early witness test (mode 1: p /\ !q nodes only), deadlock test
@@ -622,10 +621,10 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
   int* cur = live + K.kl * d2;
   int* nxt = cur + K.kp * d2;
   int* locs = nxt + K.kp * d2;
-  int* ints = locs + HORA_MAX_PROCS;
-  int* tl = ints + HORA_MAX_INTS;
-  int* ti = tl + HORA_MAX_PROCS;
-  int* inv = ti + HORA_MAX_INTS;
+  int* ints = locs + K.np;
+  int* tl = ints + K.ni;
+  int* ti = tl + K.np;
+  int* inv = ti + K.ni;
   int* tinv = inv + 3 * K.ic;
   int* rec = tinv + 3 * K.ic;
   const long b = blockIdx.x;
@@ -881,11 +880,12 @@ struct GfpBanks {
 
 // LDS capacities of one k_gfp launch: zones of the value W[v] (kw), of the candidate federation
 // (kc), of each subtraction piece list (kp), and invariant constraints (ic) of each location vector.
+// np, ni: the network's processes and integer variables (two discrete states are held).
 struct GfpCaps {
-  int kw, kc, kp, ic;
+  int kw, kc, kp, ic, np, ni;
   __host__ __device__ int drop() const { return kc > kp ? kc : kp; }
   __host__ __device__ int ints(int d2) const {
-    return (5 + kw + kc + 2 * kp) * d2 + 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * 3 * ic + drop();
+    return (4 + kw + kc + 2 * kp) * d2 + 2 * (np + ni) + 2 * 3 * ic + drop();
   }
 };
 
@@ -931,15 +931,15 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
   int* tmp = pz + d2;
   int* r = tmp + d2;
   int* t = r + d2;
-  int* Wv = t + 2 * d2;
+  int* Wv = t + d2;
   int* cand = Wv + K.kw * d2;
   int* cur = cand + K.kc * d2;
   int* nxt = cur + K.kp * d2;
   int* locs = nxt + K.kp * d2;
-  int* ints = locs + HORA_MAX_PROCS;
-  int* tlocs = ints + HORA_MAX_INTS;
-  int* tints = tlocs + HORA_MAX_PROCS;
-  int* inv = tints + HORA_MAX_INTS;
+  int* ints = locs + K.np;
+  int* tlocs = ints + K.ni;
+  int* tints = tlocs + K.np;
+  int* inv = tints + K.ni;
   int* tinv = inv + 3 * K.ic;
   int* drop = tinv + 3 * K.ic;
   const int s = list[blockIdx.x];
@@ -1222,19 +1222,22 @@ bool live_caps(int d2, LiveArgs& a) {
 
 // Stop-set launches, like the fixpoint's, run first with small lists and rerun the nodes that outgrow
 // them with the full ones. HORA_HIP_STOP_CAPS=kl,kp,ic,rc sets the small lists (escape zones, piece
-// zones, invariant constraints, transitions).
-StopCaps stop_caps_small(int d2) {
-  StopCaps k{2, 2, 32, 64};
+// zones, invariant constraints, transitions). Invariant and transition lists never exceed the
+// network's bounds (DevNet::icap, rcap), and the discrete states take the network's sizes.
+StopCaps stop_caps_small(const DevNet& dn) {
+  StopCaps k{2, 2, std::min(32, dn.icap), std::min(64, dn.rcap), dn.np, std::max(dn.ni, 1)};
   if (const char* e = std::getenv("HORA_HIP_STOP_CAPS")) {
-    StopCaps o{};
+    StopCaps o = k;
     if (std::sscanf(e, "%d,%d,%d,%d", &o.kl, &o.kp, &o.ic, &o.rc) == 4 && o.kl >= 1 && o.kp >= 1 && o.ic >= 1 &&
-        o.ic <= kInvCap && o.rc >= 1 && o.rc <= kRecipeCap && o.ints(d2) <= 64 * 1024 / 4)
+        o.ic <= kInvCap && o.rc >= 1 && o.rc <= kRecipeCap && o.ints(dn.d2) <= 64 * 1024 / 4)
       k = o;
   }
   return k;
 }
 
-StopCaps stop_caps_full(const LiveArgs& a) { return StopCaps{a.kl, a.kp, kInvCap, kRecipeCap}; }
+StopCaps stop_caps_full(const LiveArgs& a, const DevNet& dn) {
+  return StopCaps{a.kl, a.kp, dn.icap, dn.rcap, dn.np, std::max(dn.ni, 1)};
+}
 
 // k_stop over `n` nodes (list[b], or first_id + b without a list) in `mode`; the caller reads the
 // counters afterwards. Mode 0 writes the stop sets of the closure into L's stop-set pool.
@@ -1243,7 +1246,7 @@ void launch_stop(hora_hip_ctx* c, LiveState& L, const int* list, long n, long lo
   const int d2 = c->d2;
   L.rlist.ensure(n);
   HIP_OK(hipMemsetAsync(&L.lc.p->rcnt, 0, sizeof(int), c->stream));
-  const StopCaps KA = stop_caps_small(d2), KB = stop_caps_full(L.args);
+  const StopCaps KA = stop_caps_small(c->dn), KB = stop_caps_full(L.args, c->dn);
   int* scnt = mode == 0 ? L.scnt.p : nullptr;
   int* soff = mode == 0 ? L.soff.p : nullptr;
   int* sz = mode == 0 ? L.sz.p : nullptr;
@@ -1289,18 +1292,20 @@ void compact_values(hora_hip_ctx* c, LiveState& L, int ns, const int* sel, long 
 // Fixpoint launches run first with small LDS lists, so that many wavefronts share a CU; the few
 // nodes that outgrow them rerun with the full lists of live_caps. HORA_HIP_GFP_CAPS=kw,kc,kp,ic sets
 // the small lists (zones of the value, of the candidates, of each piece list; invariant constraints).
-GfpCaps gfp_caps_small(int d2) {
-  GfpCaps k{2, 8, 4, 32};
+GfpCaps gfp_caps_small(const DevNet& dn) {
+  GfpCaps k{2, 8, 4, std::min(32, dn.icap), dn.np, std::max(dn.ni, 1)};
   if (const char* e = std::getenv("HORA_HIP_GFP_CAPS")) {
-    GfpCaps o{};
+    GfpCaps o = k;
     if (std::sscanf(e, "%d,%d,%d,%d", &o.kw, &o.kc, &o.kp, &o.ic) == 4 && o.kw >= 1 && o.kw <= kLiveW &&
-        o.kc >= 1 && o.kp >= 2 && o.ic >= 1 && o.ic <= kInvCap && o.ints(d2) <= 64 * 1024 / 4)
+        o.kc >= 1 && o.kp >= 2 && o.ic >= 1 && o.ic <= kInvCap && o.ints(dn.d2) <= 64 * 1024 / 4)
       k = o;
   }
   return k;
 }
 
-GfpCaps gfp_caps_full(const LiveArgs& a) { return GfpCaps{kLiveW, a.kc, a.kp, kInvCap}; }
+GfpCaps gfp_caps_full(const LiveArgs& a, const DevNet& dn) {
+  return GfpCaps{kLiveW, a.kc, a.kp, dn.icap, dn.np, std::max(dn.ni, 1)};
+}
 
 int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_live_stats* st) {
   const auto t0 = std::chrono::steady_clock::now();
@@ -1718,7 +1723,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       int* sel_in = L.sel0.p; int* sel_out = L.sel1.p;
       int* chg_in = L.chg0.p; int* chg_out = L.chg1.p;
       L.dlist.ensure(ns); L.rlist.ensure(ns);
-      const GfpCaps KA = gfp_caps_small(d2), KB = gfp_caps_full(L.args);
+      const GfpCaps KA = gfp_caps_small(c->dn), KB = gfp_caps_full(L.args, c->dn);
       int round = 0;
       long long retried = 0;
       int compactions = 0;
