@@ -1768,34 +1768,55 @@ constexpr size_t kPinBytes = 64 * 1024;
 
 // A small device-to-host read through the context's pinned staging buffer: a direct DMA into pinned
 // memory, not a copy through the runtime's staging of pageable memory.
-void read_dev(hora_hip_ctx* c, void* dst, const void* src, size_t n) {
+// Two reads share one synchronisation (n2 = 0: one read).
+void read_dev(hora_hip_ctx* c, void* dst, const void* src, size_t n, void* dst2 = nullptr, const void* src2 = nullptr,
+              size_t n2 = 0) {
   if (!c->hpin) HIP_OK(hipHostMalloc(&c->hpin, kPinBytes, hipHostMallocDefault));
-  if (n > kPinBytes) throw HipFail("read_dev: request larger than the staging buffer");
+  const size_t o2 = (n + 255) & ~(size_t)255;
+  if (o2 + n2 > kPinBytes) throw HipFail("read_dev: request larger than the staging buffer");
   HIP_OK(hipMemcpyAsync(c->hpin, src, n, hipMemcpyDeviceToHost, c->stream));
+  if (n2) HIP_OK(hipMemcpyAsync((char*)c->hpin + o2, src2, n2, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   std::memcpy(dst, c->hpin, n);
+  if (n2) std::memcpy(dst2, (char*)c->hpin + o2, n2);
 }
 
 }  // namespace
 
 namespace {
 
-long long scan_exclusive(hora_hip_ctx* c, const int* in, long n, long long* out) {
+// Exclusive scan of in[0, n) into out, the total into scan_total[slot] (no synchronisation).
+void scan_launch(hora_hip_ctx* c, const int* in, long n, long long* out, int slot) {
   const long per = (long)kScanBlock * kScanItems;
   const long nb = std::max<long>(1, (n + per - 1) / per);
   c->bsum.ensure(nb);
-  c->scan_total.ensure(1);
+  c->scan_total.ensure(2);
   if (n > 0) {
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, c->stream, in, n, c->bsum.p);
   } else {
     HIP_OK(hipMemsetAsync(c->bsum.p, 0, sizeof(long long), c->stream));
   }
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, c->stream, c->bsum.p, nb, c->scan_total.p);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, c->stream, c->bsum.p, nb, c->scan_total.p + slot);
   if (n > 0)
     hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanBlock), 0, c->stream, in, n, c->bsum.p, out);
+}
+
+long long scan_exclusive(hora_hip_ctx* c, const int* in, long n, long long* out) {
+  scan_launch(c, in, n, out, 0);
   long long total = 0;
   read_dev(c, &total, c->scan_total.p, sizeof(total));
   return total;
+}
+
+// Two scans, one synchronisation.
+void scan_exclusive2(hora_hip_ctx* c, const int* in1, long n, long long* out1, const int* in2, long long* out2,
+                     long long* t1, long long* t2) {
+  scan_launch(c, in1, n, out1, 0);
+  scan_launch(c, in2, n, out2, 1);
+  long long t[2] = {0, 0};
+  read_dev(c, t, c->scan_total.p, sizeof(t));
+  *t1 = t[0];
+  *t2 = t[1];
 }
 
 inline unsigned blocks_for(long n, int bs) { return (unsigned)std::max<long>(1, (n + bs - 1) / bs); }
@@ -1863,9 +1884,7 @@ struct Tally {
   unsigned long long operator[](int k) const { return v[k]; }
 };
 
-Tally read_counters(hora_hip_ctx* c) {
-  Counters hc;
-  read_dev(c, &hc, c->ctr.p, sizeof(Counters));
+Tally tally_of(const Counters& hc) {
   Tally t;
   for (int s = 0; s < kStripes; ++s)
     for (int k = 0; k < C_NSTAT; ++k) t.v[k] += hc.st[s][k];
@@ -1875,6 +1894,22 @@ Tally read_counters(hora_hip_ctx* c) {
   t.nlight = hc.nlight;
   t.nheavy = hc.nheavy;
   return t;
+}
+
+Tally read_counters(hora_hip_ctx* c) {
+  Counters hc;
+  read_dev(c, &hc, c->ctr.p, sizeof(Counters));
+  return tally_of(hc);
+}
+
+// A scan and the counters in one synchronisation.
+long long scan_and_counters(hora_hip_ctx* c, const int* in, long n, long long* out, Tally* t) {
+  scan_launch(c, in, n, out, 0);
+  long long total = 0;
+  Counters hc;
+  read_dev(c, &total, c->scan_total.p, sizeof(total), &hc, c->ctr.p, sizeof(Counters));
+  *t = tally_of(hc);
+  return total;
 }
 
 // Per-level device time and algorithmic HBM bytes (int32 entries, 4 B each):
@@ -1991,11 +2026,11 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     HIP_OK(hipEventRecord(c->ev[1], c->stream));
     // ---- compaction of non-empty successors
     c->pos.ensure(R + 1);
-    const long Gall = R > 0 ? scan_exclusive(c, c->s_valid.p, R, c->pos.p) : 0;
+    Tally hc;
+    const long Gall = scan_and_counters(c, c->s_valid.p, R, c->pos.p, &hc);
     c->prof.expand_ms += ev_ms(c->ev[0], c->ev[1]);
     c->prof.expand_bytes += 4.0 * d2 * (double)((init ? 0 : F) + Gall);
     c->prof.expand_launches += 1;
-    Tally hc = read_counters(c);
     if (!init) st->expanded += (u64)F;
     long G = Gall;
     const bool found = hc.first_target != 0x7fffffff;
@@ -2040,8 +2075,8 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
                        c->caps.p, c->segs.p);
     c->pos.ensure(T + 1);
     c->pos2.ensure(T + 1);
-    const long long W = scan_exclusive(c, c->caps.p, T, c->pos.p);
-    const long long Sseg = scan_exclusive(c, c->segs.p, T, c->pos2.p);
+    long long W = 0, Sseg = 0;
+    scan_exclusive2(c, c->caps.p, T, c->pos.p, c->segs.p, c->pos2.p, &W, &Sseg);
     // capacity: one zone id per work-list position, the reserved chain segments
     if (zcount + (u64)W > P.zcap) c->zarena.grow_keep((size_t)(zcount + (u64)W) * 2 * d2, c->stream);
     if (ncount + (u64)Sseg > P.ncap) c->nodes.grow_keep((size_t)(ncount + (u64)Sseg) * 2, c->stream);
@@ -2084,12 +2119,11 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     c->flag_i.ensure(G);
     hipLaunchKernelGGL(k_flag_survivors, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, c->survived.p, G, c->flag_i.p);
     c->pos.ensure(G + 1);
-    const long F2 = scan_exclusive(c, c->flag_i.p, G, c->pos.p);
+    const long F2 = scan_and_counters(c, c->flag_i.p, G, c->pos.p, &hc);
     c->fr_dk2.ensure(F2);
     c->fr_zid2.ensure(F2);
     hipLaunchKernelGGL(k_build_frontier, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, c->survived.p, c->pos.p, G,
                        c->sj.p, c->s_dk.p, c->fr_dk2.p, c->fr_zid2.p);
-    hc = read_counters(c);
     account(c, hc, prev, G, T, F2);
     if (c->trace)
       std::fprintf(stderr, "level %d F=%ld R=%ld G=%ld probe_cov=%llu T=%d (light %d heavy %d) W=%lld F2=%ld | exp %.3f probe %.3f merge %.3f ms | reads p=%llu m=%llu unions=%llu\n",
