@@ -1174,7 +1174,8 @@ __host__ __device__ inline MergeLayout merge_layout(int d2, bool heavy) {
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_merge(const DevNet* __restrict__ Np, Passed P, const u32* __restrict__ list, int nlist,
                                               int* __restrict__ wlist, const int* __restrict__ sj,
                                               int* __restrict__ szone, int* __restrict__ survived,
-                                              Counters* ctr, int heavy, u64 zbase) {
+                                              Counters* ctr, int heavy, int* __restrict__ zscr,
+                                              int* __restrict__ used) {
   extern __shared__ int sm[];
   const DevNet& N = *Np;
   const int d = N.d, d2 = N.d2, l = threadIdx.x;
@@ -1397,23 +1398,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
     ++n_ins;
     csync();
   }
-  // Live new zones move to the arena, each at the zone id reserved for its work-list position
-  // (zbase + noff + k); the chain is compacted oldest first into this key's reserved segment.
+  // Live new zones go to the level's scratch slot of their work-list position (noff + k), named
+  // kNewTag | slot in the chain and in `survived` until k_zmove gives them compact zone ids; the
+  // chain is compacted oldest first into this key's reserved segment.
   wsync();
   u32* out = P.nodes + seg;
-  const u64 zk = zbase + P.noff[h];
+  const u32 sk = (u32)P.noff[h];
   int w = 0;
   for (int m = 0; m < len; ++m) {
     const u32 id = ch[m];
     if (id & kDead) continue;
     if (id & kNewTag) {
       const u32 kpos = id & (kNewTag - 1);
-      const u32 zid = (u32)(zk + kpos);
+      const u32 slot = sk + kpos;
       const int* src = zp(m);
-      int* dst = P.zarena + (size_t)zid * d2;
+      int* dst = zscr + (size_t)slot * d2;
       for (int e = l; e < d2; e += kWave) dst[e] = src[e];
       wsync();
-      if (l == 0) { out[w] = zid; survived[lst[kpos]] = (int)zid; }
+      if (l == 0) {
+        out[w] = kNewTag | slot;
+        survived[lst[kpos]] = (int)(kNewTag | slot);
+        used[slot] = 1;
+      }
     } else {
       wsync();
       if (l == 0) out[w] = id;
@@ -1447,6 +1453,30 @@ __global__ void k_classify(Passed P, const u32* __restrict__ touched, int T, u32
   if (ok && !lt) heavy[ih] = h;
 }
 
+// Zone ids for the level's surviving new zones, compact and in slot order: slot -> zbase + rank.
+// One wavefront per touched key walks its chain: each provisional entry kNewTag | slot gets its
+// zone id, and the zone moves from the scratch slot into the arena.
+__global__ __launch_bounds__(64) void k_zassign(Passed P, const u32* __restrict__ touched, int T,
+                                                const long long* __restrict__ rank, const int* __restrict__ zscr,
+                                                u64 zbase, int d2) {
+  const int t = blockIdx.x;
+  if (t >= T) return;
+  const u32 h = touched[t];
+  u32* c = P.nodes + P.off[h];
+  const u32 len = P.len[h];
+  const int l = threadIdx.x;
+  for (u32 m = 0; m < len; ++m) {
+    const u32 id = c[m];
+    if (!(id & kNewTag)) continue;
+    const u32 slot = id & (kNewTag - 1);
+    const u64 zid = zbase + (u64)rank[slot];
+    const int* src = zscr + (size_t)slot * d2;
+    int* dst = P.zarena + (size_t)zid * d2;
+    for (int e = l; e < d2; e += kWave) dst[e] = src[e];
+    if (l == 0) c[m] = (u32)zid;
+  }
+}
+
 // next frontier from survivors, in compacted-successor order
 __global__ void k_flag_survivors(const int* __restrict__ survived, long G, int* __restrict__ flag) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1455,14 +1485,15 @@ __global__ void k_flag_survivors(const int* __restrict__ survived, long G, int* 
 
 __global__ void k_build_frontier(const int* __restrict__ survived, const long long* __restrict__ pos, long G,
                                  const int* __restrict__ sj, const u64* __restrict__ sdk,
-                                 u64* __restrict__ fr_dk, u32* __restrict__ fr_zid) {
+                                 u64* __restrict__ fr_dk, u32* __restrict__ fr_zid,
+                                 const long long* __restrict__ rank, u64 zbase) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= G) return;
   const int id = survived[g];
   if (id < 0) return;
   const long p = pos[g];
   fr_dk[p] = sdk[sj[g]];
-  fr_zid[p] = (u32)id;
+  fr_zid[p] = (u32)(zbase + (u64)rank[(u32)id & (kNewTag - 1)]);   // survivors are this level's new zones
 }
 
 __global__ void k_rehash(Passed Pold, Passed Pnew, u64 Hold, Counters* ctr) {
@@ -1750,7 +1781,8 @@ struct hora_hip_ctx {
   DBuf<u64> s_dk;
   DBuf<long> gbk;
   DBuf<u32> gticket, touched, klight, kheavy;
-  DBuf<long long> pos, pos2, bsum, scan_total;
+  DBuf<long long> pos, pos2, bsum, scan_total, zrank;
+  DBuf<int> zscr, zused;      // a level's surviving new zones before they get zone ids
   DBuf<Counters> ctr;
   // profile of the last search
   hora_reach_profile prof{};
@@ -1902,14 +1934,17 @@ Tally read_counters(hora_hip_ctx* c) {
   return tally_of(hc);
 }
 
-// A scan and the counters in one synchronisation.
-long long scan_and_counters(hora_hip_ctx* c, const int* in, long n, long long* out, Tally* t) {
+// A scan and the counters in one synchronisation; `slot1`, if given, receives the total of a scan
+// launched before into slot 1.
+long long scan_and_counters(hora_hip_ctx* c, const int* in, long n, long long* out, Tally* t,
+                            long long* slot1 = nullptr) {
   scan_launch(c, in, n, out, 0);
-  long long total = 0;
+  long long total[2] = {0, 0};
   Counters hc;
-  read_dev(c, &total, c->scan_total.p, sizeof(total), &hc, c->ctr.p, sizeof(Counters));
+  read_dev(c, total, c->scan_total.p, sizeof(total), &hc, c->ctr.p, sizeof(Counters));
   *t = tally_of(hc);
-  return total;
+  if (slot1) *slot1 = total[1];
+  return total[0];
 }
 
 // Per-level device time and algorithmic HBM bytes (int32 entries, 4 B each):
@@ -2077,7 +2112,8 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     c->pos2.ensure(T + 1);
     long long W = 0, Sseg = 0;
     scan_exclusive2(c, c->caps.p, T, c->pos.p, c->segs.p, c->pos2.p, &W, &Sseg);
-    // capacity: one zone id per work-list position, the reserved chain segments
+    // capacity: the reserved chain segments, and zone ids for at most W surviving new zones (the
+    // ids are handed out compactly after the merge)
     if (zcount + (u64)W > P.zcap) c->zarena.grow_keep((size_t)(zcount + (u64)W) * 2 * d2, c->stream);
     if (ncount + (u64)Sseg > P.ncap) c->nodes.grow_keep((size_t)(ncount + (u64)Sseg) * 2, c->stream);
     P = passed_view(c);
@@ -2086,7 +2122,11 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     c->wlist.ensure(W + 1);
     hipLaunchKernelGGL(k_scatter, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p,
                        c->gticket.p, G, c->wlist.p);
-    // ---- ordered merge of the kept successors: light keys, then heavy keys
+    // ---- ordered merge of the kept successors: light keys, then heavy keys; surviving new zones
+    // wait in a scratch slot per work-list position
+    c->zscr.ensure((size_t)std::max<long long>(W, 1) * d2);
+    c->zused.ensure(W + 1);
+    HIP_OK(hipMemsetAsync(c->zused.p, 0, (size_t)(W + 1) * sizeof(int), c->stream));
     c->survived.ensure(G);
     HIP_OK(hipMemsetAsync(c->survived.p, 0xff, (size_t)G * sizeof(int), c->stream));
     c->klight.ensure(T);
@@ -2101,13 +2141,20 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     HIP_OK(hipEventRecord(c->ev[3], c->stream));
     if (nlh[0] > 0)
       hipLaunchKernelGGL(k_merge, dim3(nlh[0]), dim3(64), lds_l, c->stream, c->dnp, P, c->klight.p, nlh[0],
-                         c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 0, zcount);
+                         c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 0, c->zscr.p, c->zused.p);
     if (nlh[1] > 0)
       hipLaunchKernelGGL(k_merge, dim3(nlh[1]), dim3(64), lds_h, c->stream, c->dnp, P, c->kheavy.p, nlh[1],
-                         c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 1, zcount);
+                         c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 1, c->zscr.p, c->zused.p);
     HIP_OK(hipEventRecord(c->ev[4], c->stream));
-    zcount += (u64)W;
     ncount += (u64)Sseg;
+    // zone ids: the surviving new zones of the level, compact, in slot order (the count is read
+    // with the frontier's scan below)
+    c->zrank.ensure(W + 1);
+    scan_launch(c, c->zused.p, W, c->zrank.p, 1);
+    const u64 zbase = zcount;
+    if (T > 0)
+      hipLaunchKernelGGL(k_zassign, dim3((unsigned)T), dim3(64), 0, c->stream, P, c->touched.p, T, c->zrank.p,
+                         c->zscr.p, zbase, d2);
     if (found) {
       hc = read_counters(c);
       account(c, hc, prev, G, T, 0);
@@ -2119,11 +2166,13 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     c->flag_i.ensure(G);
     hipLaunchKernelGGL(k_flag_survivors, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, c->survived.p, G, c->flag_i.p);
     c->pos.ensure(G + 1);
-    const long F2 = scan_and_counters(c, c->flag_i.p, G, c->pos.p, &hc);
+    long long nsurv = 0;
+    const long F2 = scan_and_counters(c, c->flag_i.p, G, c->pos.p, &hc, &nsurv);
+    zcount += (u64)nsurv;
     c->fr_dk2.ensure(F2);
     c->fr_zid2.ensure(F2);
     hipLaunchKernelGGL(k_build_frontier, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, c->survived.p, c->pos.p, G,
-                       c->sj.p, c->s_dk.p, c->fr_dk2.p, c->fr_zid2.p);
+                       c->sj.p, c->s_dk.p, c->fr_dk2.p, c->fr_zid2.p, c->zrank.p, zbase);
     account(c, hc, prev, G, T, F2);
     if (c->trace)
       std::fprintf(stderr, "level %d F=%ld R=%ld G=%ld probe_cov=%llu T=%d (light %d heavy %d) W=%lld F2=%ld | exp %.3f probe %.3f merge %.3f ms | reads p=%llu m=%llu unions=%llu\n",
