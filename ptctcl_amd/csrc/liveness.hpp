@@ -982,52 +982,64 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
   u32 covm = 0;
   const u32 full = (nW >= 32) ? 0xffffffffu : ((1u << nW) - 1);
   int nc = 0;
-  for (long long e = e0; e < e1; ++e) {
-    const int tt = A.etgt[e];
-    if (!A.in_s[tt]) continue;
-    const int st = A.sidx[tt];
-    const int bt = B.sel_in[st];
-    const int nY = wcnt(bt)[st];
-    if (nY == 0) continue;
-    wsync();
-    decode_key(N, A.gdk[tt], tlocs, tints);
-    wsync();
-    int fzt = 0, ntinv = 0;
-    u64 amt = 0;
-    wave_loc_info(N, tlocs, tints, &fzt, &amt, tinv, &ntinv, K.ic);
-    if (ntinv > K.ic) { overflow(ERR_RECIPES); return; }
-    const u64 dead_t = ~amt & cm;
-    const int se = A.ese[e], re = A.ere[e];
-    const u64 rm = edge_reset_mask(N, se) | (re >= 0 ? edge_reset_mask(N, re) : 0ull);
-    const int* Ys = B.wz + (size_t)woff(bt)[st] * d2;
-    for (int y = 0; y < nY; ++y) {
-      ++pcalls;
-      // exact_predecessor (operators.cpp:257-306)
-      zcopy_w(pz, Ys + y * d2, d2);
-      z_free_many(pz, d, dead_t, tmp);
-      if (!z_apply_list(pz, d, tinv, ntinv)) continue;
-      if (!fzt) {
-        z_down(pz, d);
+  for (long long eb = e0; eb < e1; eb += kWave) {
+    // One successor edge per lane: its target's value (count, pool offset) in the bank selected
+    // for this round, its key, transition and resets, all read in one round of loads; the edges
+    // whose target holds a non-empty value are then taken in order.
+    const long long e = eb + l;
+    int nYl = 0, offl = 0, sel = -1, rel = -1;
+    u64 dkl = 0, rml = 0;
+    if (e < e1) {
+      const int tt = A.etgt[e];
+      if (A.in_s[tt]) {
+        const int st = A.sidx[tt];
+        const int bt = B.sel_in[st];
+        nYl = wcnt(bt)[st];
+        if (nYl > 0) {
+          offl = woff(bt)[st];
+          dkl = A.gdk[tt];
+          sel = A.ese[e];
+          rel = A.ere[e];
+          rml = edge_reset_mask(N, sel) | (rel >= 0 ? edge_reset_mask(N, rel) : 0ull);
+        }
+      }
+    }
+    for (u64 act = __ballot(nYl > 0); act; act &= act - 1) {
+      const int q = __builtin_ctzll(act);
+      const int nY = bcast(nYl, q), se = bcast(sel, q), re = bcast(rel, q);
+      const u64 rm = bcast64(rml, q);
+      wsync();
+      decode_key(N, bcast64(dkl, q), tlocs, tints);
+      wsync();
+      int fzt = 0, ntinv = 0;
+      u64 amt = 0;
+      wave_loc_info(N, tlocs, tints, &fzt, &amt, tinv, &ntinv, K.ic);
+      if (ntinv > K.ic) { overflow(ERR_RECIPES); return; }
+      const u64 dead_t = ~amt & cm;
+      const int* Ys = B.wz + (size_t)bcast(offl, q) * d2;
+      for (int y = 0; y < nY; ++y) {
+        ++pcalls;
+        // exact_predecessor (operators.cpp:257-306)
+        zcopy_w(pz, Ys + y * d2, d2);
+        z_free_many(pz, d, dead_t, tmp);
         if (!z_apply_list(pz, d, tinv, ntinv)) continue;
+        if (!fzt) {
+          z_down(pz, d);
+          if (!z_apply_list(pz, d, tinv, ntinv)) continue;
+        }
+        if (rm && !z_zero_free(pz, d, rm, tmp)) continue;   // resets to zero, then freed
+        if (!apply_edge_guard(N, se, pz)) continue;
+        if (re >= 0 && !apply_edge_guard(N, re, pz)) continue;
+        if (!fzv) z_down(pz, d);
+        if (!z_apply_list(pz, d, inv, ninv)) continue;
+        for (int k = 0; k < nW; ++k)
+          if (!((covm >> k) & 1) && z_includes(pz, Wv + k * d2, d)) covm |= 1u << k;
+        if (covm == full) { keep(); return; }   // every member covered: the value cannot move
+        if (nc >= K.kc) nc = fed_compact(cand, nc, d2, d, drop);
+        if (nc >= K.kc) { overflow(ERR_CAPPED); return; }
+        zcopy_w(cand + nc * d2, pz, d2);
+        ++nc;
       }
-      bool ok = true;
-      for (u64 m = rm; m && ok; m &= m - 1) {
-        const int rc = __builtin_ctzll(m);
-        ok = z_constrain(pz, d, rc, 0, kLe0) && z_constrain(pz, d, 0, rc, kLe0);
-      }
-      if (!ok) continue;
-      z_free_many(pz, d, rm, tmp);
-      if (!apply_edge_guard(N, se, pz)) continue;
-      if (re >= 0 && !apply_edge_guard(N, re, pz)) continue;
-      if (!fzv) z_down(pz, d);
-      if (!z_apply_list(pz, d, inv, ninv)) continue;
-      for (int k = 0; k < nW; ++k)
-        if (!((covm >> k) & 1) && z_includes(pz, Wv + k * d2, d)) covm |= 1u << k;
-      if (covm == full) { keep(); return; }   // every member covered: the value cannot move
-      if (nc >= K.kc) nc = fed_compact(cand, nc, d2, d, drop);
-      if (nc >= K.kc) { overflow(ERR_CAPPED); return; }
-      zcopy_w(cand + nc * d2, pz, d2);
-      ++nc;
     }
   }
   nc = fed_compact(cand, nc, d2, d, drop);

@@ -277,6 +277,45 @@ __device__ __forceinline__ void z_free_many(int* z, int d, uint64_t mask, int* t
   for (int e = l; e < d2; e += kWave) z[e] = tmp[e];
 }
 
+// Z /\ (x_c = 0 for every clock c of `mask`), then those clocks freed: the exact predecessor's
+// "resets to zero, then free" in two passes. The equalities merge M = mask + {0} into one node
+// with zero-weight edges, so on a canonical Z a shortest path enters M once:
+// z'[i][j] = min(z[i][j], min_{a in M} z[i][a] + min_{b in M} z[b][j]), and the conjunction is
+// empty iff some z[a][b] (a != b in M) admits no zero difference. The result is the canonical
+// zone the clock-by-clock constrain and free produce. `tmp` is 2*d ints of LDS scratch.
+__device__ __forceinline__ bool z_zero_free(int* z, int d, uint64_t mask, int* tmp) {
+  wsync();
+  const int l = lane_id(), d2 = d * d;
+  const uint64_t M = mask | 1ull;
+  bool bad = false;
+  for (int e = l; e < d2; e += kWave) {
+    const int i = ediv(e, d), j = e - i * d;
+    if (i != j && ((M >> i) & 1) && ((M >> j) & 1) && z[e] < kLe0) bad = true;
+  }
+  if (__any(bad)) return false;
+  if (l < d) {
+    int rmin = kInf, cmin = kInf;
+    for (uint64_t m = M; m; m &= m - 1) {
+      const int a = __builtin_ctzll(m);
+      rmin = min(rmin, z[l * d + a]);
+      cmin = min(cmin, z[a * d + l]);
+    }
+    tmp[l] = rmin;
+    tmp[d + l] = cmin;
+  }
+  wsync();
+  for (int e = l; e < d2; e += kWave) {
+    const int i = ediv(e, d), j = e - i * d;
+    int v;
+    if (i == j) v = kLe0;
+    else if ((mask >> i) & 1) v = kInf;
+    else if ((mask >> j) & 1) v = min(z[i * d], radd(tmp[i], tmp[d]));
+    else v = min(z[e], radd(tmp[i], tmp[d + j]));
+    z[e] = v;
+  }
+  return true;
+}
+
 // A two-clock cycle through both zones is negative: the zones are certainly disjoint.
 __device__ __forceinline__ bool z_disjoint_quick(const int* a, const int* b, int d) {
   wsync();
