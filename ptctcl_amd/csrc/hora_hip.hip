@@ -595,7 +595,7 @@ __device__ __forceinline__ u64 bcast64(u64 v, int lane) {
 // the compiler picks by itself.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_expand(const DevNet* __restrict__ Np, const u64* __restrict__ fr_dk,
                                                 const u32* __restrict__ fr_zid, const int* __restrict__ zarena,
-                                                int F, const int* __restrict__ rec_off,
+                                                int F, const long long* __restrict__ rec_off,
                                                 const int* __restrict__ rec_cnt, ExpandOut out,
                                                 Counters* ctr, int init) {
   extern __shared__ int smem[];
@@ -762,7 +762,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   // (rec_cnt[i] = 0) writes nothing
   const int nr = min(build_recipes(N, locs, ints, (fz_src & 1) != 0, rec, N.rcap), rec_cnt[i]);
   wsync();
-  const int base = rec_off[i];
+  const int base = (int)rec_off[i];
   for (int b0 = 0; b0 < nr; b0 += kWave) {
     int p1 = -1, g1 = 0, p2 = -1, g2 = 0, fz = 0, td = 0;
     u64 am = 0, rm = 0, key = 0;
@@ -866,9 +866,11 @@ __global__ void k_offsets_to_int(const long long* __restrict__ in, long n, int* 
 }
 
 // compaction of the non-empty successor slots: sj[pos[r]] = r where valid[r]
+// (with `ctr`, also clears the touched-key count for the level's tickets)
 __global__ void k_compact_valid(const int* __restrict__ valid, const long long* __restrict__ pos, long R,
-                                int* __restrict__ sj) {
+                                int* __restrict__ sj, Counters* ctr) {
   const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r == 0 && ctr) ctr->ntouched = 0;
   if (r < R && valid[r]) sj[pos[r]] = (int)r;
 }
 
@@ -907,10 +909,15 @@ __device__ __forceinline__ long find_or_insert(Passed P, u64 dk, bool* inserted)
 }
 
 // successor -> passed-list slot of its discrete key (inserting new keys)
+// (with `survived`, also clears the successor's survivor id and frontier flag)
 __global__ void k_find(Passed P, const int* __restrict__ sj, const u64* __restrict__ sdk, long G,
-                       long* __restrict__ gbk, Counters* ctr) {
+                       long* __restrict__ gbk, Counters* ctr, int* __restrict__ survived, int* __restrict__ flag) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   bool ins = false;
+  if (g < G && survived) {
+    survived[g] = -1;
+    flag[g] = 0;
+  }
   if (g < G) {
     const long h = find_or_insert(P, sdk[sj[g]], &ins);
     if (h < 0) atomicExch(&ctr->err, ERR_HASH_FULL);
@@ -1020,9 +1027,11 @@ __host__ __device__ inline u64 list_cap(u64 n) {
 // Per touched key: its work-list capacity, and the chain segment it reserves for this level
 // (twice the working chain, so that a working chain longer than the LDS copy fits beside its
 // cache-slot table).
+// (with `ctr`, also clears the light/heavy key counts)
 __global__ void k_touched_caps(Passed P, const u32* __restrict__ touched, int T, int* __restrict__ caps,
-                               int* __restrict__ segs) {
+                               int* __restrict__ segs, Counters* ctr) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0 && ctr) { ctr->nlight = 0; ctr->nheavy = 0; }
   if (t >= T) return;
   const u32 h = touched[t];
   caps[t] = (int)list_cap(P.ncnt[h]);
@@ -1175,7 +1184,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
                                               int* __restrict__ wlist, const int* __restrict__ sj,
                                               int* __restrict__ szone, int* __restrict__ survived,
                                               Counters* ctr, int heavy, int* __restrict__ zscr,
-                                              int* __restrict__ used) {
+                                              int* __restrict__ used, int* __restrict__ flag) {
   extern __shared__ int sm[];
   const DevNet& N = *Np;
   const int d = N.d, d2 = N.d2, l = threadIdx.x;
@@ -1418,6 +1427,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
       if (l == 0) {
         out[w] = kNewTag | slot;
         survived[lst[kpos]] = (int)(kNewTag | slot);
+        flag[lst[kpos]] = 1;
         used[slot] = 1;
       }
     } else {
@@ -1477,12 +1487,7 @@ __global__ __launch_bounds__(64) void k_zassign(Passed P, const u32* __restrict_
   }
 }
 
-// next frontier from survivors, in compacted-successor order
-__global__ void k_flag_survivors(const int* __restrict__ survived, long G, int* __restrict__ flag) {
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g < G) flag[g] = survived[g] >= 0 ? 1 : 0;
-}
-
+// next frontier from survivors (flagged by k_merge), in compacted-successor order
 __global__ void k_build_frontier(const int* __restrict__ survived, const long long* __restrict__ pos, long G,
                                  const int* __restrict__ sj, const u64* __restrict__ sdk,
                                  u64* __restrict__ fr_dk, u32* __restrict__ fr_zid,
@@ -1966,7 +1971,7 @@ void account(hora_hip_ctx* c, const Tally& hc, const Tally& prev, long G, int T,
 void launch_expand(hora_hip_ctx* c, int wpb, long F, ExpandOut eo, bool init) {
   const size_t lds = (size_t)wpb * expand_lds_ints(c->d2, c->dn.rcap, c->dn.icap) * sizeof(int);
   hipLaunchKernelGGL(k_expand, dim3(blocks_for(F, wpb)), dim3(64 * wpb), lds, c->stream, c->dnp, c->fr_dk.p,
-                     c->fr_zid.p, c->zarena.p, (int)F, c->rec_off.p, c->rec_cnt.p, eo, c->ctr.p, init ? 1 : 0);
+                     c->fr_zid.p, c->zarena.p, (int)F, c->pos.p, c->rec_cnt.p, eo, c->ctr.p, init ? 1 : 0);
 }
 
 // Wavefronts per k_expand workgroup. More wavefronts per workgroup lift the per-CU workgroup limit on
@@ -2044,10 +2049,9 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
       ++level;
       hipLaunchKernelGGL(k_count, dim3(F), dim3(64), count_lds_bytes(c->dn.rcap), c->stream, c->dnp,
                          c->fr_dk.p, (int)F, c->rec_cnt.p, c->ctr.p);
-      R = scan_exclusive(c, c->rec_cnt.p, F, c->pos.p);
-      hipLaunchKernelGGL(k_offsets_to_int, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->pos.p, F, c->rec_off.p);
+      R = scan_exclusive(c, c->rec_cnt.p, F, c->pos.p);   // slot offsets, read by k_expand
     } else {
-      HIP_OK(hipMemsetAsync(c->rec_off.p, 0, sizeof(int), c->stream));
+      HIP_OK(hipMemsetAsync(c->pos.p, 0, sizeof(long long), c->stream));
     }
     // ---- expansion
     c->s_valid.ensure(R);
@@ -2084,7 +2088,8 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     }
     if (G == 0) break;
     c->sj.ensure(Gall);
-    hipLaunchKernelGGL(k_compact_valid, dim3(blocks_for(R, 256)), dim3(256), 0, c->stream, c->s_valid.p, c->pos.p, R, c->sj.p);
+    hipLaunchKernelGGL(k_compact_valid, dim3(blocks_for(R, 256)), dim3(256), 0, c->stream, c->s_valid.p, c->pos.p, R, c->sj.p,
+                       c->ctr.p);
     while ((double)(hc[C_KEYS] + (u64)G) > 0.6 * (double)c->H) grow_table(c);
     Passed P = passed_view(c);
     // ---- keys, cover pre-filter, per-key work lists
@@ -2092,10 +2097,11 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     c->gticket.ensure(G);
     c->touched.ensure(G);
     c->covered.ensure(G);
-    HIP_OK(hipMemsetAsync(&c->ctr.p->ntouched, 0, sizeof(int), c->stream));
+    c->survived.ensure(G);
+    c->flag_i.ensure(G);
     HIP_OK(hipEventRecord(c->ev[2], c->stream));
     hipLaunchKernelGGL(k_find, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->sj.p, c->s_dk.p, G,
-                       c->gbk.p, c->ctr.p);
+                       c->gbk.p, c->ctr.p, c->survived.p, c->flag_i.p);
     HIP_OK(hipEventRecord(c->ev[5], c->stream));
     hipLaunchKernelGGL(k_probe, dim3(blocks_for(G, 4)), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p,
                        c->s_dk.p, c->s_zone.p, G, c->covered.p, c->ctr.p);
@@ -2107,7 +2113,7 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     c->caps.ensure(T);
     c->segs.ensure(T);
     hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
-                       c->caps.p, c->segs.p);
+                       c->caps.p, c->segs.p, c->ctr.p);
     c->pos.ensure(T + 1);
     c->pos2.ensure(T + 1);
     long long W = 0, Sseg = 0;
@@ -2127,11 +2133,8 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     c->zscr.ensure((size_t)std::max<long long>(W, 1) * d2);
     c->zused.ensure(W + 1);
     HIP_OK(hipMemsetAsync(c->zused.p, 0, (size_t)(W + 1) * sizeof(int), c->stream));
-    c->survived.ensure(G);
-    HIP_OK(hipMemsetAsync(c->survived.p, 0xff, (size_t)G * sizeof(int), c->stream));
     c->klight.ensure(T);
     c->kheavy.ensure(T);
-    HIP_OK(hipMemsetAsync(&c->ctr.p->nlight, 0, 2 * sizeof(int), c->stream));
     hipLaunchKernelGGL(k_classify, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
                        c->klight.p, c->kheavy.p, c->ctr.p, kLightMax);
     int nlh[2] = {0, 0};
@@ -2141,10 +2144,12 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     HIP_OK(hipEventRecord(c->ev[3], c->stream));
     if (nlh[0] > 0)
       hipLaunchKernelGGL(k_merge, dim3(nlh[0]), dim3(64), lds_l, c->stream, c->dnp, P, c->klight.p, nlh[0],
-                         c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 0, c->zscr.p, c->zused.p);
+                         c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 0, c->zscr.p, c->zused.p,
+                         c->flag_i.p);
     if (nlh[1] > 0)
       hipLaunchKernelGGL(k_merge, dim3(nlh[1]), dim3(64), lds_h, c->stream, c->dnp, P, c->kheavy.p, nlh[1],
-                         c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 1, c->zscr.p, c->zused.p);
+                         c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 1, c->zscr.p, c->zused.p,
+                         c->flag_i.p);
     HIP_OK(hipEventRecord(c->ev[4], c->stream));
     ncount += (u64)Sseg;
     // zone ids: the surviving new zones of the level, compact, in slot order (the count is read
@@ -2163,8 +2168,6 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
       break;
     }
     // ---- next frontier
-    c->flag_i.ensure(G);
-    hipLaunchKernelGGL(k_flag_survivors, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, c->survived.p, G, c->flag_i.p);
     c->pos.ensure(G + 1);
     long long nsurv = 0;
     const long F2 = scan_and_counters(c, c->flag_i.p, G, c->pos.p, &hc, &nsurv);

@@ -1433,6 +1433,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
         hipLaunchKernelGGL(k_offsets_to_int, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->pos.p, F, c->rec_off.p);
       } else {
         HIP_OK(hipMemsetAsync(c->rec_off.p, 0, sizeof(int), c->stream));
+        HIP_OK(hipMemsetAsync(c->pos.p, 0, sizeof(long long), c->stream));   // k_expand's slot offset
       }
       c->s_valid.ensure(R);
       c->s_tgt.ensure(R);
@@ -1459,7 +1460,8 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       long F2 = 0;
       if (G > 0) {
         c->sj.ensure(G);
-        hipLaunchKernelGGL(k_compact_valid, dim3(blocks_for(R, 256)), dim3(256), 0, c->stream, c->s_valid.p, c->pos.p, R, c->sj.p);
+        hipLaunchKernelGGL(k_compact_valid, dim3(blocks_for(R, 256)), dim3(256), 0, c->stream, c->s_valid.p, c->pos.p, R, c->sj.p,
+                           (Counters*)nullptr);
         {
           const Tally hc = read_counters(c);
           while ((double)(hc[C_KEYS] + (u64)G) > 0.6 * (double)c->H) grow_table(c);
@@ -1475,7 +1477,8 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
         HIP_OK(hipMemsetAsync(L.fnew.p, 0, (size_t)G * sizeof(int), c->stream));
         HIP_OK(hipMemsetAsync(L.tslot.p, 0xff, (size_t)G * sizeof(int), c->stream));
         HIP_OK(hipMemsetAsync(&c->ctr.p->ntouched, 0, sizeof(int), c->stream));
-        hipLaunchKernelGGL(k_find, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->sj.p, c->s_dk.p, G, c->gbk.p, c->ctr.p);
+        hipLaunchKernelGGL(k_find, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->sj.p, c->s_dk.p, G, c->gbk.p, c->ctr.p,
+                           (int*)nullptr, (int*)nullptr);
         HIP_OK(hipEventRecord(c->ev[1], c->stream));
         hipLaunchKernelGGL(k_gprobe, dim3(blocks_for(G, 4)), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p, c->s_dk.p,
                            c->s_zone.p, G, c->zarena.p, c->covered.p, L.cov_old.p, c->ctr.p, alu);
@@ -1487,7 +1490,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
           c->caps.ensure(T);
           c->segs.ensure(T);
           hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
-                             c->caps.p, c->segs.p);
+                             c->caps.p, c->segs.p, (Counters*)nullptr);
           c->pos2.ensure(T + 1);
           const long long Wl = scan_exclusive(c, c->caps.p, T, c->pos2.p);
           hipLaunchKernelGGL(k_gsegs, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T, c->segs.p);
