@@ -216,8 +216,10 @@ enum {
 
 typedef struct hora_live_query {
   int32_t kind;
-  int32_t early;             /* LEADSTO / DEADLOCK: settle at the first witness state
-                                (leadsto_witness, src/liveness/construction.cpp:786-813)           */
+  int32_t early;             /* settle at the first witness state, checked after each level:
+                                LEADSTO a p /\ !q node with a stop valuation (leadsto_witness,
+                                src/liveness/construction.cpp:786-813), EG a psi node with one
+                                (online_eg_terminal_witness, :748-767), DEADLOCK a deadlock     */
   int32_t cover;             /* HORA_COVER_AUTO: a_LU covers unless a reachable discrete state can
                                 seed a deadlock atom (online_seed_key, src/liveness/refinement.cpp:
                                 176-282), then the a_M graph; HORA_COVER_AM: the a_M graph always */

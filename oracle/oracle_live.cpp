@@ -32,7 +32,8 @@
 //                    fair schedule reaches the same sets (the device runs Jacobi rounds).
 //   readouts         certificate_in / accepting_closure (fixpoint.cpp:979-1101); eg_closure /
 //                    eg_verdict (src/liveness/api.cpp:169-188, 290-337); early witness
-//                    leadsto_witness (construction.cpp:786-813), checked after each level.
+//                    leadsto_witness (construction.cpp:786-813), online_eg_terminal_witness
+//                    (:748-767) and the deadlock scan, checked after each level.
 //
 // Federations are lists of zones; only the sets they denote matter to the verdict and to the
 // per-node emptiness the tests compare (zone difference splits along the subtrahend's facets in
@@ -502,6 +503,10 @@ struct Live {
       if (early_on && kind == HORA_LIVE_DEADLOCK) {
         for (int v : frontier)
           if (state_has_deadlock(v, n.inv, nullptr)) { st.early = 1; return 0; }
+      }
+      if (early_on && kind == HORA_LIVE_EG) {   // online_eg_terminal_witness, construction.cpp:748-767
+        for (int v : frontier)
+          if (!stop_set(v).empty()) { st.early = 1; return 1; }   // a maximal psi-run ends at v
       }
       frontier.swap(next);
     }

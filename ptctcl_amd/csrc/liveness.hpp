@@ -631,7 +631,9 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
   if (b >= n) return;
   const long v = list ? list[b] : first_id + b;
   const u64 dk = A.gdk[v];
-  if (mode == 1 && !(pred_eval(N, A.p, dk) && !pred_eval(N, A.q, dk))) return;
+  // mode 1 tests the witness positions: p /\ !q nodes (leads-to), psi nodes (E[] psi)
+  if (mode == 1 && !(A.kind == HORA_LIVE_EG ? pred_eval(N, A.p, dk) : pred_eval(N, A.p, dk) && !pred_eval(N, A.q, dk)))
+    return;
   // a capacity of K exceeded: rerun the node with the full capacities, or fail without them
   auto overflow = [&](int err) __attribute__((always_inline)) {
     if (retry) { if (l == 0) retry[atomicAdd(&lc->rcnt, 1)] = (int)v; }
@@ -1575,18 +1577,21 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
         }
         L.nodes += F2;
       }
-      // early witness on the level just expanded (leadsto_witness / the deadlock scan)
-      if (!init && q->early && (q->kind == HORA_LIVE_LEADSTO || q->kind == HORA_LIVE_DEADLOCK)) {
+      // early witness on the level just expanded: a p /\ !q node with a non-empty stop set refutes
+      // p --> q (leadsto_witness, construction.cpp:786-813); a psi node with one ends a maximal
+      // psi-run, so E[] psi holds (online_eg_terminal_witness, construction.cpp:748-767); a
+      // deadlocked valuation refutes A[] not deadlock
+      if (!init && q->early) {
         sync_graph_args(L);
         L.args.gzone = c->zarena.p;
         upload_args(c, L);
-        launch_stop(c, L, nullptr, (long)F, (long long)fbase, q->kind == HORA_LIVE_LEADSTO ? 1 : 2);
+        launch_stop(c, L, nullptr, (long)F, (long long)fbase, q->kind == HORA_LIVE_DEADLOCK ? 2 : 1);
         const LiveCtr h = read_lc(c, L);
         if (h.err) {
           c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a stop set" : "invariant or transition table too large";
           return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
         }
-        if (h.hit) { early = 0; st->early = 1; }
+        if (h.hit) { early = q->kind == HORA_LIVE_EG ? 1 : 0; st->early = 1; }
       }
       {
         const Tally hc = read_counters(c);

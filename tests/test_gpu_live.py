@@ -82,6 +82,14 @@ def test_fischer_eg_recorded(N):
         assert r == want
 
 
+@pytest.mark.parametrize("N", [4, 6])
+def test_eg_early_witness(N):
+    net = models.fischer(N)
+    for early in (True, False):
+        r, st = check(net, lv.LiveQuery(lv.EG, P.true(), early=early), f"E[] true early={early}")
+        assert r and st.early == (1 if early else 0)
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
 @pytest.mark.parametrize("N", [4, 6, 8, 10])
 def test_token_ring(N, seed):

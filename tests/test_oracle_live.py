@@ -12,6 +12,8 @@
   timelock, forced progress, a cycle witness that no single state certifies, a partial deadlock.
 * Both covering routes give every verdict; the route itself follows the seed condition of
   online_seed_key (src/liveness/refinement.cpp:176-282).
+* The early witnesses (leadsto_witness, online_eg_terminal_witness) settle the query the fixpoint
+  would settle.
 """
 import numpy as np
 import pytest
@@ -165,6 +167,19 @@ def test_route_choice():
     net, _ = live_models.partial_deadlock()
     r, st = binding.live(net, lv.leads_to(P.at(net, "A", "a"), P.at(net, "A", "c"), early=False))
     assert not r and st.route == 1 and st.seed_keys == 1
+
+
+def test_eg_terminal_witness():
+    """E[] true on fischer: a state where every process idles lets time diverge, a maximal run
+    that stays in psi, so the construction settles E[] at that level without the fixpoint; without
+    the early witness the fixpoint gives the same verdict."""
+    net = models.fischer(4)
+    q = lv.LiveQuery(lv.EG, P.true())
+    r, st = binding.live(net, q)
+    assert r and st.early == 1 and st.closure == 0
+    q.early = False
+    r, st = binding.live(net, q)
+    assert r and st.early == 0 and st.closure > 0
 
 
 def test_early_witness_needs_no_fixpoint():
