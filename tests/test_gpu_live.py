@@ -153,6 +153,20 @@ def test_fischer7_full_graph_parity():
         assert r and st.outer_nodes == GOLD[gold]["7"]["nodes"]
 
 
+def test_fischer8_full_graph_parity():
+    """At fischer 8 (64,534 nodes) the oracle still finishes in about a second per query: the two
+    violated leads-to queries, the deadlock query and an E[] query, all without early witnesses, so
+    the whole fixpoint runs, node for node against the oracle."""
+    net = models.fischer(8)
+    a = lambda loc: P.at(net, "P0", loc)   # noqa: E731
+    for label, q, want in (("cs-->req full", lv.leads_to(a("cs"), a("req"), early=False), False),
+                           ("wait-->cs full", lv.leads_to(a("wait"), a("cs"), early=False), False),
+                           ("deadlock full", lv.not_deadlock(False), True),
+                           ("E[] !P1.cs", lv.LiveQuery(lv.EG, ~P.at(net, "P1", "cs"), early=False), True)):
+        r, _ = check(net, q, label)
+        assert r == want, label
+
+
 def test_partial_deadlock_takes_the_am_route():
     net, _ = live_models.partial_deadlock()
     q = lv.leads_to(P.at(net, "A", "a"), P.at(net, "A", "c"), early=False)
