@@ -349,8 +349,10 @@ __device__ __forceinline__ bool z_equal(const int* a, const int* b, int d) {
 
 // p \ y as disjoint zones appended to the list `out` (stride `zs` ints, `*n` entries, at most
 // `cap`): y's facets (i, j) in row-major order; where y is tighter than the remainder r, split off
-// r /\ not(x_i - x_j <| y_ij) and keep r /\ (x_i - x_j <| y_ij). `r`, `t` are d*d LDS scratch.
-// Returns false if the list would exceed `cap`.
+// r /\ not(x_i - x_j <| y_ij) and keep r /\ (x_i - x_j <| y_ij). The facets are taken a wavefront
+// of entries at a time: one ballot names those tighter than the remainder (the remainder only
+// shrinks, so no other entry of the chunk can cut it later), and each is checked again when its
+// turn comes. `r`, `t` are d*d LDS scratch. Returns false if the list would exceed `cap`.
 __device__ __forceinline__ bool z_subtract(const int* p, const int* y, int d, int* out, int zs, int* n, int cap,
                                            int* r, int* t) {
   const int l = lane_id(), d2 = d * d;
@@ -363,12 +365,16 @@ __device__ __forceinline__ bool z_subtract(const int* p, const int* y, int d, in
   }
   wsync();
   for (int e = l; e < d2; e += kWave) r[e] = p[e];
-  for (int i = 0; i < d; ++i)
-    for (int j = 0; j < d; ++j) {
-      if (i == j) continue;
+  for (int base = 0; base < d2; base += kWave) {
+    wsync();
+    const int e0 = base + l;
+    uint64_t cut = __ballot(e0 < d2 && y[e0] < r[e0]);   // diagonal entries never cut: both hold <= 0 there
+    for (; cut; cut &= cut - 1) {
+      const int f = base + __builtin_ctzll(cut);
+      const int i = ediv(f, d), j = f - i * d;
       wsync();
-      const int yij = y[i * d + j];
-      if (yij >= r[i * d + j]) continue;
+      const int yij = y[f];
+      if (yij >= r[f]) continue;   // implied by the facets taken since
       wsync();
       for (int e = l; e < d2; e += kWave) t[e] = r[e];
       if (z_constrain(t, d, j, i, r_neg(yij))) {
@@ -379,6 +385,7 @@ __device__ __forceinline__ bool z_subtract(const int* p, const int* y, int d, in
       }
       if (!z_constrain(r, d, i, j, yij)) return true;
     }
+  }
   return true;
 }
 
