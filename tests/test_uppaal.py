@@ -1,0 +1,125 @@
+"""The UPPAAL XML reader (ptctcl_amd/uppaal.py) on the reference's own benchmark models (CPU only).
+
+* fischer<N>.xml, written by the reference's generator (benchmarks/scripts/generators/
+  gen_fischer.py), reads to the same encoded network as models.fischer(N), array for array.
+* The oracle's explored counts on the copied models equal the counts the reference recorded for
+  them (tests/golden/xml_reach_counts.json, each entry citing its CSV line): arbiter, phil,
+  twolock, csma (binary and broadcast synchronisation, shared integers, several clocks per
+  process) next to fischer.
+* Queries of the recorded forms turn into the targets they mean; constructs outside the subset
+  raise ModelError instead of reading as some other model.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import binding
+from ptctcl_amd import models, uppaal
+from ptctcl_amd.network import OP_EQ, OP_LT, Target, c_gt, c_le
+
+HERE = os.path.dirname(__file__)
+GOLD = json.load(open(os.path.join(HERE, "golden", "xml_reach_counts.json")))["models"]
+
+
+def load(name):
+    return uppaal.load(os.path.join(HERE, "golden", "models", name + ".xml"))
+
+
+@pytest.mark.parametrize("N", [4, 8, 12])
+def test_fischer_xml_is_the_generated_network(N):
+    a, b = load(f"fischer{N}").encode().arrays, models.fischer(N).encode().arrays
+    assert a.keys() == b.keys()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("name", ["arbiter4", "arbiter8", "arbiter12", "phil4", "phil6", "twolock4", "twolock6",
+                                  "twolock8", "csma4", "csma6", "csma8", "fischer4", "fischer8"])
+def test_oracle_counts_equal_recorded(name):
+    g = GOLD[name]
+    net = load(name)
+    inv, tgt = uppaal.safety_target(net, g["query"])
+    reached, st = binding.reach(net, tgt, 0)
+    assert inv and (not reached) == g["verdict_satisfied"]
+    assert st["explored"] == g["states"], (name, st["explored"], g["cite"])
+
+
+def test_order_sensitive_sizes_within_recorded_range():
+    """phil from 8 processes: the reference's runs record different counts (the search order
+    decides which zones a union absorbs); the sequential oracle's lies within what they span."""
+    g = GOLD["phil8"]
+    net = load("phil8")
+    reached, st = binding.reach(net, uppaal.safety_target(net, g["query"])[1], 0)
+    lo, hi = g["range"]
+    assert not reached and lo <= st["explored"] <= hi
+
+
+def test_every_fixture_reads():
+    for name, g in GOLD.items():
+        net = load(name)
+        uppaal.safety_target(net, g["query"])
+        assert len(net.procs) >= 1 and net.nclocks >= 1
+
+
+def test_local_declarations_and_constants():
+    """hddi: template-local clocks named per instance, constants in guards and invariants."""
+    net = load("hddi2")
+    assert net.clock_names == ["0", "RING.x", "ST1.x", "ST1.y", "ST1.z", "ST2.x", "ST2.y", "ST2.z"]
+    ring = net.procs[0]
+    assert ring.locations[0].invariant == [c_le(1, 0)]          # x <= td, td = 0
+    st1 = net.procs[1]
+    # x >= SA && z >= TRTT with SA = 20, TRTT = 120 on the second transition
+    assert sorted(st1.edges[1].guard) == sorted([(0, 2, (-20 << 1) | 1), (0, 4, (-120 << 1) | 1)])
+
+
+def test_query_forms():
+    net = load("arbiter4")
+    inv, t = uppaal.safety_target(net, "A[] not (C1.cs && C2.cs)")
+    assert inv and t.loc_req == [net.resolve("C1", "cs"), net.resolve("C2", "cs")]
+    inv, t = uppaal.safety_target(net, "A[] true")
+    assert inv and t.never
+    inv, t = uppaal.safety_target(net, "A[] x0 <= 4")
+    assert inv and t.clock_cond == [c_gt(1, 4)]
+    inv, t = uppaal.safety_target(net, "E<> C0.cs and x1 < 3")
+    assert not inv and t.loc_req == [net.resolve("C0", "cs")] and t.clock_cond == [(1 + 1, 0, 3 << 1)]
+    net = load("twolock4")
+    inv, t = uppaal.safety_target(net, "E<> a == 2 && P1.cs")
+    assert t.int_cond == [(0, OP_EQ, 2)] and t.loc_req == [net.resolve("P1", "cs")]
+    inv, t = uppaal.safety_target(net, "A[] b >= 3")
+    assert t.int_cond == [(1, OP_LT, 3)]
+
+
+XML = """<nta><declaration>{decl}</declaration><template><name>P</name>{param}
+<location id="a"><name>a</name></location><location id="b"><name>b</name></location><init ref="a"/>
+<transition><source ref="a"/><target ref="b"/>{labels}</transition></template><system>system P;</system></nta>"""
+
+
+@pytest.mark.parametrize("decl,param,labels,what", [
+    ("clock x; int L;", "", '<label kind="assignment">L = (1 - L) % 2</label>', "constant"),
+    ("clock x; int[0,3] i;", "", '<label kind="guard">x &gt; 1 || i == 2</label>', "conjunctions"),
+    ("clock x; chan c[2];", "", '<label kind="synchronisation">c[0]!</label>', "arrays"),
+    ("clock x; typedef int[0,1] id_t;", "", "", "declaration"),
+    ("clock x;", "<parameter>int id</parameter>", "", "parameters"),
+    ("clock x;", "", '<label kind="select">e : int[0,1]</label>', "select"),
+    ("clock x;", "", '<label kind="assignment">x = 3</label>', "reset to 0"),
+    ("clock x; int[0,3] i;", "", '<label kind="guard">i == j</label>', "constant"),
+])
+def test_outside_the_subset_is_refused(decl, param, labels, what):
+    with pytest.raises(uppaal.ModelError, match=what):
+        uppaal.parse_xml(XML.format(decl=decl, param=param, labels=labels))
+
+
+def test_declaration_forms():
+    net = uppaal.parse_xml(XML.format(
+        decl="const int N = 3; const int M = N * 2 - 1; clock x, y; int[0,N+1] i = 2, j; bool f = true; "
+             "broadcast chan go; urgent chan u;",
+        param="", labels='<label kind="guard">x - y &lt;= M &amp;&amp; (i != N)</label>'
+                         '<label kind="assignment">i := N - 1, y = 0</label>'
+                         '<label kind="synchronisation">go!</label>'))
+    assert [(v.name, v.lo, v.hi, v.init) for v in net.intvars] == [("i", 0, 4, 2), ("j", 0, 4, 0), ("f", 0, 1, 1)]
+    assert [(c.name, c.broadcast, c.urgent) for c in net.channels] == [("go", True, False), ("u", False, True)]
+    e = net.procs[0].edges[0]
+    assert e.guard == [(1, 2, (5 << 1) | 1)] and e.int_guard == [(0, 1, 3)]
+    assert e.int_updates == [(0, 2)] and e.resets == [2] and (e.chan, e.dir) == (0, 1)
