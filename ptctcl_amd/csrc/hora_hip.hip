@@ -78,6 +78,7 @@ struct DevNet {
   const int* chan_bcast;  // one int per channel
   const int* chan_urgent;
   int has_urgent_chan;
+  int has_bcast;          // some channel is a broadcast channel
   const int* Lx;          // extrapolation bounds [d] (global bounds widened by the query)
   const int* Ux;
   const int* lloc;        // per-location a_LU bounds [total_locs*d]
@@ -238,46 +239,6 @@ __device__ __forceinline__ int frozen_flags(const DevNet& N, const int* locs, co
   for (int p = 0; p < N.np; ++p) f |= N.loc_flags[N.locbase[p] + uni(locs[p])] & 3;
   if (!(f & 2) && urgent_sync_enabled(N, locs, ints)) f |= 2;
   return f;
-}
-
-// Enumerate transition recipes of a discrete state in gen_succ_reuse's order: local edges
-// (process order, then edge order), then binary synchronisations (sender process and edge, then
-// receiver process and edge). `fn(se, re)` gets global edge ids (re = -1 for a local step).
-// Broadcast channels are refused at hora_hip_open.
-template <class F>
-__device__ __forceinline__ void enum_recipes(const DevNet& N, const int* locs, const int* ints, bool src_c, F&& fn) {
-  for (int p = 0; p < N.np; ++p) {
-    const int lp = uni(locs[p]);
-    if (src_c && !committed_at(N, p, lp)) continue;
-    const int g = N.locbase[p] + lp;
-    for (int t = N.le_off[g]; t < N.le_off[g + 1]; ++t) {
-      const int e = N.le_edges[t];
-      if (N.edge_chan[e] >= 0) continue;
-      if (!int_ok(N, e, ints)) continue;
-      fn(e, -1);
-    }
-  }
-  for (int i = 0; i < N.np; ++i) {
-    const int gi = N.locbase[i] + uni(locs[i]);
-    const bool i_comm = committed_at(N, i, locs[i]);
-    for (int t = N.le_off[gi]; t < N.le_off[gi + 1]; ++t) {
-      const int se = N.le_edges[t];
-      const int ch = N.edge_chan[se];
-      if (ch < 0 || N.edge_dir[se] != 1 || N.chan_bcast[ch]) continue;
-      if (!int_ok(N, se, ints)) continue;
-      for (int j = 0; j < N.np; ++j) {
-        if (j == i) continue;
-        const int gj = N.locbase[j] + uni(locs[j]);
-        for (int t2 = N.le_off[gj]; t2 < N.le_off[gj + 1]; ++t2) {
-          const int re = N.le_edges[t2];
-          if (N.edge_chan[re] != ch || N.edge_dir[re] != -1) continue;
-          if (src_c && !(i_comm || committed_at(N, j, locs[j]))) continue;
-          if (!int_ok(N, re, ints)) continue;
-          fn(se, re);
-        }
-      }
-    }
-  }
 }
 
 // Conjoin the combined invariant of `locs` (all_invariants + apply_all, semantics.hpp:82-86).
@@ -445,11 +406,80 @@ __device__ __forceinline__ u64 edge_reset_mask(const DevNet& N, int e) {
   return m;
 }
 
+// One part of a broadcast (build_broadcast_parts, semantics.hpp:275-303): the receiving edges on
+// channel `ch` that the integer guards enable at global location gj, and the options they give the
+// process: receive on one of them (nrecv), or, when each carries a clock guard, stay out in a zone
+// where all of them fail -- one negated atom per guard, nnone = the product of the guard sizes.
+struct BPart {
+  int nrecv, nnone;
+};
+
+constexpr int kRecipeCap = 256;    // transitions of one discrete state held in LDS
+
+__device__ __forceinline__ BPart bcast_part(const DevNet& N, int gj, int ch, const int* ints) {
+  BPart b{0, 1};
+  for (int t = N.le_off[gj]; t < N.le_off[gj + 1]; ++t) {
+    const int re = N.le_edges[t];
+    if (N.edge_chan[re] != ch || N.edge_dir[re] != -1 || !int_ok(N, re, ints)) continue;
+    ++b.nrecv;
+    b.nnone = min(b.nnone * (N.cg_off[re + 1] - N.cg_off[re]), kRecipeCap + 1);
+  }
+  if (b.nrecv == 0) b.nnone = 0;
+  return b;
+}
+
+// The k-th enabled receiving edge of a part.
+__device__ __forceinline__ int bcast_recv_edge(const DevNet& N, int gj, int ch, const int* ints, int k) {
+  for (int t = N.le_off[gj]; t < N.le_off[gj + 1]; ++t) {
+    const int re = N.le_edges[t];
+    if (N.edge_chan[re] != ch || N.edge_dir[re] != -1 || !int_ok(N, re, ints)) continue;
+    if (k == 0) return re;
+    --k;
+  }
+  return -1;
+}
+
+// Option `combo` of broadcast sender edge se, part by part in process order: the options of a part
+// are its receiving edges in edge order, then its stay-out conjunctions; the first part is the
+// fastest digit of `combo` (the odometer of semantics.hpp:275-303). fn(j, gj, re, q) gets the
+// receiving edge, or re = -1 and the index q of the stay-out conjunction.
+template <class F>
+__device__ __forceinline__ void bcast_walk(const DevNet& N, const int* locs, const int* ints, int se, int combo,
+                                           F&& fn) {
+  const int i = N.edge_proc[se], ch = N.edge_chan[se];
+  for (int j = 0; j < N.np; ++j) {
+    if (j == i) continue;
+    const int gj = N.locbase[j] + locs[j];
+    const BPart b = bcast_part(N, gj, ch, ints);
+    if (b.nrecv == 0) continue;
+    const int nopt = b.nrecv + b.nnone;
+    const int pick = combo % nopt;
+    combo /= nopt;
+    fn(j, gj, pick < b.nrecv ? bcast_recv_edge(N, gj, ch, ints, pick) : -1, pick - b.nrecv);
+  }
+}
+
+// Stay-out conjunction q of the part at gj: one negated atom of each receiving edge's clock guard,
+// the first edge's atom the most significant digit of q.
+__device__ __forceinline__ bool apply_bcast_none(const DevNet& N, int gj, int ch, const int* ints, int q, int* z) {
+  for (int t = N.le_off[gj + 1] - 1; t >= N.le_off[gj]; --t) {
+    const int re = N.le_edges[t];
+    if (N.edge_chan[re] != ch || N.edge_dir[re] != -1 || !int_ok(N, re, ints)) continue;
+    const int a = N.cg_off[re], sz = N.cg_off[re + 1] - a;
+    const int c = a + q % sz;
+    q /= sz;
+    if (!z_constrain(z, N.d, N.cg_data[3 * c + 1], N.cg_data[3 * c], r_neg(N.cg_data[3 * c + 2]))) return false;
+  }
+  return true;
+}
+
 // Transition recipes of a discrete state, in gen_succ_reuse's order (engine.cpp:541-718), into
 // `rec` as (sender edge, receiver edge or -1) pairs: local edges in process order -- each lane
 // enumerates its own process, a prefix sum places them -- then binary synchronisations (sender
-// process and edge, then receiver process and edge). Broadcast channels are refused at open.
+// process and edge, then receiver process and edge), then broadcasts (sender process and edge,
+// then the combinations of bcast_walk in order), written (sender edge, -2 - combination).
 // Returns the number of recipes, or -1 if more than `cap`.
+template <bool BC = true>
 __device__ __forceinline__ int build_recipes(const DevNet& N, const int* locs, const int* ints, bool src_c,
                                              int* rec, int cap) {
   const int l = lane_id();
@@ -503,10 +533,38 @@ __device__ __forceinline__ int build_recipes(const DevNet& N, const int* locs, c
       }
     }
   }
+  if (!BC || !N.has_bcast) return n;
+  for (int i = 0; i < N.np; ++i) {
+    const int gi = N.locbase[i] + locs[i];
+    const bool i_comm = committed_at(N, i, locs[i]);
+    for (int t = N.le_off[gi]; t < N.le_off[gi + 1]; ++t) {
+      const int se = N.le_edges[t];
+      const int ch = N.edge_chan[se];
+      if (ch < 0 || N.edge_dir[se] != 1 || !N.chan_bcast[ch]) continue;
+      if (!int_ok(N, se, ints)) continue;
+      int nopt = 1;   // lane j: the options of process j (1 when it takes no part)
+      if (l < N.np && l != i) {
+        const BPart b = bcast_part(N, N.locbase[l] + locs[l], ch, ints);
+        if (b.nrecv > 0) nopt = b.nrecv + b.nnone;
+      }
+      int combos = 1;
+      for (int j = 0; j < N.np; ++j) combos = min(combos * __builtin_amdgcn_readlane(nopt, j), cap + 1);
+      if (n + combos > cap) return -1;
+      for (int c = 0; c < combos; ++c) {
+        // from a committed state, the sender or some receiving process must be committed
+        bool keep = !src_c || i_comm;
+        if (!keep)
+          bcast_walk(N, locs, ints, se, c, [&](int j, int, int re, int) {
+            if (re >= 0 && committed_at(N, j, locs[j])) keep = true;
+          });
+        if (!keep) continue;
+        if (l == 0) { rec[2 * n] = se; rec[2 * n + 1] = -2 - c; }
+        ++n;
+      }
+    }
+  }
   return n;
 }
-
-constexpr int kRecipeCap = 256;    // transitions of one discrete state held in LDS
 
 __global__ __launch_bounds__(64) void k_count(const DevNet* __restrict__ Np, const u64* __restrict__ fr_dk, int F,
                                               int* __restrict__ cnt, Counters* ctr) {
@@ -540,15 +598,14 @@ __host__ __device__ constexpr int expand_lds_ints(int d2, int rcap, int icap) {
 // LDS of k_count: one discrete state and the recipes.
 inline size_t count_lds_bytes(int rcap) { return (size_t)(HORA_MAX_PROCS + HORA_MAX_INTS + 2 * rcap) * sizeof(int); }
 
-// z_apply_list over the source state's invariant triples, skipping those of processes p1 and p2
-// (which moved): `tag[k]` is the process of triple k.
-__device__ __forceinline__ bool z_apply_list_skip(int* z, int d, const int* cs, const int* tag, int n, int p1,
-                                                  int p2) {
+// z_apply_list over the source state's invariant triples, skipping those of the processes in
+// `moved`: `tag[k]` is the process of triple k.
+__device__ __forceinline__ bool z_apply_list_skip(int* z, int d, const int* cs, const int* tag, int n, u64 moved) {
   for (int base = 0; base < n; base += kWave) {
     wsync();
     const int k = base + lane_id();
     bool cand = false;
-    if (k < n && tag[k] != p1 && tag[k] != p2) cand = cs[3 * k + 2] < z[cs[3 * k] * d + cs[3 * k + 1]];
+    if (k < n && !((moved >> tag[k]) & 1)) cand = cs[3 * k + 2] < z[cs[3 * k] * d + cs[3 * k + 1]];
     for (u64 b = __ballot(cand); b; b &= b - 1) {
       const int c = base + __builtin_ctzll(b);
       if (!z_constrain(z, d, cs[3 * c], cs[3 * c + 1], cs[3 * c + 2])) return false;
@@ -564,13 +621,14 @@ __device__ __forceinline__ bool apply_loc_inv(const DevNet& N, int g, int* z) {
   return true;
 }
 
-// The combined invariant of a successor: the source state's, minus the moved processes' old
-// locations, plus their new locations' (g1, g2 global locations; p = -1 for none).
+// The combined invariant of a successor: the source state's, minus the old locations of the
+// processes in `moved`, plus the new locations g1, g2 (global locations; -1 for none). The
+// receivers of a broadcast add theirs after this.
 __device__ __forceinline__ bool apply_succ_inv(const DevNet& N, int* z, const int* sinv, const int* stag, int ns,
-                                               int p1, int g1, int p2, int g2) {
-  if (!z_apply_list_skip(z, N.d, sinv, stag, ns, p1, p2)) return false;
-  if (p1 >= 0 && !apply_loc_inv(N, g1, z)) return false;
-  if (p2 >= 0 && !apply_loc_inv(N, g2, z)) return false;
+                                               u64 moved, int g1, int g2) {
+  if (!z_apply_list_skip(z, N.d, sinv, stag, ns, moved)) return false;
+  if (g1 >= 0 && !apply_loc_inv(N, g1, z)) return false;
+  if (g2 >= 0 && !apply_loc_inv(N, g2, z)) return false;
   return true;
 }
 
@@ -592,7 +650,9 @@ __device__ __forceinline__ u64 bcast64(u64 v, int lane) {
 // values with readlane. The combined invariant of a successor is the source state's triples
 // without the moved processes', plus the moved processes' new locations'.
 // Eight wavefronts per SIMD (64 VGPRs, no spills to memory) measured ~5 % faster than the seven
-// the compiler picks by itself.
+// the compiler picks by itself. BC: the network has broadcast channels; that instance also walks
+// the parts of broadcast recipes (more registers, launched only for such networks).
+template <bool BC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_expand(const DevNet* __restrict__ Np, const u64* __restrict__ fr_dk,
                                                 const u32* __restrict__ fr_zid, const int* __restrict__ zarena,
                                                 int F, const long long* __restrict__ rec_off,
@@ -657,26 +717,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     }
   }
   const u64 clocks = ((1ull << N.nclocks) - 1) << 1;
+  // the invariant of the successor of transition (se, re); a broadcast's receivers add theirs
+  auto succ_inv = [&](int se, int re, u64 moved, int g1, int g2) __attribute__((always_inline)) -> bool {
+    if (!apply_succ_inv(N, z, sinv, stag, ns, moved, g1, g2)) return false;
+    bool ok = true;
+    if (BC && re <= -2)
+      bcast_walk(N, locs, ints, se, -2 - re, [&](int j, int, int rj, int) {
+        if (ok && rj >= 0) ok = apply_loc_inv(N, N.locbase[j] + N.edge_tgt[rj], z);
+      });
+    return ok;
+  };
   // the zone side of one successor; `fz`, `am` describe the target location vector
-  auto finish = [&](int slot, int se, int re, int p1, int g1, int p2, int g2, int fz, u64 am, u64 rm, u64 key,
+  auto finish = [&](int slot, int se, int re, u64 moved, int g1, int g2, int fz, u64 am, u64 rm, u64 key,
                     int tdisc) __attribute__((always_inline)) {
     wsync();
     z_copy(z, src, d2);
     bool ok = true;
     if (se >= 0) {
-      // guards (sender then receiver), then every reset of both edges at once
+      // guards (sender, then the receiver or each broadcast part), then every reset at once
       ok = apply_edge_guard(N, se, z);
       if (ok && re >= 0) ok = apply_edge_guard(N, re, z);
+      if (BC && ok && re <= -2)
+        bcast_walk(N, locs, ints, se, -2 - re, [&](int, int gj, int rj, int q) {
+          if (ok) ok = rj >= 0 ? apply_edge_guard(N, rj, z) : apply_bcast_none(N, gj, N.edge_chan[se], ints, q, z);
+        });
       if (ok) z_reset_many(z, d, rm, tmp);
     }
     // close_succ_zone (semantics.hpp:522-568): invariant, delay unless frozen, invariant,
     // inactive clocks reset, Extra_LU+ when on. A canonical zone that survives the invariant stays
     // non-empty through the delay, the second invariant, the resets and the extrapolation.
-    if (ok) ok = apply_succ_inv(N, z, sinv, stag, ns, p1, g1, p2, g2);
+    if (ok) ok = succ_inv(se, re, moved, g1, g2);
     if (ok) {
       if (!fz) {
         z_up(z, d);
-        apply_succ_inv(N, z, sinv, stag, ns, p1, g1, p2, g2);
+        succ_inv(se, re, moved, g1, g2);
       }
       z_reset_many(z, d, ~(am | N.qbits) & clocks, tmp);
       if (N.lu_plus) z_extra_lu_plus(z, d, N.Lx, N.Ux, row0);
@@ -704,36 +778,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     }
   };
   // the discrete side of transition (se, re) from the source state, in the calling lane
-  auto describe = [&](int se, int re, int* p1, int* g1, int* p2, int* g2, int* fz, u64* am, u64* rm, u64* key,
+  auto describe = [&](int se, int re, u64* moved, int* g1, int* g2, int* fz, u64* am, u64* rm, u64* key,
                       int* tdisc) __attribute__((always_inline)) {
-    const int q1 = N.edge_proc[se], t1 = N.edge_tgt[se];
-    const int q2 = re >= 0 ? N.edge_proc[re] : -1, t2 = re >= 0 ? N.edge_tgt[re] : 0;
-    const int h1 = N.locbase[q1] + t1, h2 = q2 >= 0 ? N.locbase[q2] + t2 : 0;
-    u64 k = dk;
-    k = (k & ~(N.kmask[q1] << N.kshift[q1])) | ((u64)(unsigned)t1 << N.kshift[q1]);
-    if (q2 >= 0) k = (k & ~(N.kmask[q2] << N.kshift[q2])) | ((u64)(unsigned)t2 << N.kshift[q2]);
-    u64 upd = 0;   // integer variables the transition assigns (bit v), in order sender then receiver
-    for (int pass = 0; pass < 2; ++pass) {
-      const int e = pass == 0 ? se : re;
-      if (e < 0) continue;
+    u64 k = dk, mv = 0, r = 0;
+    int fl = 0;
+    u64 a = 0;
+    // process q moves along edge e: its location field, integer updates (in order: sender, then
+    // the receiver or the broadcast parts in process order), flags, active clocks and resets
+    auto move = [&](int e) __attribute__((always_inline)) {
+      const int q = N.edge_proc[e], t = N.edge_tgt[e], h = N.locbase[q] + t;
+      k = (k & ~(N.kmask[q] << N.kshift[q])) | ((u64)(unsigned)t << N.kshift[q]);
       for (int c = N.iu_off[e]; c < N.iu_off[e + 1]; ++c) {
         const int v = N.iu_data[2 * c], val = N.iu_data[2 * c + 1];
         const int f = N.np + v;
         k = (k & ~(N.kmask[f] << N.kshift[f])) | ((u64)(unsigned)(val - N.int_lo[v]) << N.kshift[f]);
-        upd |= 1ull << v;
       }
-    }
-    int fl = (N.loc_flags[h1] & 3) | (q2 >= 0 ? (N.loc_flags[h2] & 3) : 0);
-    u64 a = N.amask[h1] | (q2 >= 0 ? N.amask[h2] : 0ull);
+      fl |= N.loc_flags[h] & 3;
+      a |= N.amask[h];
+      for (int c = N.rs_off[e]; c < N.rs_off[e + 1]; ++c) r |= 1ull << N.rs_clk[c];
+      mv |= 1ull << q;
+      return h;
+    };
+    const int h1 = move(se);
+    const int h2 = re >= 0 ? move(re) : -1;
+    if (BC && re <= -2) bcast_walk(N, locs, ints, se, -2 - re, [&](int, int, int rj, int) { if (rj >= 0) move(rj); });
     for (int p = 0; p < N.np; ++p)
-      if (p != q1 && p != q2) {
+      if (!((mv >> p) & 1)) {
         fl |= sflags[p];
         a |= samask[p];
       }
-    u64 r = 0;
-    for (int c = N.rs_off[se]; c < N.rs_off[se + 1]; ++c) r |= 1ull << N.rs_clk[c];
-    if (re >= 0)
-      for (int c = N.rs_off[re]; c < N.rs_off[re + 1]; ++c) r |= 1ull << N.rs_clk[c];
     // discrete part of the target test (reachable_net's match, engine.cpp:2275-2287)
     int td = 1;
     for (int t = 0; t < N.nlr && td; ++t) {
@@ -746,7 +819,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
       const int x = (int)((k >> N.kshift[N.np + v]) & N.kmask[N.np + v]) + N.int_lo[v];
       td = atom_holds(x, N.ic[3 * t + 1], N.ic[3 * t + 2]);
     }
-    *p1 = q1; *g1 = h1; *p2 = q2; *g2 = h2; *fz = fl; *am = a; *rm = r; *key = k; *tdisc = td;
+    *moved = mv; *g1 = h1; *g2 = h2; *fz = fl; *am = a; *rm = r; *key = k; *tdisc = td;
   };
   if (init) {
     // no transition: the initial location vector, its invariant, delay unless frozen, ...
@@ -755,18 +828,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     for (int t = 0; t < N.nic && td; ++t) td = atom_holds(ints[N.ic[3 * t]], N.ic[3 * t + 1], N.ic[3 * t + 2]);
     u64 am0 = 0;
     for (int p = 0; p < N.np; ++p) am0 |= samask[p];
-    finish(0, -1, -1, -1, 0, -1, 0, fz_src, am0, 0ull, dk, td);
+    finish(0, -1, -1, 0ull, -1, -1, fz_src, am0, 0ull, dk, td);
     return;
   }
   // the slots reserved for this state bound what is written: a state filtered out after counting
   // (rec_cnt[i] = 0) writes nothing
-  const int nr = min(build_recipes(N, locs, ints, (fz_src & 1) != 0, rec, N.rcap), rec_cnt[i]);
+  const int nr = min(build_recipes<BC>(N, locs, ints, (fz_src & 1) != 0, rec, N.rcap), rec_cnt[i]);
   wsync();
   const int base = (int)rec_off[i];
   for (int b0 = 0; b0 < nr; b0 += kWave) {
-    int p1 = -1, g1 = 0, p2 = -1, g2 = 0, fz = 0, td = 0;
-    u64 am = 0, rm = 0, key = 0;
-    if (b0 + l < nr) describe(rec[2 * (b0 + l)], rec[2 * (b0 + l) + 1], &p1, &g1, &p2, &g2, &fz, &am, &rm, &key, &td);
+    int g1 = -1, g2 = -1, fz = 0, td = 0;
+    u64 moved = 0, am = 0, rm = 0, key = 0;
+    if (b0 + l < nr) describe(rec[2 * (b0 + l)], rec[2 * (b0 + l) + 1], &moved, &g1, &g2, &fz, &am, &rm, &key, &td);
     const int nb = min(kWave, nr - b0);
     for (int r = 0; r < nb; ++r) {
       const int se = rec[2 * (b0 + r)], re = rec[2 * (b0 + r) + 1];
@@ -778,17 +851,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
         if (l < N.ni) niv[l] = ints[l];
         wsync();
         if (l == 0) {
-          nl[N.edge_proc[se]] = N.edge_tgt[se];
-          if (re >= 0) nl[N.edge_proc[re]] = N.edge_tgt[re];
-          for (int c = N.iu_off[se]; c < N.iu_off[se + 1]; ++c) niv[N.iu_data[2 * c]] = N.iu_data[2 * c + 1];
-          if (re >= 0)
-            for (int c = N.iu_off[re]; c < N.iu_off[re + 1]; ++c) niv[N.iu_data[2 * c]] = N.iu_data[2 * c + 1];
+          auto step = [&](int e) {
+            nl[N.edge_proc[e]] = N.edge_tgt[e];
+            for (int c = N.iu_off[e]; c < N.iu_off[e + 1]; ++c) niv[N.iu_data[2 * c]] = N.iu_data[2 * c + 1];
+          };
+          step(se);
+          if (re >= 0) step(re);
+          if (BC && re <= -2) bcast_walk(N, locs, ints, se, -2 - re, [&](int, int, int rj, int) { if (rj >= 0) step(rj); });
         }
         wsync();
         if (urgent_sync_enabled(N, nl, niv)) fzr |= 2;
       }
-      finish(base + b0 + r, se, re, bcast(p1, r), bcast(g1, r), bcast(p2, r), bcast(g2, r), fzr,
-             bcast64(am, r), bcast64(rm, r), bcast64(key, r), bcast(td, r));
+      finish(base + b0 + r, se, re, bcast64(moved, r), bcast(g1, r), bcast(g2, r), fzr, bcast64(am, r),
+             bcast64(rm, r), bcast64(key, r), bcast(td, r));
     }
   }
 }
@@ -1970,8 +2045,14 @@ void account(hora_hip_ctx* c, const Tally& hc, const Tally& prev, long G, int T,
 
 void launch_expand(hora_hip_ctx* c, int wpb, long F, ExpandOut eo, bool init) {
   const size_t lds = (size_t)wpb * expand_lds_ints(c->d2, c->dn.rcap, c->dn.icap) * sizeof(int);
-  hipLaunchKernelGGL(k_expand, dim3(blocks_for(F, wpb)), dim3(64 * wpb), lds, c->stream, c->dnp, c->fr_dk.p,
-                     c->fr_zid.p, c->zarena.p, (int)F, c->pos.p, c->rec_cnt.p, eo, c->ctr.p, init ? 1 : 0);
+  if (c->dn.has_bcast)
+    hipLaunchKernelGGL(k_expand<true>, dim3(blocks_for(F, wpb)), dim3(64 * wpb), lds, c->stream, c->dnp,
+                       c->fr_dk.p, c->fr_zid.p, c->zarena.p, (int)F, c->pos.p, c->rec_cnt.p, eo, c->ctr.p,
+                       init ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_expand<false>, dim3(blocks_for(F, wpb)), dim3(64 * wpb), lds, c->stream, c->dnp,
+                       c->fr_dk.p, c->fr_zid.p, c->zarena.p, (int)F, c->pos.p, c->rec_cnt.p, eo, c->ctr.p,
+                       init ? 1 : 0);
 }
 
 // Wavefronts per k_expand workgroup. More wavefronts per workgroup lift the per-CU workgroup limit on
@@ -2254,7 +2335,33 @@ void expand_caps(const hora_net_enc* e, int* rcap, int* icap) {
     local += lm;
     inv += im;
   }
+  // a broadcast's combinations: per process, its most options at one location (receiving edges,
+  // plus the stay-out conjunctions, at most the product of the non-empty guard sizes: integer
+  // guards may disable the unguarded receivers; 1 if it never takes part)
+  std::vector<long long> omax((size_t)np * std::max(nch, 1), 1);
   for (int ch = 0; ch < nch; ++ch) {
+    if (!(e->chan_bcast && e->chan_bcast[ch])) continue;
+    for (int p = 0; p < np; ++p)
+      for (int g = e->locbase[p]; g < e->locbase[p + 1]; ++g) {
+        long long nr = 0, none = 1;
+        for (int k = 0; k < e->nedges; ++k)
+          if (e->edge_chan[k] == ch && e->edge_dir[k] == -1 && e->locbase[e->edge_proc[k]] + e->edge_src[k] == g) {
+            ++nr;
+            none = std::min<long long>(none * std::max(1, e->cg_off[k + 1] - e->cg_off[k]), kRecipeCap + 1);
+          }
+        if (nr > 0) omax[(size_t)p * nch + ch] = std::max(omax[(size_t)p * nch + ch], nr + none);
+      }
+  }
+  for (int ch = 0; ch < nch; ++ch) {
+    if (e->chan_bcast && e->chan_bcast[ch]) {
+      for (int p = 0; p < np; ++p) {
+        long long prod = smax[(size_t)p * nch + ch];
+        for (int q = 0; q < np && prod > 0; ++q)
+          if (q != p) prod = std::min<long long>(prod * omax[(size_t)q * nch + ch], kRecipeCap + 1);
+        bin = std::min<long long>(bin + prod, kRecipeCap + 1);
+      }
+      continue;
+    }
     long long rtot = 0;
     for (int p = 0; p < np; ++p) rtot += rmax[(size_t)p * nch + ch];
     for (int p = 0; p < np; ++p) bin += smax[(size_t)p * nch + ch] * (rtot - rmax[(size_t)p * nch + ch]);
@@ -2278,8 +2385,6 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
   if (d > HORA_MAX_DIM) throw ScopeFail("zone dimension above HORA_MAX_DIM");
   if (c->np > HORA_MAX_PROCS || c->np < 1) throw ScopeFail("process count out of range");
   if (c->ni > HORA_MAX_INTS) throw ScopeFail("too many integer variables");
-  for (int ch = 0; ch < e->nchans; ++ch)
-    if (e->chan_bcast && e->chan_bcast[ch]) throw ScopeFail("broadcast channels are not covered by the device path");
   auto diag = [](const int32_t* data, long n) {
     for (long k = 0; k < n; ++k)
       if (data[3 * k] > 0 && data[3 * k + 1] > 0) return true;
@@ -2411,6 +2516,8 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
   N.cg_off = B + o_cg_off; N.cg_data = B + o_cg_data; N.rs_off = B + o_rs_off; N.rs_clk = B + o_rs_clk;
   N.ig_off = B + o_ig_off; N.ig_data = B + o_ig_data; N.iu_off = B + o_iu_off; N.iu_data = B + o_iu_data;
   N.chan_bcast = B + o_bc; N.chan_urgent = B + o_ur; N.has_urgent_chan = has_urgent;
+  N.has_bcast = 0;
+  for (int ch = 0; ch < e->nchans; ++ch) N.has_bcast |= bc[ch] != 0;
   N.Lx = B + o_Lx; N.Ux = B + o_Ux; N.lloc = c->lloc.p; N.uloc = c->uloc.p; N.Lq = B + o_Lq; N.Uq = B + o_Uq;
   N.amask = (const u64*)(B + o_amask); N.qbits = qbits; N.kshift = B + o_kshift; N.kmask = (const u64*)(B + o_kmask);
   N.int_lo = B + o_int_lo;

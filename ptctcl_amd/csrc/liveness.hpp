@@ -1333,6 +1333,11 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     }
   }
   if (c->dn.ncc > 0) { c->err = "open the context with a target without clock constraints for liveness"; return HORA_HIP_E_ARG; }
+  if (c->dn.has_bcast) {
+    // the graph records one (sender, receiver) pair per edge, which a broadcast does not fit
+    c->err = "broadcast channels are not covered by the liveness path";
+    return HORA_HIP_E_SCOPE;
+  }
   if (q->kind < HORA_LIVE_LEADSTO || q->kind > HORA_LIVE_DEADLOCK) return HORA_HIP_E_ARG;
   if (!c->live) c->live = new LiveState();
   LiveState& L = *c->live;
