@@ -566,26 +566,91 @@ __device__ __forceinline__ int build_recipes(const DevNet& N, const int* locs, c
   return n;
 }
 
-__global__ __launch_bounds__(64) void k_count(const DevNet* __restrict__ Np, const u64* __restrict__ fr_dk, int F,
-                                              int* __restrict__ cnt, Counters* ctr) {
-  extern __shared__ int sm[];
-  int* locs = sm;
-  int* ints = locs + HORA_MAX_PROCS;
-  int* rec = ints + HORA_MAX_INTS;
-  const DevNet& N = *Np;
-  const int i = blockIdx.x;
-  if (i >= F) return;
-  decode_key(N, fr_dk[i], locs, ints);
-  wsync();
-  int fz;
-  u64 am;
-  int ninv;
-  wave_loc_info(N, locs, ints, &fz, &am, nullptr, &ninv, 0);   // flags and mask only
-  const int r = build_recipes(N, locs, ints, (fz & 1) != 0, rec, N.rcap);
-  if (threadIdx.x == 0) {
-    cnt[i] = r < 0 ? 0 : r;
-    if (r < 0) atomicExch(&ctr->err, ERR_RECIPES);
+// The number of recipes build_recipes lists for the state with key dk (-1 where it would fail for
+// `cap`), one thread per state: the location and variable fields are read from the key as needed.
+// Broadcast combinations are counted by product, the committed rule by its complement: from a
+// committed state a combination whose sender is not committed survives unless every committed
+// process that takes part stays out.
+__device__ __forceinline__ int count_recipes(const DevNet& N, u64 dk, int cap) {
+  auto loc = [&](int p) { return (int)((dk >> N.kshift[p]) & N.kmask[p]); };
+  auto iok = [&](int e) {
+    for (int c = N.ig_off[e]; c < N.ig_off[e + 1]; ++c) {
+      const int v = N.ig_data[3 * c];
+      const int x = (int)((dk >> N.kshift[N.np + v]) & N.kmask[N.np + v]) + N.int_lo[v];
+      if (!atom_holds(x, N.ig_data[3 * c + 1], N.ig_data[3 * c + 2])) return false;
+    }
+    return true;
+  };
+  auto comm = [&](int p) { return (N.loc_flags[N.locbase[p] + loc(p)] & 1) != 0; };
+  bool src_c = false;
+  for (int p = 0; p < N.np; ++p) src_c |= comm(p);
+  long long n = 0;
+  for (int p = 0; p < N.np; ++p) {
+    if (src_c && !comm(p)) continue;
+    const int g = N.locbase[p] + loc(p);
+    for (int t = N.le_off[g]; t < N.le_off[g + 1]; ++t) {
+      const int e = N.le_edges[t];
+      if (N.edge_chan[e] < 0 && iok(e)) ++n;
+    }
   }
+  if (n > cap) return -1;
+  for (int i = 0; i < N.np; ++i) {
+    const int gi = N.locbase[i] + loc(i);
+    const bool i_comm = comm(i);
+    for (int t = N.le_off[gi]; t < N.le_off[gi + 1]; ++t) {
+      const int se = N.le_edges[t];
+      const int ch = N.edge_chan[se];
+      if (ch < 0 || N.edge_dir[se] != 1 || N.chan_bcast[ch] || !iok(se)) continue;
+      for (int j = 0; j < N.np; ++j) {
+        if (j == i) continue;
+        if (src_c && !(i_comm || comm(j))) continue;
+        const int gj = N.locbase[j] + loc(j);
+        for (int t2 = N.le_off[gj]; t2 < N.le_off[gj + 1]; ++t2) {
+          const int re = N.le_edges[t2];
+          if (N.edge_chan[re] == ch && N.edge_dir[re] == -1 && iok(re)) ++n;
+        }
+      }
+    }
+  }
+  if (n > cap) return -1;
+  if (!N.has_bcast) return (int)n;
+  for (int i = 0; i < N.np; ++i) {
+    const int gi = N.locbase[i] + loc(i);
+    const bool i_comm = comm(i);
+    for (int t = N.le_off[gi]; t < N.le_off[gi + 1]; ++t) {
+      const int se = N.le_edges[t];
+      const int ch = N.edge_chan[se];
+      if (ch < 0 || N.edge_dir[se] != 1 || !N.chan_bcast[ch] || !iok(se)) continue;
+      long long combos = 1, all_out = 1;   // all_out: combinations where every committed part stays out
+      for (int j = 0; j < N.np; ++j) {
+        if (j == i) continue;
+        const int gj = N.locbase[j] + loc(j);
+        int nrecv = 0, nnone = 1;
+        for (int t2 = N.le_off[gj]; t2 < N.le_off[gj + 1]; ++t2) {
+          const int re = N.le_edges[t2];
+          if (N.edge_chan[re] != ch || N.edge_dir[re] != -1 || !iok(re)) continue;
+          ++nrecv;
+          nnone = min(nnone * (N.cg_off[re + 1] - N.cg_off[re]), kRecipeCap + 1);
+        }
+        if (nrecv == 0) continue;
+        const long long nopt = nrecv + nnone;
+        combos = min(combos * nopt, (long long)cap + 1);
+        all_out = min(all_out * (comm(j) ? (long long)nnone : nopt), (long long)cap + 1);
+      }
+      if (n + combos > cap) return -1;   // the limit build_recipes applies before filtering
+      n += (src_c && !i_comm) ? combos - all_out : combos;
+    }
+  }
+  return (int)n;
+}
+
+__global__ __launch_bounds__(256) void k_count(const DevNet* __restrict__ Np, const u64* __restrict__ fr_dk, int F,
+                                               int* __restrict__ cnt, Counters* ctr) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= F) return;
+  const int r = count_recipes(*Np, fr_dk[i], Np->rcap);
+  cnt[i] = r < 0 ? 0 : r;
+  if (r < 0) atomicExch(&ctr->err, ERR_RECIPES);
 }
 
 // LDS layout of k_expand: 3 zones + row-0 scratch + 2 discrete states + recipes (rcap) + the source
@@ -595,8 +660,6 @@ __host__ __device__ constexpr int expand_lds_ints(int d2, int rcap, int icap) {
          3 * HORA_MAX_PROCS + 2;
 }
 
-// LDS of k_count: one discrete state and the recipes.
-inline size_t count_lds_bytes(int rcap) { return (size_t)(HORA_MAX_PROCS + HORA_MAX_INTS + 2 * rcap) * sizeof(int); }
 
 // z_apply_list over the source state's invariant triples, skipping those of the processes in
 // `moved`: `tag[k]` is the process of triple k.
@@ -2128,8 +2191,8 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     if (!init) {
       st->explored += (u64)F;
       ++level;
-      hipLaunchKernelGGL(k_count, dim3(F), dim3(64), count_lds_bytes(c->dn.rcap), c->stream, c->dnp,
-                         c->fr_dk.p, (int)F, c->rec_cnt.p, c->ctr.p);
+      hipLaunchKernelGGL(k_count, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->dnp, c->fr_dk.p, (int)F,
+                         c->rec_cnt.p, c->ctr.p);
       R = scan_exclusive(c, c->rec_cnt.p, F, c->pos.p);   // slot offsets, read by k_expand
     } else {
       HIP_OK(hipMemsetAsync(c->pos.p, 0, sizeof(long long), c->stream));

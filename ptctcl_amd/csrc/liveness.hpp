@@ -1432,7 +1432,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       c->pos.ensure(F + 1);
       if (!init) {
         ++st->levels;
-        hipLaunchKernelGGL(k_count, dim3(F), dim3(64), count_lds_bytes(c->dn.rcap), c->stream, c->dnp, c->fr_dk.p, (int)F,
+        hipLaunchKernelGGL(k_count, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->dnp, c->fr_dk.p, (int)F,
                            c->rec_cnt.p, c->ctr.p);
         hipLaunchKernelGGL(k_gfilter, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, eg ? 1 : 0,
                            c->fr_dk.p, (int)F, c->rec_cnt.p, L.expanded.p, (long)fbase);
