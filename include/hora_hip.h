@@ -260,8 +260,8 @@ typedef struct hora_live_stats {
 } hora_live_stats;
 
 /* Decide one liveness query on the context's network. Writes 1 to *holds if the property holds.
- * Returns HORA_HIP_E_SCOPE for what the device liveness path does not cover (d > 23, broadcast
- * channels), HORA_HIP_E_CAPPED when a federation outgrew the device capacity. */
+ * Returns HORA_HIP_E_SCOPE for what the device liveness path does not cover (d > 23),
+ * HORA_HIP_E_CAPPED when a federation outgrew the device capacity. */
 int hora_hip_live(hora_hip_ctx* ctx, const hora_live_query* q, int32_t* holds, hora_live_stats* stats);
 
 /* Inspection of the last hora_hip_live's graph and fixpoint (no counterpart in the reference:
@@ -275,7 +275,7 @@ typedef struct hora_live_graph {
   int64_t* row_off;          /* [nodes+1] edges of node v are [row_off[v], row_off[v+1])        */
   int32_t* etgt;             /* [edges] target node                                             */
   int32_t* ese;              /* [edges] first participant edge (sender or local)                */
-  int32_t* ere;              /* [edges] receiver edge or -1                                     */
+  int32_t* ere;              /* [edges] receiver edge, -1, or -2 - c for broadcast combination c */
   int8_t* expanded;          /* [nodes]                                                         */
   int8_t* in_s;              /* [nodes] in the fixpoint's closure                               */
   int8_t* w_nonempty;        /* [nodes] converged value non-empty                               */

@@ -7,7 +7,10 @@
 //
 // What is restated, and from where (paths relative to the HORA source tree):
 //   transitions      tagged_successors, src/liveness/operators.cpp:55-206 (local, then binary
-//                    synchronisation; the order of gen_succ_reuse). No extrapolation on this lane
+//                    synchronisation, then broadcast with the receivers' options of
+//                    build_broadcast_parts, semantics.hpp:275-303; the order of gen_succ_reuse).
+//                    An edge records (sender or local edge, receiver edge, or -2 - c for
+//                    combination c of a broadcast). No extrapolation on this lane
 //                    (emode = ExNone, src/liveness/engine.cpp:256-276).
 //   graph            LivenessEngine::intern / expand, src/liveness/construction.cpp:863-1086,
 //                    on its append-only branch: a successor goes to the first (oldest) stored node
@@ -148,6 +151,56 @@ struct Live {
     for (int c = 0; c < d; ++c) M[c] = std::max(L2[c], U2[c]);
   }
 
+  // Broadcast parts of sender edge se from the discrete state (sl, si) (build_broadcast_parts,
+  // semantics.hpp:275-303): per other process with an integer-enabled receiving edge, in process
+  // order, its options: each receiving edge, then each way of staying out where all of their
+  // clock guards fail (one negated atom per guard, the first guard the most significant).
+  struct BOpt { int edge; std::vector<Cons> extra; };
+  using BParts = std::vector<std::pair<int, std::vector<BOpt>>>;
+  BParts bparts(const int* sl, const int* si, int se) const {
+    BParts parts;
+    const int i = n.edges[se].proc, ch = n.edges[se].chan;
+    for (int j = 0; j < n.np; ++j) {
+      if (j == i) continue;
+      std::vector<BOpt> opts;
+      std::vector<const std::vector<Cons>*> guards;
+      for (int ej : n.edges_of[n.gl(j, sl[j])]) {
+        const Edge& re = n.edges[ej];
+        if (re.chan == ch && re.dir == -1 && int_ok(re.ig, si)) {
+          opts.push_back({ej, {}});
+          guards.push_back(&re.guard);
+        }
+      }
+      if (opts.empty()) continue;
+      std::vector<std::vector<Cons>> dnf{{}};
+      for (const auto* g : guards) {
+        if (g->empty()) { dnf.clear(); break; }
+        std::vector<std::vector<Cons>> next;
+        for (const auto& acc : dnf)
+          for (const auto& c : *g) { auto cp = acc; cp.push_back(negate(c)); next.push_back(cp); }
+        dnf.swap(next);
+      }
+      for (auto& dd : dnf) opts.push_back({-1, dd});
+      parts.push_back({j, opts});
+    }
+    return parts;
+  }
+  static long bcombos(const BParts& parts) {
+    long c = 1;
+    for (const auto& p : parts) c *= static_cast<long>(p.second.size());
+    return c;
+  }
+  // the option of each part in combination c, the first part the fastest digit
+  static std::vector<const BOpt*> bpick(const BParts& parts, long c) {
+    std::vector<const BOpt*> pick;
+    for (const auto& p : parts) {
+      const long k = static_cast<long>(p.second.size());
+      pick.push_back(&p.second[c % k]);
+      c /= k;
+    }
+    return pick;
+  }
+
   // successors with their transition (se, re): tagged_successors on the a_LU lane
   template <class F>
   void succ_rec(const int* sl, const int* si, const raw_t* sz, F&& emit) const {
@@ -197,6 +250,42 @@ struct Live {
             if (!int_ok(re.ig, si)) continue;
             fire(ei, ej);
           }
+        }
+      }
+    for (int i = 0; i < n.np; ++i)
+      for (int ei : n.edges_of[n.gl(i, sl[i])]) {
+        const Edge& se = n.edges[ei];
+        if (se.chan < 0 || se.dir != 1 || !n.bcast[se.chan]) continue;
+        if (!int_ok(se.ig, si)) continue;
+        const BParts parts = bparts(sl, si, ei);
+        const long combos = bcombos(parts);
+        for (long c = 0; c < combos; ++c) {
+          const auto pick = bpick(parts, c);
+          bool part_c = E.loc_committed(i, sl[i]);
+          for (std::size_t t = 0; t < parts.size(); ++t)
+            if (pick[t]->edge >= 0 && E.loc_committed(parts[t].first, sl[parts[t].first])) part_c = true;
+          if (src_c && !part_c) continue;
+          std::copy(sl, sl + n.np, nl.begin());
+          std::copy(si, si + n.ni, niv.begin());
+          nl[i] = se.tgt;
+          for (const auto& u : se.iu) niv[u.var] = u.val;
+          std::copy(sz, sz + d2, z.begin());
+          bool ok = apply_all(z.data(), d, se.guard);
+          for (std::size_t t = 0; ok && t < parts.size(); ++t)
+            ok = pick[t]->edge >= 0 ? apply_all(z.data(), d, n.edges[pick[t]->edge].guard)
+                                    : apply_all(z.data(), d, pick[t]->extra);
+          if (!ok) continue;
+          for (int r : se.resets) dbm_reset(z.data(), d, r);
+          for (std::size_t t = 0; t < parts.size(); ++t) {
+            if (pick[t]->edge < 0) continue;
+            const Edge& re = n.edges[pick[t]->edge];
+            nl[parts[t].first] = re.tgt;
+            for (const auto& u : re.iu) niv[u.var] = u.val;
+            for (int r : re.resets) dbm_reset(z.data(), d, r);
+          }
+          if (!E.close_succ_zone(nl.data(), niv.data(), z.data(), false)) continue;
+          if (dbm_empty(z.data(), d)) continue;
+          emit(nl.data(), niv.data(), z.data(), ei, static_cast<int>(-2 - c));
         }
       }
   }
@@ -306,13 +395,14 @@ struct Live {
     Fed live;
     bool covered = false;
     std::vector<int> tl(n.np);
-    auto add_src = [&](std::vector<raw_t>& g, int se, int re) {
+    // the moving edges of one escape: their resets, and the target location vector
+    auto add_src = [&](std::vector<raw_t>& g, const std::vector<int>& moves) {
       std::uint64_t rm = 0;
-      for (int r : n.edges[se].resets) rm |= std::uint64_t(1) << r;
-      if (re >= 0) for (int r : n.edges[re].resets) rm |= std::uint64_t(1) << r;
       std::copy(locs, locs + n.np, tl.begin());
-      tl[n.edges[se].proc] = n.edges[se].tgt;
-      if (re >= 0) tl[n.edges[re].proc] = n.edges[re].tgt;
+      for (int ed : moves) {
+        for (int r : n.edges[ed].resets) rm |= std::uint64_t(1) << r;
+        tl[n.edges[ed].proc] = n.edges[ed].tgt;
+      }
       for (int pr = 0; pr < n.np; ++pr)
         for (const auto& c : inv[n.gl(pr, tl[pr])]) {
           const int i = ((rm >> c.i) & 1) ? 0 : c.i;
@@ -331,7 +421,7 @@ struct Live {
         if (e.chan >= 0 || !int_ok(e.ig, ints)) continue;
         std::copy(Zd.begin(), Zd.end(), g.begin());
         if (!apply_all(g.data(), d, e.guard)) continue;
-        add_src(g, ei, -1);
+        add_src(g, {ei});
         if (covered) break;
       }
     }
@@ -351,9 +441,35 @@ struct Live {
             if (!int_ok(re.ig, ints)) continue;
             std::copy(Zd.begin(), Zd.end(), g.begin());
             if (!apply_all(g.data(), d, se.guard) || !apply_all(g.data(), d, re.guard)) continue;
-            add_src(g, ei, ej);
+            add_src(g, {ei, ej});
             if (covered) break;
           }
+        }
+      }
+    // broadcast escapes (deadlock.cpp:201-235): every combination of the receivers' options
+    for (int i = 0; i < n.np && !covered; ++i)
+      for (int ei : n.edges_of[n.gl(i, locs[i])]) {
+        if (covered) break;
+        const Edge& se = n.edges[ei];
+        if (se.chan < 0 || se.dir != 1 || !n.bcast[se.chan]) continue;
+        if (!int_ok(se.ig, ints)) continue;
+        const BParts parts = bparts(locs, ints, ei);
+        const long combos = bcombos(parts);
+        for (long c = 0; c < combos && !covered; ++c) {
+          const auto pick = bpick(parts, c);
+          bool part_c = E.loc_committed(i, locs[i]);
+          for (std::size_t t = 0; t < parts.size(); ++t)
+            if (pick[t]->edge >= 0 && E.loc_committed(parts[t].first, locs[parts[t].first])) part_c = true;
+          if (src_c && !part_c) continue;
+          std::copy(Zd.begin(), Zd.end(), g.begin());
+          bool ok = apply_all(g.data(), d, se.guard);
+          std::vector<int> moves{ei};
+          for (std::size_t t = 0; ok && t < parts.size(); ++t) {
+            if (pick[t]->edge < 0) { ok = apply_all(g.data(), d, pick[t]->extra); continue; }
+            ok = apply_all(g.data(), d, n.edges[pick[t]->edge].guard);
+            moves.push_back(pick[t]->edge);
+          }
+          if (ok) add_src(g, moves);
         }
       }
     if (stuck) stuck->clear();
@@ -428,19 +544,29 @@ struct Live {
       dbm_down(Y, d);
       if (!E.apply_inv(tl, Y)) return false;
     }
+    // the participating edges, and a broadcast's stay-out constraints (operators.cpp:283-300)
+    std::vector<int> parts{se};
+    std::vector<Cons> abstain;
+    if (re >= 0) parts.push_back(re);
+    if (re <= -2) {
+      const BParts bp = bparts(sl, si, se);
+      const auto pick = bpick(bp, -2L - re);
+      for (const BOpt* o : pick) {
+        if (o->edge >= 0) parts.push_back(o->edge);
+        else abstain.insert(abstain.end(), o->extra.begin(), o->extra.end());
+      }
+    }
     std::uint64_t rm = 0;
-    for (int k = 0; k < 2; ++k) {
-      const int ed = k == 0 ? se : re;
-      if (ed < 0) continue;
+    for (int ed : parts)
       for (int r : n.edges[ed].resets) {
         if (!dbm_constrain(Y, d, r, 0, kLe0)) return false;
         if (!dbm_constrain(Y, d, 0, r, kLe0)) return false;
         rm |= std::uint64_t(1) << r;
       }
-    }
     dbm_free_many(Y, d, rm);
-    if (!apply_all(Y, d, n.edges[se].guard)) return false;
-    if (re >= 0 && !apply_all(Y, d, n.edges[re].guard)) return false;
+    for (int ed : parts)
+      if (!apply_all(Y, d, n.edges[ed].guard)) return false;
+    if (!apply_all(Y, d, abstain)) return false;
     if (!frozen(sl, si)) dbm_down(Y, d);
     if (!E.apply_inv(sl, Y)) return false;
     return true;
@@ -720,8 +846,6 @@ int oracle_live(const hora_net_enc* e, const hora_live_query* q, int32_t* holds,
     never.never = 1;
     h->E = std::make_unique<orc::Engine>(e, &never);
     if (h->E->n.has_diag) { g_err = "diagonal constraints"; return HORA_HIP_E_SCOPE; }
-    for (int c = 0; c < h->E->n.nch; ++c)
-      if (h->E->n.bcast[c]) { g_err = "broadcast channels"; return HORA_HIP_E_SCOPE; }
     h->L = std::make_unique<Live>(*h->E, q);
     const int rc = h->L->run(holds, q->cover);
     if (st) *st = h->L->st;

@@ -34,7 +34,7 @@ namespace {
 
 constexpr int kLiveMaxDim = 23;
 constexpr int ERR_CAPPED = 5;
-constexpr int kLiveW = 8;       // zones in one value W[v]
+constexpr int kLiveW = 32;      // zones in one value W[v] at most (a u32 mask covers them)
 constexpr u32 kProbeSerial = 16;  // node segments up to this length are probed one node at a time
 
 struct DevPred {
@@ -75,6 +75,7 @@ struct LiveArgs {
   const int* in_s;            // [N] 1 = in the closure
   const int* sidx;            // [N] index into the S-indexed arrays, or -1
   int kl, kp, kc;             // federation capacities (LDS lists), set from the dimension
+  int kw;                     // zones in one value W[v] (the banks' capacity), set from the dimension
 };
 
 struct LiveCtr {
@@ -727,17 +728,33 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
     zcopy_w(g, Zd, d2);
     if (!apply_edge_guard(N, se, g)) continue;
     if (re >= 0 && !apply_edge_guard(N, re, g)) continue;
+    u64 rm = edge_reset_mask(N, se) | (re >= 0 ? edge_reset_mask(N, re) : 0ull);
+    if (re <= -2) {
+      // a broadcast (deadlock.cpp:201-235): each part's receiving edge, or its stay-out conjunction
+      bool ok = true;
+      bcast_walk(N, locs, ints, se, -2 - re, [&](int, int gj, int rj, int q) {
+        if (!ok) return;
+        if (rj >= 0) {
+          ok = apply_edge_guard(N, rj, g);
+          rm |= edge_reset_mask(N, rj);
+        } else {
+          ok = apply_bcast_none(N, gj, N.edge_chan[se], ints, q, g);
+        }
+      });
+      if (!ok) continue;
+    }
     wsync();
     if (l < N.np) tl[l] = locs[l];
     wsync();
     if (l == 0) {
       tl[N.edge_proc[se]] = N.edge_tgt[se];
       if (re >= 0) tl[N.edge_proc[re]] = N.edge_tgt[re];
+      if (re <= -2)
+        bcast_walk(N, locs, ints, se, -2 - re, [&](int j, int, int rj, int) { if (rj >= 0) tl[j] = N.edge_tgt[rj]; });
     }
     wsync();
     const int nt = wave_gather_inv(N, doff, ddat, tl, tinv);
     if (nt > K.ic) { overflow(ERR_RECIPES); return; }
-    const u64 rm = edge_reset_mask(N, se) | (re >= 0 ? edge_reset_mask(N, re) : 0ull);
     bool ok = true;
     for (int c = 0; c < nt && ok; ++c) {
       const int ci = tinv[3 * c], cj = tinv[3 * c + 1], cb = tinv[3 * c + 2];
@@ -1003,6 +1020,10 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
           sel = A.ese[e];
           rel = A.ere[e];
           rml = edge_reset_mask(N, sel) | (rel >= 0 ? edge_reset_mask(N, rel) : 0ull);
+          if (rel <= -2)   // a broadcast: the participating receivers reset too
+            bcast_walk(N, locs, ints, sel, -2 - rel, [&](int, int, int rj, int) {
+              if (rj >= 0) rml |= edge_reset_mask(N, rj);
+            });
         }
       }
     }
@@ -1032,6 +1053,15 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
         if (rm && !z_zero_free(pz, d, rm, tmp)) continue;   // resets to zero, then freed
         if (!apply_edge_guard(N, se, pz)) continue;
         if (re >= 0 && !apply_edge_guard(N, re, pz)) continue;
+        if (re <= -2) {
+          // a broadcast's receivers: their guards, and the stay-out conjunctions of those that
+          // did not take part (operators.cpp:290-300)
+          bool ok = true;
+          bcast_walk(N, locs, ints, se, -2 - re, [&](int, int gj, int rj, int q) {
+            if (ok) ok = rj >= 0 ? apply_edge_guard(N, rj, pz) : apply_bcast_none(N, gj, N.edge_chan[se], ints, q, pz);
+          });
+          if (!ok) continue;
+        }
         if (!fzv) z_down(pz, d);
         if (!z_apply_list(pz, d, inv, ninv)) continue;
         for (int k = 0; k < nW; ++k)
@@ -1093,7 +1123,7 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
     }
   }
   nn = fed_compact(cur, nn, d2, d, drop);
-  if (nn > kLiveW) {                  // the bank's capacity, the same in every launch
+  if (nn > A.kw) {                    // the bank's capacity, the same in every launch
     if (l == 0) atomicExch(&lc->err, ERR_CAPPED);
     keep();
     return;
@@ -1221,13 +1251,16 @@ void sync_graph_args(LiveState& L) {
 }
 
 // Full federation capacities from the LDS budget (64 KiB per workgroup), the same for every list,
-// sized so that both StopCaps and GfpCaps at these capacities fit. False when fewer than four zones
-// per list fit (zone dimension above 23).
+// sized so that both StopCaps and GfpCaps at these capacities fit (at most 256 zones: small zone
+// dimensions leave room for the many escapes of a broadcast's combinations), and the capacity of
+// one value: 8 zones, 32 up to dimension 8 (where a broadcast's stay-out conjunctions split
+// values further). False when fewer than four zones per list fit (zone dimension above 23).
 bool live_caps(int d2, LiveArgs& a) {
   const int budget = 64 * 1024 / 4;
   const int misc = 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * 3 * kInvCap + 2 * kRecipeCap + 128;
   const int zones = (budget - misc) / d2;
-  const int k = std::min(64, (zones - 6 - kLiveW) / 3);
+  a.kw = d2 <= 64 ? kLiveW : 8;
+  const int k = std::min(256, (zones - 6 - a.kw) / 3);
   a.kl = k;
   a.kp = k;
   a.kc = k;
@@ -1318,7 +1351,7 @@ GfpCaps gfp_caps_small(const DevNet& dn) {
 }
 
 GfpCaps gfp_caps_full(const LiveArgs& a, const DevNet& dn) {
-  return GfpCaps{kLiveW, a.kc, a.kp, dn.icap, dn.np, std::max(dn.ni, 1)};
+  return GfpCaps{a.kw, a.kc, a.kp, dn.icap, dn.np, std::max(dn.ni, 1)};
 }
 
 int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_live_stats* st) {
@@ -1333,11 +1366,6 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     }
   }
   if (c->dn.ncc > 0) { c->err = "open the context with a target without clock constraints for liveness"; return HORA_HIP_E_ARG; }
-  if (c->dn.has_bcast) {
-    // the graph records one (sender, receiver) pair per edge, which a broadcast does not fit
-    c->err = "broadcast channels are not covered by the liveness path";
-    return HORA_HIP_E_SCOPE;
-  }
   if (q->kind < HORA_LIVE_LEADSTO || q->kind > HORA_LIVE_DEADLOCK) return HORA_HIP_E_ARG;
   if (!c->live) c->live = new LiveState();
   LiveState& L = *c->live;
@@ -1791,7 +1819,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
           r.rcnt = 0;
           r.pfull = 0;
           HIP_OK(hipMemcpyAsync(L.lc.p, &r, sizeof(LiveCtr), hipMemcpyHostToDevice, c->stream));
-          compact_values(c, L, ns, sel_in, (long long)kLiveW * nd + (tight ? 0 : ns / 2));
+          compact_values(c, L, ns, sel_in, (long long)L.args.kw * nd + (tight ? 0 : ns / 2));
           ++compactions;
         }
         retried += h.rcnt;

@@ -191,3 +191,36 @@ def test_early_witness_needs_no_fixpoint():
     a, b = P.at(net, "A", "a"), P.at(net, "A", "b")
     r, st = binding.live(net, lv.leads_to(a, b))
     assert not r and st.early == 1 and st.closure == 0       # stop witness at the first state
+
+
+def _csma(n):
+    import os
+    from ptctcl_amd import uppaal
+    return uppaal.load(os.path.join(os.path.dirname(__file__), "golden", "models", f"csma{n}.xml"))
+
+
+@pytest.mark.parametrize("n", [4, 6])
+def test_csma_broadcast_recorded_verdicts(n):
+    """csma synchronises over a broadcast channel. `Bus.Transmitting --> Bus.Idle` satisfied
+    (benchmarks/results/live/hora_backward/hora_live_t1.csv:36-41), `A[] not deadlock` satisfied
+    (results/deadlock/hora/hora_deadlock_t1.csv:36-41), `Bus.Transmitting --> Bus.Collided`
+    violated by a cycle (no early witness) and `Bus.Idle --> Bus.Collided` violated at the initial
+    state (benchmarks/queries/live_violated/README.md:38-48), and the four convex `E[]` rows of
+    benchmarks/queries/eg/results-4-5.txt:22-25 (csma4, all satisfied)."""
+    net = _csma(n)
+    T, I, Col = P.at(net, "Bus", "Transmitting"), P.at(net, "Bus", "Idle"), P.at(net, "Bus", "Collided")
+    r, st = binding.live(net, lv.leads_to(T, I))
+    assert r and not st.capped
+    r, st = binding.live(net, lv.not_deadlock())
+    assert r
+    r, st = binding.live(net, lv.leads_to(T, Col))
+    assert not r and st.early == 0
+    r, st = binding.live(net, lv.leads_to(I, Col))
+    assert not r and st.early == 1 and st.outer_nodes < 10
+    if n == 4:
+        y0 = net.clock_names.index("y0")
+        for psi in (P.true(), ~Col, P.at(net, "S0", "Wait") | P.at(net, "S0", "Send"),
+                    ~(P.at(net, "S0", "Send") & P.at(net, "S1", "Send"))):
+            rnet, q = lv.exists_always(net, psi, [c_le(y0, 40)])
+            r, st = binding.live(rnet, q)
+            assert r and not st.capped
