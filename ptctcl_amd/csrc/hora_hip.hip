@@ -998,6 +998,30 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int* __restrict_
   }
 }
 
+// The whole scan in one workgroup, for n <= kScanBlock * kScanItems: one launch instead of three.
+__global__ __launch_bounds__(kScanBlock) void k_scan_small(const int* __restrict__ in, long n,
+                                                           long long* __restrict__ out,
+                                                           long long* __restrict__ total) {
+  __shared__ long long s[kScanBlock];
+  long long carry = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const long idx = (long)k * kScanBlock + threadIdx.x;
+    const long long v = idx < n ? in[idx] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < kScanBlock; o <<= 1) {
+      const long long t = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (idx < n) out[idx] = carry + s[threadIdx.x] - v;
+    carry += s[kScanBlock - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
 __global__ void k_offsets_to_int(const long long* __restrict__ in, long n, int* __restrict__ out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = (int)in[i];
@@ -1964,8 +1988,12 @@ namespace {
 void scan_launch(hora_hip_ctx* c, const int* in, long n, long long* out, int slot) {
   const long per = (long)kScanBlock * kScanItems;
   const long nb = std::max<long>(1, (n + per - 1) / per);
-  c->bsum.ensure(nb);
   c->scan_total.ensure(2);
+  if (n <= per) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kScanBlock), 0, c->stream, in, n, out, c->scan_total.p + slot);
+    return;
+  }
+  c->bsum.ensure(nb);
   if (n > 0) {
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, c->stream, in, n, c->bsum.p);
   } else {
@@ -1981,17 +2009,6 @@ long long scan_exclusive(hora_hip_ctx* c, const int* in, long n, long long* out)
   long long total = 0;
   read_dev(c, &total, c->scan_total.p, sizeof(total));
   return total;
-}
-
-// Two scans, one synchronisation.
-void scan_exclusive2(hora_hip_ctx* c, const int* in1, long n, long long* out1, const int* in2, long long* out2,
-                     long long* t1, long long* t2) {
-  scan_launch(c, in1, n, out1, 0);
-  scan_launch(c, in2, n, out2, 1);
-  long long t[2] = {0, 0};
-  read_dev(c, t, c->scan_total.p, sizeof(t));
-  *t1 = t[0];
-  *t2 = t[1];
 }
 
 inline unsigned blocks_for(long n, int bs) { return (unsigned)std::max<long>(1, (n + bs - 1) / bs); }
@@ -2258,10 +2275,24 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     c->segs.ensure(T);
     hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
                        c->caps.p, c->segs.p, c->ctr.p);
+    // light and heavy keys (k_merge's two shapes) are known once the chains are counted: the
+    // classification rides on the same synchronisation as the two scans
+    c->klight.ensure(T);
+    c->kheavy.ensure(T);
+    hipLaunchKernelGGL(k_classify, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                       c->klight.p, c->kheavy.p, c->ctr.p, kLightMax);
     c->pos.ensure(T + 1);
     c->pos2.ensure(T + 1);
     long long W = 0, Sseg = 0;
-    scan_exclusive2(c, c->caps.p, T, c->pos.p, c->segs.p, c->pos2.p, &W, &Sseg);
+    int nlh[2] = {0, 0};
+    {
+      scan_launch(c, c->caps.p, T, c->pos.p, 0);
+      scan_launch(c, c->segs.p, T, c->pos2.p, 1);
+      long long tt[2] = {0, 0};
+      read_dev(c, tt, c->scan_total.p, sizeof(tt), nlh, &c->ctr.p->nlight, sizeof(nlh));
+      W = tt[0];
+      Sseg = tt[1];
+    }
     // capacity: the reserved chain segments, and zone ids for at most W surviving new zones (the
     // ids are handed out compactly after the merge)
     if (zcount + (u64)W > P.zcap) c->zarena.grow_keep((size_t)(zcount + (u64)W) * 2 * d2, c->stream);
@@ -2277,12 +2308,6 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     c->zscr.ensure((size_t)std::max<long long>(W, 1) * d2);
     c->zused.ensure(W + 1);
     HIP_OK(hipMemsetAsync(c->zused.p, 0, (size_t)(W + 1) * sizeof(int), c->stream));
-    c->klight.ensure(T);
-    c->kheavy.ensure(T);
-    hipLaunchKernelGGL(k_classify, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
-                       c->klight.p, c->kheavy.p, c->ctr.p, kLightMax);
-    int nlh[2] = {0, 0};
-    read_dev(c, nlh, &c->ctr.p->nlight, sizeof(nlh));
     const size_t lds_l = merge_layout(d2, false).total() * sizeof(int);
     const size_t lds_h = merge_layout(d2, true).total() * sizeof(int);
     HIP_OK(hipEventRecord(c->ev[3], c->stream));
