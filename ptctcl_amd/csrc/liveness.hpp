@@ -94,9 +94,25 @@ struct LiveCtr {
   int sfull;                  // a stop set did not fit the pool
   int ptop;                   // zones taken from the value pool
   int pfull;                  // a value did not fit the pool
+  int wcnt;                   // early-witness candidates of the level
 };
 
 // ---------------------------------------------------------------- forward construction
+
+// The nodes of [first_id, first_id + n) where an early witness can stand: p /\ !q (leads-to) or
+// psi (E[] psi), listed for k_stop's mode 1 so that the others take no workgroup.
+__global__ void k_wfilter(const DevNet* __restrict__ Np, const LiveArgs* __restrict__ Ap, long n, long long first_id,
+                          int* __restrict__ list, LiveCtr* lc) {
+  const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const LiveArgs& A = *Ap;
+  bool keep = false;
+  if (b < n) {
+    const u64 dk = A.gdk[first_id + b];
+    keep = A.kind == HORA_LIVE_EG ? pred_eval(*Np, A.p, dk) : pred_eval(*Np, A.p, dk) && !pred_eval(*Np, A.q, dk);
+  }
+  const int idx = wave_append(&lc->wcnt, keep);
+  if (keep) list[idx] = (int)(first_id + b);
+}
 
 __global__ void k_gfilter(const DevNet* __restrict__ Np, const LiveArgs* __restrict__ A, int use_psi,
                           const u64* __restrict__ fr_dk, int F, int* __restrict__ cnt,
@@ -1210,7 +1226,8 @@ struct LiveState {
   DBuf<signed char> expanded, wne;
   long long nodes = 0, edges = 0;
   // backward pass
-  DBuf<int> in_s, slist, sidx, scnt, soff, sz, wcnt0, wcnt1, woff0, woff1, sel0, sel1, chg0, chg1, heads, dlist, rlist;
+  DBuf<int> in_s, slist, sidx, scnt, soff, sz, wcnt0, wcnt1, woff0, woff1, sel0, sel1, chg0, chg1, heads, dlist, rlist,
+      wlist;
   DBuf<int> wpool[2], wtmp;   // value pools (current, compaction target); per-node counts
   DBuf<long long> wnoff;      // compaction offsets
   int wcur = 0;
@@ -1618,7 +1635,16 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
         sync_graph_args(L);
         L.args.gzone = c->zarena.p;
         upload_args(c, L);
-        launch_stop(c, L, nullptr, (long)F, (long long)fbase, q->kind == HORA_LIVE_DEADLOCK ? 2 : 1);
+        if (q->kind == HORA_LIVE_DEADLOCK) {
+          launch_stop(c, L, nullptr, (long)F, (long long)fbase, 2);
+        } else {
+          L.wlist.ensure(F);
+          HIP_OK(hipMemsetAsync(&L.lc.p->wcnt, 0, sizeof(int), c->stream));
+          hipLaunchKernelGGL(k_wfilter, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, (long)F,
+                             (long long)fbase, L.wlist.p, L.lc.p);
+          const int nw = read_lc(c, L).wcnt;
+          launch_stop(c, L, L.wlist.p, (long)nw, 0, 1);
+        }
         const LiveCtr h = read_lc(c, L);
         if (h.err) {
           c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a stop set" : "invariant or transition table too large";
