@@ -22,6 +22,7 @@ from ptctcl_amd import hip, liveness as lv, models
 from ptctcl_amd.network import c_le
 
 import live_models
+import live_xml_cases
 
 pytestmark = pytest.mark.gpu
 P = lv.DPred
@@ -183,3 +184,16 @@ def test_fischer_counts_at_size(N):
     print(f"fischer{N}", r, st, flush=True)
     assert r and st.nonempty == 0 and st.route == 0
     assert (st.outer_nodes, st.fwd_edges, st.closure) == (g["nodes"], g["edges"], g["closure"])
+
+
+@pytest.mark.parametrize("case", live_xml_cases.CASES, ids=lambda c: f"{c[0]}:{c[1]}")
+def test_reference_models_recorded_verdicts(case):
+    """The reference's liveness rows on its own models (tests/live_xml_cases.py cites each): the
+    recorded verdict and witness shape, and the oracle's graph and fixpoint, node for node."""
+    name, text, holds, shape, _ = case
+    net = live_xml_cases.load(name)
+    for early in (True, False):
+        r, st = check(net, live_xml_cases.leads_to(net, text, early), f"{name} {text}")
+        assert r == holds
+        if early and not holds:
+            live_xml_cases.check_shape(shape, st)

@@ -23,6 +23,7 @@ from ptctcl_amd import hip, liveness as lv, models
 from ptctcl_amd.network import c_le
 
 import live_models
+import live_xml_cases
 from test_oracle_dbm import INF, Grid, le, rand_zone
 
 P = lv.DPred
@@ -224,3 +225,15 @@ def test_csma_broadcast_recorded_verdicts(n):
             rnet, q = lv.exists_always(net, psi, [c_le(y0, 40)])
             r, st = binding.live(rnet, q)
             assert r and not st.capped
+
+
+@pytest.mark.parametrize("case", live_xml_cases.CASES, ids=lambda c: f"{c[0]}:{c[1]}")
+def test_reference_models_recorded_verdicts(case):
+    """The reference's liveness rows on its own models (tests/live_xml_cases.py cites each)."""
+    name, text, holds, shape, _ = case
+    net = live_xml_cases.load(name)
+    for early in (True, False):
+        r, st = binding.live(net, live_xml_cases.leads_to(net, text, early))
+        assert r == holds and not st.capped, (name, text, early)
+        if early and not holds:
+            live_xml_cases.check_shape(shape, st)
