@@ -1,7 +1,10 @@
 """The reference's liveness queries on its benchmark models, with the verdicts it records (test
 fixtures; the models are the copies in tests/golden/models/).
 
-Satisfied rows: benchmarks/results/live/hora_backward/hora_live_t1.csv (line per row below).
+Satisfied leads-to rows: benchmarks/results/live/hora_backward/hora_live_t1.csv (line per row).
+`A[] not deadlock`: benchmarks/results/deadlock/hora/hora_deadlock_t1.csv (line per row).
+`E[]` with a clock part: benchmarks/queries/eg/results-4-5.txt (line per row; hora and verifyta
+agree on every one).
 Violated rows: benchmarks/queries/live_violated/*.q, "deliberately falsified variants"
 (benchmarks/queries/README.md:12); the witness shapes are the ones
 benchmarks/queries/live_violated/README.md:29-48 names: a stop witness found during the
@@ -10,6 +13,7 @@ construction, a cycle no single state certifies (no early witness), or the initi
 import os
 
 from ptctcl_amd import liveness as lv, uppaal
+from ptctcl_amd.network import c_le
 
 HERE = os.path.dirname(__file__)
 P = lv.DPred
@@ -53,3 +57,38 @@ def check_shape(shape, st):
         assert st.early == 1
     elif shape == INITIAL:
         assert st.early == 1 and st.outer_nodes < 10
+
+
+# (model, deadlock-free, citation)
+DEADLOCK = [
+    ("arbiter4", True, "hora_deadlock_t1.csv:2"),
+    ("grc4", False, "hora_deadlock_t1.csv:146"),
+    ("hddi2", True, "hora_deadlock_t1.csv:138"),
+    ("phil4", False, "hora_deadlock_t1.csv:155"),
+    ("single4", True, "hora_deadlock_t1.csv:92"),
+    ("twolock4", True, "hora_deadlock_t1.csv:101"),
+    ("csma4", True, "hora_deadlock_t1.csv:36"),
+]
+
+# (model, clock, bound, discrete part of psi, holds, citation): E[] psi with psi = part && clock <= bound
+EG = [
+    ("arbiter4", "x0", 40, lambda n: P.true(), True, "results-4-5.txt:12"),
+    ("arbiter4", "x0", 40, lambda n: ~P.at(n, "C1", "cs"), True, "results-4-5.txt:13"),
+    ("arbiter4", "x0", 40, lambda n: P.at(n, "C0", "idle") | P.at(n, "C0", "want"), False, "results-4-5.txt:14"),
+    ("arbiter4", "x0", 40, lambda n: ~(P.at(n, "C1", "cs") & P.at(n, "C2", "cs")), True, "results-4-5.txt:15"),
+    ("phil4", "c0", 40, lambda n: P.true(), True, "results-4-5.txt:32"),
+    ("phil4", "c0", 40, lambda n: ~P.at(n, "P1", "Eat"), True, "results-4-5.txt:33"),
+    ("phil4", "c0", 40, lambda n: ~(P.at(n, "P0", "Eat") & P.at(n, "P1", "Eat")), True, "results-4-5.txt:34"),
+    ("grc4", "x0", 15, lambda n: P.true(), True, "results-4-5.txt:40"),
+    ("grc4", "x0", 15, lambda n: ~P.at(n, "T0", "in"), True, "results-4-5.txt:41"),
+    ("grc4", "x0", 15, lambda n: P.at(n, "T0", "far") | P.at(n, "T0", "near"), True, "results-4-5.txt:42"),
+    ("grc4", "x0", 15, lambda n: ~(P.at(n, "T0", "in") & P.at(n, "T1", "in")), True, "results-4-5.txt:43"),
+    ("twolock4", "x0", 30, lambda n: P.true(), True, "results-4-5.txt:50"),
+    ("twolock4", "x0", 30, lambda n: ~P.at(n, "P1", "cs"), True, "results-4-5.txt:51"),
+    ("twolock4", "x0", 30, lambda n: P.at(n, "P0", "idle") | P.at(n, "P0", "waitA"), False, "results-4-5.txt:52"),
+]
+
+
+def eg_query(net, clock, bound, part):
+    """(network restricted by clock <= bound, E[] query)."""
+    return lv.exists_always(net, part(net), [c_le(net.clock_names.index(clock), bound)])

@@ -197,3 +197,20 @@ def test_reference_models_recorded_verdicts(case):
         assert r == holds
         if early and not holds:
             live_xml_cases.check_shape(shape, st)
+
+
+@pytest.mark.parametrize("case", live_xml_cases.DEADLOCK, ids=lambda c: c[0])
+def test_reference_models_recorded_deadlock(case):
+    name, want, _ = case
+    net = live_xml_cases.load(name)
+    for early in (True, False):
+        r, _ = check(net, lv.not_deadlock(early), f"{name} deadlock")
+        assert r == want
+
+
+@pytest.mark.parametrize("case", live_xml_cases.EG, ids=lambda c: c[5])
+def test_reference_models_recorded_eg(case):
+    name, clock, bound, part, want, _ = case
+    rnet, q = live_xml_cases.eg_query(live_xml_cases.load(name), clock, bound, part)
+    r, _ = check(rnet, q, f"{name} EG")
+    assert r == want

@@ -237,3 +237,20 @@ def test_reference_models_recorded_verdicts(case):
         assert r == holds and not st.capped, (name, text, early)
         if early and not holds:
             live_xml_cases.check_shape(shape, st)
+
+
+@pytest.mark.parametrize("case", live_xml_cases.DEADLOCK, ids=lambda c: c[0])
+def test_reference_models_recorded_deadlock(case):
+    name, want, _ = case
+    net = live_xml_cases.load(name)
+    for early in (True, False):
+        r, st = binding.live(net, lv.not_deadlock(early))
+        assert r == want and not st.capped, (name, early)
+
+
+@pytest.mark.parametrize("case", live_xml_cases.EG, ids=lambda c: c[5])
+def test_reference_models_recorded_eg(case):
+    name, clock, bound, part, want, _ = case
+    rnet, q = live_xml_cases.eg_query(live_xml_cases.load(name), clock, bound, part)
+    r, st = binding.live(rnet, q)
+    assert r == want and not st.capped
