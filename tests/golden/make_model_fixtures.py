@@ -22,10 +22,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSV = "benchmarks/results/reach/hora/hora_t16.csv"
 OTHER = ("benchmarks/results/reach/hora/hora_t24.csv", "benchmarks/results/reach/hora/hora_t32.csv")
 MODELS = {
-    "arbiter": [4, 8, 12, 16],
+    "arbiter": [4, 8, 12, 16, 18],
     "phil": [4, 6, 8, 10, 12],
-    "twolock": [4, 6, 8, 10, 12],
-    "csma": [4, 6, 8, 10],
+    "twolock": [4, 6, 8, 10, 12, 13],
+    "csma": [4, 6, 8, 10, 14, 16],
     "fischer": [4, 8, 12],
     "grc": [4, 5],
     "single": [4, 5, 6],

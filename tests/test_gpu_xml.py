@@ -43,7 +43,8 @@ def test_counters_equal_oracle(name):
         assert getattr(st_h, k) == st_o[k], k
 
 
-@pytest.mark.parametrize("name", ["arbiter16", "twolock10", "twolock12", "fischer12", "csma10"])
+@pytest.mark.parametrize("name", ["arbiter16", "arbiter18", "twolock10", "twolock12", "twolock13", "fischer12",
+                                  "csma10", "csma14", "csma16"])
 def test_recorded_counts_at_size(name):
     net, tgt = load(name)
     with hip.HipContext(net, tgt) as ctx:
