@@ -506,7 +506,9 @@ __device__ __forceinline__ int build_recipes(const DevNet& N, const int* locs, c
         ++off;
       }
     }
-  // binary synchronisations: senders spread over lanes by process, receivers walked in order
+  // binary synchronisations: per sender edge (process order, then edge order), the receivers of
+  // every other process at once, one lane per receiving process, placed by a prefix sum (the
+  // receiver process, then edge order)
   const u64 has_send = __ballot(l < N.np && N.le_off[g + 1] > N.le_off[g]);
   if (!has_send) return total;
   int n = total;
@@ -518,19 +520,26 @@ __device__ __forceinline__ int build_recipes(const DevNet& N, const int* locs, c
       const int ch = N.edge_chan[se];
       if (ch < 0 || N.edge_dir[se] != 1 || N.chan_bcast[ch]) continue;
       if (!int_ok(N, se, ints)) continue;
-      for (int j = 0; j < N.np; ++j) {
-        if (j == i) continue;
-        const int gj = N.locbase[j] + locs[j];
-        for (int t2 = N.le_off[gj]; t2 < N.le_off[gj + 1]; ++t2) {
+      // lane l: the receiving edges of process l (g is its location)
+      int rc = 0;
+      const bool part = l < N.np && l != i && (!src_c || i_comm || comm);
+      if (part)
+        for (int t2 = N.le_off[g]; t2 < N.le_off[g + 1]; ++t2) {
           const int re = N.le_edges[t2];
-          if (N.edge_chan[re] != ch || N.edge_dir[re] != -1) continue;
-          if (src_c && !(i_comm || committed_at(N, j, locs[j]))) continue;
-          if (!int_ok(N, re, ints)) continue;
-          if (n >= cap) return -1;
-          if (l == 0) { rec[2 * n] = se; rec[2 * n + 1] = re; }
-          ++n;
+          if (N.edge_chan[re] == ch && N.edge_dir[re] == -1 && int_ok(N, re, ints)) ++rc;
         }
-      }
+      int tot = 0;
+      int o = wave_excl_scan(rc, &tot);
+      if (n + tot > cap) return -1;
+      if (rc > 0)
+        for (int t2 = N.le_off[g]; t2 < N.le_off[g + 1]; ++t2) {
+          const int re = N.le_edges[t2];
+          if (N.edge_chan[re] != ch || N.edge_dir[re] != -1 || !int_ok(N, re, ints)) continue;
+          rec[2 * (n + o)] = se;
+          rec[2 * (n + o) + 1] = re;
+          ++o;
+        }
+      n += tot;
     }
   }
   if (!BC || !N.has_bcast) return n;
