@@ -260,8 +260,7 @@ typedef struct hora_live_stats {
 } hora_live_stats;
 
 /* Decide one liveness query on the context's network. Writes 1 to *holds if the property holds.
- * Returns HORA_HIP_E_SCOPE for what the device liveness path does not cover (d > 23),
- * HORA_HIP_E_CAPPED when a federation outgrew the device capacity. */
+ * Returns HORA_HIP_E_CAPPED when a federation outgrew the device capacity. */
 int hora_hip_live(hora_hip_ctx* ctx, const hora_live_query* q, int32_t* holds, hora_live_stats* stats);
 
 /* Inspection of the last hora_hip_live's graph and fixpoint (no counterpart in the reference:

@@ -32,7 +32,10 @@
 
 namespace {
 
-constexpr int kLiveMaxDim = 23;
+constexpr int kLiveMaxDim = HORA_MAX_DIM;
+// LDS one workgroup may take on gfx950; the full-capacity launches go above 64 KiB only where 64 KiB
+// would hold fewer than four zones per list (zone dimension above 23).
+constexpr int kLdsWorkgroupMax = 160 * 1024;
 constexpr int ERR_CAPPED = 5;
 constexpr int kLiveW = 32;      // zones in one value W[v] at most (a u32 mask covers them)
 constexpr u32 kProbeSerial = 16;  // node segments up to this length are probed one node at a time
@@ -75,6 +78,7 @@ struct LiveArgs {
   const int* in_s;            // [N] 1 = in the closure
   const int* sidx;            // [N] index into the S-indexed arrays, or -1
   int kl, kp, kc;             // federation capacities (LDS lists), set from the dimension
+  int skp;                    // pieces of a stop set's zone difference (its own LDS budget)
   int kw;                     // zones in one value W[v] (the banks' capacity), set from the dimension
 };
 
@@ -1267,18 +1271,24 @@ void sync_graph_args(LiveState& L) {
   L.args.expanded = L.expanded.p;
 }
 
-// Full federation capacities from the LDS budget (64 KiB per workgroup), the same for every list,
-// sized so that both StopCaps and GfpCaps at these capacities fit (at most 256 zones: small zone
-// dimensions leave room for the many escapes of a broadcast's combinations), and the capacity of
-// one value: 8 zones, 32 up to dimension 8 (where a broadcast's stay-out conjunctions split
-// values further). False when fewer than four zones per list fit (zone dimension above 23).
+// Full federation capacities from the LDS budget (64 KiB per workgroup, or up to 160 KiB where 64
+// KiB holds fewer than four zones per list), the same for every list, sized so that both StopCaps
+// and GfpCaps at these capacities fit (at most 256 zones: small zone dimensions leave room for the
+// many escapes of a broadcast's combinations), and the capacity of one value: 8 zones, 32 up to
+// dimension 8 (where a broadcast's stay-out conjunctions split values further). False when fewer
+// than four zones per list fit.
 bool live_caps(int d2, LiveArgs& a) {
-  const int budget = 64 * 1024 / 4;
   const int misc = 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * 3 * kInvCap + 2 * kRecipeCap + 128;
-  const int zones = (budget - misc) / d2;
   a.kw = d2 <= 64 ? kLiveW : 8;
-  const int k = std::min(256, (zones - 6 - a.kw) / 3);
-  a.kl = k;
+  int k = 0, ks = 0;
+  for (const int budget : {64 * 1024 / 4, kLdsWorkgroupMax / 4}) {
+    const int zones = (budget - misc) / d2;
+    k = std::min(256, (zones - 6 - a.kw) / 3);
+    ks = std::min(256, (zones - 6) / 3);     // k_stop holds no value: its lists take the rest
+    if (k >= 4) break;
+  }
+  a.skp = ks;
+  a.kl = ks;
   a.kp = k;
   a.kc = k;
   return k >= 4;
@@ -1300,7 +1310,7 @@ StopCaps stop_caps_small(const DevNet& dn) {
 }
 
 StopCaps stop_caps_full(const LiveArgs& a, const DevNet& dn) {
-  return StopCaps{a.kl, a.kp, dn.icap, dn.rcap, dn.np, std::max(dn.ni, 1)};
+  return StopCaps{a.kl, a.skp, dn.icap, dn.rcap, dn.np, std::max(dn.ni, 1)};
 }
 
 // k_stop over `n` nodes (list[b], or first_id + b without a list) in `mode`; the caller reads the
@@ -1358,6 +1368,7 @@ void compact_values(hora_hip_ctx* c, LiveState& L, int ns, const int* sel, long 
 // the small lists (zones of the value, of the candidates, of each piece list; invariant constraints).
 GfpCaps gfp_caps_small(const DevNet& dn) {
   GfpCaps k{2, 8, 4, std::min(32, dn.icap), dn.np, std::max(dn.ni, 1)};
+  if (k.ints(dn.d2) > 64 * 1024 / 4) k = GfpCaps{1, 4, 2, k.ic, k.np, k.ni};   // zone dimension above 27
   if (const char* e = std::getenv("HORA_HIP_GFP_CAPS")) {
     GfpCaps o = k;
     if (std::sscanf(e, "%d,%d,%d,%d", &o.kw, &o.kc, &o.kp, &o.ic) == 4 && o.kw >= 1 && o.kw <= kLiveW &&
@@ -1378,10 +1389,13 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
   {
     LiveArgs probe{};
     if (c->d > kLiveMaxDim || !live_caps(c->d2, probe)) {
-      c->err = "liveness on the device covers zone dimension up to 23";
+      c->err = "liveness on the device covers zone dimension up to 32";
       return HORA_HIP_E_SCOPE;
     }
   }
+  // the full-capacity launches may ask for more than 64 KiB of LDS
+  HIP_OK(hipFuncSetAttribute((const void*)k_stop, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsWorkgroupMax));
+  HIP_OK(hipFuncSetAttribute((const void*)k_gfp, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsWorkgroupMax));
   if (c->dn.ncc > 0) { c->err = "open the context with a target without clock constraints for liveness"; return HORA_HIP_E_ARG; }
   if (q->kind < HORA_LIVE_LEADSTO || q->kind > HORA_LIVE_DEADLOCK) return HORA_HIP_E_ARG;
   if (!c->live) c->live = new LiveState();
