@@ -24,6 +24,7 @@ it cannot represent rather than verify a different model.
 """
 from __future__ import annotations
 
+import os
 import re
 import xml.etree.ElementTree as ET
 from typing import Dict, List, Optional, Tuple
@@ -373,9 +374,9 @@ def parse_xml(text: str, name: str = "") -> Network:
 
 
 def load(path: str) -> Network:
+    """The network of the model file at `path`, named after the file."""
     with open(path, encoding="utf-8") as f:
         text = f.read()
-    import os
     return parse_xml(text, os.path.splitext(os.path.basename(path))[0])
 
 
