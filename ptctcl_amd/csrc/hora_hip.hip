@@ -1384,7 +1384,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
   if (n <= (u32)kWave) {
     const int v = (int)l < (int)n ? lst[l] : 0x7fffffff;
     int rank = 0;
-    for (int j = 0; j < kWave; ++j) rank += (__shfl(v, j) < v) ? 1 : 0;
+    for (int j = 0; j < (int)n; ++j) rank += (__shfl(v, j) < v) ? 1 : 0;   // lanes from n on hold the maximum
     if ((int)l < (int)n) sbuf[rank] = v;
     lst = sbuf;
   } else if (n <= (u32)ml.sort_cap) {
