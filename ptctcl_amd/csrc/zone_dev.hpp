@@ -218,29 +218,38 @@ __device__ __forceinline__ bool z_try_union(int* a, const int* b, int d, int* fa
     np += __popcll(mp);
   }
   wsync();
-  bool bad = false;
   if (nf > 0 && np > 0) {
+    // a wavefront of pairs at a time; the first crossing point found ends the test for all lanes
     const int pairs = nf * np;
-    for (int t = l; t < pairs && !bad; t += kWave) {
+    for (int base = 0; base < pairs; base += kWave) {
       // pass 1: facet of a from FA, entry from PA, compared against b
-      const int f = fa[t / np], e = pa[t % np];
-      const int i = ediv(f, d), j = f - i * d, p = ediv(e, d), q = e - p * d;
-      const int neg = r_neg(a[f]);
-      const int hpj = max(a[p * d + j], b[p * d + j]);
-      const int hiq = max(a[i * d + q], b[i * d + q]);
-      if (radd(radd(hpj, neg), hiq) > b[e]) bad = true;
+      const int t = base + l;
+      bool bad = false;
+      if (t < pairs) {
+        const int f = fa[t / np], e = pa[t % np];
+        const int i = ediv(f, d), j = f - i * d, p = ediv(e, d), q = e - p * d;
+        const int neg = r_neg(a[f]);
+        const int hpj = max(a[p * d + j], b[p * d + j]);
+        const int hiq = max(a[i * d + q], b[i * d + q]);
+        bad = radd(radd(hpj, neg), hiq) > b[e];
+      }
+      if (__any(bad)) return false;
     }
-    for (int t = l; t < pairs && !bad; t += kWave) {
+    for (int base = 0; base < pairs; base += kWave) {
       // pass 2: facet of b from PA, entry from FA, compared against a
-      const int f = pa[t / nf], e = fa[t % nf];
-      const int i = ediv(f, d), j = f - i * d, p = ediv(e, d), q = e - p * d;
-      const int neg = r_neg(b[f]);
-      const int hpj = max(a[p * d + j], b[p * d + j]);
-      const int hiq = max(a[i * d + q], b[i * d + q]);
-      if (radd(radd(hpj, neg), hiq) > a[e]) bad = true;
+      const int t = base + l;
+      bool bad = false;
+      if (t < pairs) {
+        const int f = pa[t / nf], e = fa[t % nf];
+        const int i = ediv(f, d), j = f - i * d, p = ediv(e, d), q = e - p * d;
+        const int neg = r_neg(b[f]);
+        const int hpj = max(a[p * d + j], b[p * d + j]);
+        const int hiq = max(a[i * d + q], b[i * d + q]);
+        bad = radd(radd(hpj, neg), hiq) > a[e];
+      }
+      if (__any(bad)) return false;
     }
   }
-  if (__any(bad)) return false;
   wsync();
   for (int e = l; e < d2; e += kWave) a[e] = max(a[e], b[e]);
   return true;
