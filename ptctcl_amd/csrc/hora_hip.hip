@@ -146,6 +146,26 @@ __device__ __forceinline__ void stat_add(Counters* c, int k, unsigned long long 
 
 // Wavefront-aggregated append: lanes with `pred` get consecutive indices from `*counter`, with one
 // atomic per wavefront. Every lane of the wavefront must call it.
+// Workgroup-aggregated append (blockDim.x a multiple of 64, at most 1024): one atomic per workgroup.
+// Every thread of the workgroup must call it; the order across workgroups is that of the atomics.
+__device__ __forceinline__ int block_append(int* counter, bool pred) {
+  __shared__ int wcnt[16], wbase;
+  const u64 b = __ballot(pred);
+  const int wid = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63), nw = (int)(blockDim.x >> 6);
+  __syncthreads();   // a previous call's readers are done with wcnt and wbase
+  if (lane == 0) wcnt[wid] = __popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < nw; ++w) tot += wcnt[w];
+    wbase = tot ? atomicAdd(counter, tot) : 0;
+  }
+  __syncthreads();
+  int base = wbase;
+  for (int w = 0; w < wid; ++w) base += wcnt[w];
+  return pred ? base + __popcll(b & ((1ull << lane) - 1)) : -1;
+}
+
 __device__ __forceinline__ int wave_append(int* counter, bool pred) {
   const u64 b = __ballot(pred);
   if (!b) return -1;
@@ -1184,7 +1204,7 @@ __global__ void k_ticket(Passed P, const long* __restrict__ gbk, const int* __re
     gticket[g] = t;
     first = t == 0;
   }
-  const int idx = wave_append(&ctr->ntouched, first);
+  const int idx = block_append(&ctr->ntouched, first);
   if (first) touched[idx] = (u32)h;
 }
 
@@ -1628,8 +1648,8 @@ __global__ void k_classify(Passed P, const u32* __restrict__ touched, int T, u32
   const bool ok = t < T;
   const u32 w = ok ? P.len[h] + P.ncnt[h] : 0u;
   const bool lt = ok && w <= (u32)light_max && P.ncnt[h] <= 64u;
-  const int il = wave_append(&ctr->nlight, lt);
-  const int ih = wave_append(&ctr->nheavy, ok && !lt);
+  const int il = block_append(&ctr->nlight, lt);
+  const int ih = block_append(&ctr->nheavy, ok && !lt);
   if (lt) light[il] = h;
   if (ok && !lt) heavy[ih] = h;
 }
@@ -2276,7 +2296,7 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     hipLaunchKernelGGL(k_probe, dim3(blocks_for(G, 4)), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p,
                        c->s_dk.p, c->s_zone.p, G, c->covered.p, c->ctr.p);
     HIP_OK(hipEventRecord(c->ev[6], c->stream));
-    hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p, G,
+    hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 1024)), dim3(1024), 0, c->stream, P, c->gbk.p, c->covered.p, G,
                        c->gticket.p, c->touched.p, c->ctr.p);
     int T = 0;
     read_dev(c, &T, &c->ctr.p->ntouched, sizeof(int));
@@ -2288,7 +2308,7 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     // classification rides on the same synchronisation as the two scans
     c->klight.ensure(T);
     c->kheavy.ensure(T);
-    hipLaunchKernelGGL(k_classify, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+    hipLaunchKernelGGL(k_classify, dim3(blocks_for(T, 1024)), dim3(1024), 0, c->stream, P, c->touched.p, T,
                        c->klight.p, c->kheavy.p, c->ctr.p, kLightMax);
     c->pos.ensure(T + 1);
     c->pos2.ensure(T + 1);
