@@ -114,7 +114,7 @@ __global__ void k_wfilter(const DevNet* __restrict__ Np, const LiveArgs* __restr
     const u64 dk = A.gdk[first_id + b];
     keep = A.kind == HORA_LIVE_EG ? pred_eval(*Np, A.p, dk) : pred_eval(*Np, A.p, dk) && !pred_eval(*Np, A.q, dk);
   }
-  const int idx = wave_append(&lc->wcnt, keep);
+  const int idx = block_append(&lc->wcnt, keep);
   if (keep) list[idx] = (int)(first_id + b);
 }
 
@@ -325,8 +325,8 @@ __global__ void k_gsplit(Passed P, const u32* __restrict__ touched, int T, u32* 
   const bool ok = t < T;
   const u32 h = ok ? touched[t] : 0u;
   const bool lt = ok && P.ncnt[h] <= (u32)kLightLive;
-  const int il = wave_append(&ctr->nlight, lt);
-  const int ih = wave_append(&ctr->nheavy, ok && !lt);
+  const int il = block_append(&ctr->nlight, lt);
+  const int ih = block_append(&ctr->nheavy, ok && !lt);
   if (lt) light[il] = h;
   if (ok && !lt) heavy[ih] = h;
 }
@@ -522,7 +522,7 @@ __global__ __launch_bounds__(64) void k_glink(Passed P, const u32* __restrict__ 
 __global__ void k_heads(Passed P, long H, int* __restrict__ heads, LiveCtr* lc) {
   const long h = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool ok = h < H && P.key[h] != kEmptyKey && P.len[h] > 0;
-  const int idx = wave_append(&lc->ns, ok);
+  const int idx = block_append(&lc->ns, ok);
   if (ok) heads[idx] = (int)P.nodes[P.off[h]];
 }
 
@@ -852,7 +852,7 @@ __global__ void k_sseed(const DevNet* __restrict__ Np, const LiveArgs* __restric
     else seed = v == 0 && pred_eval(*Np, A.p, dk);
   }
   if (v < N) in_s[v] = seed ? 1 : 0;
-  const int idx = wave_append(&lc->ns, seed);
+  const int idx = block_append(&lc->ns, seed);
   if (seed) slist[idx] = (int)v;
 }
 
@@ -948,7 +948,7 @@ __global__ void k_gdirty(const LiveArgs* __restrict__ Ap, const int* __restrict_
       }
     }
   }
-  const int idx = wave_append(&lc->dcnt, dirty);
+  const int idx = block_append(&lc->dcnt, dirty);
   if (s >= ns) return;
   if (dirty) dlist[idx] = s;
   else { B.sel_out[s] = bi; B.chg_out[s] = 0; }
