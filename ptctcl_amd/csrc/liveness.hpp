@@ -82,6 +82,8 @@ struct LiveArgs {
   int kw;                     // zones in one value W[v] (the banks' capacity), set from the dimension
 };
 
+constexpr int kLiveStripes = 64;
+
 struct LiveCtr {
   int hit;                    // early witness / deadlock found
   int err;
@@ -99,6 +101,10 @@ struct LiveCtr {
   int ptop;                   // zones taken from the value pool
   int pfull;                  // a value did not fit the pool
   int wcnt;                   // early-witness candidates of the level
+  // the fixpoint's per-node statistics, striped over cache lines (one node per workgroup would
+  // otherwise meet every other on one address): kLiveStripes x {pre_calls, products, changed},
+  // one 64-byte line each; read_lc sums them into the fields above
+  unsigned long long stripe[3][kLiveStripes][8];
 };
 
 // ---------------------------------------------------------------- forward construction
@@ -986,8 +992,8 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
   const int bi = B.sel_in[s];
   unsigned long long pcalls = 0, prods = 0;
   auto stats = [&]() __attribute__((always_inline)) {
-    if (l == 0 && pcalls) atomicAdd(&lc->pre_calls, pcalls);
-    if (l == 0 && prods) atomicAdd(&lc->products, prods);
+    if (l == 0 && pcalls) atomicAdd(&lc->stripe[0][blockIdx.x % kLiveStripes][0], pcalls);
+    if (l == 0 && prods) atomicAdd(&lc->stripe[1][blockIdx.x % kLiveStripes][0], prods);
   };
   auto keep = [&]() __attribute__((always_inline)) {
     stats();
@@ -1163,7 +1169,7 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
     woff(bo)[s] = off;
     B.sel_out[s] = bo;
     B.chg_out[s] = 1;
-    atomicAdd(&lc->changed, 1);
+    atomicAdd(&lc->stripe[2][blockIdx.x % kLiveStripes][0], 1ull);
   }
 }
 
@@ -1254,6 +1260,12 @@ void grow_keep_to(DBuf<T>& b, size_t n, hipStream_t s) {
 LiveCtr read_lc(hora_hip_ctx* c, LiveState& L) {
   LiveCtr h;
   read_dev(c, &h, L.lc.p, sizeof(LiveCtr));
+  unsigned long long sum[3] = {0, 0, 0};
+  for (int k = 0; k < 3; ++k)
+    for (int s = 0; s < kLiveStripes; ++s) sum[k] += h.stripe[k][s][0];
+  h.pre_calls = sum[0];
+  h.products = sum[1];
+  h.changed = (int)sum[2];
   return h;
 }
 
@@ -1822,8 +1834,8 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       int compactions = 0;
       for (;;) {
         const auto tr0 = std::chrono::steady_clock::now();
-        // changed; dcnt and rcnt are adjacent in LiveCtr
-        HIP_OK(hipMemsetAsync(&L.lc.p->changed, 0, sizeof(int), c->stream));
+        // changed (its stripes); dcnt and rcnt are adjacent in LiveCtr
+        HIP_OK(hipMemsetAsync(&L.lc.p->stripe[2][0][0], 0, sizeof(L.lc.p->stripe[2]), c->stream));
         HIP_OK(hipMemsetAsync(&L.lc.p->dcnt, 0, 2 * sizeof(int), c->stream));
         auto banks = [&]() {
           return GfpBanks{L.wcnt0.p, L.wcnt1.p, L.woff0.p, L.woff1.p, L.wpool[L.wcur].p, (int)L.wcap,
@@ -1853,9 +1865,9 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
           // the pool ran out: drop the round's writes, compact with room for every value at its
           // largest, and run the round again from the same values
           LiveCtr r = h;
-          r.pre_calls = h0.pre_calls;
-          r.products = h0.products;
-          r.changed = 0;
+          std::memcpy(r.stripe[0], h0.stripe[0], sizeof(r.stripe[0]));
+          std::memcpy(r.stripe[1], h0.stripe[1], sizeof(r.stripe[1]));
+          std::memset(r.stripe[2], 0, sizeof(r.stripe[2]));
           r.rcnt = 0;
           r.pfull = 0;
           HIP_OK(hipMemcpyAsync(L.lc.p, &r, sizeof(LiveCtr), hipMemcpyHostToDevice, c->stream));
