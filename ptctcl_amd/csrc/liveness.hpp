@@ -1237,7 +1237,7 @@ struct LiveState {
   long long nodes = 0, edges = 0;
   // backward pass
   DBuf<int> in_s, slist, sidx, scnt, soff, sz, wcnt0, wcnt1, woff0, woff1, sel0, sel1, chg0, chg1, heads, dlist, rlist,
-      wlist;
+      rlist2, wlist;
   DBuf<int> wpool[2], wtmp;   // value pools (current, compaction target); per-node counts
   DBuf<long long> wnoff;      // compaction offsets
   int wcur = 0;
@@ -1339,11 +1339,24 @@ void launch_stop(hora_hip_ctx* c, LiveState& L, const int* list, long n, long lo
   const int scap = mode == 0 ? (int)L.scap : 0;
   hipLaunchKernelGGL(k_stop, dim3(n), dim3(64), KA.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp, list, n,
                      first_id, mode, KA, L.rlist.p, scnt, soff, sz, scap, L.lc.p);
-  const LiveCtr h = read_lc(c, L);
-  if (h.rcnt > 0 && !h.err)
-    hipLaunchKernelGGL(k_stop, dim3(h.rcnt), dim3(64), KB.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
-                       (const int*)L.rlist.p, (long)h.rcnt, (long long)0, mode, KB, (int*)nullptr, scnt, soff, sz,
-                       scap, L.lc.p);
+  LiveCtr h = read_lc(c, L);
+  if (h.rcnt <= 0 || h.err) return;
+  // a middle tier between the small lists and the full ones: still several wavefronts per CU
+  const StopCaps KM{6, 6, KB.ic, KB.rc, KB.np, KB.ni};
+  const int* full_list = L.rlist.p;
+  if (KM.kl < KB.kl && KM.kp < KB.kp && KM.ints(d2) <= 64 * 1024 / 4 / 2) {
+    const int nm = h.rcnt;
+    L.rlist2.ensure(nm);
+    HIP_OK(hipMemsetAsync(&L.lc.p->rcnt, 0, sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_stop, dim3(nm), dim3(64), KM.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+                       (const int*)L.rlist.p, (long)nm, (long long)0, mode, KM, L.rlist2.p, scnt, soff, sz, scap,
+                       L.lc.p);
+    h = read_lc(c, L);
+    if (h.rcnt <= 0 || h.err) return;
+    full_list = L.rlist2.p;
+  }
+  hipLaunchKernelGGL(k_stop, dim3(h.rcnt), dim3(64), KB.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+                     full_list, (long)h.rcnt, (long long)0, mode, KB, (int*)nullptr, scnt, soff, sz, scap, L.lc.p);
 }
 
 // HORA_HIP_LIVE_POOL=tight: the stop-set and value pools start at the least they can hold and grow
