@@ -1342,9 +1342,10 @@ void launch_stop(hora_hip_ctx* c, LiveState& L, const int* list, long n, long lo
   LiveCtr h = read_lc(c, L);
   if (h.rcnt <= 0 || h.err) return;
   // a middle tier between the small lists and the full ones: still several wavefronts per CU
+  // (not under an override of the small lists, whose runs exercise the full lists)
   const StopCaps KM{6, 6, KB.ic, KB.rc, KB.np, KB.ni};
   const int* full_list = L.rlist.p;
-  if (KM.kl < KB.kl && KM.kp < KB.kp && KM.ints(d2) <= 64 * 1024 / 4 / 2) {
+  if (!std::getenv("HORA_HIP_STOP_CAPS") && KM.kl < KB.kl && KM.kp < KB.kp && KM.ints(d2) <= 64 * 1024 / 4 / 2) {
     const int nm = h.rcnt;
     L.rlist2.ensure(nm);
     HIP_OK(hipMemsetAsync(&L.lc.p->rcnt, 0, sizeof(int), c->stream));
@@ -1842,6 +1843,11 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       int* chg_in = L.chg0.p; int* chg_out = L.chg1.p;
       L.dlist.ensure(ns); L.rlist.ensure(ns);
       const GfpCaps KA = gfp_caps_small(c->dn), KB = gfp_caps_full(L.args, c->dn);
+      // a middle tier between the small lists and the full ones (at most 32 KiB, so that several
+      // wavefronts share a CU); not under an override of the small lists, whose runs exercise the full ones
+      const GfpCaps KM{std::min(4, KB.kw), std::min(16, KB.kc), std::min(8, KB.kp), KB.ic, KB.np, KB.ni};
+      const bool gfp_mid = !std::getenv("HORA_HIP_GFP_CAPS") && KM.kw >= KA.kw && KM.kc >= KA.kc && KM.kp >= KA.kp &&
+                           (KM.kc < KB.kc || KM.kp < KB.kp || KM.kw < KB.kw) && KM.ints(d2) <= 64 * 1024 / 4 / 2;
       int round = 0;
       long long retried = 0;
       int compactions = 0;
@@ -1869,9 +1875,19 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
             hipLaunchKernelGGL(k_gfp, dim3(nd), dim3(64), KA.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
                                L.slist.p, L.dlist.p, KA, banks(), L.rlist.p, L.lc.p);
           h = read_lc(c, L);
+          const int* full_list = L.rlist.p;
+          if (h.rcnt > 0 && !h.err && !h.pfull && gfp_mid) {
+            const int nm = h.rcnt;
+            L.rlist2.ensure(nm);
+            HIP_OK(hipMemsetAsync(&L.lc.p->rcnt, 0, sizeof(int), c->stream));
+            hipLaunchKernelGGL(k_gfp, dim3(nm), dim3(64), KM.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+                               L.slist.p, L.rlist.p, KM, banks(), L.rlist2.p, L.lc.p);
+            h = read_lc(c, L);
+            full_list = L.rlist2.p;
+          }
           if (h.rcnt > 0 && !h.err && !h.pfull) {
             hipLaunchKernelGGL(k_gfp, dim3(h.rcnt), dim3(64), KB.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
-                               L.slist.p, L.rlist.p, KB, banks(), (int*)nullptr, L.lc.p);
+                               L.slist.p, full_list, KB, banks(), (int*)nullptr, L.lc.p);
             h = read_lc(c, L);
           }
           if (!h.pfull || h.err) break;
