@@ -206,7 +206,7 @@ typedef struct hora_dpred {
   const int32_t* atoms;      /* 4 int32 per atom */
 } hora_dpred;
 
-enum { HORA_COVER_AUTO = 0, HORA_COVER_AM = 1 };
+enum { HORA_COVER_AUTO = 0, HORA_COVER_AM = 1, HORA_COVER_ONLINE = 2 };
 
 enum {
   HORA_LIVE_LEADSTO = 0,     /* p --> q: holds iff no reachable p-state has a maximal run avoiding q */
@@ -220,9 +220,16 @@ typedef struct hora_live_query {
                                 LEADSTO a p /\ !q node with a stop valuation (leadsto_witness,
                                 src/liveness/construction.cpp:786-813), EG a psi node with one
                                 (online_eg_terminal_witness, :748-767), DEADLOCK a deadlock     */
-  int32_t cover;             /* HORA_COVER_AUTO: a_LU covers unless a reachable discrete state can
-                                seed a deadlock atom (online_seed_key, src/liveness/refinement.cpp:
-                                176-282), then the a_M graph; HORA_COVER_AM: the a_M graph always */
+  int32_t cover;             /* HORA_COVER_AUTO: the reference's route (decide_leads_to,
+                                src/liveness/api.cpp:357-447): the online construction (a_LU covers,
+                                src/liveness/construction.cpp:480-683) unless a reachable discrete
+                                state can seed a deadlock atom (online_seed_key, src/liveness/
+                                refinement.cpp:176-282), then the a_M graph; p --> q on a network
+                                where D is provably a_LU (structurally_empty_d, src/liveness/
+                                internal.cpp:59-103) takes the base construction (:863-1003), which
+                                the device does not run (HORA_HIP_E_SCOPE: the caller falls back).
+                                HORA_COVER_ONLINE: as AUTO, but the online construction there too.
+                                HORA_COVER_AM: the a_M graph always. */
   int32_t pad;
   hora_dpred p;              /* LEADSTO: p.  EG: psi's discrete part.  DEADLOCK: unused           */
   hora_dpred q;              /* LEADSTO: q.  Otherwise unused                                     */
@@ -250,6 +257,10 @@ typedef struct hora_live_stats {
   int64_t fed_max;           /* widest converged value, in zones                                */
   int64_t products;          /* W /\ cand intersections formed (fixpoint.cpp:842-850)          */
   int64_t chain_reads;       /* stored or queued zones read by the cover tests of the construction */
+  int64_t retired;           /* records retired by a newer record of their key that covers them
+                                (online_compact_key, src/liveness/construction.cpp:426-477)       */
+  int64_t unions;            /* successful convex unions of a successor with an active record    */
+  int64_t enlarged;          /* A[] not deadlock: unexpanded records enlarged in place            */
   int32_t early;             /* 1: settled by a witness state during the construction           */
   int32_t capped;            /* 1: a federation outgrew its capacity; no verdict                */
   int32_t route;             /* covering of the graph decided on: 0 a_LU, 1 a_M                 */

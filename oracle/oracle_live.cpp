@@ -486,10 +486,14 @@ struct Live {
     return any;
   }
 
-  Fed stop_set(int v) const {
+  Fed stop_set(int v) const { return stop_set_at(v, row_off[v + 1] == row_off[v]); }
+
+  // stop_set of node v, told whether v has a successor (leadsto_witness, construction.cpp:786-813,
+  // takes it from the successors just generated)
+  Fed stop_set_at(int v, bool no_successor) const {
     Fed f;
     if (!restricted) {
-      if (row_off[v + 1] == row_off[v] || time_diverges(v)) { f.assign(Z(v), Z(v) + d2); return f; }
+      if (no_successor || time_diverges(v)) { f.assign(Z(v), Z(v) + d2); return f; }
       state_has_deadlock(v, n.inv, &f);
       return f;
     }
@@ -643,6 +647,307 @@ struct Live {
     for (std::size_t v = 0; v < dk.size(); ++v) row_off[v + 1] = row_off[v] + ecnt[v];
   }
 
+  // ---- the online construction (a_LU lane, no refinement atom) -------------------------------
+  // LivenessEngine::online_base_forward / online_drain_forward_work (refinement.cpp:935-1065),
+  // online_expand (:610-781) and online_intern (construction.cpp:480-683) with the map D empty,
+  // which is what structurally_empty_d / the seed test establish for the networks this lane runs:
+  //   * a FIFO worklist of active records in activation (= creation) order; a record retired
+  //     before its turn is skipped (refinement.cpp:1013-1014);
+  //   * each successor of the record being expanded, in tagged_successors order, goes to the
+  //     NEWEST active record of its key that a_LU-covers it (the active list is prepended on
+  //     activation, construction.cpp:385-396), whether or not that record includes it
+  //     (construction.cpp:515-535: inclusion, or a revocable cover with no atom to revoke it);
+  //   * otherwise the successor is joined with every active record but the root whose union with
+  //     it is one zone, in newest-first passes repeated until a pass joins nothing
+  //     (construction.cpp:607-640), and the result is appended as a new active record
+  //     (online_append, :309-327, the count `outer_nodes`); every other active record but the
+  //     root that it a_LU-covers is retired (online_compact_key, :426-477).
+  // Not restated: the in-place enlargement the reference applies under A[] not deadlock only
+  // (construction.cpp:590-605: an active unexpanded record that the successor includes takes the
+  // successor's zone). It changes which records are stored, not whether a deadlocked valuation is
+  // reachable; the recorded deadlock counts come from a route this restatement does not reproduce.
+  // A retired record keeps its zone and, if it was expanded, its edges; an edge to a record that
+  // was never expanded is redirected along the retirements to the active record that covers it
+  // (online_finalize_graph, refinement.cpp:1071-1160, online_resolve_cover, construction.cpp:359-378).
+  std::vector<char> active;
+  std::vector<int> retired_by;
+  std::vector<int> depth;
+  std::unordered_map<std::uint64_t, std::vector<int>> act;   // key -> its active records, oldest first
+  bool base_route = false;
+
+  void key_bounds(std::uint64_t key, std::int32_t* L2, std::int32_t* U2) const {
+    int locs[64], ints[64];
+    E.decode(key, locs, ints);
+    E.state_bounds(locs, L2, U2);
+  }
+
+  void retire_covered(int v) {   // online_compact_key
+    std::int32_t L2[64], U2[64];
+    key_bounds(dk[v], L2, U2);
+    auto& A = act[dk[v]];
+    std::vector<int> keep;
+    keep.reserve(A.size());
+    for (int j : A) {
+      if (j != 0 && j != v && dbm_alu_le(Z(j), Z(v), d, L2, U2)) {
+        active[j] = 0;
+        retired_by[j] = v;
+        ++st.retired;
+        continue;
+      }
+      keep.push_back(j);
+    }
+    A.swap(keep);
+  }
+
+  int online_intern(std::uint64_t key, std::vector<raw_t>& s, int dep, std::deque<int>& work) {
+    std::int32_t L2[64], U2[64];
+    key_bounds(key, L2, U2);
+    {
+      auto& A = act[key];
+      for (auto it = A.rbegin(); it != A.rend(); ++it) {
+        ++st.chain_reads;
+        if (dbm_alu_le(s.data(), Z(*it), d, L2, U2)) { ++st.covered; return *it; }
+      }
+      for (bool grew = true; grew;) {
+        grew = false;
+        for (auto it = A.rbegin(); it != A.rend(); ++it) {
+          const int j = *it;
+          if (j == 0 || dbm_includes(s.data(), Z(j), d)) continue;
+          if (dbm_try_union(s.data(), Z(j), d)) { grew = true; ++st.unions; }
+        }
+      }
+    }
+    const int id = new_node(key, s.data());
+    active.push_back(1);
+    retired_by.push_back(-1);
+    depth.push_back(dep);
+    act[key].push_back(id);
+    work.push_back(id);
+    retire_covered(id);
+    return id;
+  }
+
+  // ---- the base construction (networks where D is provably a_LU) ------------------------------
+  // LivenessEngine::base_forward (api.cpp:37-117), expand (construction.cpp:1034-1096) and intern
+  // (:863-1003), which decide_leads_to takes when structurally_empty_d holds (api.cpp:392):
+  // a FIFO worklist of unexpanded nodes; each successor goes to the OLDEST node of its key that
+  // a_LU-covers it, else the oldest node it a_LU-covers is overwritten with it in place, else the
+  // oldest node whose union with it is one zone absorbs it, else it is appended. A node rewritten
+  // by a successor of another node loses its edges and is expanded again.
+  std::vector<std::vector<std::array<int, 3>>> adj;
+
+  int base_intern(std::uint64_t key, std::vector<raw_t>& s, bool* grew) {
+    std::int32_t L2[64], U2[64];
+    key_bounds(key, L2, U2);
+    *grew = false;
+    auto it = chains.find(key);
+    if (it != chains.end()) {
+      const std::vector<int>& b = it->second;
+      for (int j : b) {
+        ++st.chain_reads;
+        if (dbm_alu_le(s.data(), Z(j), d, L2, U2)) { ++st.covered; return j; }
+      }
+      for (int j : b)
+        if (dbm_alu_le(Z(j), s.data(), d, L2, U2)) {
+          std::copy(s.begin(), s.end(), zones.begin() + static_cast<std::ptrdiff_t>(j) * d2);
+          ++st.enlarged;
+          *grew = true;
+          return j;
+        }
+      std::vector<raw_t> t(d2);
+      for (int j : b) {
+        std::copy(Z(j), Z(j) + d2, t.begin());
+        if (dbm_try_union(t.data(), s.data(), d)) {
+          std::copy(t.begin(), t.end(), zones.begin() + static_cast<std::ptrdiff_t>(j) * d2);
+          ++st.unions;
+          *grew = true;
+          return j;
+        }
+      }
+    }
+    const int id = new_node(key, s.data());
+    adj.emplace_back();
+    return id;
+  }
+
+  int build_base() {
+    std::vector<int> l0(n.np), i0(n.ni);
+    for (int pr = 0; pr < n.np; ++pr) l0[pr] = n.loc_init[pr];
+    for (int v = 0; v < n.ni; ++v) i0[v] = n.int_init[v];
+    std::vector<raw_t> z0(d2);
+    dbm_zero(z0.data(), d);
+    if (!E.close_succ_zone(l0.data(), i0.data(), z0.data(), false) || dbm_empty(z0.data(), d)) return -1;
+    adj.clear();
+    new_node(E.encode(l0.data(), i0.data()), z0.data());
+    adj.emplace_back();
+    std::deque<int> work{0};
+    std::vector<char> inq{1};
+    std::vector<int> sl(n.np), si(n.ni);
+    struct Succ { std::uint64_t key; std::vector<raw_t> z; int se, re; };
+    std::vector<Succ> succ;
+    int result = -1;
+    while (!work.empty()) {
+      const int u = work.front();
+      work.pop_front();
+      inq[u] = 0;
+      if (!expanded[u]) {
+        expanded[u] = 1;
+        ++st.expanded_nodes;
+        state(u, sl.data(), si.data());
+        succ.clear();
+        {
+          const std::vector<raw_t> zu(Z(u), Z(u) + d2);
+          succ_rec(sl.data(), si.data(), zu.data(), [&](const int* nl, const int* niv, const raw_t* z, int se, int re) {
+            succ.push_back({E.encode(nl, niv), std::vector<raw_t>(z, z + d2), se, re});
+          });
+        }
+        if (early_on) {
+          const bool none = succ.empty();
+          bool hit = false;
+          if (kind == HORA_LIVE_DEADLOCK) hit = state_has_deadlock(u, n.inv, nullptr);
+          else if (kind == HORA_LIVE_LEADSTO) hit = is_p(u) && !is_q(u) && !stop_set_at(u, none).empty();
+          if (hit) {
+            st.early = 1;
+            result = 0;
+            break;
+          }
+        }
+        for (auto& s : succ) {
+          ++st.generated;
+          bool grew = false;
+          const int v = base_intern(s.key, s.z, &grew);
+          adj[u].push_back({v, s.se, s.re});
+          if (grew) {
+            if (v != u) {
+              expanded[v] = 0;
+              adj[v].clear();
+            } else {
+              expanded[u] = 0;
+            }
+          }
+        }
+      }
+      inq.resize(dk.size(), 0);
+      for (const auto& e : adj[u])
+        if (!expanded[e[0]] && !inq[e[0]]) { inq[e[0]] = 1; work.push_back(e[0]); }
+    }
+    // the edges in node order
+    etgt.clear(); ese.clear(); ere.clear();
+    for (std::size_t v = 0; v < dk.size(); ++v) {
+      ecnt[v] = static_cast<std::int64_t>(adj[v].size());
+      for (const auto& e : adj[v]) { etgt.push_back(e[0]); ese.push_back(e[1]); ere.push_back(e[2]); }
+    }
+    return result;
+  }
+
+  // structurally_empty_d (src/liveness/internal.cpp:59-103): no urgency and no upper invariant, or
+  // a single possible discrete state whose universe seed has no partial deadlock region
+  bool structurally_empty() const {
+    bool time_open = !n.has_urgent_chan, one_key = true;
+    for (int L = 0; L < n.tl; ++L) {
+      if (n.loc_flags[L] & 3) time_open = false;
+      for (const auto& c : n.inv[L])
+        if (c.i > 0 && c.j == 0 && c.b < kInf) time_open = false;
+    }
+    for (int pr = 0; pr < n.np; ++pr)
+      if (n.locbase[pr + 1] - n.locbase[pr] != 1) one_key = false;
+    for (int v = 0; v < n.ni; ++v)
+      if (n.int_lo[v] != n.int_hi[v]) one_key = false;
+    if (time_open) return true;
+    if (!one_key) return false;
+    std::vector<int> l0(n.np), i0(n.ni);
+    for (int pr = 0; pr < n.np; ++pr) l0[pr] = n.loc_init[pr];
+    for (int v = 0; v < n.ni; ++v) i0[v] = n.int_init[v];
+    std::vector<raw_t> u(d2, kInf);
+    for (int j = 0; j < d; ++j) u[j] = kLe0;
+    for (int i = 0; i < d; ++i) u[i * d + i] = kLe0;
+    std::vector<raw_t> iz(u);
+    if (!apply_inv_table(n.inv, l0.data(), iz.data())) return true;
+    Fed stuck, esc;
+    if (!state_has_deadlock(l0.data(), i0.data(), u.data(), n.inv, &stuck, &esc)) return true;
+    if (!frozen(l0.data(), i0.data())) {
+      bool open = true;
+      for (int i = 1; i < d; ++i)
+        if (iz[i * d] != kInf) open = false;
+      if (open) return true;
+    }
+    std::vector<raw_t> t(d2);
+    for (std::size_t k = 0; k < esc.size() / d2; ++k)
+      if (dbm_intersect(t.data(), &esc[k * d2], iz.data(), d)) return false;
+    return true;
+  }
+
+  // Returns 1/0 when an early witness settled the query (value = holds), -1 otherwise.
+  int build_online() {
+    std::vector<int> l0(n.np), i0(n.ni);
+    for (int pr = 0; pr < n.np; ++pr) l0[pr] = n.loc_init[pr];
+    for (int v = 0; v < n.ni; ++v) i0[v] = n.int_init[v];
+    std::vector<raw_t> z0(d2);
+    dbm_zero(z0.data(), d);
+    if (!E.close_succ_zone(l0.data(), i0.data(), z0.data(), false) || dbm_empty(z0.data(), d)) return -1;
+    std::deque<int> work;
+    active.clear(); retired_by.clear(); depth.clear(); act.clear();
+    {
+      const std::uint64_t k0 = E.encode(l0.data(), i0.data());
+      new_node(k0, z0.data());
+      active.push_back(1);
+      retired_by.push_back(-1);
+      depth.push_back(0);
+      act[k0].push_back(0);
+      work.push_back(0);
+    }
+    struct Succ { std::uint64_t key; std::vector<raw_t> z; int se, re; };
+    std::vector<Succ> succ;
+    std::vector<int> sl(n.np), si(n.ni);
+    int result = -1, max_depth = -1;
+    while (!work.empty()) {
+      const int u = work.front();
+      work.pop_front();
+      if (!active[u] || expanded[u]) continue;
+      if (kind == HORA_LIVE_EG && !psi(u)) continue;   // outside psi: stored, never expanded
+      expanded[u] = 1;
+      ++st.expanded_nodes;
+      max_depth = std::max(max_depth, depth[u]);
+      state(u, sl.data(), si.data());
+      succ.clear();
+      {
+        const std::vector<raw_t> zu(Z(u), Z(u) + d2);
+        succ_rec(sl.data(), si.data(), zu.data(), [&](const int* nl, const int* niv, const raw_t* z, int se, int re) {
+          succ.push_back({E.encode(nl, niv), std::vector<raw_t>(z, z + d2), se, re});
+        });
+      }
+      // early witnesses at the record being expanded (online_expand, refinement.cpp:634-654)
+      if (early_on) {
+        const bool none = succ.empty();
+        bool hit = false;
+        if (kind == HORA_LIVE_DEADLOCK) hit = state_has_deadlock(u, n.inv, nullptr);
+        else if (kind == HORA_LIVE_LEADSTO) hit = is_p(u) && !is_q(u) && !stop_set_at(u, none).empty();
+        else hit = !stop_set_at(u, none).empty();
+        if (hit) {
+          st.early = 1;
+          result = kind == HORA_LIVE_EG ? 1 : 0;
+          break;
+        }
+      }
+      for (auto& s : succ) {
+        ++st.generated;
+        const int t = online_intern(s.key, s.z, depth[u] + 1, work);
+        etgt.push_back(t);
+        ese.push_back(s.se);
+        ere.push_back(s.re);
+        ++ecnt[u];
+      }
+    }
+    st.levels = max_depth + 1;
+    // edges to records never expanded go to the active record that covers them
+    for (auto& t : etgt) {
+      int r = t;
+      while (!expanded[r] && retired_by[r] >= 0) r = retired_by[r];
+      t = r;
+    }
+    return result;
+  }
+
   // ---- backward pass ------------------------------------------------------------------------------
   std::vector<int> closure() {
     const int N = static_cast<int>(dk.size());
@@ -774,9 +1079,10 @@ struct Live {
   int run(int32_t* holds, int cover) {
     const auto t0 = std::chrono::steady_clock::now();
     cover_am = cover == HORA_COVER_AM;
-    int er = build();
+    base_route = cover == HORA_COVER_AUTO && kind == HORA_LIVE_LEADSTO && structurally_empty();
+    int er = cover_am ? build() : base_route ? build_base() : build_online();
     if (er < 0 && !cover_am) {
-      // the a_LU graph stands unless one of its discrete states can seed a deadlock atom
+      // the online graph stands unless one of its discrete states can seed a deadlock atom
       std::int32_t seeds = 0;
       for (const auto& kv : chains) seeds += key_seeds(kv.first) ? 1 : 0;
       st.seed_keys = seeds;

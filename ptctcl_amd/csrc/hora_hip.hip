@@ -1962,6 +1962,9 @@ struct hora_hip_ctx {
   DevNet dn{};
   DevNet* dnp = nullptr;   // device copy of dn, refreshed when dn changes
   std::vector<int> h_loc_init, h_int_init, h_kshift, h_int_lo;
+  // the liveness lane of the reference takes its base construction here (structurally_empty_d,
+  // src/liveness/internal.cpp:59-103): 1 time-open network, 2 one possible discrete state
+  int base_route = 0;
   std::vector<u64> h_kmask;
   int has_target_clock = 0;
   // search state
@@ -2526,6 +2529,21 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
   c->h_loc_init = vec(e->loc_init, c->np);
   c->h_int_init = vec(e->int_init, c->ni);
   c->h_int_lo = vec(e->int_lo, c->ni);
+  {
+    bool time_open = true, one_key = true;
+    for (int ch = 0; ch < e->nchans; ++ch)
+      if (e->chan_urgent && e->chan_urgent[ch]) time_open = false;
+    for (int g = 0; g < c->tl; ++g) {
+      if (e->loc_flags[g] & 3) time_open = false;
+      for (int k = e->inv_off[g]; k < e->inv_off[g + 1]; ++k)
+        if (e->inv_data[3 * k] > 0 && e->inv_data[3 * k + 1] == 0 && e->inv_data[3 * k + 2] < HORA_INF) time_open = false;
+    }
+    for (int p = 0; p < c->np; ++p)
+      if (e->locbase[p + 1] - e->locbase[p] != 1) one_key = false;
+    for (int v = 0; v < c->ni; ++v)
+      if (e->int_lo[v] != e->int_hi[v]) one_key = false;
+    c->base_route = time_open ? 1 : one_key ? 2 : 0;
+  }
 
   // bounds: global L/U widened by the query (extend_bounds, engine.cpp:739-752), query floors
   std::vector<int> Lx = vec(e->lbound, d), Ux = vec(e->ubound, d), Lq(d, HORA_ALU_NEG_INF), Uq(d, HORA_ALU_NEG_INF);

@@ -33,11 +33,11 @@
 namespace {
 
 constexpr int kLiveMaxDim = HORA_MAX_DIM;
-// LDS one workgroup may take on gfx950; the full-capacity launches go above 64 KiB only where 64 KiB
-// would hold fewer than four zones per list (zone dimension above 23).
+// LDS one workgroup may take on gfx950: the full-capacity launches (the nodes that outgrow the small
+// and middle lists) take all of it.
 constexpr int kLdsWorkgroupMax = 160 * 1024;
 constexpr int ERR_CAPPED = 5;
-constexpr int kLiveW = 32;      // zones in one value W[v] at most (a u32 mask covers them)
+constexpr int kLiveW = 64;      // zones in one value W[v] at most (a u64 mask covers them)
 constexpr u32 kProbeSerial = 16;  // node segments up to this length are probed one node at a time
 
 struct DevPred {
@@ -80,6 +80,8 @@ struct LiveArgs {
   int kl, kp, kc;             // federation capacities (LDS lists), set from the dimension
   int skp;                    // pieces of a stop set's zone difference (its own LDS budget)
   int kw;                     // zones in one value W[v] (the banks' capacity), set from the dimension
+  const int* recnt;           // online construction: [N] successors of a node (its row length), else null
+  signed char* whit;          // online construction: [N] early witness found at the node, else null
 };
 
 constexpr int kLiveStripes = 64;
@@ -101,6 +103,16 @@ struct LiveCtr {
   int ptop;                   // zones taken from the value pool
   int pfull;                  // a value did not fit the pool
   int wcnt;                   // early-witness candidates of the level
+  // online construction, per interning of a level
+  int nev;                    // stored records retired (event list length)
+  int nuni;                   // convex unions
+  int nret;                   // records retired
+  int hzn;                    // widened new zones taken from the pool
+  int hzfull;                 // the pool was too small
+  int vchanged;               // the skip marks of the frontier changed
+  int wfirst;                 // first frontier record that is an early witness
+  int nexp;                   // frontier records expanded
+  int nkeys;                  // discrete states with an active record
   // the fixpoint's per-node statistics, striped over cache lines (one node per workgroup would
   // otherwise meet every other on one address): kLiveStripes x {pre_calls, products, changed},
   // one 64-byte line each; read_lc sums them into the fields above
@@ -173,7 +185,7 @@ __global__ __launch_bounds__(64) void k_gprobe(const DevNet* __restrict__ Np, Pa
                                                const int* __restrict__ sj, const u64* __restrict__ sdk,
                                                const int* __restrict__ szone, long G, const int* __restrict__ gzone,
                                                int* __restrict__ covered, int* __restrict__ cov_old, Counters* ctr,
-                                               int alu) {
+                                               int alu, int newest) {
   __shared__ int Lg[4][HORA_MAX_DIM], Ug[4][HORA_MAX_DIM], lg[4][HORA_MAX_PROCS];
   const DevNet& N = *Np;
   const int lane = threadIdx.x, q = lane / kGroup, t = lane % kGroup;
@@ -196,7 +208,8 @@ __global__ __launch_bounds__(64) void k_gprobe(const DevNet* __restrict__ Np, Pa
   const int* U = Ug[q];
   const u32* seg = P.nodes + (len > 0 ? P.off[h] : 0);
   int found = -1, reads = 0;
-  for (u32 m = 0; m < len; ++m) {   // group-uniform loop
+  for (u32 k = 0; k < len; ++k) {   // group-uniform loop
+    const u32 m = newest ? len - 1 - k : k;
     const int* M = gzone + (size_t)seg[m] * d2;
     ++reads;
     bool wit = false;   // S <=_aLU M fails (the predicate of z_alu_le)
@@ -676,7 +689,7 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
   const int* zv = A.gzone + (size_t)v * d2;
   auto out_whole = [&]() __attribute__((always_inline)) {
     if (mode == 0) { if (l == 0) scnt[A.sidx[v]] = -1; }
-    else if (l == 0) atomicExch(&lc->hit, 1);
+    else if (l == 0) { atomicExch(&lc->hit, 1); if (A.whit) A.whit[v] = 1; }
   };
   auto out_none = [&]() __attribute__((always_inline)) {
     if (mode == 0 && l == 0) scnt[A.sidx[v]] = 0;
@@ -708,9 +721,12 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
     }
     zv = zu;
   }
+  // the deadlock scan of a finished graph reads the expanded nodes only
+  if (mode == 2 && !list && !A.expanded[v]) return;
   if (mode == 0 || mode == 1) {   // the stop set proper (stop_set); modes 2, 3 test deadlock only
     if (!A.restricted) {
-      if (A.grow[v + 1] == A.grow[v] || time_diverges()) { out_whole(); return; }
+      const bool nosucc = A.recnt ? A.recnt[v] == 0 : A.grow[v + 1] == A.grow[v];
+      if (nosucc || time_diverges()) { out_whole(); return; }
     } else {
       if (time_diverges()) { out_whole(); return; }
       // orig_time_open (construction.cpp:1164-1191)
@@ -832,7 +848,7 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
     if (meets && l == 0) atomicAdd(&lc->hit, 1);
     return;
   }
-  if (mode != 0) { if (l == 0) atomicExch(&lc->hit, 1); return; }
+  if (mode != 0) { if (l == 0) { atomicExch(&lc->hit, 1); if (A.whit) A.whit[v] = 1; } return; }
   // the zones go to the stop-set pool; when it is full the caller grows it and runs the phase again
   const int s = A.sidx[v];
   int off = 0;
@@ -1024,8 +1040,8 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
   wave_loc_info(N, locs, ints, &fzv, &amv, inv, &ninv, K.ic);
   if (ninv > K.ic) { overflow(ERR_RECIPES); return; }
   const u64 cm = ((1ull << N.nclocks) - 1) << 1;
-  u32 covm = 0;
-  const u32 full = (nW >= 32) ? 0xffffffffu : ((1u << nW) - 1);
+  u64 covm = 0;
+  const u64 full = (nW >= 64) ? ~0ull : ((1ull << nW) - 1);
   int nc = 0;
   for (long long eb = e0; eb < e1; eb += kWave) {
     // One successor edge per lane: its target's value (count, pool offset) in the bank selected
@@ -1091,7 +1107,7 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
         if (!fzv) z_down(pz, d);
         if (!z_apply_list(pz, d, inv, ninv)) continue;
         for (int k = 0; k < nW; ++k)
-          if (!((covm >> k) & 1) && z_includes(pz, Wv + k * d2, d)) covm |= 1u << k;
+          if (!((covm >> k) & 1) && z_includes(pz, Wv + k * d2, d)) covm |= 1ull << k;
         if (covm == full) { keep(); return; }   // every member covered: the value cannot move
         if (nc >= K.kc) nc = fed_compact(cand, nc, d2, d, drop);
         if (nc >= K.kc) { overflow(ERR_CAPPED); return; }
@@ -1220,6 +1236,8 @@ __global__ void k_gverdict(const DevNet* __restrict__ Np, const LiveArgs* __rest
 
 }  // namespace
 
+#include "online.hpp"
+
 // ---------------------------------------------------------------- host driver
 
 struct LiveState {
@@ -1235,6 +1253,13 @@ struct LiveState {
   DBuf<int> etgt, ese, ere, s_se, s_re, cov_old, tslot, fnew, newlist;
   DBuf<signed char> expanded, wne;
   long long nodes = 0, edges = 0;
+  // online construction: record state, per-level interning scratch
+  DBuf<signed char> ractive, whit;
+  DBuf<int> rret, recnt, lret, ev, cret, fhz, hz, fskip, fnext, valid2, nflag, wl;
+  DBuf<long long> pos2, npos;
+  DBuf<u64> soff_save;
+  DBuf<u32> slen_save;
+  long long hzcap = 0;
   // backward pass
   DBuf<int> in_s, slist, sidx, scnt, soff, sz, wcnt0, wcnt1, woff0, woff1, sel0, sel1, chg0, chg1, heads, dlist, rlist,
       rlist2, wlist;
@@ -1283,22 +1308,20 @@ void sync_graph_args(LiveState& L) {
   L.args.expanded = L.expanded.p;
 }
 
-// Full federation capacities from the LDS budget (64 KiB per workgroup, or up to 160 KiB where 64
-// KiB holds fewer than four zones per list), the same for every list, sized so that both StopCaps
-// and GfpCaps at these capacities fit (at most 256 zones: small zone dimensions leave room for the
-// many escapes of a broadcast's combinations), and the capacity of one value: 8 zones, 32 up to
-// dimension 8 (where a broadcast's stay-out conjunctions split values further). False when fewer
-// than four zones per list fit.
+// Full federation capacities from gfx950's whole LDS budget of one workgroup (160 KiB: only the
+// nodes that outgrow the small and middle lists run at this size), the same for every list, sized
+// so that both StopCaps and GfpCaps at these capacities fit (at most 256 zones: small zone
+// dimensions leave room for the many escapes of a broadcast's combinations), and the capacity of
+// one value (8 to 32 zones, below). False when fewer than four zones per list fit.
 bool live_caps(int d2, LiveArgs& a) {
   const int misc = 2 * (HORA_MAX_PROCS + HORA_MAX_INTS) + 2 * 3 * kInvCap + 2 * kRecipeCap + 128;
-  a.kw = d2 <= 64 ? kLiveW : 8;
-  int k = 0, ks = 0;
-  for (const int budget : {64 * 1024 / 4, kLdsWorkgroupMax / 4}) {
-    const int zones = (budget - misc) / d2;
-    k = std::min(256, (zones - 6 - a.kw) / 3);
-    ks = std::min(256, (zones - 6) / 3);     // k_stop holds no value: its lists take the rest
-    if (k >= 4) break;
-  }
+  const int zones = (kLdsWorkgroupMax / 4 - misc) / d2;
+  // the widest value the lists leave room for: 64 zones while 32 candidate and piece zones still fit
+  // (zone dimension up to 15), then 32 while 16 fit (up to 21), then 16 (up to 29), then 8
+  a.kw = kLiveW;
+  while (a.kw > 8 && (zones - 6 - a.kw) / 3 < a.kw / 2) a.kw /= 2;
+  const int k = std::min(256, (zones - 6 - a.kw) / 3);
+  const int ks = std::min(256, (zones - 6) / 3);     // k_stop holds no value: its lists take the rest
   a.skp = ks;
   a.kl = ks;
   a.kp = k;
@@ -1408,6 +1431,354 @@ GfpCaps gfp_caps_full(const LiveArgs& a, const DevNet& dn) {
   return GfpCaps{a.kw, a.kc, a.kp, dn.icap, dn.np, std::max(dn.ni, 1)};
 }
 
+// Grows a per-record array to n entries, filling what is new with `fill` bytes.
+template <class T>
+void grow_rec(DBuf<T>& b, size_t n, int fill, hipStream_t s) {
+  if (n <= b.cap && b.p) return;
+  const size_t old = b.p ? b.cap : 0;
+  b.grow_keep(std::max<size_t>(n, 1024), s);
+  HIP_OK(hipMemsetAsync(b.p + old, fill, (b.cap - old) * sizeof(T), s));
+}
+
+// One construction of the graph by the online rule (online.hpp). Returns HORA_HIP_OK or an error
+// code; *early receives the verdict of an early witness, or -1.
+int construct_online(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, hora_live_stats* st, int* early) {
+  const int d2 = c->d2;
+  const bool eg = q->kind == HORA_LIVE_EG;
+  *early = -1;
+  {
+    const int32_t seeds = st->seed_keys;
+    std::memset(st, 0, sizeof(*st));
+    st->seed_keys = seeds;
+  }
+  HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
+  c->dn.lu_plus = 0;
+  HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
+  alloc_table(c, std::max(16, c->hbits));
+  c->ctr.ensure(1);
+  HIP_OK(hipMemsetAsync(c->ctr.p, 0, sizeof(Counters), c->stream));
+  {
+    const int ft = 0x7fffffff;
+    HIP_OK(hipMemcpyAsync(&c->ctr.p->first_target, &ft, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  }
+  const size_t ncap0 = 1 << 16, ecap0 = 1 << 18;
+  L.gdk.ensure(ncap0);
+  L.expanded.ensure(ncap0);
+  L.ractive.ensure(ncap0);
+  L.rret.ensure(ncap0);
+  L.recnt.ensure(ncap0);
+  L.lret.ensure(ncap0);
+  L.whit.ensure(ncap0);
+  HIP_OK(hipMemsetAsync(L.expanded.p, 0, L.expanded.cap, c->stream));
+  HIP_OK(hipMemsetAsync(L.whit.p, 0, L.whit.cap, c->stream));
+  HIP_OK(hipMemsetAsync(L.recnt.p, 0, L.recnt.cap * sizeof(int), c->stream));
+  HIP_OK(hipMemsetAsync(L.lret.p, 0xff, L.lret.cap * sizeof(int), c->stream));
+  c->nodes.ensure(1 << 20);
+  u64 ncount = 0;
+  c->zarena.ensure(ncap0 * d2);
+  L.etgt.ensure(ecap0); L.ese.ensure(ecap0); L.ere.ensure(ecap0);
+  L.nodes = 0;
+  L.edges = 0;
+  L.ns = 0;
+  if (L.hzcap <= 0) L.hzcap = 1024;
+  L.args.recnt = L.recnt.p;
+  L.args.whit = L.whit.p;
+
+  u64 dk0 = 0;
+  for (int p = 0; p < c->np; ++p) dk0 |= (u64)(unsigned)c->h_loc_init[p] << c->h_kshift[p];
+  for (int v = 0; v < c->ni; ++v) dk0 |= (u64)(unsigned)(c->h_int_init[v] - c->h_int_lo[v]) << c->h_kshift[c->np + v];
+  c->fr_dk.ensure(1);
+  c->fr_zid.ensure(1);
+  HIP_OK(hipMemcpyAsync(c->fr_dk.p, &dk0, sizeof(u64), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemsetAsync(c->fr_zid.p, 0, sizeof(u32), c->stream));
+
+  long F = 1;
+  bool init = true;
+  long long rounds_extra = 0;
+  for (;;) {
+    L.args.recnt = L.recnt.p;
+    L.args.whit = L.whit.p;
+    sync_graph_args(L);
+    L.args.gzone = c->zarena.p;
+    upload_args(c, L);
+    // ---- expansion of every frontier record
+    long R = 1;
+    c->rec_cnt.ensure(F);
+    c->rec_off.ensure(F);
+    c->pos.ensure(F + 1);
+    if (!init) {
+      hipLaunchKernelGGL(k_count, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->dnp, c->fr_dk.p, (int)F,
+                         c->rec_cnt.p, c->ctr.p);
+      if (eg)
+        hipLaunchKernelGGL(k_ogfilter, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, c->fr_dk.p,
+                           (int)F, c->rec_cnt.p);
+      R = scan_exclusive(c, c->rec_cnt.p, F, c->pos.p);
+      hipLaunchKernelGGL(k_offsets_to_int, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->pos.p, F, c->rec_off.p);
+    } else {
+      HIP_OK(hipMemsetAsync(c->rec_off.p, 0, sizeof(int), c->stream));
+      HIP_OK(hipMemsetAsync(c->pos.p, 0, sizeof(long long), c->stream));   // k_expand's slot offset
+    }
+    c->s_valid.ensure(R);
+    c->s_tgt.ensure(R);
+    c->s_dk.ensure(R);
+    c->s_zone.ensure((size_t)R * d2);
+    L.s_se.ensure(R);
+    L.s_re.ensure(R);
+    ExpandOut eo{c->s_valid.p, c->s_dk.p, c->s_zone.p, c->s_tgt.p, L.s_se.p, L.s_re.p};
+    if (R > 0) launch_expand(c, expand_tuned(c, F, eo, init), F, eo, init);
+    c->pos.ensure(R + 1);
+    const long long Gall = R > 0 ? scan_exclusive(c, c->s_valid.p, R, c->pos.p) : 0;
+    {
+      const Tally hc = read_counters(c);
+      if (hc.err) { c->err = "a state has more transitions or invariant constraints than the device holds"; return HORA_HIP_E_MEMORY; }
+    }
+    // ---- early witnesses of the frontier records, each judged at its own turn
+    if (!init) {
+      hipLaunchKernelGGL(k_ofsucc, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->rec_off.p, (int)F, R, c->pos.p,
+                         Gall, c->fr_zid.p, L.recnt.p);
+      if (q->early) {
+        L.wl.ensure(F);
+        HIP_OK(hipMemsetAsync(&L.lc.p->wcnt, 0, sizeof(int), c->stream));
+        hipLaunchKernelGGL(k_owlist, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, (int)F,
+                           c->fr_dk.p, c->fr_zid.p, L.wl.p, L.lc.p);
+        const int nw = read_lc(c, L).wcnt;
+        launch_stop(c, L, L.wl.p, (long)nw, 0, q->kind == HORA_LIVE_DEADLOCK ? 2 : 1);
+        const LiveCtr h = read_lc(c, L);
+        if (h.err) {
+          c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a stop set" : "invariant or transition table too large";
+          st->capped = h.err == ERR_CAPPED;
+          return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
+        }
+      }
+    }
+    // ---- interning of the level, repeated until the skip marks of the frontier are stable
+    L.fskip.ensure(F);
+    L.fnext.ensure(F);
+    HIP_OK(hipMemsetAsync(L.fskip.p, 0, (size_t)F * sizeof(int), c->stream));
+    HIP_OK(hipMemsetAsync(L.fnext.p, 0, (size_t)F * sizeof(int), c->stream));
+    L.valid2.ensure(std::max<long>(R, 1));
+    L.pos2.ensure(R + 1);
+    long G = 0;
+    int T = 0, wfirst = INT32_MAX;
+    LiveCtr hl{};
+    for (;;) {
+      wfirst = INT32_MAX;
+      if (!init && q->early) {
+        HIP_OK(hipMemcpyAsync(&L.lc.p->wfirst, &wfirst, sizeof(int), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_owfirst, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->fr_zid.p, (int)F, L.fskip.p,
+                           L.whit.p, L.lc.p);
+        wfirst = read_lc(c, L).wfirst;
+      }
+      if (R > 0)
+        hipLaunchKernelGGL(k_oexcl, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->rec_off.p, (int)F, R,
+                           L.fskip.p, wfirst, c->s_valid.p, L.valid2.p);
+      G = R > 0 ? (long)scan_exclusive(c, L.valid2.p, R, L.pos2.p) : 0;
+      HIP_OK(hipMemsetAsync(&L.lc.p->nev, 0, 5 * sizeof(int), c->stream));   // nev, nuni, nret, hzn, hzfull
+      T = 0;
+      if (G > 0) {
+        c->sj.ensure(G);
+        hipLaunchKernelGGL(k_compact_valid, dim3(blocks_for(R, 256)), dim3(256), 0, c->stream, L.valid2.p, L.pos2.p, R,
+                           c->sj.p, c->ctr.p);
+        {
+          const Tally hc = read_counters(c);
+          while ((double)(hc[C_KEYS] + (u64)G) > 0.6 * (double)c->H) grow_table(c);
+        }
+        Passed P = passed_view(c);
+        c->gbk.ensure(G);
+        c->gticket.ensure(G);
+        c->touched.ensure(G);
+        c->covered.ensure(G);
+        L.cov_old.ensure(G);
+        L.tslot.ensure(G);
+        L.fnew.ensure(G);
+        L.cret.ensure(G);
+        L.fhz.ensure(G);
+        HIP_OK(hipMemsetAsync(L.fnew.p, 0, (size_t)G * sizeof(int), c->stream));
+        HIP_OK(hipMemsetAsync(L.fhz.p, 0, (size_t)G * sizeof(int), c->stream));
+        HIP_OK(hipMemsetAsync(L.tslot.p, 0xff, (size_t)G * sizeof(int), c->stream));
+        HIP_OK(hipMemsetAsync(L.cret.p, 0xff, (size_t)G * sizeof(int), c->stream));
+        hipLaunchKernelGGL(k_find, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->sj.p, c->s_dk.p, G, c->gbk.p,
+                           c->ctr.p, (int*)nullptr, (int*)nullptr);
+        hipLaunchKernelGGL(k_gprobe, dim3(blocks_for(G, 4)), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p,
+                           c->s_dk.p, c->s_zone.p, G, c->zarena.p, c->covered.p, L.cov_old.p, c->ctr.p, 1, 1);
+        hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 1024)), dim3(1024), 0, c->stream, P, c->gbk.p, c->covered.p, G,
+                           c->gticket.p, c->touched.p, c->ctr.p);
+        read_dev(c, &T, &c->ctr.p->ntouched, sizeof(int));
+        if (T > 0) {
+          c->caps.ensure(T);
+          c->segs.ensure(T);
+          hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                             c->caps.p, c->segs.p, c->ctr.p);
+          c->klight.ensure(T);
+          c->kheavy.ensure(T);
+          hipLaunchKernelGGL(k_classify, dim3(blocks_for(T, 1024)), dim3(1024), 0, c->stream, P, c->touched.p, T,
+                             c->klight.p, c->kheavy.p, c->ctr.p, kLightMax);
+          c->pos2.ensure(T + 1);
+          L.segpos.ensure(T + 1);
+          const long long W = scan_exclusive(c, c->caps.p, T, c->pos2.p);
+          const long long Sseg = scan_exclusive(c, c->segs.p, T, L.segpos.p);
+          int nlh[2] = {0, 0};
+          read_dev(c, nlh, &c->ctr.p->nlight, sizeof(nlh));
+          if (ncount + (u64)Sseg > c->nodes.cap) c->nodes.grow_keep((size_t)(ncount + (u64)Sseg) * 2, c->stream);
+          P = passed_view(c);
+          hipLaunchKernelGGL(k_touched_offsets, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                             c->pos2.p, L.segpos.p, ncount);
+          ncount += (u64)Sseg;
+          L.soff_save.ensure(T);
+          L.slen_save.ensure(T);
+          hipLaunchKernelGGL(k_osave, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                             L.soff_save.p, L.slen_save.p, 0);
+          c->wlist.ensure(W + 1);
+          hipLaunchKernelGGL(k_scatter, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p,
+                             c->gticket.p, G, c->wlist.p);
+          L.hz.ensure((size_t)L.hzcap * d2);
+          L.ev.ensure(std::max<long>(G, 1) + (long)std::max<long long>(L.nodes, 1));
+          const int hzc = (int)std::min<long long>(L.hzcap, INT32_MAX);
+          const size_t lds_l = merge_layout(d2, false).total() * sizeof(int);
+          const size_t lds_h = merge_layout(d2, true).total() * sizeof(int);
+          if (nlh[0] > 0)
+            hipLaunchKernelGGL(k_omerge, dim3(nlh[0]), dim3(64), lds_l, c->stream, c->dnp, P, c->klight.p, nlh[0],
+                               c->wlist.p, c->sj.p, c->s_zone.p, L.hz.p, hzc, L.fhz.p, L.tslot.p, L.fnew.p, L.cret.p,
+                               L.lret.p, L.ev.p, L.lc.p, c->ctr.p, 0);
+          if (nlh[1] > 0)
+            hipLaunchKernelGGL(k_omerge, dim3(nlh[1]), dim3(64), lds_h, c->stream, c->dnp, P, c->kheavy.p, nlh[1],
+                               c->wlist.p, c->sj.p, c->s_zone.p, L.hz.p, hzc, L.fhz.p, L.tslot.p, L.fnew.p, L.cret.p,
+                               L.lret.p, L.ev.p, L.lc.p, c->ctr.p, 1);
+          hipLaunchKernelGGL(k_opost, dim3(G), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, (long)G, c->covered.p,
+                             c->wlist.p, c->sj.p, c->s_dk.p, c->s_zone.p, L.hz.p, L.fhz.p, L.tslot.p, c->ctr.p);
+        }
+      }
+      hl = read_lc(c, L);
+      bool again = false;
+      if (hl.hzfull) {
+        L.hzcap = std::max<long long>(2 * L.hzcap, 2LL * hl.hzn);
+        again = true;
+      } else if (!init && hl.nev > 0) {
+        HIP_OK(hipMemsetAsync(&L.lc.p->vchanged, 0, sizeof(int), c->stream));
+        hipLaunchKernelGGL(k_ovalid, dim3(blocks_for(hl.nev, 256)), dim3(256), 0, c->stream, L.ev.p, hl.nev, L.lret.p,
+                           c->fr_zid.p, (int)F, c->sj.p, c->rec_off.p, L.fnext.p);
+        hipLaunchKernelGGL(k_ovcmp, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, L.fskip.p, L.fnext.p, (int)F,
+                           L.lc.p);
+        again = read_lc(c, L).vchanged != 0;
+      }
+      if (!again) break;
+      // put the level's chains back and intern it again
+      ++rounds_extra;
+      if (T > 0) {
+        Passed P = passed_view(c);
+        hipLaunchKernelGGL(k_osave, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                           L.soff_save.p, L.slen_save.p, 1);
+      }
+      if (hl.nev > 0)
+        hipLaunchKernelGGL(k_oclear_ev, dim3(blocks_for(hl.nev, 256)), dim3(256), 0, c->stream, L.ev.p, hl.nev, L.lret.p);
+    }
+    // ---- the level's records, edges and retirements
+    long F2 = 0;
+    const long long nbase = L.nodes;
+    if (G > 0) {
+      c->pos.ensure(G + 1);
+      F2 = (long)scan_exclusive(c, L.fnew.p, G, c->pos.p);
+      const size_t nneed = (size_t)(nbase + F2 + 1);
+      grow_rec(L.gdk, nneed, 0, c->stream);
+      grow_rec(L.expanded, nneed, 0, c->stream);
+      grow_rec(L.ractive, nneed, 0, c->stream);
+      grow_rec(L.rret, nneed, 0xff, c->stream);
+      grow_rec(L.recnt, nneed, 0, c->stream);
+      grow_rec(L.lret, nneed, 0xff, c->stream);
+      grow_rec(L.whit, nneed, 0, c->stream);
+      if (nneed * d2 > c->zarena.cap) c->zarena.grow_keep(std::max(nneed * d2, c->zarena.cap * 2), c->stream);
+      if (!init && (size_t)(L.edges + G) > L.etgt.cap) {
+        const size_t ec2 = std::max((size_t)(L.edges + G), L.etgt.cap * 2);
+        grow_keep_to(L.etgt, ec2, c->stream);
+        grow_keep_to(L.ese, ec2, c->stream);
+        grow_keep_to(L.ere, ec2, c->stream);
+      }
+      L.newlist.ensure(std::max<long>(F2, 1));
+      L.nflag.ensure(std::max<long>(F2, 1));
+      hipLaunchKernelGGL(k_oassign, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, (long)G, nbase, L.edges,
+                         init ? 0 : 1, L.cov_old.p, L.tslot.p, L.fnew.p, c->pos.p, c->sj.p, L.s_se.p, L.s_re.p,
+                         L.etgt.p, L.ese.p, L.ere.p, L.newlist.p);
+      if (F2 > 0) {
+        hipLaunchKernelGGL(k_ocopy, dim3(F2), dim3(64), 0, c->stream, (long)F2, nbase, L.newlist.p, c->sj.p, c->s_dk.p,
+                           c->s_zone.p, L.hz.p, L.fhz.p, L.cret.p, d2, c->zarena.p, L.gdk.p, L.ractive.p, L.rret.p,
+                           L.nflag.p);
+        hipLaunchKernelGGL(k_oretire_new, dim3(blocks_for(F2, 256)), dim3(256), 0, c->stream, (long)F2, nbase,
+                           L.newlist.p, L.cret.p, c->pos.p, L.ractive.p, L.rret.p);
+      }
+      if (hl.nev > 0) {
+        hipLaunchKernelGGL(k_oretire_old, dim3(blocks_for(hl.nev, 256)), dim3(256), 0, c->stream, L.ev.p, hl.nev,
+                           L.lret.p, nbase, c->pos.p, L.ractive.p, L.rret.p);
+        hipLaunchKernelGGL(k_oclear_ev, dim3(blocks_for(hl.nev, 256)), dim3(256), 0, c->stream, L.ev.p, hl.nev, L.lret.p);
+      }
+      if (T > 0) {
+        Passed P = passed_view(c);
+        hipLaunchKernelGGL(k_olink, dim3(T), dim3(64), 0, c->stream, P, c->touched.p, T, c->pos.p, nbase);
+      }
+    }
+    if (!init) {
+      HIP_OK(hipMemsetAsync(&L.lc.p->nexp, 0, sizeof(int), c->stream));
+      hipLaunchKernelGGL(k_ofront, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, c->rec_off.p,
+                         (int)F, R, L.pos2.p, (long long)G, c->fr_zid.p, c->fr_dk.p, L.fskip.p, wfirst, eg ? 1 : 0,
+                         L.expanded.p, L.recnt.p, L.lc.p);
+    }
+    // next frontier: the new records still active, in creation order
+    long Fn = 0;
+    if (F2 > 0) {
+      L.npos.ensure(F2 + 1);
+      Fn = (long)scan_exclusive(c, L.nflag.p, F2, L.npos.p);
+      c->fr_dk2.ensure(std::max<long>(Fn, 1));
+      c->fr_zid2.ensure(std::max<long>(Fn, 1));
+      hipLaunchKernelGGL(k_onext, dim3(blocks_for(F2, 256)), dim3(256), 0, c->stream, (long)F2, nbase, L.nflag.p,
+                         L.npos.p, L.newlist.p, c->sj.p, c->s_dk.p, c->fr_dk2.p, c->fr_zid2.p);
+    }
+    const LiveCtr he = read_lc(c, L);
+    {
+      const Tally hc = read_counters(c);
+      if (hc.err) { c->err = hc.err == ERR_HASH_FULL ? "hash table full" : "device capacity exceeded"; return HORA_HIP_E_MEMORY; }
+      st->chain_reads = (long long)(hc[C_READS] + hc[C_PREADS]);
+    }
+    if (!init) {
+      st->expanded_nodes += he.nexp;
+      if (he.nexp > 0) ++st->levels;
+      st->generated += G;
+      st->covered += G - F2;
+      L.edges += G;
+    }
+    st->unions += hl.nuni;
+    st->retired += hl.nret;
+    L.nodes += F2;
+    if (c->trace)
+      std::fprintf(stderr, "online level F=%ld R=%ld G=%ld T=%d new=%ld next=%ld retired=%d unions=%d extra rounds so far %lld\n",
+                   F, R, G, T, F2, Fn, hl.nret, hl.nuni, rounds_extra);
+    if (wfirst != INT32_MAX) {
+      *early = eg ? 1 : 0;
+      st->early = 1;
+      break;
+    }
+    std::swap(c->fr_dk.p, c->fr_dk2.p); std::swap(c->fr_dk.cap, c->fr_dk2.cap);
+    std::swap(c->fr_zid.p, c->fr_zid2.p); std::swap(c->fr_zid.cap, c->fr_zid2.cap);
+    F = Fn;
+    init = false;
+    if (F == 0) break;
+  }
+  // edges to records never expanded go to their active coverer; rows from the edge counts
+  if (L.edges > 0)
+    hipLaunchKernelGGL(k_oresolve, dim3(blocks_for((long)L.edges, 256)), dim3(256), 0, c->stream, L.edges, L.etgt.p,
+                       L.expanded.p, L.rret.p);
+  grow_keep_to(L.grow, (size_t)(L.nodes + 1), c->stream);
+  if (L.nodes > 0) scan_launch(c, L.recnt.p, (long)L.nodes, L.grow.p, 0);
+  HIP_OK(hipMemcpyAsync(L.grow.p + L.nodes, &L.edges, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  {
+    HIP_OK(hipMemsetAsync(&L.lc.p->nkeys, 0, sizeof(int), c->stream));
+    Passed P = passed_view(c);
+    hipLaunchKernelGGL(k_okeys, dim3(blocks_for((long)c->H, 256)), dim3(256), 0, c->stream, P, (long)c->H, L.lc.p);
+    st->disc_keys = read_lc(c, L).nkeys;
+  }
+  st->outer_nodes = L.nodes;
+  st->fwd_edges = L.edges;
+  return HORA_HIP_OK;
+}
+
 int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_live_stats* st) {
   const auto t0 = std::chrono::steady_clock::now();
   std::memset(st, 0, sizeof(*st));
@@ -1469,6 +1840,8 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     const int alu = am ? 0 : 1;
     early = -1;
     rows_done = -1;
+    L.args.recnt = nullptr;
+    L.args.whit = nullptr;
     HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));   // a seed test may have counted hits
     {
       const int32_t seeds = st->seed_keys;
@@ -1573,7 +1946,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
                            (int*)nullptr, (int*)nullptr);
         HIP_OK(hipEventRecord(c->ev[1], c->stream));
         hipLaunchKernelGGL(k_gprobe, dim3(blocks_for(G, 4)), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p, c->s_dk.p,
-                           c->s_zone.p, G, c->zarena.p, c->covered.p, L.cov_old.p, c->ctr.p, alu);
+                           c->s_zone.p, G, c->zarena.p, c->covered.p, L.cov_old.p, c->ctr.p, alu, 0);
         hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p, G,
                            c->gticket.p, c->touched.p, c->ctr.p);
         int T = 0;
@@ -1717,11 +2090,23 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     }
     st->outer_nodes = L.nodes;
     st->fwd_edges = L.edges;
+    if (L.nodes > 0) {   // E[] psi: nodes outside psi were stored, not expanded
+      HIP_OK(hipMemsetAsync(&L.lc.p->nexp, 0, sizeof(int), c->stream));
+      hipLaunchKernelGGL(k_ocount, dim3(blocks_for((long)L.nodes, 256)), dim3(256), 0, c->stream, L.expanded.p,
+                         (long)L.nodes, L.lc.p);
+      st->expanded_nodes = read_lc(c, L).nexp;
+    }
     return HORA_HIP_OK;
   };
-  const bool auto_route = q->cover == HORA_COVER_AUTO;
+  const bool auto_route = q->cover != HORA_COVER_AM;
+  if (q->cover == HORA_COVER_AUTO && q->kind == HORA_LIVE_LEADSTO && c->base_route) {
+    c->err = c->base_route == 1
+                 ? "time-open network: the reference decides p --> q by its base construction, not on the device"
+                 : "one possible discrete state: the reference may decide p --> q by its base construction, not on the device";
+    return HORA_HIP_E_SCOPE;
+  }
   {
-    const int rc = construct(!auto_route);
+    const int rc = auto_route ? construct_online(c, L, q, st, &early) : construct(true);
     if (rc != HORA_HIP_OK) return rc;
   }
   if (auto_route && early < 0 && L.nodes > 0) {

@@ -59,6 +59,15 @@ class OutOfScope(HipError):
     """The network is outside the device path's scope; the reference's caller falls back to its CPU engine."""
 
 
+class Capped(HipError):
+    """A federation outgrew the device's capacity: no verdict (``GfpStats::capped``). ``stats`` holds
+    the query's counters up to that point (the whole construction when the fixpoint capped)."""
+
+    def __init__(self, code: int, msg: str, stats):
+        super().__init__(code, msg)
+        self.stats = stats
+
+
 _lib = None
 
 
@@ -137,6 +146,8 @@ class HipContext:
         r = C.c_int32(0)
         st = lv.hora_live_stats()
         rc = lib().hora_hip_live(self.h, C.byref(enc.struct), C.byref(r), C.byref(st))
+        if rc == E_CAPPED:
+            raise Capped(rc, lib().hora_hip_last_error(self.h).decode(), lv.LiveStats.from_c(st))
         if rc != E_OK:
             raise (OutOfScope if rc == E_SCOPE else HipError)(rc, lib().hora_hip_last_error(self.h).decode())
         return bool(r.value), lv.LiveStats.from_c(st)

@@ -17,7 +17,7 @@ from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from .network import Network
+from .network import INF, Network
 
 LEADSTO, EG, DEADLOCK = 0, 1, 2
 OP_EQ, OP_NE, OP_LT, OP_LE, OP_GT, OP_GE = range(6)
@@ -73,7 +73,7 @@ class hora_dpred(C.Structure):
     _fields_ = [("nterms", C.c_int32), ("term_off", C.c_void_p), ("atoms", C.c_void_p)]
 
 
-COVER_AUTO, COVER_AM = 0, 1
+COVER_AUTO, COVER_AM, COVER_ONLINE = 0, 1, 2
 
 
 class hora_live_query(C.Structure):
@@ -83,7 +83,8 @@ class hora_live_query(C.Structure):
 
 
 _STAT_I64 = ("outer_nodes", "fwd_edges", "disc_keys", "expanded_nodes", "generated", "covered", "levels",
-             "closure", "nonempty", "iters", "pre_calls", "fed_max", "products", "chain_reads")
+             "closure", "nonempty", "iters", "pre_calls", "fed_max", "products", "chain_reads",
+             "retired", "unions", "enlarged")
 
 
 class hora_live_stats(C.Structure):
@@ -117,6 +118,9 @@ class LiveStats:
     fed_max: int = 0
     products: int = 0
     chain_reads: int = 0
+    retired: int = 0
+    unions: int = 0
+    enlarged: int = 0
     early: int = 0
     capped: int = 0
     route: int = 0
@@ -176,6 +180,25 @@ class LiveQueryEnc:
             s.orig_inv_off = enc.arrays["inv_off"].ctypes.data
             s.orig_inv_data = enc.arrays["inv_data"].ctypes.data
         self.struct = s
+
+
+def base_route(net: Network) -> int:
+    """Where the reference decides ``p --> q`` by its base construction instead of the online one
+    (structurally_empty_d, src/liveness/internal.cpp:59-103), which the device does not run: 1 for a
+    time-open network (no urgency, no upper invariant), 2 for a network with one possible discrete
+    state. The device refuses ``p --> q`` with COVER_AUTO there; COVER_ONLINE runs the online
+    construction instead. Same test as the library's at open."""
+    time_open = not any(c.urgent for c in net.channels)
+    for p in net.procs:
+        for loc in p.locations:
+            if loc.urgent or loc.committed:
+                time_open = False
+            if any(i > 0 and j == 0 and b < INF for i, j, b in loc.invariant):
+                time_open = False
+    if time_open:
+        return 1
+    one_key = all(len(p.locations) == 1 for p in net.procs) and all(v.lo == v.hi for v in net.intvars)
+    return 2 if one_key else 0
 
 
 def leads_to(p: DPred, q: DPred, early: bool = True, cover: int = COVER_AUTO) -> LiveQuery:

@@ -96,7 +96,9 @@ def eg_query(net, clock, bound, part):
     return lv.exists_always(net, part(net), [c_le(net.clock_names.index(clock), bound)])
 
 
-# Rows whose fixpoint outgrows the device's federation lists at zone dimension 32 (hddi10: 31
-# clocks, lists of 8 zones in 160 KiB of LDS): the device reports HORA_HIP_E_CAPPED there, the
-# reference's `capped` (no verdict).
-DEVICE_CAPPED = {("hddi10", "ST1.station_z_sync --> ST1.station_z_idle")}
+# Rows whose fixpoint or stop sets outgrow the device's federation lists: at zone dimension 32
+# (hddi10: 31 clocks, lists of 8 to 10 zones in 160 KiB of LDS), and phil12's fixpoint, whose
+# candidate unions reach 120 zones of dimension 13 (54 fit). The device reports HORA_HIP_E_CAPPED
+# there, the reference's `capped` (no verdict).
+DEVICE_CAPPED = {("hddi10", "ST1.station_z_sync --> ST1.station_z_idle"), ("hddi10", "deadlock"),
+                 ("phil12", "P0.Eat --> P0.Dropped")}

@@ -14,7 +14,18 @@
   online_seed_key (src/liveness/refinement.cpp:176-282).
 * The early witnesses (leadsto_witness, online_eg_terminal_witness) settle the query the fixpoint
   would settle.
+* The construction's storing rule against the reference's recorded liveness rows
+  (tests/golden/live_recorded_counts.json, written from benchmarks/results/live/hora/
+  hora_live_t1.csv): the number of records expanded equals the recorded `states` on fischer,
+  arbiter, phil, twolock and hddi (the online construction, online_intern, src/liveness/
+  construction.cpp:480-683) and on single (the base construction, intern, :863-1003, which
+  structurally_empty_d selects for its one discrete state). A dropped retirement, a union not
+  taken, a cover scanned in the wrong order or a record expanded after its retirement changes
+  these counts.
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -254,3 +265,24 @@ def test_reference_models_recorded_eg(case):
     rnet, q = live_xml_cases.eg_query(live_xml_cases.load(name), clock, bound, part)
     r, st = binding.live(rnet, q)
     assert r == want and not st.capped
+
+
+RECORDED = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "live_recorded_counts.json")))
+
+
+@pytest.mark.parametrize("row", [r for r in RECORDED if r["states"] <= 50_000], ids=lambda r: r["model"])
+def test_recorded_construction_counts(row):
+    name = row["model"]
+    net = models.fischer(int(name[len("fischer"):])) if name.startswith("fischer") else live_xml_cases.load(name)
+    r, st = binding.live(net, live_xml_cases.leads_to(net, row["query"]))
+    assert r == row["holds"]
+    assert st.expanded_nodes == row["states"], (name, st.expanded_nodes, row["states"], row["source"])
+    assert st.route == 0 and st.early == 0
+
+
+def test_online_construction_retires_and_joins():
+    """The online rule on phil8: records retired by a newer covering record, convex unions taken,
+    and fewer records expanded than stored (the retired ones that never got their turn)."""
+    net = live_xml_cases.load("phil8")
+    r, st = binding.live(net, live_xml_cases.leads_to(net, "P0.Eat --> P0.Dropped"))
+    assert r and st.retired > 0 and st.unions > 0 and st.expanded_nodes < st.outer_nodes
