@@ -275,8 +275,9 @@ typedef struct hora_live_stats {
 int hora_hip_live(hora_hip_ctx* ctx, const hora_live_query* q, int32_t* holds, hora_live_stats* stats);
 
 /* Inspection of the last hora_hip_live's graph and fixpoint (no counterpart in the reference:
- * for parity tests and tools). Always writes nodes, edges and d; copies each array that is
- * non-NULL (sized from a first call with NULL arrays). Node ids are creation order. */
+ * for parity tests and tools). Writes nodes, edges and d; copies each array that is non-NULL (sized
+ * from a first call with NULL arrays). Node ids are creation order. Returns HORA_HIP_E_ARG, with
+ * zero counts, when the last hora_hip_live failed before its graph and fixpoint were complete. */
 typedef struct hora_live_graph {
   int64_t nodes, edges;
   int32_t d, pad;
@@ -291,6 +292,34 @@ typedef struct hora_live_graph {
   int8_t* w_nonempty;        /* [nodes] converged value non-empty                               */
 } hora_live_graph;
 int hora_hip_live_graph(const hora_hip_ctx* ctx, hora_live_graph* g);
+
+/* ---- backward greatest fixpoint over a caller-supplied zone graph ----
+ * Replaces LivenessEngine::solve_greatest_fixpoint (src/liveness/fixpoint.cpp:48-965) with the
+ * readouts of certificate_in / accepting_closure (fixpoint.cpp:979-1101) and eg_verdict
+ * (src/liveness/api.cpp:290-337), for a caller that built the zone graph itself (the reference's own
+ * construction, for instance): the call sites are final_verdict_violated (src/liveness/
+ * fixpoint.cpp:1110, called by decide_leads_to and run, src/liveness/api.cpp:459 and :346) and
+ * eg_verdict (api.cpp:296-304). Every array is a
+ * DEVICE pointer and stays the caller's. Keys use the packing of the context's network
+ * (hora_hip_open); node 0 is the initial node; an edge records its transition as hora_live_graph
+ * does. */
+typedef struct hora_gfp_graph {
+  int64_t nodes, edges;
+  const uint64_t* dk;        /* [nodes] packed discrete keys                                    */
+  const int32_t* zones;      /* [nodes*d*d] node zones                                           */
+  const int64_t* row_off;    /* [nodes+1] edges of node v are [row_off[v], row_off[v+1])        */
+  const int32_t* etgt;       /* [edges] target node                                             */
+  const int32_t* ese;        /* [edges] first participant edge (sender or local)                */
+  const int32_t* ere;        /* [edges] receiver edge, -1, or -2 - c for broadcast combination c */
+  const int8_t* expanded;    /* [nodes] 1 = the node's row holds all its successors              */
+} hora_gfp_graph;
+
+/* Decides q on the graph: W = W /\ (NoSucc \/ Pre(W)) over the closure the query reads (kind
+ * LEADSTO, EG or DEADLOCK; q->early and q->cover are not used). Writes the verdict to *holds, the
+ * fixpoint's counters to *stats (capped on HORA_HIP_E_CAPPED), and, when w_nonempty is not NULL, per
+ * node 1 iff its converged value is non-empty (DEVICE int8[nodes]). */
+int hora_hip_gfp_solve(hora_hip_ctx* ctx, const hora_gfp_graph* g, const hora_live_query* q, int32_t* holds,
+                       hora_live_stats* stats, int8_t* w_nonempty);
 
 /* ---- serving: verification jobs of different networks, continuous batching on one GPU ----
  * A server owns `slots` device slots (a HIP stream, device-resident passed list, frontier and

@@ -1901,42 +1901,95 @@ struct CapFail : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
+// The stream of the context the calling thread works for (one thread per context): a buffer
+// released while it works is freed after that stream's queued work, not after the whole device's,
+// so that other contexts' streams (the serving slots) keep running.
+thread_local hipStream_t tl_stream = nullptr;
+
+struct StreamScope {
+  hipStream_t prev;
+  explicit StreamScope(hipStream_t s) : prev(tl_stream) { tl_stream = s; }
+  ~StreamScope() { tl_stream = prev; }
+};
+
+// A bump allocator over caller-provided device memory. While one is installed on the calling
+// thread (ArenaScope), buffers take their memory from it instead of hipMalloc and give nothing
+// back; growing past its end throws ArenaFull.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+  void* take(size_t bytes) {
+    const size_t at = (used + 255) & ~(size_t)255;
+    if (at > cap || bytes > cap - at) return nullptr;
+    used = at + bytes;
+    return base + at;
+  }
+};
+thread_local Arena* tl_arena = nullptr;
+
+struct ArenaFull : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+struct ArenaScope {
+  Arena* prev;
+  explicit ArenaScope(Arena* a) : prev(tl_arena) { tl_arena = a; }
+  ~ArenaScope() { tl_arena = prev; }
+};
+
 template <class T>
 struct DBuf {
   T* p = nullptr;
   size_t cap = 0;
+  bool borrowed = false;   // memory from an Arena: never freed here
   DBuf() = default;
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
-    // work queued on any stream may still read the old buffer
-    if (p) { (void)hipDeviceSynchronize(); (void)hipFree(p); }
+    if (p && !borrowed) {
+      // work queued on the owner's stream may still read the old buffer
+      if (tl_stream) (void)hipStreamSynchronize(tl_stream);
+      else (void)hipDeviceSynchronize();
+      (void)hipFree(p);
+    }
     p = nullptr;
     cap = 0;
+    borrowed = false;
+  }
+  T* take(size_t c) {
+    if (tl_arena) {
+      void* q = tl_arena->take(c * sizeof(T));
+      if (!q) throw ArenaFull("the query needs more device scratch than the static scratch holds");
+      return (T*)q;
+    }
+    T* q = nullptr;
+    HIP_OK(hipMalloc(&q, c * sizeof(T)));
+    return q;
   }
   // scratch semantics: contents are not preserved on growth
   void ensure(size_t n) {
     if (n <= cap && p) return;
     release();
     size_t c = std::max<size_t>(n, 1);
-    c = c + c / 4 + 1024;
-    HIP_OK(hipMalloc(&p, c * sizeof(T)));
+    if (!tl_arena) c = c + c / 4 + 1024;   // headroom, except in a fixed scratch
+    p = take(c);
+    borrowed = tl_arena != nullptr;
     cap = c;
   }
   // persistent semantics: contents preserved on growth
   void grow_keep(size_t n, hipStream_t s) {
     if (n <= cap && p) return;
-    size_t c = std::max<size_t>(n, cap * 2);
-    T* q = nullptr;
-    HIP_OK(hipMalloc(&q, c * sizeof(T)));
+    size_t c = std::max<size_t>(n, tl_arena ? n : cap * 2);
+    T* q = take(c);
     if (p) {
       HIP_OK(hipMemcpyAsync(q, p, cap * sizeof(T), hipMemcpyDeviceToDevice, s));
       HIP_OK(hipStreamSynchronize(s));
-      (void)hipFree(p);
+      if (!borrowed) (void)hipFree(p);
     }
     p = q;
     cap = c;
+    borrowed = tl_arena != nullptr;
   }
   // synchronous copy on `s` (the owner's stream, so that other streams keep running)
   void upload(const std::vector<T>& v, hipStream_t s) {
@@ -2586,6 +2639,7 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
     HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& ev : c->ev) HIP_OK(hipEventCreate(&ev));
   }
+  if (!c->hpin) HIP_OK(hipHostMalloc(&c->hpin, kPinBytes, hipHostMallocDefault));
   const int ne = std::max(c->ne, 1);
   // all static tables except the per-location bounds go into one blob (u64 tables 8-aligned)
   std::vector<int> blob;
@@ -2720,6 +2774,7 @@ const char* hora_hip_last_error(const hora_hip_ctx* ctx) { return ctx ? ctx->err
 
 int hora_hip_reach(hora_hip_ctx* ctx, int32_t* reachable, hora_reach_stats* stats) {
   if (!ctx || !reachable || !stats || !ctx->stream) return HORA_HIP_E_ARG;
+  StreamScope scope(ctx->stream);
   try {
     return run_search(ctx, reachable, stats);
   } catch (const HipFail& ex) {
@@ -2778,8 +2833,25 @@ int hora_hip_zone_op(int32_t op, int32_t d, int64_t n, const int32_t* a, const i
 
 int hora_hip_live(hora_hip_ctx* ctx, const hora_live_query* q, int32_t* holds, hora_live_stats* stats) {
   if (!ctx || !q || !holds || !stats || !ctx->stream) return HORA_HIP_E_ARG;
+  StreamScope scope(ctx->stream);
   try {
     return run_live(ctx, q, holds, stats);
+  } catch (const HipFail& ex) {
+    ctx->err = ex.what();
+    return HORA_HIP_E_DEVICE;
+  } catch (const std::exception& ex) {
+    ctx->err = ex.what();
+    return HORA_HIP_E_INTERNAL;
+  }
+}
+
+int hora_hip_gfp_solve(hora_hip_ctx* ctx, const hora_gfp_graph* g, const hora_live_query* q, int32_t* holds,
+                       hora_live_stats* stats, int8_t* w_nonempty) {
+  if (!ctx || !q || !holds || !stats || !ctx->stream) return HORA_HIP_E_ARG;
+  StreamScope scope(ctx->stream);
+  try {
+    if (!ctx->live) ctx->live = new LiveState();
+    return gfp_solve(ctx, *ctx->live, g, q, holds, stats, w_nonempty);
   } catch (const HipFail& ex) {
     ctx->err = ex.what();
     return HORA_HIP_E_DEVICE;

@@ -1260,6 +1260,7 @@ struct LiveState {
   DBuf<u64> soff_save;
   DBuf<u32> slen_save;
   long long hzcap = 0;
+  bool graph_valid = false;   // the last query built its graph to the end (readable by live_graph)
   // backward pass
   DBuf<int> in_s, slist, sidx, scnt, soff, sz, wcnt0, wcnt1, woff0, woff1, sel0, sel1, chg0, chg1, heads, dlist, rlist,
       rlist2, wlist;
@@ -1268,8 +1269,9 @@ struct LiveState {
   int wcur = 0;
   long long scap = 0, wcap = 0;
   int ns = 0;
+  bool args_borrowed = false;   // argsp taken from an Arena
   ~LiveState() {
-    if (argsp) { (void)hipDeviceSynchronize(); (void)hipFree(argsp); }
+    if (argsp && !args_borrowed) { (void)hipDeviceSynchronize(); (void)hipFree(argsp); }
   }
 };
 
@@ -1779,24 +1781,13 @@ int construct_online(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, ho
   return HORA_HIP_OK;
 }
 
-int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_live_stats* st) {
-  const auto t0 = std::chrono::steady_clock::now();
-  std::memset(st, 0, sizeof(*st));
-  *holds = 0;
-  {
-    LiveArgs probe{};
-    if (c->d > kLiveMaxDim || !live_caps(c->d2, probe)) {
-      c->err = "liveness on the device covers zone dimension up to 32";
-      return HORA_HIP_E_SCOPE;
-    }
-  }
+// The query's tables and the kernels' arguments for one liveness query or fixpoint solve.
+int live_setup(hora_hip_ctx* c, LiveState& L, const hora_live_query* q) {
   // the full-capacity launches may ask for more than 64 KiB of LDS
   HIP_OK(hipFuncSetAttribute((const void*)k_stop, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsWorkgroupMax));
   HIP_OK(hipFuncSetAttribute((const void*)k_gfp, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsWorkgroupMax));
   if (c->dn.ncc > 0) { c->err = "open the context with a target without clock constraints for liveness"; return HORA_HIP_E_ARG; }
   if (q->kind < HORA_LIVE_LEADSTO || q->kind > HORA_LIVE_DEADLOCK) return HORA_HIP_E_ARG;
-  if (!c->live) c->live = new LiveState();
-  LiveState& L = *c->live;
   const int d2 = c->d2;
   // ---- query tables
   {
@@ -1828,9 +1819,228 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     L.args.oinv_data = L.oinv_data.p;
   }
   live_caps(d2, L.args);
-  if (!L.argsp) HIP_OK(hipMalloc(&L.argsp, sizeof(LiveArgs)));
+  if (!L.argsp) {
+    if (tl_arena) {
+      L.argsp = (LiveArgs*)tl_arena->take(sizeof(LiveArgs));
+      if (!L.argsp) throw ArenaFull("the query needs more device scratch than the static scratch holds");
+      L.args_borrowed = true;
+    } else {
+      HIP_OK(hipMalloc(&L.argsp, sizeof(LiveArgs)));
+    }
+  }
   L.lc.ensure(1);
   HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
+  return HORA_HIP_OK;
+}
+
+// The backward pass over the graph the kernels' arguments name (N nodes): closure of the query,
+// stop sets, Jacobi rounds of relax to the greatest fixpoint, readout (src/liveness/
+// fixpoint.cpp:48-965). `early` >= 0 is a verdict the construction already reached.
+int solve_graph(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, long long N, int early, int32_t* holds,
+                hora_live_stats* st) {
+  const int d2 = c->d2;
+  L.in_s.ensure(std::max<long long>(N, 1));
+  L.wne.ensure(std::max<long long>(N, 1));
+  L.sidx.ensure(std::max<long long>(N, 1));
+  HIP_OK(hipMemsetAsync(L.in_s.p, 0, sizeof(int) * std::max<long long>(N, 1), c->stream));
+  HIP_OK(hipMemsetAsync(L.wne.p, 0, std::max<long long>(N, 1), c->stream));
+  HIP_OK(hipMemsetAsync(L.sidx.p, 0xff, sizeof(int) * std::max<long long>(N, 1), c->stream));
+  L.args.in_s = L.in_s.p;
+  L.args.sidx = L.sidx.p;
+  upload_args(c, L);
+  if (early >= 0) {
+    *holds = early;
+  } else if (N == 0) {
+    *holds = q->kind == HORA_LIVE_EG ? 0 : 1;
+  } else if (q->kind == HORA_LIVE_DEADLOCK) {
+    HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
+    launch_stop(c, L, nullptr, (long)N, 0, 2);
+    const LiveCtr h = read_lc(c, L);
+    if (h.err) { c->err = "federation capacity exceeded in a deadlock test"; return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY; }
+    *holds = h.hit ? 0 : 1;
+  } else {
+    // ---- closure S
+    HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
+    L.slist.ensure(std::max<long long>(N, 1));
+    hipLaunchKernelGGL(k_sseed, dim3(blocks_for(N, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, (long)N, L.in_s.p,
+                       L.slist.p, L.lc.p);
+    int from = 0, to = read_lc(c, L).ns;
+    while (to > from) {
+      hipLaunchKernelGGL(k_sstep, dim3(blocks_for(to - from, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, L.slist.p,
+                         from, to, L.in_s.p, L.slist.p, L.lc.p);
+      from = to;
+      to = read_lc(c, L).ns;
+    }
+    const int ns = to;
+    L.ns = ns;
+    st->closure = ns;
+    if (ns > 0) {
+      hipLaunchKernelGGL(k_sindex, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.slist.p, ns, L.sidx.p);
+      // ---- stop sets (into a pool: most nodes have none; grown and run again when it overflows)
+      L.scnt.ensure(ns);
+      L.soff.ensure(ns);
+      const bool tight = pool_tight();
+      // the pool's room in zones of THIS query's dimension: a context reused for a network of
+      // another dimension keeps its buffer, whose capacity in zones then differs
+      L.scap = L.sz.p ? std::min<long long>((long long)(L.sz.cap / d2), INT32_MAX) : 0;
+      if (tight) {
+        L.sz.ensure(1);
+        L.scap = 1;
+      } else if (L.scap < std::max<long long>(ns / 4, 4096)) {
+        L.sz.ensure((size_t)std::max<long long>(ns / 4, 4096) * d2);
+        L.scap = std::min<long long>((long long)(L.sz.cap / d2), INT32_MAX);
+      }
+      const auto ts0 = std::chrono::steady_clock::now();
+      for (;;) {
+        HIP_OK(hipMemsetAsync(&L.lc.p->spool, 0, 2 * sizeof(int), c->stream));   // spool, sfull
+        launch_stop(c, L, L.slist.p, (long)ns, 0, 0);
+        const LiveCtr h = read_lc(c, L);
+        if (h.err) {
+          c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a stop set" : "invariant or transition table too large";
+          st->capped = h.err == ERR_CAPPED;
+          return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
+        }
+        if (c->trace)
+          std::fprintf(stderr, "stop sets: %d nodes, %d at full capacity, %d pool zones of %lld, %.3f ms\n", ns, h.rcnt,
+                       h.spool, L.scap,
+                       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts0).count());
+        if (!h.sfull) break;
+        L.sz.ensure((size_t)h.spool * d2);
+        L.scap = tight ? h.spool : std::min<long long>((long long)(L.sz.cap / d2), INT32_MAX);
+      }
+      // ---- fixpoint
+      L.wcnt0.ensure(ns); L.wcnt1.ensure(ns); L.woff0.ensure(ns); L.woff1.ensure(ns);
+      L.sel0.ensure(ns); L.sel1.ensure(ns); L.chg0.ensure(ns); L.chg1.ensure(ns);
+      L.wcur = 0;
+      // the seeds (one zone per node) and, mostly, room for two zones per node changed in a round
+      L.wpool[0].ensure((size_t)(3 * (size_t)ns + ns / 2) * d2);
+      L.wcap = tight ? ns : std::min<long long>((long long)(L.wpool[0].cap / d2), INT32_MAX);
+      HIP_OK(hipMemsetAsync(L.wcnt1.p, 0, sizeof(int) * ns, c->stream));
+      hipLaunchKernelGGL(k_gseed, dim3(ns), dim3(64), 0, c->stream, L.argsp, L.slist.p, ns, d2, L.wcnt0.p, L.woff0.p,
+                         L.wpool[0].p, L.sel0.p, L.chg0.p);
+      {
+        const int top = ns;
+        HIP_OK(hipMemcpyAsync(&L.lc.p->ptop, &top, sizeof(int), hipMemcpyHostToDevice, c->stream));
+      }
+      int* sel_in = L.sel0.p; int* sel_out = L.sel1.p;
+      int* chg_in = L.chg0.p; int* chg_out = L.chg1.p;
+      L.dlist.ensure(ns); L.rlist.ensure(ns);
+      const GfpCaps KA = gfp_caps_small(c->dn), KB = gfp_caps_full(L.args, c->dn);
+      // a middle tier between the small lists and the full ones (at most 32 KiB, so that several
+      // wavefronts share a CU); not under an override of the small lists, whose runs exercise the full ones
+      const GfpCaps KM{std::min(4, KB.kw), std::min(16, KB.kc), std::min(8, KB.kp), KB.ic, KB.np, KB.ni};
+      const bool gfp_mid = !std::getenv("HORA_HIP_GFP_CAPS") && KM.kw >= KA.kw && KM.kc >= KA.kc && KM.kp >= KA.kp &&
+                           (KM.kc < KB.kc || KM.kp < KB.kp || KM.kw < KB.kw) && KM.ints(d2) <= 64 * 1024 / 4 / 2;
+      int round = 0;
+      long long retried = 0;
+      int compactions = 0;
+      for (;;) {
+        const auto tr0 = std::chrono::steady_clock::now();
+        // changed (its stripes); dcnt and rcnt are adjacent in LiveCtr
+        HIP_OK(hipMemsetAsync(&L.lc.p->stripe[2][0][0], 0, sizeof(L.lc.p->stripe[2]), c->stream));
+        HIP_OK(hipMemsetAsync(&L.lc.p->dcnt, 0, 2 * sizeof(int), c->stream));
+        auto banks = [&]() {
+          return GfpBanks{L.wcnt0.p, L.wcnt1.p, L.woff0.p, L.woff1.p, L.wpool[L.wcur].p, (int)L.wcap,
+                          sel_in, sel_out, chg_in, chg_out, L.scnt.p, L.soff.p, L.sz.p};
+        };
+        hipLaunchKernelGGL(k_gdirty, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.argsp, L.slist.p, ns,
+                           round == 0 ? 1 : 0, banks(), L.dlist.p, L.lc.p);
+        const LiveCtr h0 = read_lc(c, L);
+        const int nd = h0.dcnt;
+        // most values hold one or two zones: room for two per visited node, else compact first
+        if (!tight && (long long)h0.ptop + 2LL * nd > L.wcap) {
+          compact_values(c, L, ns, sel_in, 2LL * nd + ns / 2);
+          ++compactions;
+        }
+        LiveCtr h{};
+        for (;;) {
+          if (nd > 0)
+            hipLaunchKernelGGL(k_gfp, dim3(nd), dim3(64), KA.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+                               L.slist.p, L.dlist.p, KA, banks(), L.rlist.p, L.lc.p);
+          h = read_lc(c, L);
+          const int* full_list = L.rlist.p;
+          if (h.rcnt > 0 && !h.err && !h.pfull && gfp_mid) {
+            const int nm = h.rcnt;
+            L.rlist2.ensure(nm);
+            HIP_OK(hipMemsetAsync(&L.lc.p->rcnt, 0, sizeof(int), c->stream));
+            hipLaunchKernelGGL(k_gfp, dim3(nm), dim3(64), KM.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+                               L.slist.p, L.rlist.p, KM, banks(), L.rlist2.p, L.lc.p);
+            h = read_lc(c, L);
+            full_list = L.rlist2.p;
+          }
+          if (h.rcnt > 0 && !h.err && !h.pfull) {
+            hipLaunchKernelGGL(k_gfp, dim3(h.rcnt), dim3(64), KB.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+                               L.slist.p, full_list, KB, banks(), (int*)nullptr, L.lc.p);
+            h = read_lc(c, L);
+          }
+          if (!h.pfull || h.err) break;
+          // the pool ran out: drop the round's writes, compact with room for every value at its
+          // largest, and run the round again from the same values
+          LiveCtr r = h;
+          std::memcpy(r.stripe[0], h0.stripe[0], sizeof(r.stripe[0]));
+          std::memcpy(r.stripe[1], h0.stripe[1], sizeof(r.stripe[1]));
+          std::memset(r.stripe[2], 0, sizeof(r.stripe[2]));
+          r.rcnt = 0;
+          r.pfull = 0;
+          HIP_OK(hipMemcpyAsync(L.lc.p, &r, sizeof(LiveCtr), hipMemcpyHostToDevice, c->stream));
+          compact_values(c, L, ns, sel_in, (long long)L.args.kw * nd + (tight ? 0 : ns / 2));
+          ++compactions;
+        }
+        retried += h.rcnt;
+        if (c->trace)
+          std::fprintf(stderr, "gfp round %d: %d visited, %d at full capacity, %d values changed, pre_calls so far %llu,"
+                       " pool %d of %lld zones, %.3f ms\n", round, h.dcnt, h.rcnt, h.changed,
+                       (unsigned long long)h.pre_calls, h.ptop, L.wcap,
+                       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tr0).count());
+        ++round;
+        if (h.err) {
+          st->capped = h.err == ERR_CAPPED;
+          c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in the fixpoint" : "invariant table too large";
+          return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
+        }
+        std::swap(sel_in, sel_out);
+        std::swap(chg_in, chg_out);
+        if (h.changed == 0) break;
+      }
+      st->iters = round;
+      if (c->trace)
+        std::fprintf(stderr, "gfp: %d rounds, %lld node visits at full capacity (small lists %d,%d,%d,%d), %d pool"
+                     " compactions\n", round, retried, KA.kw, KA.kc, KA.kp, KA.ic, compactions);
+      hipLaunchKernelGGL(k_gverdict, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, L.slist.p, ns,
+                         sel_in, L.wcnt0.p, L.wcnt1.p, L.wne.p, L.lc.p);
+      const LiveCtr h = read_lc(c, L);
+      st->nonempty = h.nonempty;
+      st->fed_max = h.fed_max;
+      st->pre_calls = (long long)h.pre_calls;
+      st->products = (long long)h.products;
+      *holds = h.violated ? 0 : 1;
+      if (q->kind == HORA_LIVE_EG) *holds = h.violated ? 1 : 0;
+    } else {
+      *holds = q->kind == HORA_LIVE_EG ? 0 : 1;
+    }
+  }
+  return HORA_HIP_OK;
+}
+
+int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_live_stats* st) {
+  const auto t0 = std::chrono::steady_clock::now();
+  std::memset(st, 0, sizeof(*st));
+  *holds = 0;
+  {
+    LiveArgs probe{};
+    if (c->d > kLiveMaxDim || !live_caps(c->d2, probe)) {
+      c->err = "liveness on the device covers zone dimension up to 32";
+      return HORA_HIP_E_SCOPE;
+    }
+  }
+  if (!c->live) c->live = new LiveState();
+  LiveState& L = *c->live;
+  L.graph_valid = false;
+  const int d2 = c->d2;
+  {
+    const int rc = live_setup(c, L, q);
+    if (rc != HORA_HIP_OK) return rc;
+  }
 
   // One construction of the graph under a_M covers (am) or a_LU covers. Returns HORA_HIP_OK or an
   // error code; `early` receives the verdict of an early witness, or -1.
@@ -2143,184 +2353,11 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
   const auto t1 = std::chrono::steady_clock::now();
   sync_graph_args(L);
   L.args.gzone = c->zarena.p;
-  const long long N = L.nodes;
-  L.in_s.ensure(std::max<long long>(N, 1));
-  L.wne.ensure(std::max<long long>(N, 1));
-  L.sidx.ensure(std::max<long long>(N, 1));
-  HIP_OK(hipMemsetAsync(L.in_s.p, 0, sizeof(int) * std::max<long long>(N, 1), c->stream));
-  HIP_OK(hipMemsetAsync(L.wne.p, 0, std::max<long long>(N, 1), c->stream));
-  HIP_OK(hipMemsetAsync(L.sidx.p, 0xff, sizeof(int) * std::max<long long>(N, 1), c->stream));
-  L.args.in_s = L.in_s.p;
-  L.args.sidx = L.sidx.p;
-  upload_args(c, L);
-  if (early >= 0) {
-    *holds = early;
-  } else if (N == 0) {
-    *holds = q->kind == HORA_LIVE_EG ? 0 : 1;
-  } else if (q->kind == HORA_LIVE_DEADLOCK) {
-    HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
-    launch_stop(c, L, nullptr, (long)N, 0, 2);
-    const LiveCtr h = read_lc(c, L);
-    if (h.err) { c->err = "federation capacity exceeded in a deadlock test"; return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY; }
-    *holds = h.hit ? 0 : 1;
-  } else {
-    // ---- closure S
-    HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
-    L.slist.ensure(std::max<long long>(N, 1));
-    hipLaunchKernelGGL(k_sseed, dim3(blocks_for(N, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, (long)N, L.in_s.p,
-                       L.slist.p, L.lc.p);
-    int from = 0, to = read_lc(c, L).ns;
-    while (to > from) {
-      hipLaunchKernelGGL(k_sstep, dim3(blocks_for(to - from, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, L.slist.p,
-                         from, to, L.in_s.p, L.slist.p, L.lc.p);
-      from = to;
-      to = read_lc(c, L).ns;
-    }
-    const int ns = to;
-    L.ns = ns;
-    st->closure = ns;
-    if (ns > 0) {
-      hipLaunchKernelGGL(k_sindex, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.slist.p, ns, L.sidx.p);
-      // ---- stop sets (into a pool: most nodes have none; grown and run again when it overflows)
-      L.scnt.ensure(ns);
-      L.soff.ensure(ns);
-      const bool tight = pool_tight();
-      if (tight) {
-        L.sz.ensure(1);
-        L.scap = 1;
-      } else if (L.scap < std::max<long long>(ns / 4, 4096)) {
-        L.sz.ensure((size_t)std::max<long long>(ns / 4, 4096) * d2);
-        L.scap = std::min<long long>((long long)(L.sz.cap / d2), INT32_MAX);
-      }
-      const auto ts0 = std::chrono::steady_clock::now();
-      for (;;) {
-        HIP_OK(hipMemsetAsync(&L.lc.p->spool, 0, 2 * sizeof(int), c->stream));   // spool, sfull
-        launch_stop(c, L, L.slist.p, (long)ns, 0, 0);
-        const LiveCtr h = read_lc(c, L);
-        if (h.err) {
-          c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a stop set" : "invariant or transition table too large";
-          st->capped = h.err == ERR_CAPPED;
-          return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
-        }
-        if (c->trace)
-          std::fprintf(stderr, "stop sets: %d nodes, %d at full capacity, %d pool zones of %lld, %.3f ms\n", ns, h.rcnt,
-                       h.spool, L.scap,
-                       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts0).count());
-        if (!h.sfull) break;
-        L.sz.ensure((size_t)h.spool * d2);
-        L.scap = tight ? h.spool : std::min<long long>((long long)(L.sz.cap / d2), INT32_MAX);
-      }
-      // ---- fixpoint
-      L.wcnt0.ensure(ns); L.wcnt1.ensure(ns); L.woff0.ensure(ns); L.woff1.ensure(ns);
-      L.sel0.ensure(ns); L.sel1.ensure(ns); L.chg0.ensure(ns); L.chg1.ensure(ns);
-      L.wcur = 0;
-      // the seeds (one zone per node) and, mostly, room for two zones per node changed in a round
-      L.wpool[0].ensure((size_t)(3 * (size_t)ns + ns / 2) * d2);
-      L.wcap = tight ? ns : std::min<long long>((long long)(L.wpool[0].cap / d2), INT32_MAX);
-      HIP_OK(hipMemsetAsync(L.wcnt1.p, 0, sizeof(int) * ns, c->stream));
-      hipLaunchKernelGGL(k_gseed, dim3(ns), dim3(64), 0, c->stream, L.argsp, L.slist.p, ns, d2, L.wcnt0.p, L.woff0.p,
-                         L.wpool[0].p, L.sel0.p, L.chg0.p);
-      {
-        const int top = ns;
-        HIP_OK(hipMemcpyAsync(&L.lc.p->ptop, &top, sizeof(int), hipMemcpyHostToDevice, c->stream));
-      }
-      int* sel_in = L.sel0.p; int* sel_out = L.sel1.p;
-      int* chg_in = L.chg0.p; int* chg_out = L.chg1.p;
-      L.dlist.ensure(ns); L.rlist.ensure(ns);
-      const GfpCaps KA = gfp_caps_small(c->dn), KB = gfp_caps_full(L.args, c->dn);
-      // a middle tier between the small lists and the full ones (at most 32 KiB, so that several
-      // wavefronts share a CU); not under an override of the small lists, whose runs exercise the full ones
-      const GfpCaps KM{std::min(4, KB.kw), std::min(16, KB.kc), std::min(8, KB.kp), KB.ic, KB.np, KB.ni};
-      const bool gfp_mid = !std::getenv("HORA_HIP_GFP_CAPS") && KM.kw >= KA.kw && KM.kc >= KA.kc && KM.kp >= KA.kp &&
-                           (KM.kc < KB.kc || KM.kp < KB.kp || KM.kw < KB.kw) && KM.ints(d2) <= 64 * 1024 / 4 / 2;
-      int round = 0;
-      long long retried = 0;
-      int compactions = 0;
-      for (;;) {
-        const auto tr0 = std::chrono::steady_clock::now();
-        // changed (its stripes); dcnt and rcnt are adjacent in LiveCtr
-        HIP_OK(hipMemsetAsync(&L.lc.p->stripe[2][0][0], 0, sizeof(L.lc.p->stripe[2]), c->stream));
-        HIP_OK(hipMemsetAsync(&L.lc.p->dcnt, 0, 2 * sizeof(int), c->stream));
-        auto banks = [&]() {
-          return GfpBanks{L.wcnt0.p, L.wcnt1.p, L.woff0.p, L.woff1.p, L.wpool[L.wcur].p, (int)L.wcap,
-                          sel_in, sel_out, chg_in, chg_out, L.scnt.p, L.soff.p, L.sz.p};
-        };
-        hipLaunchKernelGGL(k_gdirty, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, L.argsp, L.slist.p, ns,
-                           round == 0 ? 1 : 0, banks(), L.dlist.p, L.lc.p);
-        const LiveCtr h0 = read_lc(c, L);
-        const int nd = h0.dcnt;
-        // most values hold one or two zones: room for two per visited node, else compact first
-        if (!tight && (long long)h0.ptop + 2LL * nd > L.wcap) {
-          compact_values(c, L, ns, sel_in, 2LL * nd + ns / 2);
-          ++compactions;
-        }
-        LiveCtr h{};
-        for (;;) {
-          if (nd > 0)
-            hipLaunchKernelGGL(k_gfp, dim3(nd), dim3(64), KA.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
-                               L.slist.p, L.dlist.p, KA, banks(), L.rlist.p, L.lc.p);
-          h = read_lc(c, L);
-          const int* full_list = L.rlist.p;
-          if (h.rcnt > 0 && !h.err && !h.pfull && gfp_mid) {
-            const int nm = h.rcnt;
-            L.rlist2.ensure(nm);
-            HIP_OK(hipMemsetAsync(&L.lc.p->rcnt, 0, sizeof(int), c->stream));
-            hipLaunchKernelGGL(k_gfp, dim3(nm), dim3(64), KM.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
-                               L.slist.p, L.rlist.p, KM, banks(), L.rlist2.p, L.lc.p);
-            h = read_lc(c, L);
-            full_list = L.rlist2.p;
-          }
-          if (h.rcnt > 0 && !h.err && !h.pfull) {
-            hipLaunchKernelGGL(k_gfp, dim3(h.rcnt), dim3(64), KB.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
-                               L.slist.p, full_list, KB, banks(), (int*)nullptr, L.lc.p);
-            h = read_lc(c, L);
-          }
-          if (!h.pfull || h.err) break;
-          // the pool ran out: drop the round's writes, compact with room for every value at its
-          // largest, and run the round again from the same values
-          LiveCtr r = h;
-          std::memcpy(r.stripe[0], h0.stripe[0], sizeof(r.stripe[0]));
-          std::memcpy(r.stripe[1], h0.stripe[1], sizeof(r.stripe[1]));
-          std::memset(r.stripe[2], 0, sizeof(r.stripe[2]));
-          r.rcnt = 0;
-          r.pfull = 0;
-          HIP_OK(hipMemcpyAsync(L.lc.p, &r, sizeof(LiveCtr), hipMemcpyHostToDevice, c->stream));
-          compact_values(c, L, ns, sel_in, (long long)L.args.kw * nd + (tight ? 0 : ns / 2));
-          ++compactions;
-        }
-        retried += h.rcnt;
-        if (c->trace)
-          std::fprintf(stderr, "gfp round %d: %d visited, %d at full capacity, %d values changed, pre_calls so far %llu,"
-                       " pool %d of %lld zones, %.3f ms\n", round, h.dcnt, h.rcnt, h.changed,
-                       (unsigned long long)h.pre_calls, h.ptop, L.wcap,
-                       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tr0).count());
-        ++round;
-        if (h.err) {
-          st->capped = h.err == ERR_CAPPED;
-          c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in the fixpoint" : "invariant table too large";
-          return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
-        }
-        std::swap(sel_in, sel_out);
-        std::swap(chg_in, chg_out);
-        if (h.changed == 0) break;
-      }
-      st->iters = round;
-      if (c->trace)
-        std::fprintf(stderr, "gfp: %d rounds, %lld node visits at full capacity (small lists %d,%d,%d,%d), %d pool"
-                     " compactions\n", round, retried, KA.kw, KA.kc, KA.kp, KA.ic, compactions);
-      hipLaunchKernelGGL(k_gverdict, dim3(blocks_for(ns, 256)), dim3(256), 0, c->stream, c->dnp, L.argsp, L.slist.p, ns,
-                         sel_in, L.wcnt0.p, L.wcnt1.p, L.wne.p, L.lc.p);
-      const LiveCtr h = read_lc(c, L);
-      st->nonempty = h.nonempty;
-      st->fed_max = h.fed_max;
-      st->pre_calls = (long long)h.pre_calls;
-      st->products = (long long)h.products;
-      *holds = h.violated ? 0 : 1;
-      if (q->kind == HORA_LIVE_EG) *holds = h.violated ? 1 : 0;
-    } else {
-      *holds = q->kind == HORA_LIVE_EG ? 0 : 1;
-    }
+  {
+    const int rc = solve_graph(c, L, q, L.nodes, early, holds, st);
+    if (rc != HORA_HIP_OK) return rc;
   }
+  L.graph_valid = true;
   const auto t2 = std::chrono::steady_clock::now();
   st->build_seconds = std::chrono::duration<double>(t1 - t0).count();
   st->gfp_seconds = std::chrono::duration<double>(t2 - t1).count();
@@ -2328,11 +2365,63 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
   return HORA_HIP_OK;
 }
 
+// hora_hip_gfp_solve: the backward pass over a graph the caller built, in device memory.
+int gfp_solve(hora_hip_ctx* c, LiveState& L, const hora_gfp_graph* g, const hora_live_query* q, int32_t* holds,
+              hora_live_stats* st, int8_t* w_nonempty) {
+  const auto t0 = std::chrono::steady_clock::now();
+  std::memset(st, 0, sizeof(*st));
+  *holds = 0;
+  if (!g || g->nodes < 0 || g->edges < 0 || g->nodes > INT32_MAX ||
+      (g->nodes > 0 && (!g->dk || !g->zones || !g->row_off || !g->expanded)) ||
+      (g->edges > 0 && (!g->etgt || !g->ese || !g->ere))) {
+    c->err = "graph arrays missing or sizes out of range";
+    return HORA_HIP_E_ARG;
+  }
+  {
+    LiveArgs probe{};
+    if (c->d > kLiveMaxDim || !live_caps(c->d2, probe)) {
+      c->err = "liveness on the device covers zone dimension up to 32";
+      return HORA_HIP_E_SCOPE;
+    }
+  }
+  {
+    const int rc = live_setup(c, L, q);
+    if (rc != HORA_HIP_OK) return rc;
+  }
+  L.graph_valid = false;   // live_graph describes the last hora_hip_live's own graph
+  L.args.gdk = g->dk;
+  L.args.gzone = g->zones;
+  L.args.grow = (const long long*)g->row_off;
+  L.args.etgt = g->etgt;
+  L.args.ese = g->ese;
+  L.args.ere = g->ere;
+  L.args.expanded = (const signed char*)g->expanded;
+  L.args.recnt = nullptr;
+  L.args.whit = nullptr;
+  {
+    const int rc = solve_graph(c, L, q, g->nodes, -1, holds, st);
+    if (rc != HORA_HIP_OK) return rc;
+  }
+  if (w_nonempty && g->nodes > 0)
+    HIP_OK(hipMemcpyAsync(w_nonempty, L.wne.p, (size_t)g->nodes, hipMemcpyDeviceToDevice, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  st->outer_nodes = g->nodes;
+  st->fwd_edges = g->edges;
+  st->gfp_seconds = st->seconds =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return HORA_HIP_OK;
+}
+
 int live_graph(const hora_hip_ctx* c, hora_live_graph* g) {
   const LiveState* L = c->live;
+  g->d = c->d;
+  if (L && !L->graph_valid) {   // the last query stopped before its graph was complete
+    g->nodes = 0;
+    g->edges = 0;
+    return HORA_HIP_E_ARG;
+  }
   g->nodes = L ? L->nodes : 0;
   g->edges = L ? L->edges : 0;
-  g->d = c->d;
   if (!L) return HORA_HIP_OK;
   auto cp = [](void* dst, const void* src, size_t bytes) {
     if (dst && bytes) HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));

@@ -130,7 +130,9 @@ namespace {
 void run_job(hora_hip_ctx* c, Job* j) {
   hora_serve_job_result& r = j->res;
   try {
+    StreamScope scope(c->stream);   // null before the slot's first job: then a device-wide wait
     open_ctx(&j->data.net, j->kind == 0 ? &j->data.target : nullptr, c);
+    tl_stream = c->stream;
     int32_t v = 0;
     if (j->kind == 0) r.status = run_search(c, &v, &r.reach);
     else r.status = run_live(c, &j->data.query, &v, &r.live);

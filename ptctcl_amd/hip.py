@@ -18,6 +18,8 @@ import os
 from dataclasses import dataclass
 from typing import Optional
 
+import numpy as np
+
 from . import liveness as lv
 from .network import NetEnc, Network, Target, TargetEnc, hora_reach_stats
 
@@ -31,7 +33,7 @@ E_OK, E_ARG, E_SCOPE, E_DEVICE, E_MEMORY, E_INTERNAL, E_CAPPED = 0, -1, -2, -3, 
 
 EXPORTS = ("hora_hip_abi_version", "hora_hip_build_info", "hora_hip_open", "hora_hip_close",
            "hora_hip_last_error", "hora_hip_reach", "hora_hip_reach_profile", "hora_hip_zone_op",
-           "hora_hip_live", "hora_hip_live_graph", "hora_hip_server_open", "hora_hip_server_submit_reach",
+           "hora_hip_live", "hora_hip_live_graph", "hora_hip_gfp_solve", "hora_hip_server_open", "hora_hip_server_submit_reach",
            "hora_hip_server_submit_live", "hora_hip_server_wait", "hora_hip_server_now", "hora_hip_server_close")
 
 
@@ -90,6 +92,8 @@ def lib():
         L.hora_hip_live.argtypes = [C.c_void_p, C.POINTER(lv.hora_live_query), C.POINTER(C.c_int32),
                                     C.POINTER(lv.hora_live_stats)]
         L.hora_hip_live_graph.argtypes = [C.c_void_p, C.POINTER(lv.hora_live_graph)]
+        L.hora_hip_gfp_solve.argtypes = [C.c_void_p, C.POINTER(lv.hora_gfp_graph), C.POINTER(lv.hora_live_query),
+                                         C.POINTER(C.c_int32), C.POINTER(lv.hora_live_stats), C.c_void_p]
         _lib = L
     return _lib
 
@@ -151,6 +155,30 @@ class HipContext:
         if rc != E_OK:
             raise (OutOfScope if rc == E_SCOPE else HipError)(rc, lib().hora_hip_last_error(self.h).decode())
         return bool(r.value), lv.LiveStats.from_c(st)
+
+    def gfp_solve(self, graph: dict, query: "lv.LiveQuery", entry: str = "hora_hip_gfp_solve"):
+        """The backward pass over a zone graph (``hora_hip_gfp_solve``): `graph` holds the arrays of
+        :func:`liveness.graph_arrays` (dk, zones, row_off, etgt, ese, ere, expanded) on the host; they
+        are copied to device memory, and the result is (holds, LiveStats, per-node non-empty flags)."""
+        import torch
+        dev = torch.device("cuda")
+        host = {k: np.ascontiguousarray(graph[k]) for k in ("dk", "zones", "row_off", "etgt", "ese", "ere", "expanded")}
+        host["dk"] = host["dk"].view(np.int64)   # the same 64-bit keys, in a type torch moves
+        t = {k: torch.from_numpy(v).to(dev) for k, v in host.items()}
+        n, e = int(graph["row_off"].shape[0]) - 1, int(graph["etgt"].shape[0])
+        wne = torch.zeros(max(n, 1), dtype=torch.int8, device=dev)
+        g = lv.hora_gfp_graph(n, e, *(t[k].data_ptr() for k in ("dk", "zones", "row_off", "etgt", "ese", "ere",
+                                                              "expanded")))
+        enc = query.encode()
+        r = C.c_int32(0)
+        st = lv.hora_live_stats()
+        torch.cuda.synchronize()
+        rc = getattr(lib(), entry)(self.h, C.byref(g), C.byref(enc.struct), C.byref(r), C.byref(st), wne.data_ptr())
+        if rc == E_CAPPED:
+            raise Capped(rc, lib().hora_hip_last_error(self.h).decode(), lv.LiveStats.from_c(st))
+        if rc != E_OK:
+            raise (OutOfScope if rc == E_SCOPE else HipError)(rc, lib().hora_hip_last_error(self.h).decode())
+        return bool(r.value), lv.LiveStats.from_c(st), wne[:n].cpu().numpy()
 
     def live_graph(self) -> dict:
         """The last liveness query's zone graph and fixpoint emptiness, copied to host arrays."""

@@ -93,6 +93,12 @@ class hora_live_stats(C.Structure):
                [(k, C.c_double) for k in ("seconds", "build_seconds", "gfp_seconds")]
 
 
+class hora_gfp_graph(C.Structure):
+    _fields_ = [("nodes", C.c_int64), ("edges", C.c_int64), ("dk", C.c_void_p), ("zones", C.c_void_p),
+                ("row_off", C.c_void_p), ("etgt", C.c_void_p), ("ese", C.c_void_p), ("ere", C.c_void_p),
+                ("expanded", C.c_void_p)]
+
+
 class hora_live_graph(C.Structure):
     _fields_ = [("nodes", C.c_int64), ("edges", C.c_int64), ("d", C.c_int32), ("pad", C.c_int32),
                 ("dk", C.c_void_p), ("zones", C.c_void_p), ("row_off", C.c_void_p), ("etgt", C.c_void_p),

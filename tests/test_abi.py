@@ -34,10 +34,10 @@ def test_struct_layout_matches_c(tmp_path):
     import subprocess
     from ptctcl_amd.network import hora_net_enc, hora_reach_stats, hora_target
     from ptctcl_amd.hip import hora_reach_profile
-    from ptctcl_amd.liveness import hora_dpred, hora_live_graph, hora_live_query, hora_live_stats
+    from ptctcl_amd.liveness import hora_dpred, hora_gfp_graph, hora_live_graph, hora_live_query, hora_live_stats
     from ptctcl_amd.serve import hora_serve_job_result
     structs = (hora_net_enc, hora_target, hora_reach_stats, hora_reach_profile, hora_dpred, hora_live_query,
-               hora_live_stats, hora_live_graph, hora_serve_job_result)
+               hora_live_stats, hora_live_graph, hora_gfp_graph, hora_serve_job_result)
     cc = shutil.which("gcc") or shutil.which("cc")
     if cc is None:
         import pytest

@@ -242,6 +242,9 @@ def test_reference_models_recorded_verdicts(case):
         with hip.HipContext(net, None) as ctx:
             with pytest.raises(hip.HipError, match="capacity"):
                 ctx.live(live_xml_cases.leads_to(net, text, False))
+            # a query that stopped before its fixpoint finished leaves no graph to read
+            with pytest.raises(hip.HipError, match="hora_hip_live_graph"):
+                ctx.live_graph()
         return
     for early in (True, False):
         r, st = check(net, live_xml_cases.leads_to(net, text, early), f"{name} {text}")
