@@ -321,6 +321,18 @@ typedef struct hora_gfp_graph {
 int hora_hip_gfp_solve(hora_hip_ctx* ctx, const hora_gfp_graph* g, const hora_live_query* q, int32_t* holds,
                        hora_live_stats* stats, int8_t* w_nonempty);
 
+/* hora_hip_gfp_solve without any device allocation. No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The same computation and results as hora_hip_gfp_solve, with every working
+ * buffer carved from one file-scope device array of 256 MiB per GPU (held until the process ends).
+ * A graph of more nodes than fit (256 MiB / (80 + 16 d^2) bytes: 96,421 nodes at zone
+ * dimension 13), or a run whose pools outgrow the array, returns HORA_HIP_E_MEMORY with the reason in
+ * hora_hip_last_error. Every call on a GPU shares that GPU's array: calls that overlap on one GPU,
+ * from any contexts, race and must be serialised by the caller. The context must have been opened
+ * (hora_hip_open allocates, once). */
+int hora_hip_gfp_solve_noalloc(hora_hip_ctx* ctx, const hora_gfp_graph* g, const hora_live_query* q,
+                               int32_t* holds, hora_live_stats* stats, int8_t* w_nonempty);
+
 /* ---- serving: verification jobs of different networks, continuous batching on one GPU ----
  * A server owns `slots` device slots (a HIP stream, device-resident passed list, frontier and
  * successor buffers each, kept across jobs). Submitted jobs wait in arrival order; a job is

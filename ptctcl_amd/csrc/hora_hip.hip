@@ -33,6 +33,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -2726,6 +2727,49 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
 #include "liveness.hpp"
 #include "serve.hpp"
 
+// ---------------------------------------------------------------- the allocation-free fixpoint
+
+// Device scratch of hora_hip_gfp_solve_noalloc: every buffer that call needs is carved from it
+// (Arena), so it calls no hipMalloc. One array per GPU (each device that loads the library holds
+// its own copy until the process ends); the address of this device's copy comes from
+// hipGetSymbolAddress, once per device, and is kept in noalloc_base[].
+constexpr size_t kNoAllocScratchBytes = 256ull << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocDevices = 64;
+std::mutex noalloc_mu;
+void* noalloc_base[kNoAllocDevices] = {};
+
+// The device address of this device's scratch, or null.
+void* noalloc_scratch(std::string* err) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocDevices) {
+    *err = "no current HIP device for the static scratch";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  if (!noalloc_base[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    noalloc_base[dev] = p;
+  }
+  return noalloc_base[dev];
+}
+
+// Largest graph the scratch takes: its per-node arrays and a value pool of a few zones per node.
+long long noalloc_max_nodes(int d2) { return (long long)(kNoAllocScratchBytes / (size_t)(80 + 16 * d2)); }
+
+template <class T>
+void swap_buf(DBuf<T>& a, DBuf<T>& b) {
+  std::swap(a.p, b.p);
+  std::swap(a.cap, b.cap);
+  std::swap(a.borrowed, b.borrowed);
+}
+
 extern "C" {
 
 int hora_hip_abi_version(void) { return HORA_HIP_ABI_VERSION; }
@@ -2859,6 +2903,47 @@ int hora_hip_gfp_solve(hora_hip_ctx* ctx, const hora_gfp_graph* g, const hora_li
     ctx->err = ex.what();
     return HORA_HIP_E_INTERNAL;
   }
+}
+
+int hora_hip_gfp_solve_noalloc(hora_hip_ctx* ctx, const hora_gfp_graph* g, const hora_live_query* q,
+                               int32_t* holds, hora_live_stats* stats, int8_t* w_nonempty) {
+  if (!ctx || !q || !holds || !stats || !ctx->stream || !g) return HORA_HIP_E_ARG;
+  std::memset(stats, 0, sizeof(*stats));
+  *holds = 0;
+  if (g->nodes > noalloc_max_nodes(ctx->d2)) {
+    ctx->err = "graph of " + std::to_string(g->nodes) + " nodes beyond the static scratch's maximum of " +
+               std::to_string(noalloc_max_nodes(ctx->d2)) + " at this zone dimension";
+    return HORA_HIP_E_MEMORY;
+  }
+  void* base = noalloc_scratch(&ctx->err);
+  if (!base) return HORA_HIP_E_DEVICE;
+  StreamScope scope(ctx->stream);
+  Arena arena;
+  arena.base = (char*)base;
+  arena.cap = kNoAllocScratchBytes;
+  LiveState L;
+  // the context's scan buffers give way to ones in the scratch for the call
+  DBuf<long long> scan_total, bsum;
+  swap_buf(ctx->scan_total, scan_total);
+  swap_buf(ctx->bsum, bsum);
+  int rc;
+  try {
+    ArenaScope in_scratch(&arena);
+    rc = gfp_solve(ctx, L, g, q, holds, stats, w_nonempty);
+  } catch (const ArenaFull& ex) {
+    ctx->err = ex.what();
+    rc = HORA_HIP_E_MEMORY;
+  } catch (const HipFail& ex) {
+    ctx->err = ex.what();
+    rc = HORA_HIP_E_DEVICE;
+  } catch (const std::exception& ex) {
+    ctx->err = ex.what();
+    rc = HORA_HIP_E_INTERNAL;
+  }
+  (void)hipStreamSynchronize(ctx->stream);   // nothing of this call still reads the scratch
+  swap_buf(ctx->scan_total, scan_total);
+  swap_buf(ctx->bsum, bsum);
+  return rc;
 }
 
 int hora_hip_live_graph(const hora_hip_ctx* ctx, hora_live_graph* g) {

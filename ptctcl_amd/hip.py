@@ -33,7 +33,8 @@ E_OK, E_ARG, E_SCOPE, E_DEVICE, E_MEMORY, E_INTERNAL, E_CAPPED = 0, -1, -2, -3, 
 
 EXPORTS = ("hora_hip_abi_version", "hora_hip_build_info", "hora_hip_open", "hora_hip_close",
            "hora_hip_last_error", "hora_hip_reach", "hora_hip_reach_profile", "hora_hip_zone_op",
-           "hora_hip_live", "hora_hip_live_graph", "hora_hip_gfp_solve", "hora_hip_server_open", "hora_hip_server_submit_reach",
+           "hora_hip_live", "hora_hip_live_graph", "hora_hip_gfp_solve", "hora_hip_gfp_solve_noalloc",
+           "hora_hip_server_open", "hora_hip_server_submit_reach",
            "hora_hip_server_submit_live", "hora_hip_server_wait", "hora_hip_server_now", "hora_hip_server_close")
 
 
@@ -94,6 +95,7 @@ def lib():
         L.hora_hip_live_graph.argtypes = [C.c_void_p, C.POINTER(lv.hora_live_graph)]
         L.hora_hip_gfp_solve.argtypes = [C.c_void_p, C.POINTER(lv.hora_gfp_graph), C.POINTER(lv.hora_live_query),
                                          C.POINTER(C.c_int32), C.POINTER(lv.hora_live_stats), C.c_void_p]
+        L.hora_hip_gfp_solve_noalloc.argtypes = L.hora_hip_gfp_solve.argtypes
         _lib = L
     return _lib
 
