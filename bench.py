@@ -21,6 +21,12 @@ import os
 import sys
 import time
 
+# The CPU baseline's OpenMP threads are pinned one per physical core, as the reference's own OMP
+# runs are described (OMP_PROC_BIND=close, OMP_PLACES=cores); the OpenMP runtime reads these when it
+# starts, so they are set before anything that loads it. A caller's own settings win.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -87,18 +93,30 @@ def physical_cores() -> int:
 
 
 def cpu_baseline(name: str, seconds: float) -> dict:
+    """The oracle's OpenMP engine on this host: one warm-up run, then the best of three, each bounded
+    to a quarter of `seconds` (a full search of the default workload takes about 5 s)."""
     from oracle import binding
     net, target, _ = workload(name)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or physical_cores()
-    done, st = binding.reach_bounded(net, target, threads, seconds)
+    per_run = seconds / 4.0
+    binding.reach_bounded(net, target, threads, per_run)
+    best = None
+    for _ in range(3):
+        done, st = binding.reach_bounded(net, target, threads, per_run)
+        rate = st["explored"] / st["seconds"] if st["seconds"] > 0 else 0.0
+        if best is None or rate > best[0]:
+            best = (rate, done, st)
+    rate, done, st = best
     return {
-        "value": st["explored"] / st["seconds"] if st["seconds"] > 0 else 0.0,
+        "value": rate,
         "unit": "states/s",
         "cores": threads,
         "kind": "port",
         "sample": (f"{'full search' if done else 'first %d levels (time-bounded)' % st['levels']} of the same "
-                   f"workload: {st['explored']} states in {st['seconds']:.2f} s, OpenMP engine "
-                   f"(level-synchronous, stripe-locked passed set, dynamic chunk 8)"),
+                   f"workload, best of 3 after one warm-up: {st['explored']} states in {st['seconds']:.2f} s, "
+                   f"OpenMP engine (level-synchronous, stripe-locked passed set, dynamic chunk 8), "
+                   f"{threads} threads, OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}, "
+                   f"OMP_PLACES={os.environ.get('OMP_PLACES')}"),
     }
 
 
@@ -295,6 +313,22 @@ def zoneops_main(a) -> int:
     return 0
 
 
+def calibrate_main(a) -> int:
+    """Reference copies for the HBM traffic counters: `--steps` device-to-device copies of 1 GiB.
+    Run under `rocprofv3 --pmc FETCH_SIZE` (then `WRITE_SIZE`); the counter per copy dispatch over
+    the bytes copied is the factor that turns the counter into bytes for kernels measured the same
+    way. Prints no benchmark line."""
+    import torch
+    n = (1 << 30) // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(a.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"copied {a.steps} x 1024 MiB ({n * 4} bytes each)", flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -316,6 +350,8 @@ def main() -> int:
         return serve_main(a)
     if a.workload == "zoneops":
         return zoneops_main(a)
+    if a.workload == "calibrate":
+        return calibrate_main(a)
 
     import torch
     from ptctcl_amd import hip
