@@ -273,3 +273,13 @@ def test_reference_models_recorded_eg(case):
     rnet, q = live_xml_cases.eg_query(live_xml_cases.load(name), clock, bound, part)
     r, _ = check(rnet, q, f"{name} EG")
     assert r == want
+
+
+def test_urgent_location_deadlock():
+    """(B, x = 0) in tests/urgent_models.py's urgent-location network can neither delay nor leave:
+    `A[] not deadlock` fails, found at its turn or by the scan, on the device as in the oracle."""
+    import urgent_models
+    net, _, _, _ = urgent_models.urgent_location()
+    for early in (True, False):
+        r, _ = check(net, lv.not_deadlock(early))
+        assert not r

@@ -79,3 +79,22 @@ def test_clock_target():
     # P0 in wait with x0 > 3 (x0 is clock 1)
     t = Target(loc_req=[net.resolve("P0", "wait")], clock_cond=[(0, 1, -3 * 2)])
     check(net, t)
+
+
+def test_adaptive_switch_level():
+    """The model whose switch level follows from the 4x rule by hand (tests/extra_cases.py): level 3,
+    on the device as in the oracle, with every counter equal."""
+    import extra_cases
+    net, target, level = extra_cases.switch_model()
+    r, st = check(net, target)
+    assert not r and st.emode_switch_level == level
+
+
+@pytest.mark.parametrize("k", range(4))
+def test_urgent_semantics(k):
+    """Urgent locations and urgent channels (tests/urgent_models.py): the derived answer and
+    explored count, on the device as in the oracle, every counter equal."""
+    import urgent_models
+    net, target, reachable, explored = urgent_models.ALL[k]()
+    r, st = check(net, target)
+    assert r == reachable and st.explored == explored

@@ -129,3 +129,12 @@ def test_full_size_properties(d):
     assert gf.min() == 1
     go, gf = run(hip.ZOP_UNION, d, z, z)
     assert gf.min() == 1 and np.array_equal(go, z)
+
+
+@pytest.mark.parametrize("case", __import__("extra_cases").EXTRA_CASES, ids=lambda c: c[0])
+def test_extra_lu_plus_known_answers(case):
+    """Hand-derived Extra_LU+ answers (tests/extra_cases.py), one per clause, on the device."""
+    _, d, z, L, U, want = case
+    go, _ = run(hip.ZOP_EXTRA_LU, d, np.asarray([z], np.int32), L=np.asarray([L], np.int32),
+                U=np.asarray([U], np.int32))
+    assert go[0].tolist() == want

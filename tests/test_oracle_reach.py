@@ -59,3 +59,31 @@ def test_token_ring_runs():
     net = models.token_ring(6, 1)
     r, st = binding.reach(net, None, 0)
     assert not r and st["explored"] > 0
+
+
+@pytest.mark.parametrize("k", range(4))
+def test_urgent_semantics_known_answers(k):
+    """Urgent locations and urgent (binary, broadcast) channels: the answers and explored counts
+    derived in tests/urgent_models.py (include/hora/detail/semantics.hpp:118-169)."""
+    import urgent_models
+    net, target, reachable, explored = urgent_models.ALL[k]()
+    r, st = binding.reach(net, target, 0)
+    assert r == reachable and st["explored"] == explored
+
+
+def test_urgency_is_what_blocks_the_target():
+    """The same networks with the urgency taken away: time passes and every target is reached, so the
+    known answers above hold only under the urgent rule."""
+    import urgent_models
+    for k in (0, 1, 3):
+        net, target, reachable, _ = urgent_models.ALL[k]()
+        assert not reachable
+        for p in net.procs:
+            for loc in p.locations:
+                loc.urgent = False
+        for ch in net.channels:
+            ch.urgent = False
+        net._analysed = False
+        net.finalize()
+        r, _ = binding.reach(net, target, 0)
+        assert r, net.name
