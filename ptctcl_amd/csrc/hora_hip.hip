@@ -131,14 +131,14 @@ __device__ __forceinline__ DevNet stage_tables(const DevNet& N, int* lds) {
 // Statistics are striped over cache lines by workgroup so that hundreds of thousands of
 // workgroups per launch do not serialise on one address; the host sums the stripes.
 constexpr int kStripes = 64;
-enum { C_SUB, C_KILL, C_UNI, C_READS, C_INS, C_KEYS, C_LIVE, C_PREADS, C_PSUB, C_NSTAT };
+enum { C_SUB, C_KILL, C_UNI, C_READS, C_INS, C_KEYS, C_LIVE, C_PREADS, C_PSUB, C_FULL, C_WOVF, C_NSTAT };
 struct Counters {
   unsigned long long st[kStripes][16];
   int err;
   int first_target;
   int ntouched;
-  int nlight, nheavy;
-  int pad[3];
+  int nlight, nheavy, npair;
+  int pad[2];
 };
 
 __device__ __forceinline__ void stat_add(Counters* c, int k, unsigned long long v) {
@@ -1223,7 +1223,7 @@ __host__ __device__ inline u64 list_cap(u64 n) {
 __global__ void k_touched_caps(Passed P, const u32* __restrict__ touched, int T, int* __restrict__ caps,
                                int* __restrict__ segs, Counters* ctr) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0 && ctr) { ctr->nlight = 0; ctr->nheavy = 0; }
+  if (t == 0 && ctr) { ctr->nlight = 0; ctr->nheavy = 0; ctr->npair = 0; }
   if (t >= T) return;
   const u32 h = touched[t];
   caps[t] = (int)list_cap(P.ncnt[h]);
@@ -1269,6 +1269,8 @@ __device__ __forceinline__ void bitonic_sort(int* a, int n) {   // n a power of 
 }
 
 constexpr u32 kNewTag = 0x40000000u;     // chain entry refers to a successor stored this level
+constexpr u32 kWideTag = 0x20000000u;    // ... whose zone a union widened (set under the pair pre-pass)
+constexpr u32 kPosMask = kWideTag - 1;   // work-list position of a new member
 
 // Lane-serial a_LU test (same predicate as z_alu_le, one lane per pair of zones): Z <=_aLU Zp.
 __device__ __forceinline__ bool alu_le_lane(const int* Z, const int* Zp, int d, const int* L, const int* U) {
@@ -1283,6 +1285,35 @@ __device__ __forceinline__ bool alu_le_lane(const int* Z, const int* Zp, int d, 
       if (ly == kAluNegInf) continue;
       const int zp = Zp[y * d + x];
       if (zp < Z[y * d + x] && radd(zp, r_lt(-ly)) < z0x) return false;
+    }
+  }
+  return true;
+}
+
+// The same predicate for d <= 16 where one zone differs per lane and sits in global memory: a
+// column of that zone (16 entries) is loaded before any entry of it is compared, so the lane waits
+// on memory once per column instead of once per entry. kLaneZ: the per-lane zone is Z (a kill
+// test), else Zp (a cover test).
+template <bool kLaneZ>
+__device__ __forceinline__ bool alu_le_lane16(const int* Z, const int* Zp, int d, const int* L, const int* U) {
+  const int* V = kLaneZ ? Z : Zp;
+  for (int x = 0; x < d; ++x) {
+    const int ux = U[x];
+    if (ux == kAluNegInf) continue;
+    if (!kLaneZ && Z[x] < r_le(-ux)) continue;
+    int col[16];
+#pragma unroll
+    for (int y = 0; y < 16; ++y) col[y] = y < d ? V[y * d + x] : 0;
+    const int z0x = kLaneZ ? col[0] : Z[x];
+    if (kLaneZ && z0x < r_le(-ux)) continue;
+#pragma unroll
+    for (int y = 0; y < 16; ++y) {
+      if (y >= d || y == x) continue;
+      const int ly = L[y];
+      if (ly == kAluNegInf) continue;
+      const int zp = kLaneZ ? Zp[y * d + x] : col[y];
+      const int zv = kLaneZ ? col[y] : Z[y * d + x];
+      if (zp < zv && radd(zp, r_lt(-ly)) < z0x) return false;
     }
   }
   return true;
@@ -1340,12 +1371,26 @@ __device__ __forceinline__ bool union_convex_lane(const int* a, const int* b, in
   return true;
 }
 
+// Heavy keys: the pair pre-pass (k_pairs) runs at d <= 16 for keys with at most kPairMaxN kept
+// successors whose sequential fold would test at least kPairMinWork pairs (below that the fold is
+// short and the pre-pass's launches cost more than they save).
+constexpr int kPairTile = 32;
+constexpr int kPairMaxN = 1 << 18;
+constexpr int kKillSlots = 4;
+constexpr int kWideCap = 1024;  // widened members tracked per key (more: full scans from then on)
+constexpr u64 kPairMinWork = 8192;
+__host__ __device__ inline bool pair_eligible(u32 n, u32 old, int d) {
+  return d <= 16 && n <= (u32)kPairMaxN && (u64)n * (u64)(old + n) >= kPairMinWork;
+}
+
 // LDS layout of k_merge, in ints: S, M, fa, pa (d2 each); L/U; decoded key; scratch; the sorted
-// work list; chain ids and cache slots; the chain-zone cache (cz zones at an odd stride).
+// work list; chain ids and cache slots; widened members (heavy keys); the chain-zone cache (cz
+// zones at an odd stride).
 struct MergeLayout {
-  int d2, zs, cz, sort_cap, chain_cap;
+  int d2, zs, cz, sort_cap, chain_cap, wide_cap;
   __host__ __device__ int total() const {
-    return 4 * d2 + 2 * HORA_MAX_DIM + HORA_MAX_PROCS + HORA_MAX_INTS + 16 + 32 + sort_cap + 2 * chain_cap + cz * zs;
+    return 4 * d2 + 2 * HORA_MAX_DIM + HORA_MAX_PROCS + HORA_MAX_INTS + 16 + 32 + sort_cap + 2 * chain_cap + wide_cap +
+           cz * zs;
   }
 };
 
@@ -1355,10 +1400,12 @@ __host__ __device__ inline MergeLayout merge_layout(int d2, bool heavy) {
   m.zs = d2 | 1;   // odd stride: lanes reading different cached zones hit different banks
   m.sort_cap = heavy ? kSortLds : 64;
   m.chain_cap = heavy ? 2048 : 64;
+  m.wide_cap = (heavy && d2 <= 256) ? kWideCap : 0;   // the pair pre-pass runs at d <= 16
   // LDS per workgroup (ints): the light budget leaves the chain cache at its two-zone minimum and
   // lets registers, not LDS, bound the resident wavefronts; both picked by timing fischer 12
-  const int budget = heavy ? 30 * 1024 / 4 : 6 * 1024 / 4;
-  const int fixed = 4 * d2 + 2 * HORA_MAX_DIM + HORA_MAX_PROCS + HORA_MAX_INTS + 16 + 32 + m.sort_cap + 2 * m.chain_cap;
+  const int budget = heavy ? 38 * 1024 / 4 : 6 * 1024 / 4;
+  const int fixed =
+      4 * d2 + 2 * HORA_MAX_DIM + HORA_MAX_PROCS + HORA_MAX_INTS + 16 + 32 + m.sort_cap + 2 * m.chain_cap + m.wide_cap;
   m.cz = max(2, min(heavy ? 1024 : 16, (budget - fixed) / m.zs));
   return m;
 }
@@ -1371,12 +1418,22 @@ __host__ __device__ inline MergeLayout merge_layout(int d2, bool heavy) {
 // lane, and evaluate the 32-zone union window in one round, taking the first success in
 // newest-first order as the sequential fold does. At the end the key's live new zones get zone
 // ids in one allocation, move to the arena, and are reported as this level's survivors.
-// Six wavefronts per SIMD (77 VGPRs, no spills) measured faster than the compiler's five.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_merge(const DevNet* __restrict__ Np, Passed P, const u32* __restrict__ list, int nlist,
+//
+// With `pre` (heavy keys), the work list arrives sorted, and for keys the pair pre-pass covered,
+// its answers replace the two long scans: a successor covered by an earlier successor of the key
+// is dropped outright, and a kept successor tests coverage only against members that a union
+// widened (the others are previous-level zones, which k_probe cleared, or earlier successors as
+// they came out of the expansion, which k_pairs cleared); its kills are the recorded ones among
+// unwidened live members, plus the widened members that it covers now.
+// Five wavefronts per SIMD: the batched loads of the lane tests need the registers (no spills).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_merge(const DevNet* __restrict__ Np, Passed P, const u32* __restrict__ list, int nlist,
                                               int* __restrict__ wlist, const int* __restrict__ sj,
                                               int* __restrict__ szone, int* __restrict__ survived,
                                               Counters* ctr, int heavy, int* __restrict__ zscr,
-                                              int* __restrict__ used, int* __restrict__ flag) {
+                                              int* __restrict__ used, int* __restrict__ flag, int pre,
+                                              const int* __restrict__ pdrop, const int* __restrict__ kcnt,
+                                              const u32* __restrict__ kslot, const long long* __restrict__ koff,
+                                              const u32* __restrict__ klist, int* __restrict__ mpos) {
   extern __shared__ int sm[];
   const DevNet& N = *Np;
   const int d = N.d, d2 = N.d2, l = threadIdx.x;
@@ -1394,7 +1451,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
   int* sbuf = cand + 32;
   u32* chl = (u32*)(sbuf + ml.sort_cap);
   int* cslot = (int*)(chl + ml.chain_cap);
-  int* cache = cslot + ml.chain_cap;
+  int* wl = cslot + ml.chain_cap;             // widened members' chain positions
+  int* cache = wl + ml.wide_cap;
   const int zs = ml.zs, cz = ml.cz;
   if ((int)blockIdx.x >= nlist) return;
 
@@ -1402,7 +1460,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
   const u32 n = P.ncnt[h];
   int* lst = wlist + P.noff[h];
   // order the kept successors by compacted index = (state, recipe) order
-  if (n <= (u32)kWave) {
+  if (pre) {
+    // sorted by k_presort
+    if (n <= (u32)ml.sort_cap) {
+      for (int k = l; k < (int)n; k += kWave) sbuf[k] = lst[k];
+      lst = sbuf;
+    }
+  } else if (n <= (u32)kWave) {
     const int v = (int)l < (int)n ? lst[l] : 0x7fffffff;
     int rank = 0;
     for (int j = 0; j < (int)n; ++j) rank += (__shfl(v, j) < v) ? 1 : 0;   // lanes from n on hold the maximum
@@ -1467,18 +1531,34 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
   }
   // cache slots are handed out upwards to new members until the cache is full
   int next_slot = ncached;
-  // new members are tagged with their position k in the sorted work list
+  // new members are tagged with their position k in the sorted work list; their zones are written
+  // to the level's scratch slot of that position (noff + k) when stored
+  const u64 zbase_k = P.noff[h];
   auto zp = [&](int pos) __attribute__((always_inline)) -> const int* {
     const int s = cs[pos];
     if (s >= 0) return cache + s * zs;
     const u32 id = ch[pos] & ~kDead;
-    return (id & kNewTag) ? szone + (size_t)sj[lst[id & (kNewTag - 1)]] * d2 : P.zarena + (size_t)id * d2;
+    return (id & kNewTag) ? zscr + (size_t)(zbase_k + (id & kPosMask)) * d2 : P.zarena + (size_t)id * d2;
   };
   int len = (int)old_len, live = (int)old_len;
   unsigned long long n_sub = 0, n_kill = 0, n_uni = 0, n_reads = 0, n_ins = 0;
-  wsync();
+  // pair pre-pass answers: `use_pairs` for the drops (exact on their own), `pairs_full` while the
+  // widened-member list is complete
+  const bool use_pairs = pre && pair_eligible(n, old_len, d);
+  bool pairs_full = use_pairs;
+  unsigned long long n_full = 0, n_wovf = 0;
+  int nw = 0;
+  const u64 pbase = P.noff[h];
+  if (use_pairs)
+    for (u32 k = l; k < n; k += kWave) mpos[pbase + k] = -1;
+  bsync();
 
   for (u32 k = 0; k < n; ++k) {
+    if (use_pairs && pdrop[pbase + k]) {
+      ++n_sub;
+      continue;
+    }
+    if (pre && !pairs_full) ++n_full;
     const int g = lst[k];
     int* sz = szone + (size_t)sj[g] * d2;
     wsync();
@@ -1489,7 +1569,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
       const bool wide = live > 8;
       // (1)+(2) covered by a live stored zone: drop
       bool covered = false;
-      if (!wide) {
+      if (pairs_full) {
+        for (int b0 = 0; b0 < nw && !covered; b0 += kWave) {
+          bool c = false;
+          if (b0 + l < nw) {
+            const int m = wl[b0 + l];
+            if (!(ch[m] & kDead)) c = d <= 16 ? alu_le_lane16<false>(S, zp(m), d, Lb, Ub) : alu_le_lane(S, zp(m), d, Lb, Ub);
+          }
+          covered = __any(c);
+        }
+        n_reads += (unsigned long long)nw;
+      } else if (!wide) {
         for (int m = len - 1; m >= 0 && !covered; --m) {
           if (ch[m] & kDead) continue;
           ++n_reads;
@@ -1499,14 +1589,54 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
         for (int base = len - 1; base >= 0 && !covered; base -= kWave) {
           const int m = base - l;
           bool c = false;
-          if (m >= 0 && !(ch[m] & kDead)) c = alu_le_lane(S, zp(m), d, Lb, Ub);
+          if (m >= 0 && !(ch[m] & kDead))
+            c = d <= 16 ? alu_le_lane16<false>(S, zp(m), d, Lb, Ub) : alu_le_lane(S, zp(m), d, Lb, Ub);
           covered = __any(c);
         }
         n_reads += (unsigned long long)live;
       }
       if (covered) { ++n_sub; continue; }
       // (3) retire stored zones covered by S
-      if (!wide) {
+      if (pairs_full) {
+        // recorded kills among live unwidened members, then the widened members as they are now
+        const int kc = kcnt[pbase + k];
+        if (kc > 0) bsync();   // member positions that lane 0 wrote to global memory
+        const u32* kl = kc <= kKillSlots ? kslot + (pbase + k) * kKillSlots : klist + koff[pbase + k];
+        for (int b0 = 0; b0 < kc; b0 += kWave) {
+          bool kill = false;
+          int m = -1;
+          if (b0 + l < kc) {
+            const u32 tag = kl[b0 + l];
+            m = (tag & kNewTag) ? mpos[pbase + (tag & kPosMask)] : (int)tag;
+            if (m >= 0) {
+              const u32 id = ch[m];
+              kill = !(id & kDead) && !(id & kWideTag);
+            }
+          }
+          const u64 b = __ballot(kill);
+          wsync();
+          if (kill) ch[m] |= kDead;
+          csync();
+          live -= __popcll(b);
+          n_kill += __popcll(b);
+        }
+        n_reads += (unsigned long long)kc;
+        for (int b0 = 0; b0 < nw; b0 += kWave) {
+          bool kw = false;
+          int mw = -1;
+          if (b0 + l < nw) {
+            mw = wl[b0 + l];
+            if (!(ch[mw] & kDead)) kw = d <= 16 ? alu_le_lane16<true>(zp(mw), S, d, Lb, Ub) : alu_le_lane(zp(mw), S, d, Lb, Ub);
+          }
+          const u64 bw = __ballot(kw);
+          wsync();
+          if (kw) ch[mw] |= kDead;
+          csync();
+          live -= __popcll(bw);
+          n_kill += __popcll(bw);
+        }
+        n_reads += (unsigned long long)nw;
+      } else if (!wide) {
         for (int m = len - 1; m >= 0; --m) {
           const u32 id = ch[m];
           if (id & kDead) continue;
@@ -1523,7 +1653,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
         for (int base = len - 1; base >= 0; base -= kWave) {
           const int m = base - l;
           bool kill = false;
-          if (m >= 0 && !(ch[m] & kDead)) kill = alu_le_lane(zp(m), S, d, Lb, Ub);
+          if (m >= 0 && !(ch[m] & kDead))
+            kill = d <= 16 ? alu_le_lane16<true>(zp(m), S, d, Lb, Ub) : alu_le_lane(zp(m), S, d, Lb, Ub);
           const u64 b = __ballot(kill);
           wsync();
           if (kill) ch[m] |= kDead;
@@ -1579,28 +1710,53 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
         }
       }
     }
-    // (5) store S as the newest member: in a free cache slot when one is left, and back in its
-    // successor slot when a union widened it
+    // (5) store S as the newest member: in its scratch slot, and in a free cache slot when one is left
     wsync();
     int slot = -1;
     if (next_slot < cz) slot = next_slot++;
     if (slot >= 0)
       for (int e = l; e < d2; e += kWave) cache[slot * zs + e] = S[e];
-    if (widened) {
-      for (int e = l; e < d2; e += kWave) sz[e] = S[e];
-      bsync();   // the slot is read back by other lanes when the zone is not cached
+    {
+      int* zdst = zscr + (size_t)(zbase_k + k) * d2;
+      for (int e = l; e < d2; e += kWave) zdst[e] = S[e];
+      if (slot < 0) bsync();   // the slot is read back by other lanes when the zone is not cached
     }
     if (l == 0) {
-      ch[len] = kNewTag | k;
+      ch[len] = kNewTag | k | ((use_pairs && widened) ? kWideTag : 0u);
       cs[len] = slot;
+      if (use_pairs) mpos[pbase + k] = len;
+    }
+    if (pairs_full && widened) {
+      if (nw == ml.wide_cap) {
+        // full: keep the members still live
+        int w2 = 0;
+        for (int b0 = 0; b0 < nw; b0 += kWave) {
+          const int m = b0 + l < nw ? wl[b0 + l] : -1;
+          const bool keep = m >= 0 && !(ch[m] & kDead);
+          const u64 b = __ballot(keep);
+          const int r = __popcll(b & ((1ull << l) - 1ull));
+          wsync();
+          if (keep) wl[w2 + r] = m;
+          wsync();
+          w2 += __popcll(b);
+        }
+        nw = w2;
+      }
+      if (nw < ml.wide_cap) {
+        if (l == 0) wl[nw] = len;
+        ++nw;
+      } else {
+        pairs_full = false;
+        n_wovf = 1;
+      }
     }
     ++len;
     ++live;
     ++n_ins;
     csync();
   }
-  // Live new zones go to the level's scratch slot of their work-list position (noff + k), named
-  // kNewTag | slot in the chain and in `survived` until k_zmove gives them compact zone ids; the
+  // Live new zones are in the level's scratch slot of their work-list position (noff + k), named
+  // kNewTag | slot in the chain and in `survived` until k_zassign gives them compact zone ids; the
   // chain is compacted oldest first into this key's reserved segment.
   wsync();
   u32* out = P.nodes + seg;
@@ -1610,11 +1766,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
     const u32 id = ch[m];
     if (id & kDead) continue;
     if (id & kNewTag) {
-      const u32 kpos = id & (kNewTag - 1);
+      const u32 kpos = id & kPosMask;
       const u32 slot = sk + kpos;
-      const int* src = zp(m);
-      int* dst = zscr + (size_t)slot * d2;
-      for (int e = l; e < d2; e += kWave) dst[e] = src[e];
       wsync();
       if (l == 0) {
         out[w] = kNewTag | slot;
@@ -1638,12 +1791,220 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
     stat_add(ctr, C_READS, n_reads);
     stat_add(ctr, C_INS, n_ins);
     stat_add(ctr, C_LIVE, (unsigned long long)((long long)w - (long long)old_len));
+    if (n_full) stat_add(ctr, C_FULL, n_full);
+    if (n_wovf) stat_add(ctr, C_WOVF, n_wovf);
   }
+}
+
+// ---------------------------------------------------------------- heavy keys: pair pre-pass
+
+// Workgroup-wide bitonic sort, ascending, n a power of two (LDS or this workgroup's global segment).
+__device__ void block_bitonic(int* a, int n) {
+  for (int k = 2; k <= n; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const int x = a[i], y = a[ixj];
+          const bool up = (i & k) == 0;
+          if ((x > y) == up) { a[i] = y; a[ixj] = x; }
+        }
+      }
+    }
+  __syncthreads();
+}
+
+constexpr int kPresortLds = 4096;
+
+// Each heavy key's work list sorted in place into compacted-index = (state, recipe) order, one
+// workgroup per key; k_pairs and k_merge then read it as it is.
+__global__ __launch_bounds__(1024) void k_presort(Passed P, const u32* __restrict__ list, int nlist,
+                                                  int* __restrict__ wlist) {
+  __shared__ int s[kPresortLds];
+  if ((int)blockIdx.x >= nlist) return;
+  const u32 h = list[blockIdx.x];
+  const int n = (int)P.ncnt[h];
+  int* lst = wlist + P.noff[h];
+  const int t = threadIdx.x, nt = blockDim.x;
+  if (n <= kPresortLds) {
+    int p2 = 1;
+    while (p2 < n) p2 <<= 1;
+    for (int k = t; k < p2; k += nt) s[k] = k < n ? lst[k] : 0x7fffffff;
+    block_bitonic(s, p2);
+    for (int k = t; k < n; k += nt) lst[k] = s[k];
+  } else {
+    // n > kSortLds: the list's capacity is a power of two (list_cap)
+    const int cap = (int)list_cap((u64)n);
+    for (int k = n + t; k < cap; k += nt) lst[k] = 0x7fffffff;
+    block_bitonic(lst, cap);
+  }
+}
+
+// Pair tiles per heavy key: kPairTile successors (rows) against kPairTile candidates (columns),
+// the columns being the key's previous-level zones and then its earlier successors (a triangle).
+__global__ void k_pair_tiles(Passed P, const u32* __restrict__ list, int nlist, int d, int* __restrict__ tcount) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlist) return;
+  const u32 h = list[t];
+  const u32 n = P.ncnt[h];
+  if (!pair_eligible(n, P.len[h], d)) {
+    tcount[t] = 0;
+    return;
+  }
+  const long long rt = (n + kPairTile - 1) / kPairTile, ot = (P.len[h] + kPairTile - 1) / kPairTile;
+  tcount[t] = (int)(rt * ot + rt * (rt + 1) / 2);
+}
+
+// The pair pre-pass of the ordered merge, all pairs of one heavy key at once. For successor j (in
+// work-list order) and
+//   a previous-level zone Z of the key: Z <=_aLU S_j records a kill of Z by S_j;
+//   an earlier successor S_i (i < j): S_j <=_aLU S_i drops S_j, else S_i <=_aLU S_j records a kill.
+// The drop is what the sequential fold (passed_set.hpp:327-415) does to S_j: S_i, stored or
+// dropped, is covered by a live zone of the key from its turn on (zones are retired only by zones
+// that cover or contain them), and simulation is transitive. The kills are used for members still
+// as stored; a member that a union widened is tested again in k_merge.
+// Workgroups walk the tiles of all heavy keys (offsets `toff`, total `*ttotal`). Mode 0 decides the
+// drops, counts each successor's kills and keeps the first kKillSlots of them; mode 1 runs again
+// for the kept successors with more, writing all of their kills to `klist` at `koff`.
+__global__ __launch_bounds__(256) void k_pairs(const DevNet* __restrict__ Np, Passed P, const u32* __restrict__ list,
+                                               int nlist, const long long* __restrict__ toff,
+                                               const long long* __restrict__ ttotal, const int* __restrict__ wlist,
+                                               const int* __restrict__ sj, const int* __restrict__ szone,
+                                               int* __restrict__ pdrop, int* __restrict__ kcnt,
+                                               u32* __restrict__ kslot, int mode, const long long* __restrict__ koff,
+                                               int* __restrict__ kpos, u32* __restrict__ klist) {
+  extern __shared__ int sm[];
+  const DevNet& N = *Np;
+  const int d = N.d, d2 = N.d2, zs = d2 | 1, t = threadIdx.x;
+  int* Rz = sm;
+  int* Cz = Rz + kPairTile * zs;
+  int* Lb = Cz + kPairTile * zs;
+  int* Ub = Lb + HORA_MAX_DIM;
+  int* locs = Ub + HORA_MAX_DIM;
+  int* ints = locs + HORA_MAX_PROCS;
+  const long long total = *ttotal;
+  for (long long tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    // owning key: the last one whose tiles start at or before this tile
+    int lo = 0, hi = nlist - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (toff[mid] <= tile) lo = mid; else hi = mid - 1;
+    }
+    const u32 h = list[lo];
+    const int n = (int)P.ncnt[h], old = (int)P.len[h];
+    const int rtn = (n + kPairTile - 1) / kPairTile, otn = (old + kPairTile - 1) / kPairTile;
+    // sibling tiles first, each row's diagonal tile first (a successor is most often covered by
+    // one just before it), then the previous-level tiles: rows dropped by then skip them
+    const long long u = tile - toff[lo];
+    const long long ntri = (long long)rtn * (rtn + 1) / 2;
+    int rt, ct;
+    bool sib;
+    if (u < ntri) {
+      const long long s2 = u;
+      rt = (int)((sqrt(8.0 * (double)s2 + 1.0) - 1.0) * 0.5);
+      while (rt > 0 && (long long)rt * (rt + 1) / 2 > s2) --rt;
+      while ((long long)(rt + 1) * (rt + 2) / 2 <= s2) ++rt;
+      ct = rt - (int)(s2 - (long long)rt * (rt + 1) / 2);
+      sib = true;
+    } else {
+      const long long u2 = u - ntri;
+      rt = (int)(u2 / otn);
+      ct = (int)(u2 - (long long)rt * otn);
+      sib = false;
+    }
+    __syncthreads();   // the previous tile's reads of LDS are done
+    {
+      // a tile whose rows are all dropped already (mode 1: that need no kill list) has nothing to decide
+      int live_row = 0;
+      if (t < kPairTile) {
+        const int j = rt * kPairTile + t;
+        const u64 pj = P.noff[h] + (u64)j;
+        live_row = j < n && !pdrop[pj] && (mode == 0 || kcnt[pj] > kKillSlots);
+      }
+      if (__syncthreads_count(live_row) == 0) continue;
+    }
+    if (t < kWave) decode_key(N, P.key[h], locs, ints);
+    __syncthreads();
+    if (t < d) {
+      int lb = 0, ub = 0;
+      if (t > 0) {
+        lb = ub = kAluNegInf;
+        for (int p = 0; p < N.np; ++p) {
+          const size_t b = (size_t)(N.locbase[p] + locs[p]) * d + t;
+          lb = max(lb, N.lloc[b]);
+          ub = max(ub, N.uloc[b]);
+        }
+        lb = max(lb, N.Lq[t]);
+        ub = max(ub, N.Uq[t]);
+      }
+      Lb[t] = lb;
+      Ub[t] = ub;
+    }
+    const int* lst = wlist + P.noff[h];
+    for (int e = t; e < kPairTile * d2; e += blockDim.x) {
+      const int r = e / d2, x = e - r * d2;
+      const int j = rt * kPairTile + r, c = ct * kPairTile + r;
+      Rz[r * zs + x] = j < n ? szone[(size_t)sj[lst[j]] * d2 + x] : 0;
+      int v = 0;
+      if (!sib && c < old) v = P.zarena[(size_t)P.nodes[P.off[h] + c] * d2 + x];
+      if (sib && c < n) v = szone[(size_t)sj[lst[c]] * d2 + x];
+      Cz[r * zs + x] = v;
+    }
+    __syncthreads();
+    const int r = t >> 3, j = rt * kPairTile + r;
+    if (j >= n) continue;
+    const u64 pj = P.noff[h] + (u64)j;
+    if (pdrop[pj]) continue;   // already dropped by another tile: its kills no longer matter
+    const int* Sj = Rz + r * zs;
+    if (mode == 1) {
+      const int kc = kcnt[pj];
+      if (kc <= kKillSlots) continue;
+      auto put = [&](u32 tag) {
+        const int s = atomicAdd(&kpos[pj], 1);
+        if (s < kc) klist[koff[pj] + s] = tag;
+      };
+      // a kept successor is covered by none of its earlier successors: only the kill direction
+      for (int q = 0; q < kPairTile / 8; ++q) {
+        const int cc = (t & 7) + 8 * q, c = ct * kPairTile + cc;
+        const int* Zc = Cz + cc * zs;
+        if (!sib) {
+          if (c < old && alu_le_lane(Zc, Sj, d, Lb, Ub)) put((u32)c);
+        } else if (c < j && alu_le_lane(Zc, Sj, d, Lb, Ub)) {
+          put(kNewTag | (u32)c);
+        }
+      }
+      continue;
+    }
+    auto record = [&](u32 tag) {
+      const int s = atomicAdd(&kcnt[pj], 1);
+      if (s < kKillSlots) kslot[pj * kKillSlots + s] = tag;
+    };
+    for (int q = 0; q < kPairTile / 8; ++q) {
+      const int cc = (t & 7) + 8 * q, c = ct * kPairTile + cc;
+      const int* Zc = Cz + cc * zs;
+      if (!sib) {
+        if (c < old && alu_le_lane(Zc, Sj, d, Lb, Ub)) record((u32)c);
+      } else if (c < j) {
+        if (alu_le_lane(Sj, Zc, d, Lb, Ub)) {
+          pdrop[pj] = 1;
+          break;
+        }
+        if (alu_le_lane(Zc, Sj, d, Lb, Ub)) record(kNewTag | (u32)c);
+      }
+    }
+  }
+}
+
+// kill-list lengths: the kept successors whose kills overflow their slots
+__global__ void k_kill_need(const int* __restrict__ pdrop, const int* __restrict__ kcnt, long W, int* __restrict__ need) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < W) need[p] = (!pdrop[p] && kcnt[p] > kKillSlots) ? kcnt[p] : 0;
 }
 
 // light/heavy classification of touched keys by working-chain size
 __global__ void k_classify(Passed P, const u32* __restrict__ touched, int T, u32* __restrict__ light,
-                           u32* __restrict__ heavy, Counters* ctr, int light_max) {
+                           u32* __restrict__ heavy, Counters* ctr, int light_max, int d) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const u32 h = t < T ? touched[t] : 0u;
   const bool ok = t < T;
@@ -1651,6 +2012,7 @@ __global__ void k_classify(Passed P, const u32* __restrict__ touched, int T, u32
   const bool lt = ok && w <= (u32)light_max && P.ncnt[h] <= 64u;
   const int il = block_append(&ctr->nlight, lt);
   const int ih = block_append(&ctr->nheavy, ok && !lt);
+  block_append(&ctr->npair, ok && !lt && pair_eligible(P.ncnt[h], P.len[h], d));
   if (lt) light[il] = h;
   if (ok && !lt) heavy[ih] = h;
 }
@@ -2036,6 +2398,12 @@ struct hora_hip_ctx {
   DBuf<u32> gticket, touched, klight, kheavy;
   DBuf<long long> pos, pos2, bsum, scan_total, zrank;
   DBuf<int> zscr, zused;      // a level's surviving new zones before they get zone ids
+  // heavy keys' pair pre-pass: drops, recorded kills and member positions per work-list position;
+  // tile counts and offsets per heavy key
+  DBuf<int> pdrop, kcnt, mpos, tcnt, kneed, kpos;
+  DBuf<u32> kslot, klist;
+  DBuf<long long> toff, koff;
+  bool pairs = true;          // HORA_HIP_PAIRS=0 turns the pre-pass off (the fold then scans in full)
   DBuf<Counters> ctr;
   // profile of the last search
   hora_reach_profile prof{};
@@ -2366,11 +2734,11 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     c->klight.ensure(T);
     c->kheavy.ensure(T);
     hipLaunchKernelGGL(k_classify, dim3(blocks_for(T, 1024)), dim3(1024), 0, c->stream, P, c->touched.p, T,
-                       c->klight.p, c->kheavy.p, c->ctr.p, kLightMax);
+                       c->klight.p, c->kheavy.p, c->ctr.p, kLightMax, c->d);
     c->pos.ensure(T + 1);
     c->pos2.ensure(T + 1);
     long long W = 0, Sseg = 0;
-    int nlh[2] = {0, 0};
+    int nlh[3] = {0, 0, 0};   // light keys, heavy keys, heavy keys for the pair pre-pass
     {
       scan_launch(c, c->caps.p, T, c->pos.p, 0);
       scan_launch(c, c->segs.p, T, c->pos2.p, 1);
@@ -2397,14 +2765,54 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
     const size_t lds_l = merge_layout(d2, false).total() * sizeof(int);
     const size_t lds_h = merge_layout(d2, true).total() * sizeof(int);
     HIP_OK(hipEventRecord(c->ev[3], c->stream));
+    // heavy keys: work lists sorted, then the pair pre-pass over each key's successors
+    const int pre = (c->pairs && nlh[2] > 0) ? 1 : 0;
+    if (pre) {
+      c->pdrop.ensure(W + 1);
+      c->kcnt.ensure(W + 1);
+      c->mpos.ensure(W + 1);
+      c->kslot.ensure((size_t)(W + 1) * kKillSlots);
+      c->tcnt.ensure(nlh[1]);
+      c->toff.ensure(nlh[1] + 1);
+      HIP_OK(hipMemsetAsync(c->pdrop.p, 0, (size_t)(W + 1) * sizeof(int), c->stream));
+      HIP_OK(hipMemsetAsync(c->kcnt.p, 0, (size_t)(W + 1) * sizeof(int), c->stream));
+      hipLaunchKernelGGL(k_presort, dim3(nlh[1]), dim3(1024), 0, c->stream, P, c->kheavy.p, nlh[1], c->wlist.p);
+      hipLaunchKernelGGL(k_pair_tiles, dim3(blocks_for(nlh[1], 256)), dim3(256), 0, c->stream, P, c->kheavy.p,
+                         nlh[1], c->d, c->tcnt.p);
+      scan_launch(c, c->tcnt.p, nlh[1], c->toff.p, 0);
+      const size_t zs = (size_t)(d2 | 1);
+      const size_t lds_p = (2 * kPairTile * zs + 2 * HORA_MAX_DIM + HORA_MAX_PROCS + HORA_MAX_INTS) * sizeof(int);
+      if (lds_p > 64 * 1024)
+        HIP_OK(hipFuncSetAttribute((const void*)k_pairs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_p));
+      hipLaunchKernelGGL(k_pairs, dim3(4096), dim3(256), lds_p, c->stream, c->dnp, P, c->kheavy.p, nlh[1], c->toff.p,
+                         c->scan_total.p, c->wlist.p, c->sj.p, c->s_zone.p, c->pdrop.p, c->kcnt.p, c->kslot.p, 0,
+                         nullptr, nullptr, nullptr);
+      // kept successors with more kills than slots: their full lists
+      c->kneed.ensure(W + 1);
+      c->koff.ensure(W + 1);
+      hipLaunchKernelGGL(k_kill_need, dim3(blocks_for(W, 256)), dim3(256), 0, c->stream, c->pdrop.p, c->kcnt.p, (long)W,
+                         c->kneed.p);
+      scan_launch(c, c->kneed.p, W, c->koff.p, 1);
+      long long K = 0;
+      read_dev(c, &K, c->scan_total.p + 1, sizeof(K));
+      if (K > 0) {
+        c->klist.ensure((size_t)K);
+        c->kpos.ensure(W + 1);
+        HIP_OK(hipMemsetAsync(c->kpos.p, 0, (size_t)(W + 1) * sizeof(int), c->stream));
+        hipLaunchKernelGGL(k_pairs, dim3(4096), dim3(256), lds_p, c->stream, c->dnp, P, c->kheavy.p, nlh[1],
+                           c->toff.p, c->scan_total.p, c->wlist.p, c->sj.p, c->s_zone.p, c->pdrop.p, c->kcnt.p,
+                           c->kslot.p, 1, c->koff.p, c->kpos.p, c->klist.p);
+      }
+    }
+    HIP_OK(hipEventRecord(c->ev[7], c->stream));
     if (nlh[0] > 0)
       hipLaunchKernelGGL(k_merge, dim3(nlh[0]), dim3(64), lds_l, c->stream, c->dnp, P, c->klight.p, nlh[0],
                          c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 0, c->zscr.p, c->zused.p,
-                         c->flag_i.p);
+                         c->flag_i.p, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     if (nlh[1] > 0)
       hipLaunchKernelGGL(k_merge, dim3(nlh[1]), dim3(64), lds_h, c->stream, c->dnp, P, c->kheavy.p, nlh[1],
                          c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 1, c->zscr.p, c->zused.p,
-                         c->flag_i.p);
+                         c->flag_i.p, pre, c->pdrop.p, c->kcnt.p, c->kslot.p, c->koff.p, c->klist.p, c->mpos.p);
     HIP_OK(hipEventRecord(c->ev[4], c->stream));
     ncount += (u64)Sseg;
     // zone ids: the surviving new zones of the level, compact, in slot order (the count is read
@@ -2433,10 +2841,12 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
                        c->sj.p, c->s_dk.p, c->fr_dk2.p, c->fr_zid2.p, c->zrank.p, zbase);
     account(c, hc, prev, G, T, F2);
     if (c->trace)
-      std::fprintf(stderr, "level %d F=%ld R=%ld G=%ld probe_cov=%llu T=%d (light %d heavy %d) W=%lld F2=%ld | exp %.3f probe %.3f merge %.3f ms | reads p=%llu m=%llu unions=%llu\n",
+      std::fprintf(stderr, "level %d F=%ld R=%ld G=%ld probe_cov=%llu T=%d (light %d heavy %d) W=%lld F2=%ld | exp %.3f probe %.3f pairs %.3f merge %.3f ms | reads p=%llu m=%llu unions=%llu full=%llu wovf=%llu\n",
                    level, F, R, G, hc[C_PSUB] - prev[C_PSUB], T, nlh[0], nlh[1], W, F2, ev_ms(c->ev[0], c->ev[1]),
-                   ev_ms(c->ev[5], c->ev[6]), ev_ms(c->ev[3], c->ev[4]), hc[C_PREADS] - prev[C_PREADS],
-                   hc[C_READS] - prev[C_READS], hc[C_UNI] - prev[C_UNI]);
+                   ev_ms(c->ev[5], c->ev[6]), ev_ms(c->ev[3], c->ev[7]), ev_ms(c->ev[7], c->ev[4]),
+                   hc[C_PREADS] - prev[C_PREADS],
+                   hc[C_READS] - prev[C_READS], hc[C_UNI] - prev[C_UNI], hc[C_FULL] - prev[C_FULL],
+                   hc[C_WOVF] - prev[C_WOVF]);
     prev = hc;
     if (hc.err) {
       c->err = hc.err == ERR_HASH_FULL ? "passed-list hash table full"
@@ -2635,6 +3045,8 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
   {
     const char* tr = std::getenv("HORA_HIP_TRACE");
     c->trace = tr && *tr == '1';
+    const char* pr = std::getenv("HORA_HIP_PAIRS");
+    c->pairs = !(pr && *pr == '0');
   }
   if (!c->stream) {
     HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

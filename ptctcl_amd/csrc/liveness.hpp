@@ -1614,7 +1614,7 @@ int construct_online(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, ho
           c->klight.ensure(T);
           c->kheavy.ensure(T);
           hipLaunchKernelGGL(k_classify, dim3(blocks_for(T, 1024)), dim3(1024), 0, c->stream, P, c->touched.p, T,
-                             c->klight.p, c->kheavy.p, c->ctr.p, kLightMax);
+                             c->klight.p, c->kheavy.p, c->ctr.p, kLightMax, c->d);
           c->pos2.ensure(T + 1);
           L.segpos.ensure(T + 1);
           const long long W = scan_exclusive(c, c->caps.p, T, c->pos2.p);
