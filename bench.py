@@ -226,8 +226,9 @@ def serve_main(a) -> int:
         "ms_per_step": round(served["span_s"] * 1000, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
         "data": "synthetic (seeded job mix; no external data)",
-        "config": {"workload": "serving: 512 jobs, types variant reach N~U[3,7], fischer reach N~U[3,10] k=N, "
-                               "fischer A[] not deadlock N~U[3,6], token-ring P0.hold --> P1.hold N~U[3,10]",
+        "config": {"workload": "serving: 512 jobs, types mutual-exclusion variant E<> P1.cs && P2.cs with k=2 "
+                               "and with k=N at N~U[3,8], variant k=N A[] not deadlock N~U[3,10], token-ring "
+                               "P0.hold --> P1.hold N~U[3,10]",
                    "slots": a.slots, "arrival_rate_per_s": round(rate, 2),
                    "one_at_a_time_capacity_per_s": round(capacity, 2),
                    "served": served, "one_at_a_time": single,

@@ -4,9 +4,9 @@
 stream with its own device-resident passed list and frontier; jobs are admitted in arrival order
 when a slot is free and the HBM footprint budget allows, and admitted jobs overlap on the device.
 
-:func:`job_mix` is the seeded serving workload: jobs mixing the plain-reachability variant, the
-batched forward search family with its deadlock query, and the token-ring liveness query, with
-sizes drawn per job, and :func:`poisson_arrivals` the arrival schedule.
+:func:`job_mix` is the seeded serving workload: jobs mixing plain reachability on the
+mutual-exclusion variant (k = 2 and k = N), that family's deadlock query, and the token-ring
+liveness query, with sizes drawn per job, and :func:`poisson_arrivals` the arrival schedule.
 """
 from __future__ import annotations
 
@@ -142,33 +142,39 @@ class JobSpec:
         return ctx.live(self.query)
 
 
-def job_mix(n: int, seed: int) -> List[JobSpec]:
-    """n seeded jobs, the four job types in equal proportion:
+FULL_SEARCH_MAX_N = 8
 
-    * plain reachability on the mutual-exclusion variant (k = 2), N ~ U[3, 7];
-    * batched forward search on fischer (k = N), N ~ U[3, 10], A[] not (P1.cs && P2.cs);
-    * A[] not deadlock on fischer (k = 2), N ~ U[3, 6];
+
+def job_mix(n: int, seed: int, full_max: int = FULL_SEARCH_MAX_N) -> List[JobSpec]:
+    """n seeded jobs, four types in equal proportion, each with its own N:
+
+    * plain reachability, E<> P1.cs && P2.cs, on the mutual-exclusion variant with k = 2 (the
+      4-process CPU case's family), N ~ U[3, full_max];
+    * the same query on the same family with k = N (the batched forward search family),
+      N ~ U[3, full_max];
+    * A[] not deadlock on that family with k = N, early witnesses, N ~ U[3, 10];
     * liveness P0.hold --> P1.hold on the seeded token ring, N ~ U[3, 10].
 
-    The variant and deadlock sizes stop lower than U[3, 10]: their graphs grow much faster with N
-    (the variant's keys carry long zone chains; the deadlock graph stores zones without extrapolation).
+    The two full searches stop at N = 8 by default: alone, one takes 0.2 s at N = 8, about 4 s at
+    N = 9 and about 16 s at N = 10, so at U[3, 10] they would hold a 512-job mix on the GPU for
+    many minutes, almost all of it in the few largest jobs.
     """
     rng = random.Random(seed)
     jobs: List[JobSpec] = []
     for k in range(n):
         t = rng.randrange(4)
         if t == 0:
-            N = rng.randint(3, 7)
+            N = rng.randint(3, full_max)
             net = models.fischer_variant(N, k=2)
-            jobs.append(JobSpec(f"variant{N}", 0, net, target=models.mutex_target(net)))
+            jobs.append(JobSpec(f"variant{N}-k2", 0, net, target=models.mutex_target(net)))
         elif t == 1:
-            N = rng.randint(3, 10)
-            net = models.fischer(N, k=N)
-            jobs.append(JobSpec(f"fischer{N}", 0, net, target=models.mutex_target(net)))
+            N = rng.randint(3, full_max)
+            net = models.fischer_variant(N, k=N)
+            jobs.append(JobSpec(f"variant{N}", 0, net, target=models.mutex_target(net)))
         elif t == 2:
-            N = rng.randint(3, 6)
-            net = models.fischer(N)
-            jobs.append(JobSpec(f"deadlock-fischer{N}", 1, net, query=lv.not_deadlock()))
+            N = rng.randint(3, 10)
+            net = models.fischer_variant(N, k=N)
+            jobs.append(JobSpec(f"deadlock-variant{N}", 1, net, query=lv.not_deadlock()))
         else:
             N = rng.randint(3, 10)
             net = models.token_ring(N, rng.randint(1, 1 << 30))

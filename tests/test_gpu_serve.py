@@ -23,7 +23,7 @@ def collect(server, n):
 
 
 def test_results_identical_alone_and_in_mix():
-    jobs = serve.job_mix(48, seed=7)
+    jobs = serve.job_mix(512, seed=7)
     ref = [alone(s) for s in jobs]
     with serve.Server(slots=6) as srv:
         ids = [s.submit(srv) for s in jobs]
