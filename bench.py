@@ -32,6 +32,12 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# k_expand is bound by instruction issue: one wave64 VALU instruction per CU per clock (four 16-lane
+# SIMDs, four clocks each), 256 CUs at 2.4 GHz. Its VALU instructions per generated successor on a
+# workload are the rocprofv3 SQ_INSTS_VALU count over one search divided by the successors, from
+# profiles/k_expand_fischer12_sq_counters.md.
+VALU_ISSUE_PEAK = 614.4   # G instructions/s
+K_EXPAND_VALU_PER_SUCCESSOR = {"fischer12": 683}
 
 
 def live_workload(name: str):
@@ -384,6 +390,13 @@ def main() -> int:
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(a.workload, kern), "kernel": kern,
                 "avg_launch_ms": round(kms / launches, 4), "bytes_per_launch": round(kbytes / launches)}
+    issue = None
+    if kern == "k_expand" and a.workload in K_EXPAND_VALU_PER_SUCCESSOR:
+        per = K_EXPAND_VALU_PER_SUCCESSOR[a.workload]
+        rate = per * (st.generated / (launches / a.steps)) / ((kms / launches) / 1000.0) / 1e9
+        issue = {"bound": "valu_issue", "achieved": round(rate, 1), "peak": VALU_ISSUE_PEAK, "unit": "G instr/s",
+                 "frac": round(rate / VALU_ISSUE_PEAK, 4), "kernel": kern, "instr_per_successor": per,
+                 "source": "profiles/k_expand_fischer12_sq_counters.md"}
     line = {
         "metric": "symbolic states/sec (full forward zone-graph search)",
         "value": round(value, 1),
@@ -400,6 +413,7 @@ def main() -> int:
         "config": dict(cfg, explored=st.explored, generated=st.generated, verdict_A_not=not reachable,
                        levels=st.levels, stored_live=st.stored_live, keys=st.keys),
         "roofline": roofline,
+        "roofline_issue": issue,
         "kernel_ms_per_step": {k: round(prof[k + "_ms"] / a.steps, 3) for k in ("expand", "probe", "merge", "other")},
     }
     ctx.close()

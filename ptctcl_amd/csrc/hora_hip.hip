@@ -101,6 +101,7 @@ struct DevNet {
   const int* blob;
   int blob_ints;
   int stage;              // 1: kernels that walk the tables stage the blob in LDS
+  int inv_upper;          // every invariant constraint is an upper bound x_a - x_0 <| r
 };
 
 // Rebase every blob pointer of N onto an LDS copy of the blob.
@@ -725,6 +726,37 @@ __device__ __forceinline__ bool apply_succ_inv(const DevNet& N, int* z, const in
   return true;
 }
 
+// After z_up no clock has a finite upper bound. Conjoining upper bounds x_a <| r, triples (a, 0, r)
+// handed out by `each`, then needs no incremental closure per triple: a shortest path through the
+// new edges a -> 0 uses one of them (a second would close a cycle through 0, never negative in a
+// non-empty zone), so the closed result is
+//   z'[i][0] = min(z[i][0], min_a z[i][a] + r_a),  z'[i][j] = min(z[i][j], z'[i][0] + z[0][j]),
+// and it is empty iff min_a z[0][a] + r_a < (<=, 0). A non-empty zone has one canonical form, so
+// this is the zone the triples give one constraint at a time. Returns false iff empty.
+template <class Each>
+__device__ __forceinline__ bool z_upper_after_up(int* z, int d, Each each) {
+  wsync();
+  const int l = lane_id();
+  int m = kInf;
+  if (l < d) each([&](int a, int r) { m = min(m, radd(z[l * d + a], r)); });
+  if (__any(l == 0 && m < kLe0)) return false;
+  const bool tight = l >= 1 && l < d && m < z[l * d];
+  if (!__any(tight)) return true;
+  wsync();
+  if (tight) z[l * d] = m;
+  wsync();
+  const int d2 = d * d;
+  for (int e = l; e < d2; e += kWave) {
+    const int i = ediv(e, d), j = e - i * d;
+    if (i == 0 || j == 0) continue;
+    const int i0 = z[i * d];
+    if (i0 >= kInf) continue;
+    const int v = radd(i0, z[j]);
+    if (v < z[e]) z[e] = v;
+  }
+  return true;
+}
+
 __device__ __forceinline__ int bcast(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 __device__ __forceinline__ u64 bcast64(u64 v, int lane) {
   const u32 lo = (u32)__builtin_amdgcn_readlane((int)(u32)v, lane);
@@ -842,8 +874,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     if (ok) ok = succ_inv(se, re, moved, g1, g2);
     if (ok) {
       if (!fz) {
+        // the invariant holds before the delay, so the zone stays non-empty after it
         z_up(z, d);
-        succ_inv(se, re, moved, g1, g2);
+        if (N.inv_upper && !(BC && re <= -2)) {
+          z_upper_after_up(z, d, [&](auto&& f) __attribute__((always_inline)) {
+            for (int k = 0; k < ns; ++k)
+              if (!((moved >> stag[k]) & 1)) f(sinv[3 * k], sinv[3 * k + 2]);
+            for (int g : {g1, g2})
+              if (g >= 0)
+                for (int c = N.inv_off[g]; c < N.inv_off[g + 1]; ++c) f(N.inv_data[3 * c], N.inv_data[3 * c + 2]);
+          });
+        } else {
+          succ_inv(se, re, moved, g1, g2);
+        }
       }
       z_reset_many(z, d, ~(am | N.qbits) & clocks, tmp);
       if (N.lu_plus) z_extra_lu_plus(z, d, N.Lx, N.Ux, row0);
@@ -3127,6 +3170,9 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
   N.blob = B;
   N.blob_ints = (int)blob.size();
   N.stage = 0;   // staging the tables in LDS measured slower than the L1/L2-resident originals
+  N.inv_upper = 1;
+  for (long k = 0; k < e->inv_off[c->tl]; ++k)
+    if (e->inv_data[3 * k + 1] != 0 || e->inv_data[3 * k] <= 0) N.inv_upper = 0;
   N.never = never; N.nlr = nlr; N.nic = nic; N.ncc = ncc;
   expand_caps(e, &N.rcap, &N.icap);
   if (!c->dnp) HIP_OK(hipMalloc(&c->dnp, sizeof(DevNet)));

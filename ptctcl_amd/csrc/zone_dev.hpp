@@ -26,21 +26,22 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1
 
 // Row of entry e of a d x d matrix, e < d*d, d <= 32, by a multiply and a shift: with
 // M = ceil(2^16 / d) = (2^16 + r) / d, r < d, floor(e*M / 2^16) = floor(e / d) since e*r < 2^16.
-// (A division by a run-time d costs a dozen vector instructions per entry.)
-__device__ __forceinline__ int ediv(int e, int d) {
-  const unsigned m = (65536u + (unsigned)d - 1u) / (unsigned)d;
-  return (int)(((unsigned)e * m) >> 16);
-}
+// (A division by a run-time d costs a dozen vector instructions per entry; even the one division
+// that forms M costs twenty instructions where it is inlined, so M comes from a table.)
+__device__ __constant__ unsigned kEdivM[33] = {
+    0,    65536, 32768, 21846, 16384, 13108, 10923, 9363, 8192, 7282, 6554, 5958, 5462, 5042, 4682, 4370, 4096,
+    3856, 3641,  3450,  3277,  3121,  2979,  2850,  2731, 2622, 2521, 2428, 2341, 2260, 2185, 2115, 2048};
+__device__ __forceinline__ int ediv(int e, int d) { return (int)(((unsigned)e * kEdivM[d]) >> 16); }
 
 __device__ __forceinline__ int r_le(int v) { return v * 2 + 1; }
 __device__ __forceinline__ int r_lt(int v) { return v * 2; }
 __device__ __forceinline__ int r_bnd(int r) { return r >> 1; }
 __device__ __forceinline__ bool r_strict(int r) { return (r & 1) == 0; }
-// Saturating bound addition: infinity absorbs; the sum is weak iff both operands are weak.
+// Saturating bound addition: infinity absorbs; the sum is weak iff both operands are weak. With
+// a = 2x + sa, b = 2y + sb the result 2(x + y) + (sa & sb) is a + b - (sa | sb).
 __device__ __forceinline__ int radd(int a, int b) {
-  if (a >= kInf || b >= kInf) return kInf;
-  const int v = (a >> 1) + (b >> 1);
-  return v * 2 + (a & b & 1);
+  if (max(a, b) >= kInf) return kInf;
+  return a + b - ((a | b) & 1);
 }
 // The raw bound of the complement of x_i - x_j <| v, stated as a bound on x_j - x_i.
 __device__ __forceinline__ int r_neg(int r) {
@@ -65,6 +66,18 @@ __device__ __forceinline__ void bsync() { __syncthreads(); }
 
 __device__ __forceinline__ void z_copy(int* __restrict__ dst, const int* __restrict__ src, int d2) {
   for (int e = lane_id(); e < d2; e += kWave) dst[e] = src[e];
+}
+
+// One pass z[i][j] = min(z[i][j], z[i][p] + z[p][j]) for a pivot p whose row and column the pass
+// leaves unchanged (z[p][p] = (<=, 0)), so lanes may read and write in any order. Branch-free: a
+// lane reads its three values, then writes the minimum (the same value where nothing changes).
+__device__ __forceinline__ void z_pivot_pass(int* z, int d, int p) {
+  const int l = lane_id(), d2 = d * d;
+  for (int e = l; e < d2; e += kWave) {
+    const int i = ediv(e, d), j = e - i * d;
+    const int ip = z[i * d + p], pj = z[p * d + j], ij = z[e];
+    z[e] = min(ij, radd(ip, pj));
+  }
 }
 
 __device__ __forceinline__ bool z_empty(const int* z, int d) {
@@ -110,30 +123,16 @@ __device__ __forceinline__ bool z_constrain(int* z, int d, int a, int b, int r) 
   wsync();
   if (l < d && col < z[l * d + b]) z[l * d + b] = col;
   wsync();
-  const int d2 = d * d;
-  for (int e = l; e < d2; e += kWave) {
-    const int i = ediv(e, d), k = e - i * d;
-    const int ib = z[i * d + b];
-    if (ib >= kInf) continue;
-    const int v = radd(ib, z[b * d + k]);
-    if (v < z[e]) z[e] = v;
-  }
+  z_pivot_pass(z, d, b);
   return true;
 }
 
 // Floyd-Warshall closure, one pivot per step. Within a step, row k and column k do not change on
 // a zone without negative cycles, so the parallel update equals the sequential one.
 __device__ __forceinline__ void z_close(int* z, int d) {
-  const int d2 = d * d, l = lane_id();
   for (int k = 0; k < d; ++k) {
     wsync();
-    for (int e = l; e < d2; e += kWave) {
-      const int i = ediv(e, d), j = e - i * d;
-      const int ik = z[i * d + k];
-      if (ik >= kInf) continue;
-      const int v = radd(ik, z[k * d + j]);
-      if (v < z[e]) z[e] = v;
-    }
+    z_pivot_pass(z, d, k);
   }
 }
 
