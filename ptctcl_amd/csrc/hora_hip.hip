@@ -1476,7 +1476,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                                               int* __restrict__ used, int* __restrict__ flag, int pre,
                                               const int* __restrict__ pdrop, const int* __restrict__ kcnt,
                                               const u32* __restrict__ kslot, const long long* __restrict__ koff,
-                                              const u32* __restrict__ klist, int* __restrict__ mpos) {
+                                              const u32* __restrict__ klist, int* __restrict__ mpos,
+                                              int* __restrict__ zdead, u64 lvl_zbase) {
   extern __shared__ int sm[];
   const DevNet& N = *Np;
   const int d = N.d, d2 = N.d2, l = threadIdx.x;
@@ -1583,6 +1584,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     const u32 id = ch[pos] & ~kDead;
     return (id & kNewTag) ? zscr + (size_t)(zbase_k + (id & kPosMask)) * d2 : P.zarena + (size_t)id * d2;
   };
+  // a stored zone that an earlier chunk of this level stored, retired here, leaves the next frontier
+  auto note_dead = [&](u32 id) __attribute__((always_inline)) {
+    if (zdead && !(id & (kNewTag | kDead)) && (u64)id >= lvl_zbase) zdead[id - lvl_zbase] = 1;
+  };
   int len = (int)old_len, live = (int)old_len;
   unsigned long long n_sub = 0, n_kill = 0, n_uni = 0, n_reads = 0, n_ins = 0;
   // pair pre-pass answers: `use_pairs` for the drops (exact on their own), `pairs_full` while the
@@ -1658,7 +1663,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
           }
           const u64 b = __ballot(kill);
           wsync();
-          if (kill) ch[m] |= kDead;
+          if (kill) {
+            note_dead(ch[m]);
+            ch[m] |= kDead;
+          }
           csync();
           live -= __popcll(b);
           n_kill += __popcll(b);
@@ -1673,7 +1681,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
           }
           const u64 bw = __ballot(kw);
           wsync();
-          if (kw) ch[mw] |= kDead;
+          if (kw) {
+            note_dead(ch[mw]);
+            ch[mw] |= kDead;
+          }
           csync();
           live -= __popcll(bw);
           n_kill += __popcll(bw);
@@ -1686,7 +1697,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
           ++n_reads;
           if (z_alu_le(zp(m), S, d, Lb, Ub)) {
             wsync();
-            if (l == 0) ch[m] = id | kDead;
+            if (l == 0) {
+              note_dead(id);
+              ch[m] = id | kDead;
+            }
             csync();
             --live;
             ++n_kill;
@@ -1700,7 +1714,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             kill = d <= 16 ? alu_le_lane16<true>(zp(m), S, d, Lb, Ub) : alu_le_lane(zp(m), S, d, Lb, Ub);
           const u64 b = __ballot(kill);
           wsync();
-          if (kill) ch[m] |= kDead;
+          if (kill) {
+            note_dead(ch[m]);
+            ch[m] |= kDead;
+          }
           csync();
           live -= __popcll(b);
           n_kill += __popcll(b);
@@ -1744,7 +1761,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         if (win >= 0) {
           const int m = cand[win];
           wsync();
-          if (l == 0) ch[m] |= kDead;
+          if (l == 0) {
+            note_dead(ch[m]);
+            ch[m] |= kDead;
+          }
           csync();
           --live;
           ++n_uni;
@@ -2084,6 +2104,40 @@ __global__ __launch_bounds__(64) void k_zassign(Passed P, const u32* __restrict_
   }
 }
 
+// Chunk boundaries: cuts[k] = first frontier state whose slots start at or after k * budget
+// (cuts[0] = 0, cuts[K] = F); `pos` is the exclusive scan of the recipe counts.
+__global__ void k_chunk_cuts(const long long* __restrict__ pos, long F, long budget, long K,
+                             long long* __restrict__ cuts) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > K) return;
+  if (k == 0) { cuts[0] = 0; return; }
+  if (k == K) { cuts[K] = F; return; }
+  const long long want = (long long)k * budget;
+  long lo = 0, hi = F;   // first index in [0, F] with pos >= want (F when none)
+  while (lo < hi) {
+    const long mid = (lo + hi) >> 1;
+    if (pos[mid] >= want) hi = mid; else lo = mid + 1;
+  }
+  cuts[k] = lo;
+}
+
+// A chunked level's next frontier: the entries whose zone no later chunk retired.
+__global__ void k_frontier_live(const u32* __restrict__ zid, long F2, const int* __restrict__ zdead, u64 zbase,
+                                int* __restrict__ flag) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < F2) flag[i] = zdead[zid[i] - zbase] ? 0 : 1;
+}
+
+__global__ void k_frontier_compact(const u64* __restrict__ dk, const u32* __restrict__ zid, long F2,
+                                   const int* __restrict__ flag, const long long* __restrict__ pos,
+                                   u64* __restrict__ dk_out, u32* __restrict__ zid_out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < F2 && flag[i]) {
+    dk_out[pos[i]] = dk[i];
+    zid_out[pos[i]] = zid[i];
+  }
+}
+
 // next frontier from survivors (flagged by k_merge), in compacted-successor order
 __global__ void k_build_frontier(const int* __restrict__ survived, const long long* __restrict__ pos, long G,
                                  const int* __restrict__ sj, const u64* __restrict__ sdk,
@@ -2407,6 +2461,117 @@ struct DBuf {
   }
 };
 
+// The stored-zone arena. It only grows, and at its largest it holds most of the GPU's memory, so
+// growing it by copying into a larger buffer (old and new alive at once) would halve what fits.
+// Where the device supports virtual memory management, the arena is a reserved address range the
+// size of the device's memory into which physical memory is mapped in 256 MiB steps as it grows:
+// no copy, no double footprint, and the address never moves. Elsewhere (or inside a fixed
+// scratch) it falls back to DBuf's copying growth.
+struct GrowArena {
+  int* p = nullptr;
+  size_t cap = 0;                 // ints mapped (or allocated)
+  DBuf<int> plain;                // fallback
+  void* va = nullptr;
+  size_t reserved = 0, mapped = 0, step = 0;
+  std::vector<std::pair<hipMemGenericAllocationHandle_t, size_t>> parts;
+  int vmm = -1;                   // -1 unknown, 0 no, 1 yes
+  hipMemAllocationProp prop{};
+  GrowArena() = default;
+  GrowArena(const GrowArena&) = delete;
+  GrowArena& operator=(const GrowArena&) = delete;
+  ~GrowArena() { release(); }
+  bool use_vmm() {
+    if (tl_arena) return false;
+    if (vmm < 0) {
+      vmm = 0;
+      int dev = 0, ok = 0;
+      if (hipGetDevice(&dev) == hipSuccess &&
+          hipDeviceGetAttribute(&ok, hipDeviceAttributeVirtualMemoryManagementSupported, dev) == hipSuccess && ok) {
+        prop = hipMemAllocationProp{};
+        prop.type = hipMemAllocationTypePinned;
+        prop.location.type = hipMemLocationTypeDevice;
+        prop.location.id = dev;
+        size_t gran = 0, total = 0;
+        if (hipMemGetAllocationGranularity(&gran, &prop, hipMemAllocationGranularityRecommended) == hipSuccess &&
+            gran > 0 && hipDeviceTotalMem(&total, dev) == hipSuccess && total > 0) {
+          step = ((size_t(256) << 20) + gran - 1) / gran * gran;
+          const size_t want = (total + step - 1) / step * step;
+          if (hipMemAddressReserve(&va, want, gran, nullptr, 0) == hipSuccess) {
+            reserved = want;
+            vmm = 1;
+          } else {
+            (void)hipGetLastError();
+            va = nullptr;
+          }
+        } else {
+          (void)hipGetLastError();
+        }
+      } else {
+        (void)hipGetLastError();
+      }
+    }
+    return vmm == 1;
+  }
+  // map physical memory until `bytes` are backed (at least an eighth more than now, so that a
+  // growing search maps in few steps)
+  void map_to(size_t bytes) {
+    if (bytes > reserved) throw HipFail("zone arena: more than the device's memory");
+    if (bytes > mapped) bytes = std::min(reserved, std::max(bytes, mapped + mapped / 8));
+    while (mapped < bytes) {
+      hipMemGenericAllocationHandle_t h;
+      HIP_OK(hipMemCreate(&h, step, &prop, 0));
+      if (hipMemMap((char*)va + mapped, step, 0, h, 0) != hipSuccess) {
+        (void)hipMemRelease(h);
+        throw HipFail("zone arena: mapping failed");
+      }
+      hipMemAccessDesc acc{};
+      acc.location = prop.location;
+      acc.flags = hipMemAccessFlagsProtReadWrite;
+      HIP_OK(hipMemSetAccess((char*)va + mapped, step, &acc, 1));
+      parts.emplace_back(h, step);
+      mapped += step;
+    }
+    p = (int*)va;
+    cap = mapped / sizeof(int);
+  }
+  void ensure(size_t n) {
+    if (use_vmm()) { map_to(std::max<size_t>(n, 1) * sizeof(int)); return; }
+    plain.ensure(n);
+    p = plain.p;
+    cap = plain.cap;
+  }
+  void grow_keep(size_t n, hipStream_t s) {
+    if (use_vmm()) {
+      // mapping more pages leaves the mapped ones, and any kernel reading them, untouched
+      map_to(std::max<size_t>(n, 1) * sizeof(int));
+      return;
+    }
+    plain.grow_keep(n, s);
+    p = plain.p;
+    cap = plain.cap;
+  }
+  void release() {
+    if (vmm == 1 && va) {
+      if (tl_stream) (void)hipStreamSynchronize(tl_stream);
+      else (void)hipDeviceSynchronize();
+      size_t off = 0;
+      for (auto& ph : parts) {
+        (void)hipMemUnmap((char*)va + off, ph.second);
+        (void)hipMemRelease(ph.first);
+        off += ph.second;
+      }
+      parts.clear();
+      (void)hipMemAddressFree(va, reserved);
+      va = nullptr;
+      reserved = mapped = 0;
+      vmm = -1;
+    }
+    plain.release();
+    p = nullptr;
+    cap = 0;
+  }
+};
+
 }  // namespace
 
 struct LiveState;             // liveness pass state (liveness.hpp)
@@ -2430,7 +2595,7 @@ struct hora_hip_ctx {
   DBuf<u64> ht_key, ht_off, ht_noff, ht_nseg;
   DBuf<u32> ht_len, ht_ncnt;
   DBuf<u32> nodes;
-  DBuf<int> zarena;
+  GrowArena zarena;
   u64 H = 0;
   int hbits = 0;
   DBuf<u64> fr_dk, fr_dk2;
@@ -2444,6 +2609,8 @@ struct hora_hip_ctx {
   // heavy keys' pair pre-pass: drops, recorded kills and member positions per work-list position;
   // tile counts and offsets per heavy key
   DBuf<int> pdrop, kcnt, mpos, tcnt, kneed, kpos;
+  DBuf<int> zdead;            // a chunked level's retirement marks for the zones it stored
+  DBuf<long long> cuts;       // chunk boundaries of a level
   DBuf<u32> kslot, klist;
   DBuf<long long> toff, koff;
   bool pairs = true;          // HORA_HIP_PAIRS=0 turns the pre-pass off (the fold then scans in full)
@@ -2620,15 +2787,16 @@ void account(hora_hip_ctx* c, const Tally& hc, const Tally& prev, long G, int T,
   c->prof.other_ms += ev_ms(c->ev[2], c->ev[5]) + ev_ms(c->ev[6], c->ev[3]);
 }
 
-void launch_expand(hora_hip_ctx* c, int wpb, long F, ExpandOut eo, bool init) {
+// Expansion of frontier states [fa, fa + F); slot offsets and counts are the chunk's own.
+void launch_expand(hora_hip_ctx* c, int wpb, long F, long fa, ExpandOut eo, bool init) {
   const size_t lds = (size_t)wpb * expand_lds_ints(c->d2, c->dn.rcap, c->dn.icap) * sizeof(int);
   if (c->dn.has_bcast)
     hipLaunchKernelGGL(k_expand<true>, dim3(blocks_for(F, wpb)), dim3(64 * wpb), lds, c->stream, c->dnp,
-                       c->fr_dk.p, c->fr_zid.p, c->zarena.p, (int)F, c->pos.p, c->rec_cnt.p, eo, c->ctr.p,
+                       c->fr_dk.p + fa, c->fr_zid.p + fa, c->zarena.p, (int)F, c->pos.p, c->rec_cnt.p, eo, c->ctr.p,
                        init ? 1 : 0);
   else
     hipLaunchKernelGGL(k_expand<false>, dim3(blocks_for(F, wpb)), dim3(64 * wpb), lds, c->stream, c->dnp,
-                       c->fr_dk.p, c->fr_zid.p, c->zarena.p, (int)F, c->pos.p, c->rec_cnt.p, eo, c->ctr.p,
+                       c->fr_dk.p + fa, c->fr_zid.p + fa, c->zarena.p, (int)F, c->pos.p, c->rec_cnt.p, eo, c->ctr.p,
                        init ? 1 : 0);
 }
 
@@ -2636,7 +2804,7 @@ void launch_expand(hora_hip_ctx* c, int wpb, long F, ExpandOut eo, bool init) {
 // resident wavefronts; fewer keep LDS allocation granular. The first level with enough states runs
 // the expansion once per candidate (the kernel is idempotent: same slots, same values) and keeps
 // the fastest for the context and zone dimension. HORA_HIP_EXPAND_WPB=1|2|4 fixes the choice.
-int expand_tuned(hora_hip_ctx* c, long F, ExpandOut eo, bool init) {
+int expand_tuned(hora_hip_ctx* c, long F, long fa, ExpandOut eo, bool init) {
   if (c->expand_wpb > 0 && c->expand_wpb_d == c->d) return c->expand_wpb;
   if (const char* e = std::getenv("HORA_HIP_EXPAND_WPB")) {
     const int w = std::atoi(e);
@@ -2649,7 +2817,7 @@ int expand_tuned(hora_hip_ctx* c, long F, ExpandOut eo, bool init) {
   for (int rep = 0; rep < 2; ++rep)
     for (int w : cand) {
       HIP_OK(hipEventRecord(c->ev[6], c->stream));
-      launch_expand(c, w, F, eo, init);
+      launch_expand(c, w, F, fa, eo, init);
       HIP_OK(hipEventRecord(c->ev[7], c->stream));
       HIP_OK(hipEventSynchronize(c->ev[7]));
       const float ms = ev_ms(c->ev[6], c->ev[7]);
@@ -2659,6 +2827,38 @@ int expand_tuned(hora_hip_ctx* c, long F, ExpandOut eo, bool init) {
   c->expand_wpb_d = c->d;
   if (c->trace) std::fprintf(stderr, "k_expand: %d wavefronts per workgroup (%.3f ms at F=%ld)\n", pick, best, F);
   return pick;
+}
+
+// Successor slots per chunk of a level: the transient per-successor buffers (the zone, its key and
+// flags, its place in the work lists and the merge's scratch copy) are sized for one chunk, about
+// 2 x 4 d^2 + 128 bytes a slot, within 16 GiB. HORA_HIP_CHUNK_SLOTS sets another bound.
+long chunk_slots(hora_hip_ctx* c) {
+  if (const char* e = std::getenv("HORA_HIP_CHUNK_SLOTS")) {
+    const long v = std::atol(e);
+    if (v > 0) return v;
+  }
+  return std::max<long>(1L << 16, (long)((16ull << 30) / (8ull * (u64)c->d2 + 128ull)));
+}
+
+// Chunk boundaries of a frontier of F states whose slot offsets (exclusive scan of the recipe
+// counts) are in c->pos and total R: cut k is the first state whose slots start at or after
+// k * budget. Empty chunks are dropped.
+std::vector<long> chunk_cuts(hora_hip_ctx* c, long F, long R, long budget) {
+  long K = (R + budget - 1) / budget;
+  if (K > 4096) {
+    budget = (R + 4095) / 4096;
+    K = (R + budget - 1) / budget;
+  }
+  c->cuts.ensure(K + 1);
+  hipLaunchKernelGGL(k_chunk_cuts, dim3(blocks_for(K + 1, 256)), dim3(256), 0, c->stream, c->pos.p, F, budget, K,
+                     c->cuts.p);
+  std::vector<long long> h(K + 1);
+  read_dev(c, h.data(), c->cuts.p, (K + 1) * sizeof(long long));
+  std::vector<long> cuts{0};
+  for (long k = 1; k <= K; ++k)
+    if ((long)h[k] > cuts.back()) cuts.push_back((long)h[k]);
+  if (cuts.back() != F) cuts.push_back(F);
+  return cuts;
 }
 
 int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
@@ -2696,8 +2896,11 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
   long F = 1;
   int level = 0;
   bool init = true;
+  bool done = false;
+  // successor slots per chunk of the frontier (transient buffers are sized for one chunk)
+  const long slot_budget = chunk_slots(c);
   for (;;) {
-    // ---- recipe counts and offsets
+    // ---- recipe counts and offsets of the whole frontier
     long R = 1;
     c->rec_cnt.ensure(F);
     c->rec_off.ensure(F);
@@ -2708,202 +2911,256 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
       hipLaunchKernelGGL(k_count, dim3(blocks_for(F, 256)), dim3(256), 0, c->stream, c->dnp, c->fr_dk.p, (int)F,
                          c->rec_cnt.p, c->ctr.p);
       R = scan_exclusive(c, c->rec_cnt.p, F, c->pos.p);   // slot offsets, read by k_expand
+      st->expanded += (u64)F;
     } else {
       HIP_OK(hipMemsetAsync(c->pos.p, 0, sizeof(long long), c->stream));
     }
-    // ---- expansion
-    c->s_valid.ensure(R);
-    c->s_tgt.ensure(R);
-    c->s_dk.ensure(R);
-    c->s_zone.ensure((size_t)R * d2);
-    ExpandOut eo{c->s_valid.p, c->s_dk.p, c->s_zone.p, c->s_tgt.p, nullptr, nullptr};
-    const int wpb = R > 0 ? expand_tuned(c, F, eo, init) : 1;
-    HIP_OK(hipEventRecord(c->ev[0], c->stream));
-    if (R > 0) launch_expand(c, wpb, F, eo, init);
-    HIP_OK(hipEventRecord(c->ev[1], c->stream));
-    // ---- compaction of non-empty successors
-    c->pos.ensure(R + 1);
-    Tally hc;
-    const long Gall = scan_and_counters(c, c->s_valid.p, R, c->pos.p, &hc);
-    c->prof.expand_ms += ev_ms(c->ev[0], c->ev[1]);
-    c->prof.expand_bytes += 4.0 * d2 * (double)((init ? 0 : F) + Gall);
-    c->prof.expand_launches += 1;
-    if (!init) st->expanded += (u64)F;
-    long G = Gall;
-    const bool found = hc.first_target != 0x7fffffff;
-    if (found) {
-      // The sequential engine inserts every successor that precedes the first target, then
-      // stops; generated counts those plus the target itself (engine.cpp:2085-2095).
-      long long before = 0;
-      if (hc.first_target > 0)
-        read_dev(c, &before, c->pos.p + hc.first_target, sizeof(before));
-      if (!init) st->generated += (u64)before + 1;
-      st->found = 1;
-      *reachable = 1;
-      G = (long)before;
-    } else if (!init) {
-      st->generated += (u64)G;
-    }
-    if (G == 0) break;
-    c->sj.ensure(Gall);
-    hipLaunchKernelGGL(k_compact_valid, dim3(blocks_for(R, 256)), dim3(256), 0, c->stream, c->s_valid.p, c->pos.p, R, c->sj.p,
-                       c->ctr.p);
-    while ((double)(hc[C_KEYS] + (u64)G) > 0.6 * (double)c->H) grow_table(c);
-    Passed P = passed_view(c);
-    // ---- keys, cover pre-filter, per-key work lists
-    c->gbk.ensure(G);
-    c->gticket.ensure(G);
-    c->touched.ensure(G);
-    c->covered.ensure(G);
-    c->survived.ensure(G);
-    c->flag_i.ensure(G);
-    HIP_OK(hipEventRecord(c->ev[2], c->stream));
-    hipLaunchKernelGGL(k_find, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->sj.p, c->s_dk.p, G,
-                       c->gbk.p, c->ctr.p, c->survived.p, c->flag_i.p);
-    HIP_OK(hipEventRecord(c->ev[5], c->stream));
-    hipLaunchKernelGGL(k_probe, dim3(blocks_for(G, 4)), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p,
-                       c->s_dk.p, c->s_zone.p, G, c->covered.p, c->ctr.p);
-    HIP_OK(hipEventRecord(c->ev[6], c->stream));
-    hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 1024)), dim3(1024), 0, c->stream, P, c->gbk.p, c->covered.p, G,
-                       c->gticket.p, c->touched.p, c->ctr.p);
-    int T = 0;
-    read_dev(c, &T, &c->ctr.p->ntouched, sizeof(int));
-    c->caps.ensure(T);
-    c->segs.ensure(T);
-    hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
-                       c->caps.p, c->segs.p, c->ctr.p);
-    // light and heavy keys (k_merge's two shapes) are known once the chains are counted: the
-    // classification rides on the same synchronisation as the two scans
-    c->klight.ensure(T);
-    c->kheavy.ensure(T);
-    hipLaunchKernelGGL(k_classify, dim3(blocks_for(T, 1024)), dim3(1024), 0, c->stream, P, c->touched.p, T,
-                       c->klight.p, c->kheavy.p, c->ctr.p, kLightMax, c->d);
-    c->pos.ensure(T + 1);
-    c->pos2.ensure(T + 1);
-    long long W = 0, Sseg = 0;
-    int nlh[3] = {0, 0, 0};   // light keys, heavy keys, heavy keys for the pair pre-pass
-    {
-      scan_launch(c, c->caps.p, T, c->pos.p, 0);
-      scan_launch(c, c->segs.p, T, c->pos2.p, 1);
-      long long tt[2] = {0, 0};
-      read_dev(c, tt, c->scan_total.p, sizeof(tt), nlh, &c->ctr.p->nlight, sizeof(nlh));
-      W = tt[0];
-      Sseg = tt[1];
-    }
-    // capacity: the reserved chain segments, and zone ids for at most W surviving new zones (the
-    // ids are handed out compactly after the merge)
-    if (zcount + (u64)W > P.zcap) c->zarena.grow_keep((size_t)(zcount + (u64)W) * 2 * d2, c->stream);
-    if (ncount + (u64)Sseg > P.ncap) c->nodes.grow_keep((size_t)(ncount + (u64)Sseg) * 2, c->stream);
-    P = passed_view(c);
-    hipLaunchKernelGGL(k_touched_offsets, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
-                       c->pos.p, c->pos2.p, ncount);
-    c->wlist.ensure(W + 1);
-    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p,
-                       c->gticket.p, G, c->wlist.p);
-    // ---- ordered merge of the kept successors: light keys, then heavy keys; surviving new zones
-    // wait in a scratch slot per work-list position
-    c->zscr.ensure((size_t)std::max<long long>(W, 1) * d2);
-    c->zused.ensure(W + 1);
-    HIP_OK(hipMemsetAsync(c->zused.p, 0, (size_t)(W + 1) * sizeof(int), c->stream));
-    const size_t lds_l = merge_layout(d2, false).total() * sizeof(int);
-    const size_t lds_h = merge_layout(d2, true).total() * sizeof(int);
-    HIP_OK(hipEventRecord(c->ev[3], c->stream));
-    // heavy keys: work lists sorted, then the pair pre-pass over each key's successors
-    const int pre = (c->pairs && nlh[2] > 0) ? 1 : 0;
-    if (pre) {
-      c->pdrop.ensure(W + 1);
-      c->kcnt.ensure(W + 1);
-      c->mpos.ensure(W + 1);
-      c->kslot.ensure((size_t)(W + 1) * kKillSlots);
-      c->tcnt.ensure(nlh[1]);
-      c->toff.ensure(nlh[1] + 1);
-      HIP_OK(hipMemsetAsync(c->pdrop.p, 0, (size_t)(W + 1) * sizeof(int), c->stream));
-      HIP_OK(hipMemsetAsync(c->kcnt.p, 0, (size_t)(W + 1) * sizeof(int), c->stream));
-      hipLaunchKernelGGL(k_presort, dim3(nlh[1]), dim3(1024), 0, c->stream, P, c->kheavy.p, nlh[1], c->wlist.p);
-      hipLaunchKernelGGL(k_pair_tiles, dim3(blocks_for(nlh[1], 256)), dim3(256), 0, c->stream, P, c->kheavy.p,
-                         nlh[1], c->d, c->tcnt.p);
-      scan_launch(c, c->tcnt.p, nlh[1], c->toff.p, 0);
-      const size_t zs = (size_t)(d2 | 1);
-      const size_t lds_p = (2 * kPairTile * zs + 2 * HORA_MAX_DIM + HORA_MAX_PROCS + HORA_MAX_INTS) * sizeof(int);
-      if (lds_p > 64 * 1024)
-        HIP_OK(hipFuncSetAttribute((const void*)k_pairs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_p));
-      hipLaunchKernelGGL(k_pairs, dim3(4096), dim3(256), lds_p, c->stream, c->dnp, P, c->kheavy.p, nlh[1], c->toff.p,
-                         c->scan_total.p, c->wlist.p, c->sj.p, c->s_zone.p, c->pdrop.p, c->kcnt.p, c->kslot.p, 0,
-                         nullptr, nullptr, nullptr);
-      // kept successors with more kills than slots: their full lists
-      c->kneed.ensure(W + 1);
-      c->koff.ensure(W + 1);
-      hipLaunchKernelGGL(k_kill_need, dim3(blocks_for(W, 256)), dim3(256), 0, c->stream, c->pdrop.p, c->kcnt.p, (long)W,
-                         c->kneed.p);
-      scan_launch(c, c->kneed.p, W, c->koff.p, 1);
-      long long K = 0;
-      read_dev(c, &K, c->scan_total.p + 1, sizeof(K));
-      if (K > 0) {
-        c->klist.ensure((size_t)K);
-        c->kpos.ensure(W + 1);
-        HIP_OK(hipMemsetAsync(c->kpos.p, 0, (size_t)(W + 1) * sizeof(int), c->stream));
-        hipLaunchKernelGGL(k_pairs, dim3(4096), dim3(256), lds_p, c->stream, c->dnp, P, c->kheavy.p, nlh[1],
-                           c->toff.p, c->scan_total.p, c->wlist.p, c->sj.p, c->s_zone.p, c->pdrop.p, c->kcnt.p,
-                           c->kslot.p, 1, c->koff.p, c->kpos.p, c->klist.p);
+    // The level in chunks of consecutive frontier states, each with at most slot_budget successor
+    // slots, expanded and merged one after another. Merging chunk by chunk is the sequential
+    // fold's order (successors in (state, recipe) order); the cover pre-filter of a later chunk
+    // tests against zones an earlier chunk stored, which precede it in that order too. A zone a
+    // later chunk retires leaves the next frontier at the end of the level.
+    std::vector<long> cuts{0, F};
+    if (R > slot_budget) cuts = chunk_cuts(c, F, R, slot_budget);
+    const bool chunked = cuts.size() > 2;
+    const u64 lvl_zbase = zcount;
+    long F2 = 0;
+    size_t zdead_n = 0;   // retirement marks cleared so far this level
+    bool found = false;
+    for (size_t ci = 0; ci + 1 < cuts.size(); ++ci) {
+      const long fa = cuts[ci], Fc = cuts[ci + 1] - cuts[ci];
+      long Rc = R;
+      if (chunked) {
+        hipLaunchKernelGGL(k_count, dim3(blocks_for(Fc, 256)), dim3(256), 0, c->stream, c->dnp, c->fr_dk.p + fa,
+                           (int)Fc, c->rec_cnt.p, c->ctr.p);
+        Rc = scan_exclusive(c, c->rec_cnt.p, Fc, c->pos.p);
+      }
+      // ---- expansion
+      c->s_valid.ensure(Rc);
+      c->s_tgt.ensure(Rc);
+      c->s_dk.ensure(Rc);
+      c->s_zone.ensure((size_t)Rc * d2);
+      ExpandOut eo{c->s_valid.p, c->s_dk.p, c->s_zone.p, c->s_tgt.p, nullptr, nullptr};
+      const int wpb = Rc > 0 ? expand_tuned(c, Fc, fa, eo, init) : 1;
+      HIP_OK(hipEventRecord(c->ev[0], c->stream));
+      if (Rc > 0) launch_expand(c, wpb, Fc, fa, eo, init);
+      HIP_OK(hipEventRecord(c->ev[1], c->stream));
+      // ---- compaction of non-empty successors
+      c->pos.ensure(Rc + 1);
+      Tally hc;
+      const long Gall = scan_and_counters(c, c->s_valid.p, Rc, c->pos.p, &hc);
+      c->prof.expand_ms += ev_ms(c->ev[0], c->ev[1]);
+      c->prof.expand_bytes += 4.0 * d2 * (double)((init ? 0 : Fc) + Gall);
+      c->prof.expand_launches += 1;
+      long G = Gall;
+      found = hc.first_target != 0x7fffffff;
+      if (found) {
+        // The sequential engine inserts every successor that precedes the first target, then
+        // stops; generated counts those plus the target itself (engine.cpp:2085-2095).
+        long long before = 0;
+        if (hc.first_target > 0)
+          read_dev(c, &before, c->pos.p + hc.first_target, sizeof(before));
+        if (!init) st->generated += (u64)before + 1;
+        st->found = 1;
+        *reachable = 1;
+        G = (long)before;
+      } else if (!init) {
+        st->generated += (u64)G;
+      }
+      if (G == 0) {
+        prev = hc;
+        if (found) break;
+        continue;
+      }
+      c->sj.ensure(Gall);
+      hipLaunchKernelGGL(k_compact_valid, dim3(blocks_for(Rc, 256)), dim3(256), 0, c->stream, c->s_valid.p, c->pos.p, Rc,
+                         c->sj.p, c->ctr.p);
+      while ((double)(hc[C_KEYS] + (u64)G) > 0.6 * (double)c->H) grow_table(c);
+      Passed P = passed_view(c);
+      // ---- keys, cover pre-filter, per-key work lists
+      c->gbk.ensure(G);
+      c->gticket.ensure(G);
+      c->touched.ensure(G);
+      c->covered.ensure(G);
+      c->survived.ensure(G);
+      c->flag_i.ensure(G);
+      HIP_OK(hipEventRecord(c->ev[2], c->stream));
+      hipLaunchKernelGGL(k_find, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->sj.p, c->s_dk.p, G,
+                         c->gbk.p, c->ctr.p, c->survived.p, c->flag_i.p);
+      HIP_OK(hipEventRecord(c->ev[5], c->stream));
+      hipLaunchKernelGGL(k_probe, dim3(blocks_for(G, 4)), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p,
+                         c->s_dk.p, c->s_zone.p, G, c->covered.p, c->ctr.p);
+      HIP_OK(hipEventRecord(c->ev[6], c->stream));
+      hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 1024)), dim3(1024), 0, c->stream, P, c->gbk.p, c->covered.p, G,
+                         c->gticket.p, c->touched.p, c->ctr.p);
+      int T = 0;
+      read_dev(c, &T, &c->ctr.p->ntouched, sizeof(int));
+      c->caps.ensure(T);
+      c->segs.ensure(T);
+      hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                         c->caps.p, c->segs.p, c->ctr.p);
+      // light and heavy keys (k_merge's two shapes) are known once the chains are counted: the
+      // classification rides on the same synchronisation as the two scans
+      c->klight.ensure(T);
+      c->kheavy.ensure(T);
+      hipLaunchKernelGGL(k_classify, dim3(blocks_for(T, 1024)), dim3(1024), 0, c->stream, P, c->touched.p, T,
+                         c->klight.p, c->kheavy.p, c->ctr.p, kLightMax, c->d);
+      c->pos.ensure(T + 1);
+      c->pos2.ensure(T + 1);
+      long long W = 0, Sseg = 0;
+      int nlh[3] = {0, 0, 0};   // light keys, heavy keys, heavy keys for the pair pre-pass
+      {
+        scan_launch(c, c->caps.p, T, c->pos.p, 0);
+        scan_launch(c, c->segs.p, T, c->pos2.p, 1);
+        long long tt[2] = {0, 0};
+        read_dev(c, tt, c->scan_total.p, sizeof(tt), nlh, &c->ctr.p->nlight, sizeof(nlh));
+        W = tt[0];
+        Sseg = tt[1];
+      }
+      // capacity: the reserved chain segments, and zone ids for at most W surviving new zones (the
+      // ids are handed out compactly after the merge; a chain entry holds an id below kWideTag)
+      if (zcount + (u64)W >= (u64)kWideTag) {
+        c->err = "more stored zones than zone ids";
+        return HORA_HIP_E_MEMORY;
+      }
+      if (zcount + (u64)W > P.zcap) c->zarena.grow_keep((size_t)(zcount + (u64)W) * d2, c->stream);
+      if (ncount + (u64)Sseg > P.ncap) c->nodes.grow_keep((size_t)(ncount + (u64)Sseg) * 2, c->stream);
+      P = passed_view(c);
+      hipLaunchKernelGGL(k_touched_offsets, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
+                         c->pos.p, c->pos2.p, ncount);
+      c->wlist.ensure(W + 1);
+      hipLaunchKernelGGL(k_scatter, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->gbk.p, c->covered.p,
+                         c->gticket.p, G, c->wlist.p);
+      // ---- ordered merge of the kept successors: light keys, then heavy keys; surviving new zones
+      // wait in a scratch slot per work-list position
+      c->zscr.ensure((size_t)std::max<long long>(W, 1) * d2);
+      c->zused.ensure(W + 1);
+      HIP_OK(hipMemsetAsync(c->zused.p, 0, (size_t)(W + 1) * sizeof(int), c->stream));
+      const size_t lds_l = merge_layout(d2, false).total() * sizeof(int);
+      const size_t lds_h = merge_layout(d2, true).total() * sizeof(int);
+      // zones this level stored in earlier chunks, which this chunk's merge may retire
+      int* zdead = ci > 0 ? c->zdead.p : nullptr;
+      HIP_OK(hipEventRecord(c->ev[3], c->stream));
+      // heavy keys: work lists sorted, then the pair pre-pass over each key's successors
+      const int pre = (c->pairs && nlh[2] > 0) ? 1 : 0;
+      if (pre) {
+        c->pdrop.ensure(W + 1);
+        c->kcnt.ensure(W + 1);
+        c->mpos.ensure(W + 1);
+        c->kslot.ensure((size_t)(W + 1) * kKillSlots);
+        c->tcnt.ensure(nlh[1]);
+        c->toff.ensure(nlh[1] + 1);
+        HIP_OK(hipMemsetAsync(c->pdrop.p, 0, (size_t)(W + 1) * sizeof(int), c->stream));
+        HIP_OK(hipMemsetAsync(c->kcnt.p, 0, (size_t)(W + 1) * sizeof(int), c->stream));
+        hipLaunchKernelGGL(k_presort, dim3(nlh[1]), dim3(1024), 0, c->stream, P, c->kheavy.p, nlh[1], c->wlist.p);
+        hipLaunchKernelGGL(k_pair_tiles, dim3(blocks_for(nlh[1], 256)), dim3(256), 0, c->stream, P, c->kheavy.p,
+                           nlh[1], c->d, c->tcnt.p);
+        scan_launch(c, c->tcnt.p, nlh[1], c->toff.p, 0);
+        const size_t zs = (size_t)(d2 | 1);
+        const size_t lds_p = (2 * kPairTile * zs + 2 * HORA_MAX_DIM + HORA_MAX_PROCS + HORA_MAX_INTS) * sizeof(int);
+        if (lds_p > 64 * 1024)
+          HIP_OK(hipFuncSetAttribute((const void*)k_pairs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_p));
+        hipLaunchKernelGGL(k_pairs, dim3(4096), dim3(256), lds_p, c->stream, c->dnp, P, c->kheavy.p, nlh[1], c->toff.p,
+                           c->scan_total.p, c->wlist.p, c->sj.p, c->s_zone.p, c->pdrop.p, c->kcnt.p, c->kslot.p, 0,
+                           nullptr, nullptr, nullptr);
+        // kept successors with more kills than slots: their full lists
+        c->kneed.ensure(W + 1);
+        c->koff.ensure(W + 1);
+        hipLaunchKernelGGL(k_kill_need, dim3(blocks_for(W, 256)), dim3(256), 0, c->stream, c->pdrop.p, c->kcnt.p,
+                           (long)W, c->kneed.p);
+        scan_launch(c, c->kneed.p, W, c->koff.p, 1);
+        long long K = 0;
+        read_dev(c, &K, c->scan_total.p + 1, sizeof(K));
+        if (K > 0) {
+          c->klist.ensure((size_t)K);
+          c->kpos.ensure(W + 1);
+          HIP_OK(hipMemsetAsync(c->kpos.p, 0, (size_t)(W + 1) * sizeof(int), c->stream));
+          hipLaunchKernelGGL(k_pairs, dim3(4096), dim3(256), lds_p, c->stream, c->dnp, P, c->kheavy.p, nlh[1],
+                             c->toff.p, c->scan_total.p, c->wlist.p, c->sj.p, c->s_zone.p, c->pdrop.p, c->kcnt.p,
+                             c->kslot.p, 1, c->koff.p, c->kpos.p, c->klist.p);
+        }
+      }
+      HIP_OK(hipEventRecord(c->ev[7], c->stream));
+      if (nlh[0] > 0)
+        hipLaunchKernelGGL(k_merge, dim3(nlh[0]), dim3(64), lds_l, c->stream, c->dnp, P, c->klight.p, nlh[0],
+                           c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 0, c->zscr.p, c->zused.p,
+                           c->flag_i.p, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, zdead, lvl_zbase);
+      if (nlh[1] > 0)
+        hipLaunchKernelGGL(k_merge, dim3(nlh[1]), dim3(64), lds_h, c->stream, c->dnp, P, c->kheavy.p, nlh[1],
+                           c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 1, c->zscr.p, c->zused.p,
+                           c->flag_i.p, pre, c->pdrop.p, c->kcnt.p, c->kslot.p, c->koff.p, c->klist.p, c->mpos.p,
+                           zdead, lvl_zbase);
+      HIP_OK(hipEventRecord(c->ev[4], c->stream));
+      ncount += (u64)Sseg;
+      // zone ids: the surviving new zones of the chunk, compact, in slot order (the count is read
+      // with the frontier's scan below)
+      c->zrank.ensure(W + 1);
+      scan_launch(c, c->zused.p, W, c->zrank.p, 1);
+      const u64 zbase = zcount;
+      if (T > 0)
+        hipLaunchKernelGGL(k_zassign, dim3((unsigned)T), dim3(64), 0, c->stream, P, c->touched.p, T, c->zrank.p,
+                           c->zscr.p, zbase, d2);
+      if (found) {
+        hc = read_counters(c);
+        account(c, hc, prev, G, T, 0);
+        if (hc.err) { c->err = "device capacity exceeded"; return HORA_HIP_E_MEMORY; }
+        prev = hc;
+        break;
+      }
+      // ---- the chunk's survivors join the next frontier
+      c->pos.ensure(G + 1);
+      long long nsurv = 0;
+      const long F2c = scan_and_counters(c, c->flag_i.p, G, c->pos.p, &hc, &nsurv);
+      zcount += (u64)nsurv;
+      c->fr_dk2.grow_keep(F2 + F2c + 1, c->stream);
+      c->fr_zid2.grow_keep(F2 + F2c + 1, c->stream);
+      hipLaunchKernelGGL(k_build_frontier, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, c->survived.p, c->pos.p,
+                         G, c->sj.p, c->s_dk.p, c->fr_dk2.p + F2, c->fr_zid2.p + F2, c->zrank.p, zbase);
+      F2 += F2c;
+      if (chunked) {
+        // retirement marks for the zones stored so far this level, cleared as they are added
+        const size_t need = (size_t)(zcount - lvl_zbase) + 1;
+        c->zdead.grow_keep(need, c->stream);
+        HIP_OK(hipMemsetAsync(c->zdead.p + zdead_n, 0, (need - zdead_n) * sizeof(int), c->stream));
+        zdead_n = need;
+      }
+      account(c, hc, prev, G, T, F2c);
+      if (c->trace)
+        std::fprintf(stderr, "level %d chunk %zu/%zu F=%ld R=%ld G=%ld probe_cov=%llu T=%d (light %d heavy %d) W=%lld F2=%ld | exp %.3f probe %.3f pairs %.3f merge %.3f ms | reads p=%llu m=%llu unions=%llu full=%llu wovf=%llu\n",
+                     level, ci + 1, cuts.size() - 1, Fc, Rc, G, hc[C_PSUB] - prev[C_PSUB], T, nlh[0], nlh[1], W, F2c,
+                     ev_ms(c->ev[0], c->ev[1]), ev_ms(c->ev[5], c->ev[6]), ev_ms(c->ev[3], c->ev[7]),
+                     ev_ms(c->ev[7], c->ev[4]), hc[C_PREADS] - prev[C_PREADS], hc[C_READS] - prev[C_READS],
+                     hc[C_UNI] - prev[C_UNI], hc[C_FULL] - prev[C_FULL], hc[C_WOVF] - prev[C_WOVF]);
+      prev = hc;
+      if (hc.err) {
+        c->err = hc.err == ERR_HASH_FULL ? "passed-list hash table full"
+                 : hc.err == ERR_ZONES_FULL ? "zone arena full"
+                 : hc.err == ERR_RECIPES ? "a state has more transitions or invariant constraints than the device holds"
+                                         : "chain node arena full";
+        return HORA_HIP_E_MEMORY;
       }
     }
-    HIP_OK(hipEventRecord(c->ev[7], c->stream));
-    if (nlh[0] > 0)
-      hipLaunchKernelGGL(k_merge, dim3(nlh[0]), dim3(64), lds_l, c->stream, c->dnp, P, c->klight.p, nlh[0],
-                         c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 0, c->zscr.p, c->zused.p,
-                         c->flag_i.p, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
-    if (nlh[1] > 0)
-      hipLaunchKernelGGL(k_merge, dim3(nlh[1]), dim3(64), lds_h, c->stream, c->dnp, P, c->kheavy.p, nlh[1],
-                         c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 1, c->zscr.p, c->zused.p,
-                         c->flag_i.p, pre, c->pdrop.p, c->kcnt.p, c->kslot.p, c->koff.p, c->klist.p, c->mpos.p);
-    HIP_OK(hipEventRecord(c->ev[4], c->stream));
-    ncount += (u64)Sseg;
-    // zone ids: the surviving new zones of the level, compact, in slot order (the count is read
-    // with the frontier's scan below)
-    c->zrank.ensure(W + 1);
-    scan_launch(c, c->zused.p, W, c->zrank.p, 1);
-    const u64 zbase = zcount;
-    if (T > 0)
-      hipLaunchKernelGGL(k_zassign, dim3((unsigned)T), dim3(64), 0, c->stream, P, c->touched.p, T, c->zrank.p,
-                         c->zscr.p, zbase, d2);
-    if (found) {
-      hc = read_counters(c);
-      account(c, hc, prev, G, T, 0);
-      if (hc.err) { c->err = "device capacity exceeded"; return HORA_HIP_E_MEMORY; }
-      prev = hc;
-      break;
+    if (found) break;
+    if (chunked && F2 > 0) {
+      // drop the frontier entries a later chunk retired, keeping storing order
+      c->flag_i.ensure(F2);
+      c->pos.ensure(F2 + 1);
+      hipLaunchKernelGGL(k_frontier_live, dim3(blocks_for(F2, 256)), dim3(256), 0, c->stream, c->fr_zid2.p, F2,
+                         c->zdead.p, lvl_zbase, c->flag_i.p);
+      const long keep = scan_exclusive(c, c->flag_i.p, F2, c->pos.p);
+      c->fr_dk.ensure(keep + 1);
+      c->fr_zid.ensure(keep + 1);
+      hipLaunchKernelGGL(k_frontier_compact, dim3(blocks_for(F2, 256)), dim3(256), 0, c->stream, c->fr_dk2.p,
+                         c->fr_zid2.p, F2, c->flag_i.p, c->pos.p, c->fr_dk.p, c->fr_zid.p);
+      F = keep;
+    } else {
+      std::swap(c->fr_dk.p, c->fr_dk2.p); std::swap(c->fr_dk.cap, c->fr_dk2.cap);
+      std::swap(c->fr_zid.p, c->fr_zid2.p); std::swap(c->fr_zid.cap, c->fr_zid2.cap);
+      F = F2;
     }
-    // ---- next frontier
-    c->pos.ensure(G + 1);
-    long long nsurv = 0;
-    const long F2 = scan_and_counters(c, c->flag_i.p, G, c->pos.p, &hc, &nsurv);
-    zcount += (u64)nsurv;
-    c->fr_dk2.ensure(F2);
-    c->fr_zid2.ensure(F2);
-    hipLaunchKernelGGL(k_build_frontier, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, c->survived.p, c->pos.p, G,
-                       c->sj.p, c->s_dk.p, c->fr_dk2.p, c->fr_zid2.p, c->zrank.p, zbase);
-    account(c, hc, prev, G, T, F2);
-    if (c->trace)
-      std::fprintf(stderr, "level %d F=%ld R=%ld G=%ld probe_cov=%llu T=%d (light %d heavy %d) W=%lld F2=%ld | exp %.3f probe %.3f pairs %.3f merge %.3f ms | reads p=%llu m=%llu unions=%llu full=%llu wovf=%llu\n",
-                   level, F, R, G, hc[C_PSUB] - prev[C_PSUB], T, nlh[0], nlh[1], W, F2, ev_ms(c->ev[0], c->ev[1]),
-                   ev_ms(c->ev[5], c->ev[6]), ev_ms(c->ev[3], c->ev[7]), ev_ms(c->ev[7], c->ev[4]),
-                   hc[C_PREADS] - prev[C_PREADS],
-                   hc[C_READS] - prev[C_READS], hc[C_UNI] - prev[C_UNI], hc[C_FULL] - prev[C_FULL],
-                   hc[C_WOVF] - prev[C_WOVF]);
-    prev = hc;
-    if (hc.err) {
-      c->err = hc.err == ERR_HASH_FULL ? "passed-list hash table full"
-               : hc.err == ERR_ZONES_FULL ? "zone arena full"
-               : hc.err == ERR_RECIPES ? "a state has more transitions or invariant constraints than the device holds"
-                                       : "chain node arena full";
-      return HORA_HIP_E_MEMORY;
-    }
-    std::swap(c->fr_dk.p, c->fr_dk2.p); std::swap(c->fr_dk.cap, c->fr_dk2.cap);
-    std::swap(c->fr_zid.p, c->fr_zid2.p); std::swap(c->fr_zid.cap, c->fr_zid2.cap);
-    F = F2;
     init = false;
     // adaptive Extra_LU+ switch (engine.cpp:2178-2184)
-    if (!c->dn.lu_plus && hc[C_KEYS] && (double)hc[C_LIVE] >= 4.0 * (double)hc[C_KEYS]) {
+    if (!c->dn.lu_plus && prev[C_KEYS] && (double)prev[C_LIVE] >= 4.0 * (double)prev[C_KEYS]) {
       c->dn.lu_plus = 1;
       HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
       HIP_OK(hipStreamSynchronize(c->stream));

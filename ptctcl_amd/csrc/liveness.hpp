@@ -1527,7 +1527,7 @@ int construct_online(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, ho
     L.s_se.ensure(R);
     L.s_re.ensure(R);
     ExpandOut eo{c->s_valid.p, c->s_dk.p, c->s_zone.p, c->s_tgt.p, L.s_se.p, L.s_re.p};
-    if (R > 0) launch_expand(c, expand_tuned(c, F, eo, init), F, eo, init);
+    if (R > 0) launch_expand(c, expand_tuned(c, F, 0, eo, init), F, 0, eo, init);
     c->pos.ensure(R + 1);
     const long long Gall = R > 0 ? scan_exclusive(c, c->s_valid.p, R, c->pos.p) : 0;
     {
@@ -2117,7 +2117,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       L.s_se.ensure(R);
       L.s_re.ensure(R);
       ExpandOut eo{c->s_valid.p, c->s_dk.p, c->s_zone.p, c->s_tgt.p, L.s_se.p, L.s_re.p};
-      if (R > 0) launch_expand(c, expand_tuned(c, F, eo, init), F, eo, init);
+      if (R > 0) launch_expand(c, expand_tuned(c, F, 0, eo, init), F, 0, eo, init);
       c->pos.ensure(R + 1);
       const long G = R > 0 ? scan_exclusive(c, c->s_valid.p, R, c->pos.p) : 0;
       if (!init) {

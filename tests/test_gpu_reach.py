@@ -39,15 +39,28 @@ def test_fischer_counts(N):
     assert not r and st.explored == GOLD["fischer_reach"][str(N)]["states"]
 
 
-@pytest.mark.parametrize("N", [10, 11, 12, 13])
+@pytest.mark.parametrize("N", [10, 11, 12, 13, 14, 15])
 def test_fischer_recorded_counts_at_size(N):
     """Past what the oracle finishes in seconds, the reference's recorded counts are the pin (its
-    16-thread runs take 1.6 s to 93 s for these sizes)."""
+    16-thread runs take 1.6 s to 1,358 s for these sizes). Fischer 15 stores 157,592,231 zones of
+    dimension 16 as dense int32 (about 161 GB of zones, 214 GiB of device memory in all): its
+    levels run in chunks and the zone arena grows by mapping memory, never by copying."""
     net = models.fischer(N)
     with hip.HipContext(net, models.mutex_target(net)) as ctx:
         r, st = ctx.reach()
     print(f"fischer{N}", r, st.explored, st.stored_live, flush=True)
     assert not r and st.explored == GOLD["fischer_reach"][str(N)]["states"]
+
+
+@pytest.mark.parametrize("N", [6, 8])
+def test_chunked_levels(N, monkeypatch):
+    """A level split into chunks of at most 2,000 successor slots, expanded and merged one after
+    another, is the sequential fold in the same order: every counter as the oracle's, on the
+    order-sensitive variant (zones an earlier chunk stored and a later one retires leave the next
+    frontier)."""
+    monkeypatch.setenv("HORA_HIP_CHUNK_SLOTS", "2000")
+    net = models.fischer_variant(N, k=N)
+    check(net, models.mutex_target(net))
 
 
 @pytest.mark.parametrize("N", [4, 5, 6, 7])
