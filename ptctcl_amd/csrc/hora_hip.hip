@@ -1462,18 +1462,21 @@ __host__ __device__ inline MergeLayout merge_layout(int d2, bool heavy) {
 // newest-first order as the sequential fold does. At the end the key's live new zones get zone
 // ids in one allocation, move to the arena, and are reported as this level's survivors.
 //
-// With `pre` (heavy keys), the work list arrives sorted, and for keys the pair pre-pass covered,
+// With PRE (heavy keys, when the pair pre-pass ran), the work list arrives sorted, and for keys the pair pre-pass covered,
 // its answers replace the two long scans: a successor covered by an earlier successor of the key
 // is dropped outright, and a kept successor tests coverage only against members that a union
 // widened (the others are previous-level zones, which k_probe cleared, or earlier successors as
 // they came out of the expansion, which k_pairs cleared); its kills are the recorded ones among
 // unwidened live members, plus the widened members that it covers now.
-// Five wavefronts per SIMD: the batched loads of the lane tests need the registers (no spills).
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_merge(const DevNet* __restrict__ Np, Passed P, const u32* __restrict__ list, int nlist,
+// Two instances: without the pair pre-pass (light keys, and heavy keys when it did not run) six
+// wavefronts per SIMD measured fastest; with it, five leave the batched lane tests (used there,
+// where chains are long and mostly outside the LDS cache) their registers.
+template <bool PRE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6))) void k_merge(const DevNet* __restrict__ Np, Passed P, const u32* __restrict__ list, int nlist,
                                               int* __restrict__ wlist, const int* __restrict__ sj,
                                               int* __restrict__ szone, int* __restrict__ survived,
                                               Counters* ctr, int heavy, int* __restrict__ zscr,
-                                              int* __restrict__ used, int* __restrict__ flag, int pre,
+                                              int* __restrict__ used, int* __restrict__ flag,
                                               const int* __restrict__ pdrop, const int* __restrict__ kcnt,
                                               const u32* __restrict__ kslot, const long long* __restrict__ koff,
                                               const u32* __restrict__ klist, int* __restrict__ mpos,
@@ -1504,7 +1507,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
   const u32 n = P.ncnt[h];
   int* lst = wlist + P.noff[h];
   // order the kept successors by compacted index = (state, recipe) order
-  if (pre) {
+  if (PRE) {
     // sorted by k_presort
     if (n <= (u32)ml.sort_cap) {
       for (int k = l; k < (int)n; k += kWave) sbuf[k] = lst[k];
@@ -1575,14 +1578,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
   }
   // cache slots are handed out upwards to new members until the cache is full
   int next_slot = ncached;
-  // new members are tagged with their position k in the sorted work list; their zones are written
-  // to the level's scratch slot of that position (noff + k) when stored
-  const u64 zbase_k = P.noff[h];
+  // new members are tagged with their position k in the sorted work list. A member's cache-slot
+  // entry is its LDS slot (>= 0), or -2 - j for a new member whose zone is in successor slot j (its
+  // own, rewritten there when a union widened it), or -1 for a stored zone read from the arena:
+  // no chain of dependent loads to find a zone.
   auto zp = [&](int pos) __attribute__((always_inline)) -> const int* {
     const int s = cs[pos];
     if (s >= 0) return cache + s * zs;
-    const u32 id = ch[pos] & ~kDead;
-    return (id & kNewTag) ? zscr + (size_t)(zbase_k + (id & kPosMask)) * d2 : P.zarena + (size_t)id * d2;
+    if (s <= -2) return szone + (size_t)(-(s + 2)) * d2;
+    return P.zarena + (size_t)(ch[pos] & ~kDead) * d2;
   };
   // a stored zone that an earlier chunk of this level stored, retired here, leaves the next frontier
   auto note_dead = [&](u32 id) __attribute__((always_inline)) {
@@ -1592,7 +1596,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
   unsigned long long n_sub = 0, n_kill = 0, n_uni = 0, n_reads = 0, n_ins = 0;
   // pair pre-pass answers: `use_pairs` for the drops (exact on their own), `pairs_full` while the
   // widened-member list is complete
-  const bool use_pairs = pre && pair_eligible(n, old_len, d);
+  const bool use_pairs = PRE && pair_eligible(n, old_len, d);
   bool pairs_full = use_pairs;
   unsigned long long n_full = 0, n_wovf = 0;
   int nw = 0;
@@ -1606,7 +1610,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
       ++n_sub;
       continue;
     }
-    if (pre && !pairs_full) ++n_full;
+    if (PRE && !pairs_full) ++n_full;
     const int g = lst[k];
     int* sz = szone + (size_t)sj[g] * d2;
     wsync();
@@ -1622,7 +1626,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
           bool c = false;
           if (b0 + l < nw) {
             const int m = wl[b0 + l];
-            if (!(ch[m] & kDead)) c = d <= 16 ? alu_le_lane16<false>(S, zp(m), d, Lb, Ub) : alu_le_lane(S, zp(m), d, Lb, Ub);
+            if (!(ch[m] & kDead)) c = (PRE && d <= 16) ? alu_le_lane16<false>(S, zp(m), d, Lb, Ub) : alu_le_lane(S, zp(m), d, Lb, Ub);
           }
           covered = __any(c);
         }
@@ -1638,7 +1642,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
           const int m = base - l;
           bool c = false;
           if (m >= 0 && !(ch[m] & kDead))
-            c = d <= 16 ? alu_le_lane16<false>(S, zp(m), d, Lb, Ub) : alu_le_lane(S, zp(m), d, Lb, Ub);
+            c = (PRE && d <= 16) ? alu_le_lane16<false>(S, zp(m), d, Lb, Ub) : alu_le_lane(S, zp(m), d, Lb, Ub);
           covered = __any(c);
         }
         n_reads += (unsigned long long)live;
@@ -1677,7 +1681,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
           int mw = -1;
           if (b0 + l < nw) {
             mw = wl[b0 + l];
-            if (!(ch[mw] & kDead)) kw = d <= 16 ? alu_le_lane16<true>(zp(mw), S, d, Lb, Ub) : alu_le_lane(zp(mw), S, d, Lb, Ub);
+            if (!(ch[mw] & kDead)) kw = (PRE && d <= 16) ? alu_le_lane16<true>(zp(mw), S, d, Lb, Ub) : alu_le_lane(zp(mw), S, d, Lb, Ub);
           }
           const u64 bw = __ballot(kw);
           wsync();
@@ -1711,7 +1715,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
           const int m = base - l;
           bool kill = false;
           if (m >= 0 && !(ch[m] & kDead))
-            kill = d <= 16 ? alu_le_lane16<true>(zp(m), S, d, Lb, Ub) : alu_le_lane(zp(m), S, d, Lb, Ub);
+            kill = (PRE && d <= 16) ? alu_le_lane16<true>(zp(m), S, d, Lb, Ub) : alu_le_lane(zp(m), S, d, Lb, Ub);
           const u64 b = __ballot(kill);
           wsync();
           if (kill) {
@@ -1773,20 +1777,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         }
       }
     }
-    // (5) store S as the newest member: in its scratch slot, and in a free cache slot when one is left
+    // (5) store S as the newest member: in a free cache slot when one is left, and back in its
+    // successor slot when a union widened it
     wsync();
     int slot = -1;
     if (next_slot < cz) slot = next_slot++;
     if (slot >= 0)
       for (int e = l; e < d2; e += kWave) cache[slot * zs + e] = S[e];
-    {
-      int* zdst = zscr + (size_t)(zbase_k + k) * d2;
-      for (int e = l; e < d2; e += kWave) zdst[e] = S[e];
-      if (slot < 0) bsync();   // the slot is read back by other lanes when the zone is not cached
+    if (widened) {
+      for (int e = l; e < d2; e += kWave) sz[e] = S[e];
+      bsync();   // the slot is read back by other lanes when the zone is not cached
     }
     if (l == 0) {
       ch[len] = kNewTag | k | ((use_pairs && widened) ? kWideTag : 0u);
-      cs[len] = slot;
+      cs[len] = slot >= 0 ? slot : -2 - sj[g];
       if (use_pairs) mpos[pbase + k] = len;
     }
     if (pairs_full && widened) {
@@ -1818,7 +1822,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     ++n_ins;
     csync();
   }
-  // Live new zones are in the level's scratch slot of their work-list position (noff + k), named
+  // Live new zones go to the level's scratch slot of their work-list position (noff + k), named
   // kNewTag | slot in the chain and in `survived` until k_zassign gives them compact zone ids; the
   // chain is compacted oldest first into this key's reserved segment.
   wsync();
@@ -1831,6 +1835,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     if (id & kNewTag) {
       const u32 kpos = id & kPosMask;
       const u32 slot = sk + kpos;
+      const int* src = zp(m);
+      int* dst = zscr + (size_t)slot * d2;
+      for (int e = l; e < d2; e += kWave) dst[e] = src[e];
       wsync();
       if (l == 0) {
         out[w] = kNewTag | slot;
@@ -3082,14 +3089,14 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
       }
       HIP_OK(hipEventRecord(c->ev[7], c->stream));
       if (nlh[0] > 0)
-        hipLaunchKernelGGL(k_merge, dim3(nlh[0]), dim3(64), lds_l, c->stream, c->dnp, P, c->klight.p, nlh[0],
+        hipLaunchKernelGGL(k_merge<false>, dim3(nlh[0]), dim3(64), lds_l, c->stream, c->dnp, P, c->klight.p, nlh[0],
                            c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 0, c->zscr.p, c->zused.p,
-                           c->flag_i.p, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, zdead, lvl_zbase);
+                           c->flag_i.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, zdead, lvl_zbase);
       if (nlh[1] > 0)
-        hipLaunchKernelGGL(k_merge, dim3(nlh[1]), dim3(64), lds_h, c->stream, c->dnp, P, c->kheavy.p, nlh[1],
-                           c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 1, c->zscr.p, c->zused.p,
-                           c->flag_i.p, pre, c->pdrop.p, c->kcnt.p, c->kslot.p, c->koff.p, c->klist.p, c->mpos.p,
-                           zdead, lvl_zbase);
+        hipLaunchKernelGGL(pre ? k_merge<true> : k_merge<false>, dim3(nlh[1]), dim3(64), lds_h, c->stream, c->dnp, P,
+                           c->kheavy.p, nlh[1], c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 1,
+                           c->zscr.p, c->zused.p, c->flag_i.p, c->pdrop.p, c->kcnt.p, c->kslot.p, c->koff.p,
+                           c->klist.p, c->mpos.p, zdead, lvl_zbase);
       HIP_OK(hipEventRecord(c->ev[4], c->stream));
       ncount += (u64)Sseg;
       // zone ids: the surviving new zones of the chunk, compact, in slot order (the count is read
