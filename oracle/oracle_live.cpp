@@ -12,19 +12,33 @@
 //                    An edge records (sender or local edge, receiver edge, or -2 - c for
 //                    combination c of a broadcast). No extrapolation on this lane
 //                    (emode = ExNone, src/liveness/engine.cpp:256-276).
-//   graph            LivenessEngine::intern / expand, src/liveness/construction.cpp:863-1086,
-//                    on its append-only branch: a successor goes to the first (oldest) stored node
-//                    of its discrete state that covers it, else it becomes a new node. The
-//                    in-place enlargement and convex-union rewrites of intern() are not taken:
-//                    the reference names append-only storage as the alternative they trade
-//                    against (construction.cpp:855-861); it stores more nodes and decides the same.
-//   covering         a_LU with per-location bounds raised to M = max(L, U) on every clock, i.e.
-//                    the a_M bisimulation the reference uses where a deadlock must be preserved
-//                    (construction.cpp:899-913, Behrmann-Bouyer-Larsen-Pelanek STTT 2006 L1/L3).
-//   order            level-synchronous: the frontier of a level is the nodes the previous level
-//                    created, in creation order; successors are processed in (frontier index,
-//                    transition index) order, so node ids are creation order. The reference's
-//                    worklist is a FIFO over the same graph (src/liveness/api.cpp:37-117).
+//   graph            three constructions, chosen as decide_leads_to / the deadlock and E[] entry
+//                    points choose them (src/liveness/api.cpp:357-447):
+//                    * online (the default): LivenessEngine::online_base_forward with online_expand
+//                      and online_intern (src/liveness/refinement.cpp:935-1065, :610-781;
+//                      construction.cpp:480-683) for a map D without atoms: a successor goes to the
+//                      newest active record of its key that a_LU-covers it, else it is joined with
+//                      the active records whose union with it is one zone and appended, retiring
+//                      the active records it covers; a record retired before its turn is not
+//                      expanded; edges to unexpanded records follow the retirements
+//                      (online_finalize_graph, refinement.cpp:1071-1160). `build_online`.
+//                    * base, for p --> q where D is provably a_LU (structurally_empty_d,
+//                      src/liveness/internal.cpp:59-103): LivenessEngine::intern
+//                      (construction.cpp:863-1003): oldest cover, else in-place enlargement of the
+//                      oldest covered node, else convex union, else append; a node rewritten by
+//                      another node's successor is expanded again. `build_base`.
+//                    * the a_M graph, when a reachable discrete state can seed a deadlock atom
+//                      (online_seed_key, refinement.cpp:176-282) or when asked for: a successor goes
+//                      to the first (oldest) node of its key that covers it under a_M (per-location
+//                      bounds raised to M = max(L, U) on every clock, a bisimulation, so deadlocks
+//                      are preserved; construction.cpp:899-913), else it becomes a new node. `build`.
+//                    The reference refines a_LU at the seeded keys only (revocable covers, deferred
+//                    recipes); that refinement is not restated, so a seeded network gets the a_M
+//                    graph's larger node count and the same verdict.
+//   order            a FIFO of records, run level by level: the frontier of a level is the records
+//                    the previous level created that are still active, in creation order; successors
+//                    are interned in (frontier index, transition index) order, so record ids are
+//                    creation order (the reference's FIFO over the same records, api.cpp:37-117).
 //   stop set         stop_set, construction.cpp:1200-1238; time_diverges :1097-1162;
 //                    orig_time_open :1164-1191; state_has_deadlock,
 //                    src/reachability/deadlock.cpp:74-268.

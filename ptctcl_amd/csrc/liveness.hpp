@@ -3,12 +3,22 @@
 // expansion kernels, passed-list hash table and scans).
 //
 // Forward: the zone graph is built level by level on the device with the expansion pipeline of
-// the reachability search, with no extrapolation and with every transition recorded. Each
-// successor goes to the first (oldest) stored node of its discrete state that covers it under a_M
-// (a_LU with per-location bounds raised to M = max(L, U): a bisimulation, so deadlocks survive the
-// quotient; src/liveness/construction.cpp:899-913), else it becomes a new node: the append-only
-// branch of LivenessEngine::intern (construction.cpp:863-1003). Node ids are creation order:
-// frontier index, then transition index, exactly as a sequential pass makes them.
+// the reachability search, with no extrapolation and with every transition recorded. Two
+// constructions run here (the route is chosen as the reference's decide_leads_to does,
+// src/liveness/api.cpp:357-447):
+//   * the online construction (online.hpp): online_intern's storing rule (a_LU covers by the newest
+//     active record, convex unions, retirement of covered records, records retired before their
+//     turn not expanded; src/liveness/construction.cpp:480-683), the reference's default;
+//   * the a_M graph, when a reachable discrete state can seed a deadlock atom (online_seed_key,
+//     src/liveness/refinement.cpp:176-282) or when asked for: each successor goes to the first
+//     (oldest) stored node of its discrete state that covers it under a_M (a_LU with per-location
+//     bounds raised to M = max(L, U): a bisimulation, so deadlocks survive the quotient;
+//     construction.cpp:899-913), else it becomes a new node.
+// p --> q on a network where D is provably a_LU takes the reference's base construction
+// (construction.cpp:863-1003, with in-place enlargement), which this pass does not run: it returns
+// HORA_HIP_E_SCOPE, and the caller falls back.
+// Node ids are creation order: frontier index, then transition index, as a sequential pass makes
+// them. The a_M graph's kernels:
 //
 //   k_gfilter  E[]: only psi nodes are expanded (src/liveness/api.cpp:83-96)
 //   k_count / k_expand / scan     successors in (state, transition) order, transition recorded
