@@ -126,7 +126,7 @@ typedef struct hora_reach_stats {
   int32_t levels;
   int32_t found;        /* 1 if a target state was reached */
   int32_t emode_switch_level; /* BFS level at which Extra_LU+ switched on, or -1 */
-  int32_t pad;
+  int32_t zone_entry_bytes;   /* bytes per stored zone entry: 2 (int16 storage) or 4 */
   double seconds;       /* search wall time, excluding set-up */
 } hora_reach_stats;
 

@@ -58,6 +58,7 @@ constexpr int kLightMax = 8;   // working-chain size up to which a key takes the
 
 struct DevNet {
   int d, d2, np, ni, nclocks, lu_plus;
+  int znar;               // the search stores its zones narrow (int16); 0 outside the search
   const int* locbase;
   const int* loc_flags;
   const int* inv_off;
@@ -185,7 +186,7 @@ __device__ __forceinline__ void wave_stat(Counters* c, int k, unsigned long long
   if ((threadIdx.x & 63) == 0) stat_add(c, k, v);
 }
 
-enum { ERR_NONE = 0, ERR_HASH_FULL = 1, ERR_ZONES_FULL = 2, ERR_NODES_FULL = 3, ERR_RECIPES = 4 };
+enum { ERR_NONE = 0, ERR_HASH_FULL = 1, ERR_ZONES_FULL = 2, ERR_NODES_FULL = 3, ERR_RECIPES = 4, ERR_NARROW = 5 };
 constexpr int kInvCap = 128;      // invariant constraints of one location vector held in LDS
 
 // ---------------------------------------------------------------- discrete state helpers
@@ -807,6 +808,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   decode_key(N, dk, locs, ints);
   if (init) {
     for (int e = l; e < d2; e += kWave) src[e] = kLe0;
+  } else if (N.znar) {
+    const short* zs = (const short*)zarena + (size_t)fr_zid[i] * nstride(d2);
+    for (int e = l; e < d2; e += kWave) src[e] = widen16(zs[e]);
   } else {
     const int* zs = zarena + (size_t)fr_zid[i] * d2;
     for (int e = l; e < d2; e += kWave) src[e] = zs[e];
@@ -1119,11 +1123,29 @@ struct Passed {
   u64* noff;       // [H] offset of this level's work list
   u64* nseg;       // [H] chain segment reserved for this level
   u32* nodes;      // chain segments (zone ids, oldest first)
-  int* zarena;     // stored zones
+  int* zarena;     // stored zones: d2 int32 entries each, or nstride(d2) int16 entries when nar
   u64 hmask;
   int hshift;
+  int nar;         // stored zones are narrow (int16)
   u64 zcap, ncap;
 };
+
+// A stored zone as an operand: int32 entries, or narrow ones widened on load.
+template <bool NAR> struct StoredZone;
+template <> struct StoredZone<false> {
+  using T = const int*;
+  static __device__ __forceinline__ T at(const Passed& P, u64 zid, int d2) { return P.zarena + zid * (u64)d2; }
+};
+template <> struct StoredZone<true> {
+  using T = Z16;
+  static __device__ __forceinline__ T at(const Passed& P, u64 zid, int d2) {
+    return Z16{(const short*)P.zarena + zid * (u64)nstride(d2)};
+  }
+};
+// Entry e of stored zone zid, whichever the storage.
+__device__ __forceinline__ int stored_entry(const Passed& P, u64 zid, int e, int d2) {
+  return P.nar ? widen16(((const short*)P.zarena)[zid * (u64)nstride(d2) + e]) : P.zarena[zid * (u64)d2 + e];
+}
 
 __device__ __forceinline__ u64 key_mix(u64 dk) { return dk * 0x9e3779b97f4a7c15ULL; }
 
@@ -1172,6 +1194,7 @@ __global__ void k_find(Passed P, const int* __restrict__ sj, const u64* __restri
 // group walks the key's previous-level chain newest first.
 constexpr int kGroup = 16;
 
+template <bool NAR>
 __global__ __launch_bounds__(64) void k_probe(const DevNet* __restrict__ Np, Passed P, const long* __restrict__ gbk,
                                               const int* __restrict__ sj, const u64* __restrict__ sdk,
                                               const int* __restrict__ szone, long G, int* __restrict__ covered,
@@ -1215,7 +1238,7 @@ __global__ __launch_bounds__(64) void k_probe(const DevNet* __restrict__ Np, Pas
   const int* L = Lg[q];
   const int* U = Ug[q];
   for (int m = (int)len - 1; m >= 0; --m) {   // group-uniform loop
-    const int* M = P.zarena + (size_t)P.nodes[P.off[h] + m] * d2;
+    const auto M = StoredZone<NAR>::at(P, P.nodes[P.off[h] + m], d2);
     ++reads;
     bool wit = false;
     for (int e = t; e < d2; e += kGroup) {
@@ -1316,7 +1339,8 @@ constexpr u32 kWideTag = 0x20000000u;    // ... whose zone a union widened (set 
 constexpr u32 kPosMask = kWideTag - 1;   // work-list position of a new member
 
 // Lane-serial a_LU test (same predicate as z_alu_le, one lane per pair of zones): Z <=_aLU Zp.
-__device__ __forceinline__ bool alu_le_lane(const int* Z, const int* Zp, int d, const int* L, const int* U) {
+template <class ZA, class ZB>
+__device__ __forceinline__ bool alu_le_lane(ZA Z, ZB Zp, int d, const int* L, const int* U) {
   for (int x = 0; x < d; ++x) {
     const int ux = U[x];
     if (ux == kAluNegInf) continue;
@@ -1337,16 +1361,15 @@ __device__ __forceinline__ bool alu_le_lane(const int* Z, const int* Zp, int d, 
 // column of that zone (16 entries) is loaded before any entry of it is compared, so the lane waits
 // on memory once per column instead of once per entry. kLaneZ: the per-lane zone is Z (a kill
 // test), else Zp (a cover test).
-template <bool kLaneZ>
-__device__ __forceinline__ bool alu_le_lane16(const int* Z, const int* Zp, int d, const int* L, const int* U) {
-  const int* V = kLaneZ ? Z : Zp;
+template <bool kLaneZ, class ZA, class ZB>
+__device__ __forceinline__ bool alu_le_lane16(ZA Z, ZB Zp, int d, const int* L, const int* U) {
   for (int x = 0; x < d; ++x) {
     const int ux = U[x];
     if (ux == kAluNegInf) continue;
     if (!kLaneZ && Z[x] < r_le(-ux)) continue;
     int col[16];
 #pragma unroll
-    for (int y = 0; y < 16; ++y) col[y] = y < d ? V[y * d + x] : 0;
+    for (int y = 0; y < 16; ++y) col[y] = y < d ? (kLaneZ ? Z[y * d + x] : Zp[y * d + x]) : 0;
     const int z0x = kLaneZ ? col[0] : Z[x];
     if (kLaneZ && z0x < r_le(-ux)) continue;
 #pragma unroll
@@ -1365,7 +1388,8 @@ __device__ __forceinline__ bool alu_le_lane16(const int* Z, const int* Zp, int d
 // Lane-serial convex-union test (the FA x PA formulation of z_try_union) for d <= 16: the entry
 // sets live in four 64-bit masks. A gap between the two zones' projections on one clock is checked
 // first: it makes the union non-convex, so it only shortens the answer.
-__device__ __forceinline__ bool union_convex_lane(const int* a, const int* b, int d) {
+template <class ZA, class ZB>
+__device__ __forceinline__ bool union_convex_lane(ZA a, ZB b, int d) {
   for (int i = 1; i < d; ++i) {
     const int bu = b[i * d], al = a[i], au = a[i * d], bl = b[i];
     if (bu < kInf && al < kInf && -r_bnd(al) > r_bnd(bu)) return false;
@@ -1392,14 +1416,14 @@ __device__ __forceinline__ bool union_convex_lane(const int* a, const int* b, in
   // pass 2: facets of b (PA) must land in a, checked on FA entries
 #pragma unroll
   for (int pass = 0; pass < 2; ++pass) {
-    const int* z = pass == 0 ? a : b;
-    const int* o = pass == 0 ? b : a;
+    auto z = [&](int e) __attribute__((always_inline)) { return pass == 0 ? a[e] : b[e]; };
+    auto o = [&](int e) __attribute__((always_inline)) { return pass == 0 ? b[e] : a[e]; };
 #pragma unroll
     for (int wf = 0; wf < 4; ++wf)
       for (u64 mf = pass == 0 ? fa[wf] : pa[wf]; mf; mf &= mf - 1) {
         const int f = wf * 64 + __builtin_ctzll(mf);
         const int i = ediv(f, d), j = f - i * d;
-        const int neg = r_neg(z[f]);
+        const int neg = r_neg(z(f));
 #pragma unroll
         for (int wp = 0; wp < 4; ++wp)
           for (u64 mp = pass == 0 ? pa[wp] : fa[wp]; mp; mp &= mp - 1) {
@@ -1407,7 +1431,7 @@ __device__ __forceinline__ bool union_convex_lane(const int* a, const int* b, in
             const int p = ediv(e, d), q = e - p * d;
             const int hpj = max(a[p * d + j], b[p * d + j]);
             const int hiq = max(a[i * d + q], b[i * d + q]);
-            if (radd(radd(hpj, neg), hiq) > o[e]) return false;
+            if (radd(radd(hpj, neg), hiq) > o(e)) return false;
           }
       }
   }
@@ -1471,7 +1495,7 @@ __host__ __device__ inline MergeLayout merge_layout(int d2, bool heavy) {
 // Two instances: without the pair pre-pass (light keys, and heavy keys when it did not run) six
 // wavefronts per SIMD measured fastest; with it, five leave the batched lane tests (used there,
 // where chains are long and mostly outside the LDS cache) their registers.
-template <bool PRE>
+template <bool PRE, bool NAR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6))) void k_merge(const DevNet* __restrict__ Np, Passed P, const u32* __restrict__ list, int nlist,
                                               int* __restrict__ wlist, const int* __restrict__ sj,
                                               int* __restrict__ szone, int* __restrict__ survived,
@@ -1573,7 +1597,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
   }
   csync();
   for (int k = 0; k < ncached; ++k) {
-    const int* src = P.zarena + (size_t)ch[old_len - ncached + k] * d2;
+    const auto src = StoredZone<NAR>::at(P, ch[old_len - ncached + k], d2);
     for (int e = l; e < d2; e += kWave) cache[k * zs + e] = src[e];
   }
   // cache slots are handed out upwards to new members until the cache is full
@@ -1587,6 +1611,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
     if (s >= 0) return cache + s * zs;
     if (s <= -2) return szone + (size_t)(-(s + 2)) * d2;
     return P.zarena + (size_t)(ch[pos] & ~kDead) * d2;
+  };
+  // f applied to member pos's zone. With narrow storage an uncached stored member is read through
+  // Z16 and every other member as int32: a branch on the member's cache-slot entry (uniform for a
+  // wavefront-wide test; per lane for the lane-serial ones, which take no cross-lane step).
+  auto zwith = [&](int pos, auto&& f) __attribute__((always_inline)) {
+    if constexpr (NAR) {
+      if (cs[pos] == -1) return f(Z16{(const short*)P.zarena + (size_t)(ch[pos] & ~kDead) * nstride(d2)});
+    }
+    return f(zp(pos));
   };
   // a stored zone that an earlier chunk of this level stored, retired here, leaves the next frontier
   auto note_dead = [&](u32 id) __attribute__((always_inline)) {
@@ -1626,7 +1659,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
           bool c = false;
           if (b0 + l < nw) {
             const int m = wl[b0 + l];
-            if (!(ch[m] & kDead)) c = (PRE && d <= 16) ? alu_le_lane16<false>(S, zp(m), d, Lb, Ub) : alu_le_lane(S, zp(m), d, Lb, Ub);
+            if (!(ch[m] & kDead))
+              c = zwith(m, [&](auto Z) { return (PRE && d <= 16) ? alu_le_lane16<false>(S, Z, d, Lb, Ub) : alu_le_lane(S, Z, d, Lb, Ub); });
           }
           covered = __any(c);
         }
@@ -1635,14 +1669,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
         for (int m = len - 1; m >= 0 && !covered; --m) {
           if (ch[m] & kDead) continue;
           ++n_reads;
-          covered = z_alu_le(S, zp(m), d, Lb, Ub);
+          covered = zwith(m, [&](auto Z) { return z_alu_le(S, Z, d, Lb, Ub); });
         }
       } else {
         for (int base = len - 1; base >= 0 && !covered; base -= kWave) {
           const int m = base - l;
           bool c = false;
           if (m >= 0 && !(ch[m] & kDead))
-            c = (PRE && d <= 16) ? alu_le_lane16<false>(S, zp(m), d, Lb, Ub) : alu_le_lane(S, zp(m), d, Lb, Ub);
+            c = zwith(m, [&](auto Z) { return (PRE && d <= 16) ? alu_le_lane16<false>(S, Z, d, Lb, Ub) : alu_le_lane(S, Z, d, Lb, Ub); });
           covered = __any(c);
         }
         n_reads += (unsigned long long)live;
@@ -1681,7 +1715,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
           int mw = -1;
           if (b0 + l < nw) {
             mw = wl[b0 + l];
-            if (!(ch[mw] & kDead)) kw = (PRE && d <= 16) ? alu_le_lane16<true>(zp(mw), S, d, Lb, Ub) : alu_le_lane(zp(mw), S, d, Lb, Ub);
+            if (!(ch[mw] & kDead))
+              kw = zwith(mw, [&](auto Z) { return (PRE && d <= 16) ? alu_le_lane16<true>(Z, S, d, Lb, Ub) : alu_le_lane(Z, S, d, Lb, Ub); });
           }
           const u64 bw = __ballot(kw);
           wsync();
@@ -1699,7 +1734,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
           const u32 id = ch[m];
           if (id & kDead) continue;
           ++n_reads;
-          if (z_alu_le(zp(m), S, d, Lb, Ub)) {
+          if (zwith(m, [&](auto Z) { return z_alu_le(Z, S, d, Lb, Ub); })) {
             wsync();
             if (l == 0) {
               note_dead(id);
@@ -1715,7 +1750,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
           const int m = base - l;
           bool kill = false;
           if (m >= 0 && !(ch[m] & kDead))
-            kill = (PRE && d <= 16) ? alu_le_lane16<true>(zp(m), S, d, Lb, Ub) : alu_le_lane(zp(m), S, d, Lb, Ub);
+            kill = zwith(m, [&](auto Z) { return (PRE && d <= 16) ? alu_le_lane16<true>(Z, S, d, Lb, Ub) : alu_le_lane(Z, S, d, Lb, Ub); });
           const u64 b = __ballot(kill);
           wsync();
           if (kill) {
@@ -1746,19 +1781,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
         int win = -1;
         if (wide && d <= 16) {
           bool ok = false;
-          if (l < nc) ok = union_convex_lane(S, zp(cand[l]), d);
+          if (l < nc) ok = zwith(cand[l], [&](auto Z) { return union_convex_lane(S, Z, d); });
           const u64 b = __ballot(ok);
           if (b) win = __builtin_ctzll(b);
           if (win >= 0) {
-            const int* mz = zp(cand[win]);
             wsync();
-            for (int e = l; e < d2; e += kWave) S[e] = max(S[e], mz[e]);
+            zwith(cand[win], [&](auto Z) {
+              for (int e = l; e < d2; e += kWave) S[e] = max(S[e], Z[e]);
+              return 0;
+            });
           }
         } else {
           for (int c = 0; c < nc && win < 0; ++c) {
-            const int* mz = zp(cand[c]);
             wsync();
-            for (int e = l; e < d2; e += kWave) M[e] = mz[e];
+            zwith(cand[c], [&](auto Z) {
+              for (int e = l; e < d2; e += kWave) M[e] = Z[e];
+              return 0;
+            });
             if (z_try_union(S, M, d, fa, pa, misc)) win = c;
           }
         }
@@ -1835,7 +1874,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
     if (id & kNewTag) {
       const u32 kpos = id & kPosMask;
       const u32 slot = sk + kpos;
-      const int* src = zp(m);
+      const int* src = zp(m);   // a new member: cached or in its successor slot
       int* dst = zscr + (size_t)slot * d2;
       for (int e = l; e < d2; e += kWave) dst[e] = src[e];
       wsync();
@@ -2017,7 +2056,7 @@ __global__ __launch_bounds__(256) void k_pairs(const DevNet* __restrict__ Np, Pa
       const int j = rt * kPairTile + r, c = ct * kPairTile + r;
       Rz[r * zs + x] = j < n ? szone[(size_t)sj[lst[j]] * d2 + x] : 0;
       int v = 0;
-      if (!sib && c < old) v = P.zarena[(size_t)P.nodes[P.off[h] + c] * d2 + x];
+      if (!sib && c < old) v = stored_entry(P, P.nodes[P.off[h] + c], x, d2);
       if (sib && c < n) v = szone[(size_t)sj[lst[c]] * d2 + x];
       Cz[r * zs + x] = v;
     }
@@ -2090,25 +2129,68 @@ __global__ void k_classify(Passed P, const u32* __restrict__ touched, int T, u32
 // Zone ids for the level's surviving new zones, compact and in slot order: slot -> zbase + rank.
 // One wavefront per touched key walks its chain: each provisional entry kNewTag | slot gets its
 // zone id, and the zone moves from the scratch slot into the arena.
+// Narrow storage: an entry that does not fit 16 bits sets ERR_NARROW, and the caller searches
+// again with int32 storage.
 __global__ __launch_bounds__(64) void k_zassign(Passed P, const u32* __restrict__ touched, int T,
                                                 const long long* __restrict__ rank, const int* __restrict__ zscr,
-                                                u64 zbase, int d2) {
+                                                u64 zbase, int d2, Counters* ctr) {
   const int t = blockIdx.x;
   if (t >= T) return;
   const u32 h = touched[t];
   u32* c = P.nodes + P.off[h];
   const u32 len = P.len[h];
   const int l = threadIdx.x;
+  bool bad = false;
   for (u32 m = 0; m < len; ++m) {
     const u32 id = c[m];
     if (!(id & kNewTag)) continue;
     const u32 slot = id & (kNewTag - 1);
     const u64 zid = zbase + (u64)rank[slot];
     const int* src = zscr + (size_t)slot * d2;
-    int* dst = P.zarena + (size_t)zid * d2;
-    for (int e = l; e < d2; e += kWave) dst[e] = src[e];
+    if (P.nar) {
+      const int ns = nstride(d2);
+      short* dst = (short*)P.zarena + (size_t)zid * ns;
+      for (int e = l; e < ns; e += kWave) {
+        const int v = e < d2 ? src[e] : 0;
+        bad |= !fits16(v);
+        dst[e] = narrow16(v);
+      }
+    } else {
+      int* dst = P.zarena + (size_t)zid * d2;
+      for (int e = l; e < d2; e += kWave) dst[e] = src[e];
+    }
     if (l == 0) c[m] = (u32)zid;
   }
+  if (__any(bad) && l == 0) atomicExch(&ctr->err, ERR_NARROW);
+}
+
+// Stored zones [a, b) rewritten from int32 to narrow storage in place, one wavefront per zone.
+// Zone z's narrow copy starts at byte 2 * ns * z, below its int32 source at 4 * d2 * z; the
+// caller picks rounds in which no zone's copy reaches the source of another zone of the same
+// round or of a later one (2 * ns * b <= 4 * d2 * a), and sources of earlier rounds are consumed
+// (stream order). A copy may overlap its own source, so the zone is read whole before any write.
+__global__ __launch_bounds__(64) void k_narrow_arena(int* arena, u64 a, int d2, Counters* ctr) {
+  const u64 z = a + blockIdx.x;
+  const int l = threadIdx.x, ns = nstride(d2);
+  const int* src = arena + z * (u64)d2;
+  int v[HORA_MAX_DIM * HORA_MAX_DIM / kWave];
+#pragma unroll
+  for (int k = 0; k < HORA_MAX_DIM * HORA_MAX_DIM / kWave; ++k) {
+    const int e = l + k * kWave;
+    v[k] = e < d2 ? src[e] : 0;
+  }
+  __syncthreads();
+  short* dst = (short*)arena + z * (u64)ns;
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k < HORA_MAX_DIM * HORA_MAX_DIM / kWave; ++k) {
+    const int e = l + k * kWave;
+    if (e < ns) {
+      bad |= !fits16(v[k]);
+      dst[e] = narrow16(v[k]);
+    }
+  }
+  if (__any(bad) && l == 0) atomicExch(&ctr->err, ERR_NARROW);
 }
 
 // Chunk boundaries: cuts[k] = first frontier state whose slots start at or after k * budget
@@ -2627,6 +2709,8 @@ struct hora_hip_ctx {
   bool trace = false;
   hipEvent_t ev[8] = {};
   LiveState* live = nullptr;
+  bool znar_ok = false;       // zones may be stored narrow (int16): constants small, no overflow seen
+  size_t znar_at = 0;         // int32 arena bytes past which a search switches to narrow storage
   int expand_wpb = 0;         // wavefronts per k_expand workgroup, picked by expand_tuned()
   int expand_wpb_d = 0;       // the zone dimension it was picked for
   void* hpin = nullptr;       // pinned host staging for the small counter reads of every level
@@ -2696,9 +2780,15 @@ Passed passed_view(hora_hip_ctx* c) {
   P.zarena = c->zarena.p;
   P.hmask = c->H - 1;
   P.hshift = 64 - c->hbits;
-  P.zcap = c->zarena.cap / (size_t)c->d2;
+  P.nar = c->dn.znar;
+  P.zcap = P.nar ? c->zarena.cap * 2 / (size_t)nstride(c->d2) : c->zarena.cap / (size_t)c->d2;
   P.ncap = c->nodes.cap;
   return P;
+}
+
+// ints of arena storage for `zones` stored zones in the search's current storage
+size_t arena_ints(const hora_hip_ctx* c, u64 zones) {
+  return c->dn.znar ? (size_t)(zones * (u64)nstride(c->d2) / 2) : (size_t)(zones * (u64)c->d2);
 }
 
 void alloc_table(hora_hip_ctx* c, int hbits) {
@@ -2868,21 +2958,42 @@ std::vector<long> chunk_cuts(hora_hip_ctx* c, long F, long R, long budget) {
   return cuts;
 }
 
-int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
+// One search; kNarrowRetry when a stored zone did not fit narrow storage.
+constexpr int kNarrowRetry = 1000;
+
+// Switch the search to narrow storage: stored zones [0, n) rewritten in place, in rounds
+// [a, b) with 2 * ns * b <= 4 * d2 * a (k_narrow_arena), then the device tables told.
+void narrow_arena(hora_hip_ctx* c, u64 n) {
+  const int d2 = c->d2, ns = nstride(d2);
+  for (u64 a = 0; a < n;) {
+    const u64 b = std::min<u64>(n, std::max<u64>(a + 1, (u64)(4 * d2) * a / (u64)(2 * ns)));
+    for (u64 lo = a; lo < b; lo += (1u << 30)) {
+      const u64 cnt = std::min<u64>(b - lo, 1u << 30);
+      hipLaunchKernelGGL(k_narrow_arena, dim3((unsigned)cnt), dim3(64), 0, c->stream, c->zarena.p, lo, d2, c->ctr.p);
+    }
+    a = b;
+  }
+  c->dn.znar = 1;
+  HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
+}
+
+int search_once(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st, int nar) {
   const auto t0 = std::chrono::steady_clock::now();
   std::memset(st, 0, sizeof(*st));
   st->emode_switch_level = -1;
+  st->zone_entry_bytes = nar ? 2 : 4;
   *reachable = 0;
   c->prof = hora_reach_profile{};
   Tally prev;
   const int d2 = c->d2;
   c->dn.lu_plus = 0;
+  c->dn.znar = nar;
   HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
 
   // passed list and arenas; a context searched before starts at the table size it grew to (slot
   // positions do not affect any result: work lists, frontiers and counts follow successor order)
   alloc_table(c, std::max(20, c->hbits));
-  c->zarena.ensure((size_t)d2 << 20);
+  c->zarena.ensure(arena_ints(c, (u64)1 << 20));
   c->nodes.ensure(1 << 22);
   c->ctr.ensure(1);
   HIP_OK(hipMemsetAsync(c->ctr.p, 0, sizeof(Counters), c->stream));
@@ -2995,8 +3106,8 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
       hipLaunchKernelGGL(k_find, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->sj.p, c->s_dk.p, G,
                          c->gbk.p, c->ctr.p, c->survived.p, c->flag_i.p);
       HIP_OK(hipEventRecord(c->ev[5], c->stream));
-      hipLaunchKernelGGL(k_probe, dim3(blocks_for(G, 4)), dim3(64), 0, c->stream, c->dnp, P, c->gbk.p, c->sj.p,
-                         c->s_dk.p, c->s_zone.p, G, c->covered.p, c->ctr.p);
+      hipLaunchKernelGGL(P.nar ? k_probe<true> : k_probe<false>, dim3(blocks_for(G, 4)), dim3(64), 0, c->stream,
+                         c->dnp, P, c->gbk.p, c->sj.p, c->s_dk.p, c->s_zone.p, G, c->covered.p, c->ctr.p);
       HIP_OK(hipEventRecord(c->ev[6], c->stream));
       hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 1024)), dim3(1024), 0, c->stream, P, c->gbk.p, c->covered.p, G,
                          c->gticket.p, c->touched.p, c->ctr.p);
@@ -3030,7 +3141,11 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
         c->err = "more stored zones than zone ids";
         return HORA_HIP_E_MEMORY;
       }
-      if (zcount + (u64)W > P.zcap) c->zarena.grow_keep((size_t)(zcount + (u64)W) * d2, c->stream);
+      if (!c->dn.znar && c->znar_ok && (zcount + (u64)W) * (u64)d2 * 4 > c->znar_at) {
+        narrow_arena(c, zcount);
+        P = passed_view(c);
+      }
+      if (zcount + (u64)W > P.zcap) c->zarena.grow_keep(arena_ints(c, zcount + (u64)W), c->stream);
       if (ncount + (u64)Sseg > P.ncap) c->nodes.grow_keep((size_t)(ncount + (u64)Sseg) * 2, c->stream);
       P = passed_view(c);
       hipLaunchKernelGGL(k_touched_offsets, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
@@ -3088,12 +3203,14 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
         }
       }
       HIP_OK(hipEventRecord(c->ev[7], c->stream));
+      const auto merge_plain = P.nar ? k_merge<false, true> : k_merge<false, false>;
+      const auto merge_pre = P.nar ? k_merge<true, true> : k_merge<true, false>;
       if (nlh[0] > 0)
-        hipLaunchKernelGGL(k_merge<false>, dim3(nlh[0]), dim3(64), lds_l, c->stream, c->dnp, P, c->klight.p, nlh[0],
+        hipLaunchKernelGGL(merge_plain, dim3(nlh[0]), dim3(64), lds_l, c->stream, c->dnp, P, c->klight.p, nlh[0],
                            c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 0, c->zscr.p, c->zused.p,
                            c->flag_i.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, zdead, lvl_zbase);
       if (nlh[1] > 0)
-        hipLaunchKernelGGL(pre ? k_merge<true> : k_merge<false>, dim3(nlh[1]), dim3(64), lds_h, c->stream, c->dnp, P,
+        hipLaunchKernelGGL(pre ? merge_pre : merge_plain, dim3(nlh[1]), dim3(64), lds_h, c->stream, c->dnp, P,
                            c->kheavy.p, nlh[1], c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 1,
                            c->zscr.p, c->zused.p, c->flag_i.p, c->pdrop.p, c->kcnt.p, c->kslot.p, c->koff.p,
                            c->klist.p, c->mpos.p, zdead, lvl_zbase);
@@ -3106,10 +3223,11 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
       const u64 zbase = zcount;
       if (T > 0)
         hipLaunchKernelGGL(k_zassign, dim3((unsigned)T), dim3(64), 0, c->stream, P, c->touched.p, T, c->zrank.p,
-                           c->zscr.p, zbase, d2);
+                           c->zscr.p, zbase, d2, c->ctr.p);
       if (found) {
         hc = read_counters(c);
         account(c, hc, prev, G, T, 0);
+        if (hc.err == ERR_NARROW) return kNarrowRetry;
         if (hc.err) { c->err = "device capacity exceeded"; return HORA_HIP_E_MEMORY; }
         prev = hc;
         break;
@@ -3139,6 +3257,7 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
                      ev_ms(c->ev[7], c->ev[4]), hc[C_PREADS] - prev[C_PREADS], hc[C_READS] - prev[C_READS],
                      hc[C_UNI] - prev[C_UNI], hc[C_FULL] - prev[C_FULL], hc[C_WOVF] - prev[C_WOVF]);
       prev = hc;
+      if (hc.err == ERR_NARROW) return kNarrowRetry;
       if (hc.err) {
         c->err = hc.err == ERR_HASH_FULL ? "passed-list hash table full"
                  : hc.err == ERR_ZONES_FULL ? "zone arena full"
@@ -3184,8 +3303,26 @@ int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
   st->killed = hc[C_KILL];
   st->unions = hc[C_UNI];
   st->chain_reads = hc[C_READS] + hc[C_PREADS];
+  st->zone_entry_bytes = c->dn.znar ? 2 : 4;
   st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return 0;
+}
+
+// Stored zones start int32 and switch to narrow (int16) storage once the arena would pass
+// znar_at bytes, when the network's constants allow it (decided at bind): int32 reads cost the
+// ordered merge a little less, and narrow storage halves the search's largest structure. If a
+// zone does not fit 16 bits after all, the search runs again with int32 storage, and the context
+// keeps int32 from then on.
+int run_search(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st) {
+  const auto t0 = std::chrono::steady_clock::now();
+  int rc = search_once(c, reachable, st, (c->znar_ok && c->znar_at == 0) ? 1 : 0);
+  if (rc == kNarrowRetry) {
+    c->znar_ok = false;
+    rc = search_once(c, reachable, st, 0);
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  c->dn.znar = 0;
+  return rc;
 }
 
 template <class T>
@@ -3336,6 +3473,26 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
     }
   }
   Lx[0] = Ux[0] = 0;
+  {
+    // narrow zone storage when the largest clock constant of the model and the query is below
+    // 4000 (engine.cpp:1596-1611); the device still checks every entry it narrows
+    int maxc = 0;
+    auto take = [&](int raw) { if (raw < HORA_INF) maxc = std::max(maxc, std::abs(raw >> 1)); };
+    for (int k = 0; k < d; ++k) {
+      if (Lx[k] != HORA_ALU_NEG_INF) maxc = std::max(maxc, std::abs(Lx[k]));
+      if (Ux[k] != HORA_ALU_NEG_INF) maxc = std::max(maxc, std::abs(Ux[k]));
+    }
+    for (int k = 0; k < e->inv_off[c->tl]; ++k) take(e->inv_data[3 * k + 2]);
+    for (int k = 0; k < e->cg_off[c->ne]; ++k) take(e->cg_data[3 * k + 2]);
+    for (int k = 0; k < ncc; ++k) take(ccv[3 * k + 2]);
+    // HORA_HIP_NARROW=0 keeps int32; =1 narrows whatever the constants (an entry that does not
+    // fit then sends the search back to int32). HORA_HIP_NARROW_AT: the int32 arena size in
+    // bytes at which a search switches (0: narrow from the start), 8 GiB by default.
+    const char* nv = std::getenv("HORA_HIP_NARROW");
+    c->znar_ok = (nv && *nv == '1') || (maxc < 4000 && !(nv && *nv == '0'));
+    const char* na = std::getenv("HORA_HIP_NARROW_AT");
+    c->znar_at = na ? (size_t)std::strtoull(na, nullptr, 10) : (size_t(8) << 30);
+  }
 
   // edges leaving each global location, ascending global edge id (NProc::edges_of order)
   std::vector<int> le_off(c->tl + 1, 0), le_edges(std::max(c->ne, 1));

@@ -1465,6 +1465,7 @@ int construct_online(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, ho
   }
   HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
   c->dn.lu_plus = 0;
+  c->dn.znar = 0;
   HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
   alloc_table(c, std::max(16, c->hbits));
   c->ctr.ensure(1);
@@ -2069,6 +2070,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       st->seed_keys = seeds;
     }
     c->dn.lu_plus = 0;
+    c->dn.znar = 0;
     HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
     alloc_table(c, std::max(16, c->hbits));
     c->ctr.ensure(1);

@@ -68,6 +68,24 @@ __device__ __forceinline__ void z_copy(int* __restrict__ dst, const int* __restr
   for (int e = lane_id(); e < d2; e += kWave) dst[e] = src[e];
 }
 
+// Narrow (int16) stored zones. Every entry of a stored zone of a model whose clock constants are
+// small is a raw bound that fits 16 bits, or infinity (the narrowing rule of the reference's GPU
+// passed set, include/hora/detail/gpu_zone_intern.hpp:36-47, src/reachability/engine.cpp:1596-1611).
+// Infinity is stored as kInf16. A narrow zone takes nstride(d2) entries, an even number, so every
+// narrow zone starts on a 4-byte boundary and its last entry shares a dword only with padding.
+// Whoever narrows a zone checks each entry (fits16) instead of trusting the constants.
+constexpr int kInf16 = 0x7fff;
+__host__ __device__ __forceinline__ int nstride(int d2) { return (d2 + 1) & ~1; }
+__device__ __forceinline__ bool fits16(int r) { return r == kInf || (r >= -32768 && r < kInf16); }
+__device__ __forceinline__ short narrow16(int r) { return (short)(r == kInf ? kInf16 : r); }
+__device__ __forceinline__ int widen16(int h) { return h == kInf16 ? kInf : h; }
+
+// A narrow stored zone as an operand: entries widened on load.
+struct Z16 {
+  const short* p;
+  __device__ __forceinline__ int operator[](int e) const { return widen16(p[e]); }
+};
+
 // One pass z[i][j] = min(z[i][j], z[i][p] + z[p][j]) for a pivot p whose row and column the pass
 // leaves unchanged (z[p][p] = (<=, 0)), so lanes may read and write in any order. Branch-free: a
 // lane reads its three values, then writes the minimum (the same value where nothing changes).
@@ -161,8 +179,10 @@ __device__ __forceinline__ void z_extra_lu_plus(int* z, int d, const int* L, con
 // Z <=_aLU Zp (Herbreteau, Srivathsan, Walukiewicz, LICS 2012): Z is not simulated iff some pair
 // x != y has U_x, L_y finite, Z[0][x] >= (<=,-U_x), Zp[y][x] < Z[y][x] and
 // Zp[y][x] + (<,-L_y) < Z[0][x]. Lanes take the pairs (y, x) = entries of the matrix, so both
-// operands are read with unit stride; either operand may be in LDS or global memory.
-__device__ __forceinline__ bool z_alu_le(const int* Z, const int* Zp, int d, const int* L, const int* U) {
+// operands are read with unit stride; either operand may be in LDS or global memory (ZA, ZB: an
+// int pointer or a narrow zone, Z16).
+template <class ZA, class ZB>
+__device__ __forceinline__ bool z_alu_le(ZA Z, ZB Zp, int d, const int* L, const int* U) {
   wsync();
   const int l = lane_id(), d2 = d * d;
   bool wit = false;

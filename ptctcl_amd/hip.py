@@ -115,6 +115,7 @@ class ReachStats:
     levels: int = 0
     found: int = 0
     emode_switch_level: int = -1
+    zone_entry_bytes: int = 4
     seconds: float = 0.0
 
     @classmethod

@@ -335,12 +335,13 @@ class hora_reach_stats(C.Structure):
         ("stored_live", C.c_uint64), ("keys", C.c_uint64), ("inserted", C.c_uint64),
         ("subsumed", C.c_uint64), ("killed", C.c_uint64), ("unions", C.c_uint64),
         ("chain_reads", C.c_uint64),
-        ("levels", C.c_int32), ("found", C.c_int32), ("emode_switch_level", C.c_int32), ("pad", C.c_int32),
+        ("levels", C.c_int32), ("found", C.c_int32), ("emode_switch_level", C.c_int32),
+        ("zone_entry_bytes", C.c_int32),
         ("seconds", C.c_double),
     ]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class NetEnc:

@@ -111,3 +111,48 @@ def test_urgent_semantics(k):
     net, target, reachable, explored = urgent_models.ALL[k]()
     r, st = check(net, target)
     assert r == reachable and st.explored == explored
+
+
+@pytest.mark.parametrize("make", [lambda: models.fischer(8), lambda: models.fischer_variant(7, k=7)])
+@pytest.mark.parametrize("at", ["0", "100000", None])
+def test_narrow_storage(make, at, monkeypatch):
+    """Stored zones switch from int32 to int16 (the reference's narrowing rule for small constants,
+    engine.cpp:1596-1611) once the int32 arena would pass HORA_HIP_NARROW_AT bytes: from the start
+    (0), in place during the search (100,000 bytes: a few hundred zones), or never at these sizes
+    (the 8 GiB default). Every counter equals the oracle's in each case; the variant exercises the
+    heavy keys' pair pre-pass and the ordered merge reading narrow zones."""
+    if at is not None:
+        monkeypatch.setenv("HORA_HIP_NARROW_AT", at)
+    net = make()
+    _, st = check(net, models.mutex_target(net))
+    assert st.zone_entry_bytes == (4 if at is None else 2)
+
+
+def test_narrow_storage_off(monkeypatch):
+    monkeypatch.setenv("HORA_HIP_NARROW_AT", "0")
+    monkeypatch.setenv("HORA_HIP_NARROW", "0")
+    net = models.fischer(7)
+    _, st = check(net, models.mutex_target(net))
+    assert st.zone_entry_bytes == 4
+
+
+@pytest.mark.parametrize("at", ["0", "20000"])
+def test_narrow_overflow_searches_again_wide(at, monkeypatch):
+    """Constants of 20,000 do not fit 16 bits (raw bound 40,001). Unforced, such a model stays
+    int32. Forced narrow, the first such entry is found where it is narrowed (a zone stored narrow,
+    or the in-place rewrite of the arena), and the search runs again with int32 zones: every
+    counter as the oracle's, on that search and on the context's next one."""
+    net = models.fischer(5, k=20000)
+    target = models.mutex_target(net)
+    monkeypatch.setenv("HORA_HIP_NARROW_AT", at)
+    _, st = check(net, target)
+    assert st.zone_entry_bytes == 4
+    monkeypatch.setenv("HORA_HIP_NARROW", "1")
+    with hip.HipContext(net, target) as ctx:
+        r1, st1 = ctx.reach()
+        r2, st2 = ctx.reach()
+    r_o, st_o = binding.reach(net, target, 0)
+    for r, s in ((r1, st1), (r2, st2)):
+        assert r == r_o and s.zone_entry_bytes == 4
+        for k in KEYS:
+            assert getattr(s, k) == st_o[k], k
