@@ -411,7 +411,9 @@ def main() -> int:
         "dtype": "int32",
         "data": "synthetic (seeded generated model; no external data)",
         "config": dict(cfg, explored=st.explored, generated=st.generated, verdict_A_not=not reachable,
-                       levels=st.levels, stored_live=st.stored_live, keys=st.keys),
+                       levels=st.levels, stored_live=st.stored_live, keys=st.keys,
+                       stored_zone_bytes=st.zone_entry_bytes * (cfg["zone_dim"] ** 2 if st.zone_entry_bytes == 4
+                                                                else (cfg["zone_dim"] ** 2 + 1) // 2 * 2)),
         "roofline": roofline,
         "roofline_issue": issue,
         "kernel_ms_per_step": {k: round(prof[k + "_ms"] / a.steps, 3) for k in ("expand", "probe", "merge", "other")},
