@@ -32,6 +32,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -2431,6 +2432,124 @@ __global__ __launch_bounds__(64) void k_zop_wave(ZopArgs A) {
   }
 }
 
+// Close and Extra_LU+ (then close) for d in {4, 8, 16, 32} with each zone held in registers:
+// G = min(64, d*d) lanes per zone (four zones per wavefront at d = 4), EPL = d*d / G entries per
+// lane. Lane t of a zone holds entries t + r*G, r < EPL: column j = t % d of the rows
+// t / d + r * (G / d), so each register row of a zone is one coalesced load. Pivot k needs z[k][j]
+// (entry k*d + j: lane k*d % G + j, register k*d / G, the same register for every lane) and z[i][k]
+// for each held row (lane (t / d) * d + k, register r): 1 + EPL cross-lane reads (ds_bpermute) per
+// pivot, where the LDS kernel reads three LDS values per entry. Row and column k do not change in
+// pivot k on a zone without negative cycles, so all lanes may read before any writes. Each
+// wavefront walks zones with a stride, loading the next group of zones while it closes this one.
+// ZW: wavefronts per workgroup (picked by zop_tuned).
+template <int D, int OP>
+__global__ __launch_bounds__(256) void k_zop_reg(ZopArgs A, long ngroups) {
+  constexpr int D2 = D * D, G = D2 < kWave ? D2 : kWave, EPL = D2 / G, ZPW = kWave / G, RS = G / D;
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  const int q = lane / G, t = lane % G, j = t % D, i0 = t / D, gb = q * G;
+  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long nw = (long)gridDim.x * (blockDim.x >> 6);
+  int cur[EPL], nxt[EPL];
+  auto load = [&](long g, int* v) __attribute__((always_inline)) {
+    const long z = g * ZPW + q;
+    const bool ok = g < ngroups && z < A.n;
+    const int* src = A.a + (ok ? z : 0) * D2;
+#pragma unroll
+    for (int r = 0; r < EPL; ++r) v[r] = ok ? src[t + r * G] : kLe0;
+  };
+  long g = wave;
+  if (g < ngroups) load(g, cur);
+  for (; g < ngroups; g += nw) {
+    if (g + nw < ngroups) load(g + nw, nxt);
+    const long z = g * ZPW + q;
+    const bool ok = z < A.n;
+    if (OP == HORA_ZOP_EXTRA_LU) {
+      // STTT 2006 Def. 6 on row 0 as it was before any change (z_extra_lu_plus)
+      const int* Lz = A.L + (ok ? z : 0) * D;
+      const int* Uz = A.U + (ok ? z : 0) * D;
+      const int r0j = __shfl(cur[0], gb + j);
+      const int Uj = Uz[j];
+      int nv[EPL];
+#pragma unroll
+      for (int r = 0; r < EPL; ++r) {
+        const int i = i0 + r * RS;
+        const int r0i = __shfl(cur[0], gb + i);
+        const int v = cur[r];
+        int w = v;
+        if (i != j && v < kInf) {
+          const int Li = Lz[i];
+          if (r_bnd(v) > Li) w = kInf;
+          else if (-r_bnd(r0i) > Li) w = kInf;
+          else if (i != 0 && -r_bnd(r0j) > Uj) w = kInf;
+          else if (v < r_lt(-Uj)) w = r_lt(-Uj);
+        }
+        nv[r] = w;
+      }
+#pragma unroll
+      for (int r = 0; r < EPL; ++r) cur[r] = nv[r];
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const int kj = __shfl(cur[(k * D) / G], gb + (k * D) % G + j);
+      int ik[EPL];
+#pragma unroll
+      for (int r = 0; r < EPL; ++r) ik[r] = __shfl(cur[r], gb + i0 * D + k);
+#pragma unroll
+      for (int r = 0; r < EPL; ++r) cur[r] = min(cur[r], radd(ik[r], kj));
+    }
+    bool neg = false;
+#pragma unroll
+    for (int r = 0; r < EPL; ++r)
+      if (i0 + r * RS == j && cur[r] < kLe0) neg = true;
+    const u64 gm = (G == kWave) ? ~0ull : (((1ull << G) - 1) << gb);
+    const bool empty = (__ballot(neg) & gm) != 0;
+    if (ok) {
+      if (A.out) {
+        int* dst = A.out + z * D2;
+#pragma unroll
+        for (int r = 0; r < EPL; ++r) dst[t + r * G] = cur[r];
+      }
+      if (t == 0 && A.flag) A.flag[z] = (OP == HORA_ZOP_CLOSE) ? (empty ? 0 : 1) : 1;
+    }
+#pragma unroll
+    for (int r = 0; r < EPL; ++r) cur[r] = nxt[r];
+  }
+}
+
+// try_union for d = 4: one lane per zone pair, 64 pairs per wavefront. Both zones of every
+// pair are staged in LDS with one coalesced load (odd stride: lanes reading their own zones hit
+// different banks); each lane runs the lane-serial convex-union test (the FA x PA formulation);
+// the hull (entrywise maximum) or the unchanged first zone goes back with one coalesced store.
+template <int D>
+__global__ __launch_bounds__(64) void k_zop_union_lane(ZopArgs A) {
+  constexpr int D2 = D * D, ZS = D2 + 1;
+  __shared__ int sa[kWave * ZS], sb[kWave * ZS];
+  const int l = threadIdx.x;
+  const long base = (long)blockIdx.x * kWave;
+  const long nz = min((long)kWave, A.n - base);
+  for (int e = l; e < nz * D2; e += kWave) {
+    const int z = e / D2, x = e - z * D2;
+    sa[z * ZS + x] = A.a[base * D2 + e];
+    sb[z * ZS + x] = A.b[base * D2 + e];
+  }
+  __syncthreads();
+  int* za = sa + l * ZS;
+  const int* zb = sb + l * ZS;
+  bool ok = false;
+  if (l < nz) {
+    ok = union_convex_lane(za, zb, D);
+    if (ok)
+      for (int x = 0; x < D2; ++x) za[x] = max(za[x], zb[x]);
+  }
+  __syncthreads();
+  if (A.out)
+    for (int e = l; e < nz * D2; e += kWave) {
+      const int z = e / D2, x = e - z * D2;
+      A.out[base * D2 + e] = sa[z * ZS + x];
+    }
+  if (l < nz && A.flag) A.flag[base + l] = ok ? 1 : 0;
+}
+
 // ---------------------------------------------------------------- host side
 
 #define HIP_OK(call)                                                                   \
@@ -3649,6 +3768,93 @@ void swap_buf(DBuf<T>& a, DBuf<T>& b) {
   std::swap(a.borrowed, b.borrowed);
 }
 
+namespace {
+
+// Launch shape of k_zop_reg: wavefronts per workgroup, and the workgroups of the grid (each
+// wavefront then walks zone groups with a stride, loading the next while it computes one).
+struct ZopShape {
+  int wpb;
+  long blocks;
+};
+
+template <int D, int OP>
+void launch_zop_reg(const ZopArgs& A, ZopShape sh, hipStream_t s) {
+  constexpr int G = D * D < kWave ? D * D : kWave;
+  const long ngroups = (A.n + kWave / G - 1) / (kWave / G);
+  const long need = (ngroups + sh.wpb - 1) / sh.wpb;
+  const long blocks = std::max<long>(1, std::min(need, sh.blocks));
+  hipLaunchKernelGGL((k_zop_reg<D, OP>), dim3((unsigned)blocks), dim3(64 * sh.wpb), 0, s, A, ngroups);
+}
+
+template <int OP>
+void launch_zop_reg_d(const ZopArgs& A, ZopShape sh, hipStream_t s) {
+  switch (A.d) {
+    case 4: launch_zop_reg<4, OP>(A, sh, s); break;
+    case 8: launch_zop_reg<8, OP>(A, sh, s); break;
+    case 16: launch_zop_reg<16, OP>(A, sh, s); break;
+    default: launch_zop_reg<32, OP>(A, sh, s); break;
+  }
+}
+
+void launch_zop_reg_any(const ZopArgs& A, ZopShape sh, hipStream_t s) {
+  if (A.op == HORA_ZOP_CLOSE) launch_zop_reg_d<HORA_ZOP_CLOSE>(A, sh, s);
+  else launch_zop_reg_d<HORA_ZOP_EXTRA_LU>(A, sh, s);
+}
+
+// The autotuner of the register kernels' launch shape, per (operation, d). The first call with at
+// least 2^16 zones, an output apart from the input (so that a rerun writes the same values) and a
+// stream that is not being captured times every candidate twice on the caller's stream and keeps
+// the fastest; every later call (and every call during graph capture) launches the pick.
+// HORA_HIP_ZOP_SHAPE=wpb,blocks fixes the shape.
+std::mutex g_zop_mu;
+std::map<int, ZopShape> g_zop_pick;
+
+ZopShape zop_shape(const ZopArgs& A, hipStream_t s) {
+  if (const char* e = std::getenv("HORA_HIP_ZOP_SHAPE")) {
+    int w = 0;
+    long b = 0;
+    if (std::sscanf(e, "%d,%ld", &w, &b) == 2 && (w == 1 || w == 2 || w == 4) && b > 0) return ZopShape{w, b};
+  }
+  const int key = A.op * 64 + A.d;
+  {
+    std::lock_guard<std::mutex> g(g_zop_mu);
+    auto it = g_zop_pick.find(key);
+    if (it != g_zop_pick.end()) return it->second;
+  }
+  const ZopShape dflt{4, 1L << 30};
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (A.n < (1L << 16) || !A.out || A.out == A.a || hipStreamIsCapturing(s, &cs) != hipSuccess ||
+      cs != hipStreamCaptureStatusNone)
+    return dflt;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return dflt;
+  std::vector<ZopShape> cand;
+  for (int w : {1, 2, 4})
+    for (long per : {2L, 8L, 1L << 30}) cand.push_back(ZopShape{w, per >= (1L << 30) ? per : per * cus * (8 / w)});
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return dflt;
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return dflt; }
+  ZopShape best = dflt;
+  float best_ms = 1e30f;
+  for (int rep = 0; rep < 2; ++rep)
+    for (const ZopShape& c : cand) {
+      (void)hipEventRecord(e0, s);
+      launch_zop_reg_any(A, c, s);
+      (void)hipEventRecord(e1, s);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best_ms) { best_ms = ms; best = c; }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> g(g_zop_mu);
+  g_zop_pick[key] = best;
+  return best;
+}
+
+}  // namespace
+
 extern "C" {
 
 int hora_hip_abi_version(void) { return HORA_HIP_ABI_VERSION; }
@@ -3746,6 +3952,11 @@ int hora_hip_zone_op(int32_t op, int32_t d, int64_t n, const int32_t* a, const i
     }
   } else if (op == HORA_ZOP_EMPTY && ld > 0) {
     hipLaunchKernelGGL(k_zop_empty, dim3(blocks_for(n, 256 >> ld)), dim3(256), 0, s, A, ld);
+  } else if ((op == HORA_ZOP_CLOSE || op == HORA_ZOP_EXTRA_LU) && ld > 0) {
+    launch_zop_reg_any(A, zop_shape(A, s), s);
+  } else if (op == HORA_ZOP_UNION && d == 4) {
+    // (at d = 8 one lane's 64-entry test is slower than a wavefront per pair: 0.8 against 1.3 TB/s)
+    hipLaunchKernelGGL(k_zop_union_lane<4>, dim3(blocks_for(n, kWave)), dim3(64), 0, s, A);
   } else {
     const long grid = std::min<long>(n, 1L << 20);
     const size_t lds = (4 * d * d + HORA_MAX_DIM + 8) * sizeof(int);

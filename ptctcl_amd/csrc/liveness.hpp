@@ -1377,10 +1377,20 @@ void launch_stop(hora_hip_ctx* c, LiveState& L, const int* list, long n, long lo
   LiveCtr h = read_lc(c, L);
   if (h.rcnt <= 0 || h.err) return;
   // a middle tier between the small lists and the full ones: still several wavefronts per CU
-  // (not under an override of the small lists, whose runs exercise the full lists)
-  const StopCaps KM{6, 6, KB.ic, KB.rc, KB.np, KB.ni};
+  // (not under an override of the small lists, whose runs exercise the full lists, unless
+  // HORA_HIP_STOP_MID_CAPS=kl,kp sets the middle tier too)
+  StopCaps KM{6, 6, KB.ic, KB.rc, KB.np, KB.ni};
+  bool mid = !std::getenv("HORA_HIP_STOP_CAPS");
+  if (const char* e = std::getenv("HORA_HIP_STOP_MID_CAPS")) {
+    int kl = 0, kp = 0;
+    if (std::sscanf(e, "%d,%d", &kl, &kp) == 2 && kl >= KA.kl && kp >= KA.kp) {
+      KM.kl = kl;
+      KM.kp = kp;
+      mid = true;
+    }
+  }
   const int* full_list = L.rlist.p;
-  if (!std::getenv("HORA_HIP_STOP_CAPS") && KM.kl < KB.kl && KM.kp < KB.kp && KM.ints(d2) <= 64 * 1024 / 4 / 2) {
+  if (mid && KM.kl < KB.kl && KM.kp < KB.kp && KM.ints(d2) <= 64 * 1024 / 4 / 2) {
     const int nm = h.rcnt;
     L.rlist2.ensure(nm);
     HIP_OK(hipMemsetAsync(&L.lc.p->rcnt, 0, sizeof(int), c->stream));
@@ -1938,9 +1948,20 @@ int solve_graph(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, long lo
       L.dlist.ensure(ns); L.rlist.ensure(ns);
       const GfpCaps KA = gfp_caps_small(c->dn), KB = gfp_caps_full(L.args, c->dn);
       // a middle tier between the small lists and the full ones (at most 32 KiB, so that several
-      // wavefronts share a CU); not under an override of the small lists, whose runs exercise the full ones
-      const GfpCaps KM{std::min(4, KB.kw), std::min(16, KB.kc), std::min(8, KB.kp), KB.ic, KB.np, KB.ni};
-      const bool gfp_mid = !std::getenv("HORA_HIP_GFP_CAPS") && KM.kw >= KA.kw && KM.kc >= KA.kc && KM.kp >= KA.kp &&
+      // wavefronts share a CU); not under an override of the small lists, whose runs exercise the
+      // full ones, unless HORA_HIP_GFP_MID_CAPS=kw,kc,kp sets the middle tier too
+      GfpCaps KM{std::min(4, KB.kw), std::min(16, KB.kc), std::min(8, KB.kp), KB.ic, KB.np, KB.ni};
+      bool mid_on = !std::getenv("HORA_HIP_GFP_CAPS");
+      if (const char* e = std::getenv("HORA_HIP_GFP_MID_CAPS")) {
+        int kw = 0, kc = 0, kp = 0;
+        if (std::sscanf(e, "%d,%d,%d", &kw, &kc, &kp) == 3 && kw <= KB.kw && kc <= KB.kc && kp <= KB.kp) {
+          KM.kw = kw;
+          KM.kc = kc;
+          KM.kp = kp;
+          mid_on = true;
+        }
+      }
+      const bool gfp_mid = mid_on && KM.kw >= KA.kw && KM.kc >= KA.kc && KM.kp >= KA.kp &&
                            (KM.kc < KB.kc || KM.kp < KB.kp || KM.kw < KB.kw) && KM.ints(d2) <= 64 * 1024 / 4 / 2;
       int round = 0;
       long long retried = 0;

@@ -137,6 +137,25 @@ def test_rerun_at_full_capacity(monkeypatch):
             assert r == want
 
 
+def test_rerun_through_middle_tier(monkeypatch):
+    """With small lists of one or two zones and middle lists of two or three, nodes pass from the
+    small launch to the middle one (the nodes that outgrow it in its own rerun list) and from there
+    to the full lists: graphs, stop sets and converged values must not change."""
+    monkeypatch.setenv("HORA_HIP_GFP_CAPS", "1,1,2,1")
+    monkeypatch.setenv("HORA_HIP_GFP_MID_CAPS", "2,2,3")
+    monkeypatch.setenv("HORA_HIP_STOP_CAPS", "1,1,1,1")
+    monkeypatch.setenv("HORA_HIP_STOP_MID_CAPS", "2,2")
+    net = models.fischer(5)
+    for label, q, want in fischer_queries(net):
+        r, _ = check(net, q, label)
+        assert r == want, label
+    for builder in live_models.ALL:
+        net, queries = builder()
+        for q, want in queries:
+            r, _ = check(net, q)
+            assert r == want
+
+
 def test_pools_grow_and_compact(monkeypatch):
     """Stop sets and fixpoint values live in pools. With pools that start at the least they can
     hold and grow by exactly what is asked, the stop-set phase runs again after growing, and rounds

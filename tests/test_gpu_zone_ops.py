@@ -115,22 +115,88 @@ def test_try_union(d):
     assert np.array_equal(eo, go)
 
 
-@pytest.mark.parametrize("d", [4, 8])
+@pytest.mark.parametrize("d", DIMS)
 def test_full_size_properties(d):
-    """2^20 zones: size-independent properties (close idempotent on canonical zones, a_LU and
-    inclusion reflexive, union with itself is convex and is the zone)."""
+    """2^20 zones at every d (a pool of distinct seeded zones tiled on the device, as in the
+    microbenchmark): size-independent properties checked on the device (close idempotent on
+    canonical zones, Extra_LU+ with bounds above every constant the identity, a_LU and inclusion
+    reflexive, union with itself convex and the zone itself)."""
     n = 1 << 20
+    pool = {4: 1 << 14, 8: 1 << 14, 16: 1 << 12, 32: 1 << 10}[d]
     rng = np.random.default_rng(700 + d)
-    z = zg.hull_zones(n, d, rng)
+    Z = dev(zg.hull_zones(pool, d, rng)).repeat(n // pool, 1)
+    out = torch.empty_like(Z)
+    flag = torch.zeros(n, dtype=torch.int32, device="cuda:0")
+    hip.zone_op(hip.ZOP_CLOSE, d, Z, out=out, flag=flag)
+    assert int(flag.min()) == 1 and torch.equal(out, Z)
+    big = torch.full((n, d), 1 << 20, dtype=torch.int32, device="cuda:0")
+    hip.zone_op(hip.ZOP_EXTRA_LU, d, Z, L=big, U=big, out=out, flag=flag)
+    assert torch.equal(out, Z)
+    Lp, Up = zg.bounds(pool, d, rng)
+    L, U = dev(Lp).repeat(n // pool, 1), dev(Up).repeat(n // pool, 1)
+    flag.zero_()
+    hip.zone_op(hip.ZOP_ALU_LE, d, Z, Z, L=L, U=U, flag=flag)
+    assert int(flag.min()) == 1
+    flag.zero_()
+    hip.zone_op(hip.ZOP_UNION, d, Z, Z, out=out, flag=flag)
+    assert int(flag.min()) == 1 and torch.equal(out, Z)
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("d", DIMS)
+@pytest.mark.parametrize("shape", ["1,1", "2,3", "4,1000000"])
+def test_register_kernels_launch_shapes(d, shape, monkeypatch):
+    """Close and Extra_LU+ at every launch shape the autotuner may pick: one wavefront walking all
+    zone groups (loading the next group while closing one), a few workgroups, one group per
+    wavefront; bit-exact against the oracle."""
+    monkeypatch.setenv("HORA_HIP_ZOP_SHAPE", shape)
+    rng = np.random.default_rng(800 + d)
+    z = zg.unclose(zg.hull_zones(301, d, rng), d, rng)
+    eo, ef = binding.zone_op(hip.ZOP_CLOSE, d, z)
     go, gf = run(hip.ZOP_CLOSE, d, z)
-    assert gf.min() == 1 and np.array_equal(go, z)
-    L, U = zg.bounds(n, d, rng)
-    _, gf = run(hip.ZOP_ALU_LE, d, z, z, L=L, U=U)
-    assert gf.min() == 1
-    go, gf = run(hip.ZOP_UNION, d, z, z)
-    assert gf.min() == 1 and np.array_equal(go, z)
+    assert np.array_equal(ef, gf) and np.array_equal(eo, go)
+    z = zg.hull_zones(301, d, rng)
+    L, U = zg.bounds(301, d, rng)
+    eo, _ = binding.zone_op(hip.ZOP_EXTRA_LU, d, z, L=L, U=U)
+    go, _ = run(hip.ZOP_EXTRA_LU, d, z, L=L, U=U)
+    assert np.array_equal(eo, go)
 
 
+@pytest.mark.parametrize("d", [4, 16])
+def test_autotuned_and_graph_captured(d):
+    """The first call with 2^17 zones tunes the register kernels' launch shape; the same calls
+    captured in a HIP graph (through torch's graph capture of the current stream) and replayed
+    three times, with eager work and synchronisation between replays, give the eager answer each
+    time, bit-exact against the oracle on a sample."""
+    n = 1 << 17
+    pool = 1 << 12
+    rng = np.random.default_rng(900 + d)
+    zp = zg.unclose(zg.hull_zones(pool, d, rng), d, rng)
+    Z = dev(zp).repeat(n // pool, 1)
+    ref = torch.empty_like(Z)
+    flag_ref = torch.zeros(n, dtype=torch.int32, device="cuda:0")
+    hip.zone_op(hip.ZOP_CLOSE, d, Z, out=ref, flag=flag_ref)      # tunes
+    torch.cuda.synchronize()
+    eo, ef = binding.zone_op(hip.ZOP_CLOSE, d, zp)
+    assert np.array_equal(ref[:pool].cpu().numpy(), eo) and np.array_equal(flag_ref[:pool].cpu().numpy(), ef)
+    out = torch.zeros_like(Z)
+    flag = torch.zeros(n, dtype=torch.int32, device="cuda:0")
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        hip.zone_op(hip.ZOP_CLOSE, d, Z, out=out, flag=flag, stream=s.cuda_stream)   # warm-up off-graph
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        hip.zone_op(hip.ZOP_CLOSE, d, Z, out=out, flag=flag, stream=torch.cuda.current_stream().cuda_stream)
+    for rep in range(3):
+        out.fill_(7)
+        flag.fill_(-1)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref) and torch.equal(flag, flag_ref), rep
 @pytest.mark.parametrize("case", __import__("extra_cases").EXTRA_CASES, ids=lambda c: c[0])
 def test_extra_lu_plus_known_answers(case):
     """Hand-derived Extra_LU+ answers (tests/extra_cases.py), one per clause, on the device."""
