@@ -6,17 +6,21 @@ Mirrors the reference's ``load_network`` (src/frontend/model_reader.cpp:987-1261
   (``broadcast``, ``urgent``), ``const int``, ``int[lo,hi]`` and ``bool`` variables with constant
   initialisers; comments ``//`` and ``/* */``;
 * templates without parameters, instantiated in the order of the ``system`` line (one process per
-  template in document order without one); clocks, variables and channels are numbered in
+  template in document order without one), directly or through argument-free instantiations
+  (``P2 = P1();`` before the ``system`` line); clocks, variables and channels are numbered in
   declaration order, globals first, then each instance's locals in process order;
 * locations with a name (else the id), a conjunction of clock upper bounds as invariant, and the
   ``committed`` / ``urgent`` flags; the ``init`` reference;
 * transitions: a guard that is a conjunction (``&&``, ``and``) of clock bounds, clock differences
   and integer comparisons against constant expressions; a ``ch!`` / ``ch?`` synchronisation; an
-  assignment list of clock resets to 0 and integer variables set to constants (``=`` or ``:=``).
+  assignment list of clock resets to 0 and integer variables set to integer expressions (``=`` or
+  ``:=``). An expression that reads integer variables (the reference's general update,
+  NEdge::gen_update) is compiled into constant updates: the edge becomes one edge per valuation of
+  what it reads (``_Builder.split_expression_edges``).
 
-Anything outside the subset (template parameters, ``select``, arrays, functions, expression
-updates, disjunctive guards) raises :class:`ModelError` naming it, as the reference refuses what
-it cannot represent rather than verify a different model.
+Anything outside the subset (template parameters, ``select``, arrays, functions, disjunctive
+guards) raises :class:`ModelError` naming it, as the reference refuses what it cannot represent
+rather than verify a different model.
 
 :func:`safety_target` turns the reachability queries the reference's benchmarks record
 (``A[] not (P.l && Q.m)``, ``A[] true``, ``A[] x <= c``, ``E<> ...``) into a :class:`Target`
@@ -24,6 +28,7 @@ it cannot represent rather than verify a different model.
 """
 from __future__ import annotations
 
+import itertools
 import os
 import re
 import xml.etree.ElementTree as ET
@@ -82,10 +87,11 @@ class _Scope:
 
 
 class _Expr:
-    """Constant integer expressions: + - * / % unary minus, parentheses, named constants."""
+    """Integer expressions: + - * / % unary minus, parentheses, named constants, and (given a
+    valuation `env` of the integer variables by index) integer variables."""
 
-    def __init__(self, toks: List[str], scope: _Scope):
-        self.t, self.i, self.sc = toks, 0, scope
+    def __init__(self, toks: List[str], scope: _Scope, env: Optional[Dict[int, int]] = None):
+        self.t, self.i, self.sc, self.env = toks, 0, scope, env
 
     def peek(self):
         return self.t[self.i] if self.i < len(self.t) else None
@@ -133,17 +139,31 @@ class _Expr:
         if tok.isdigit():
             return int(tok)
         c = self.sc.find("consts", tok)
-        if c is None:
+        if c is not None:
+            return c
+        v = self.sc.find("ints", tok) if self.env is not None else None
+        if v is None:
             raise ModelError(f"not a constant: {tok}")
-        return c
+        return self.env[v]
 
 
-def _const(toks: List[str], scope: _Scope) -> int:
-    e = _Expr(toks, scope)
+def _const(toks: List[str], scope: _Scope, env: Optional[Dict[int, int]] = None) -> int:
+    e = _Expr(toks, scope, env)
     v = e.expr()
     if e.peek() is not None:
         raise ModelError(f"unexpected {e.peek()!r} in {' '.join(toks)!r}")
     return v
+
+
+def _reads(toks: List[str], scope: _Scope) -> List[int]:
+    """The integer variables an expression names, in first-use order."""
+    out: List[int] = []
+    for t in toks:
+        if scope.find("consts", t) is None:
+            v = scope.find("ints", t)
+            if v is not None and v not in out:
+                out.append(v)
+    return out
 
 
 def _split_top(toks: List[str], sep: str) -> List[List[str]]:
@@ -165,6 +185,9 @@ class _Builder:
         self.clock_names = ["0"]
         self.intvars: List[IntVar] = []
         self.channels: List[Channel] = []
+        # edges whose updates read integer variables: id(edge) -> (edge, [(var, tokens, scope) or
+        # (var, None, constant)] in assignment order, from the first such update on)
+        self.pending: Dict[int, Tuple[Edge, list]] = {}
 
     # ---- declarations (model_reader.cpp parse_decl_block)
     def declare(self, text: str, scope: _Scope, prefix: str) -> None:
@@ -281,28 +304,140 @@ class _Builder:
                     raise ModelError(f"clock assignments other than reset to 0: {text.strip()}")
                 edge.resets.append(scope.find("clocks", name))
             elif scope.find("ints", name) is not None:
-                try:
-                    v = _const(rhs, scope)
-                except ModelError:
-                    raise ModelError(f"integer updates must be constant: {text.strip()}") from None
-                edge.int_updates.append((scope.find("ints", name), v))
+                var = scope.find("ints", name)
+                if _reads(rhs, scope):
+                    # an expression over integer variables: compiled away by split_expression_edges
+                    _Expr(rhs, scope, {v: 0 for v in _reads(rhs, scope)})   # names resolve
+                    self.pending.setdefault(id(edge), (edge, []))[1].append((var, rhs, scope))
+                    continue
+                v = _const(rhs, scope)
+                if id(edge) in self.pending:   # after an expression update: keeps its place in order
+                    self.pending[id(edge)][1].append((var, None, v))
+                else:
+                    edge.int_updates.append((var, v))
             else:
                 raise ModelError(f"assignment to an unknown name: {text.strip()}")
+
+    # ---- expression updates (model_reader.cpp's general updates, NEdge::gen_update)
+    def split_expression_edges(self, procs: List["Process"]) -> None:
+        """Compile every update that reads integer variables into constant updates: the edge is
+        replaced, in place, by one copy per valuation of the variables its updates read (each copy
+        guarded by that valuation, its updates evaluated in order with C's truncating division).
+        In any state exactly one copy is enabled, with the successor the original edge gives, and
+        the copies stand where the edge stood, so successors keep their order. A valuation whose
+        update leaves a variable's declared range disables its copy (a faulting update kills the
+        edge). The valuations range over the values each variable can take: its initial value and
+        every value an update can give it, closed under the updates (at most 4,096 per variable)."""
+        if not self.pending:
+            return
+        vals = {k: {v.init} for k, v in enumerate(self.intvars)}
+        for p in procs:
+            for e in p.edges:
+                for var, val in e.int_updates:
+                    vals[var].add(val)
+
+        def reads(seq):
+            r: List[int] = []
+            for _, toks, sc in seq:
+                for v in (_reads(toks, sc) if toks is not None else []):
+                    if v not in r:
+                        r.append(v)
+            return r
+
+        def valuations(seq):
+            r = reads(seq)
+            n = 1
+            for v in r:
+                n *= len(vals[v])
+            if n > 100000:
+                raise ModelError("an expression update over more than 100,000 valuations")
+            for vs in itertools.product(*[sorted(vals[v]) for v in r]):
+                yield r, dict(zip(r, vs))
+
+        def evaluate(e, seq, env):
+            env = dict(env)
+            for var, val in e.int_updates:      # constant updates written before the first expression
+                env[var] = val
+            out = []
+            for var, toks, arg in seq:
+                x = arg if toks is None else _const(toks, arg, env)
+                iv = self.intvars[var]
+                if not iv.lo <= x <= iv.hi:
+                    return None
+                env[var] = x
+                out.append((var, x))
+            return out
+
+        # the values' closure under the updates, semi-naively: each pass evaluates only the
+        # valuations that use a value found in the previous pass
+        old = {k: set() for k in vals}
+        new = {k: set(v) for k, v in vals.items()}
+        first = True
+        while first or any(new.values()):
+            added = {k: set() for k in vals}
+            for e, seq in self.pending.values():
+                r = reads(seq)
+                pools_list = [[]] if not r and first else []
+                for k in range(len(r)):
+                    pools_list.append([(old[r[i]] | new[r[i]]) if i < k else new[r[i]] if i == k else old[r[i]]
+                                       for i in range(len(r))])
+                for pools in pools_list:
+                    for vs in itertools.product(*[sorted(p) for p in pools]):
+                        for var, x in evaluate(e, seq, dict(zip(r, vs))) or []:
+                            if x not in old[var] and x not in new[var]:
+                                added[var].add(x)
+            for k in vals:
+                old[k] |= new[k]
+                new[k] = added[k]
+                if len(old[k]) + len(new[k]) > 4096:
+                    raise ModelError(f"updates give {self.intvars[k].name} more than 4,096 values")
+            first = False
+        vals = old
+        for p in procs:
+            new_edges = []
+            for e in p.edges:
+                if id(e) not in self.pending:
+                    new_edges.append(e)
+                    continue
+                seq = self.pending[id(e)][1]
+                for r, env in valuations(seq):
+                    if not all(_atom_holds(env[v], op, c) for v, op, c in e.int_guard if v in env):
+                        continue   # the edge's own guard disables it at this valuation
+                    out = evaluate(e, seq, env)
+                    if out is None:
+                        continue
+                    new_edges.append(Edge(e.src, e.tgt, list(e.guard), list(e.resets),
+                                          list(e.int_guard) + [(v, OP_EQ, env[v]) for v in r],
+                                          list(e.int_updates) + out, e.chan, e.dir))
+            p.edges = new_edges
+        self.pending = {}
+
+
+def _atom_holds(x: int, op: int, c: int) -> bool:
+    return {OP_EQ: x == c, OP_NE: x != c, OP_LT: x < c, OP_LE: x <= c, OP_GT: x > c, OP_GE: x >= c}[op]
 
 
 def _text(el) -> str:
     return (el.text or "") if el is not None else ""
 
 
-def _system_order(text: str) -> Optional[List[str]]:
+def _system_order(text: str) -> Tuple[Optional[List[str]], Dict[str, str]]:
+    """The process names of the `system` line, and the instantiations before it (`P = T();`:
+    instance name -> template name; instances of templates without parameters only)."""
     body = _strip_comments(text)
     m = re.search(r"\bsystem\b([^;]*);", body)
     if not m:
-        return None
-    if body[:m.start()].strip():
-        raise ModelError("process instantiations in the system block are not supported")
+        return None, {}
+    inst: Dict[str, str] = {}
+    for stmt in body[:m.start()].split(";"):
+        if not stmt.strip():
+            continue
+        mi = re.fullmatch(r"\s*([A-Za-z_]\w*)\s*=\s*([A-Za-z_]\w*)\s*\(\s*\)\s*", stmt)
+        if not mi:
+            raise ModelError(f"process instantiations with arguments are not supported: {stmt.strip()}")
+        inst[mi.group(1)] = mi.group(2)
     names = [n.strip() for n in m.group(1).replace("<", ",").split(",")]
-    return [n for n in names if n]
+    return [n for n in names if n], inst
 
 
 def parse_xml(text: str, name: str = "") -> Network:
@@ -320,11 +455,11 @@ def parse_xml(text: str, name: str = "") -> Network:
         templates[tn] = t
         order_doc.append(tn)
     sysel = root.find("system")
-    order = _system_order(_text(sysel)) if sysel is not None else None
+    order, inst = _system_order(_text(sysel)) if sysel is not None else (None, {})
     order = order if order else order_doc
     procs = []
     for pname in order:
-        t = templates.get(pname)
+        t = templates.get(inst.get(pname, pname))
         if t is None:
             raise ModelError(f"system declaration names unknown process or template: {pname}")
         if _text(t.find("parameter")).strip():
@@ -369,6 +504,7 @@ def parse_xml(text: str, name: str = "") -> Network:
                     raise ModelError(f"unsupported label kind: {kind}")
             edges.append(e)
         procs.append(Process(pname, locs, edges, lid.get(init.get("ref"), 0) if init is not None else 0))
+    b.split_expression_edges(procs)
     net = Network(len(b.clock_names) - 1, b.intvars, b.channels, procs, clock_names=b.clock_names, name=name)
     return net.finalize()
 

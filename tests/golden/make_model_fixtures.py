@@ -30,8 +30,11 @@ MODELS = {
     "grc": [4, 5],
     "single": [4, 5, 6],
     "hddi": [2, 10],
+    "viking": [4, 6, 10, 12, 15, 17, 20],
 }
-COUNTED = ("arbiter", "phil", "twolock", "csma", "fischer")
+# viking's counts are also verifyta's and tchecker's under a_LU (reach/uppaal_dfs/verifyta_dfs_t1.csv:
+# 141-152, reach/tchecker_cov/tchecker_cov_t1.csv:143-150)
+COUNTED = ("arbiter", "phil", "twolock", "csma", "fischer", "viking")
 
 
 def main():

@@ -31,7 +31,8 @@ def load(name):
 
 @pytest.mark.parametrize("name", ["arbiter4", "arbiter8", "arbiter12", "phil4", "phil6", "phil8", "phil10",
                                   "twolock4", "twolock6", "twolock8", "fischer8", "grc4", "grc5", "single4",
-                                  "single5", "single6", "hddi2", "hddi10", "csma4", "csma6", "csma8"])
+                                  "single5", "single6", "hddi2", "hddi10", "csma4", "csma6", "csma8", "viking4",
+                                  "viking6", "viking10"])
 def test_counters_equal_oracle(name):
     net, tgt = load(name)
     r_o, st_o = binding.reach(net, tgt, 0)
@@ -44,7 +45,7 @@ def test_counters_equal_oracle(name):
 
 
 @pytest.mark.parametrize("name", ["arbiter16", "arbiter18", "twolock10", "twolock12", "twolock13", "fischer12",
-                                  "csma10", "csma14", "csma16"])
+                                  "csma10", "csma14", "csma16", "viking12", "viking15", "viking17", "viking20"])
 def test_recorded_counts_at_size(name):
     net, tgt = load(name)
     with hip.HipContext(net, tgt) as ctx:

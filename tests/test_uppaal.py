@@ -17,7 +17,7 @@ import pytest
 
 from oracle import binding
 from ptctcl_amd import models, uppaal
-from ptctcl_amd.network import OP_EQ, OP_LT, Target, c_gt, c_le
+from ptctcl_amd.network import OP_EQ, OP_NE, OP_LT, Target, c_gt, c_le
 
 HERE = os.path.dirname(__file__)
 GOLD = json.load(open(os.path.join(HERE, "golden", "xml_reach_counts.json")))["models"]
@@ -36,7 +36,8 @@ def test_fischer_xml_is_the_generated_network(N):
 
 
 @pytest.mark.parametrize("name", ["arbiter4", "arbiter8", "arbiter12", "phil4", "phil6", "twolock4", "twolock6",
-                                  "twolock8", "csma4", "csma6", "csma8", "fischer4", "fischer8"])
+                                  "twolock8", "csma4", "csma6", "csma8", "fischer4", "fischer8", "viking4",
+                                  "viking6", "viking10"])
 def test_oracle_counts_equal_recorded(name):
     g = GOLD[name]
     net = load(name)
@@ -97,7 +98,7 @@ XML = """<nta><declaration>{decl}</declaration><template><name>P</name>{param}
 
 
 @pytest.mark.parametrize("decl,param,labels,what", [
-    ("clock x; int L;", "", '<label kind="assignment">L = (1 - L) % 2</label>', "constant"),
+    ("clock x; int L;", "", '<label kind="assignment">L = L + 1</label>', "more than 4,096"),
     ("clock x; int[0,3] i;", "", '<label kind="guard">x &gt; 1 || i == 2</label>', "conjunctions"),
     ("clock x; chan c[2];", "", '<label kind="synchronisation">c[0]!</label>', "arrays"),
     ("clock x; typedef int[0,1] id_t;", "", "", "declaration"),
@@ -109,6 +110,30 @@ XML = """<nta><declaration>{decl}</declaration><template><name>P</name>{param}
 def test_outside_the_subset_is_refused(decl, param, labels, what):
     with pytest.raises(uppaal.ModelError, match=what):
         uppaal.parse_xml(XML.format(decl=decl, param=param, labels=labels))
+
+
+def test_expression_updates_split_by_valuation():
+    """An update that reads integer variables becomes one edge per valuation of what it reads,
+    in the edge's place, each guarded by its valuation and setting constants. viking's torch:
+    `L = (1 - L) % 2` on `int L` (UPPAAL's default range), where L takes the values 0 and 1."""
+    net = uppaal.load(os.path.join(os.path.dirname(__file__), "golden", "models", "viking4.xml"))
+    split = [(e.int_guard, e.int_updates) for e in net.procs[-1].edges if e.int_updates]
+    assert split == [([(0, OP_EQ, 0)], [(0, 1)]), ([(0, OP_EQ, 1)], [(0, 0)])]
+    # j := 1 first (seen by the expression), then i := (i - j) / 2 + i % 2 in C's truncating
+    # arithmetic: from i = -4 the values are -4 -> (-5)/2 + 0 = -2 -> (-3)/2 + 0 = -1 ->
+    # (-2)/2 + (-1) = -2 (floor division would give -3 at the first step)
+    net = uppaal.parse_xml(XML.format(
+        decl="clock x; int[-4,4] i = -4; int[0,2] j;", param="",
+        labels='<label kind="guard">j != 2</label>'
+               '<label kind="assignment">j = 1, i = (i - j) / 2 + i % 2</label>'))
+    f = {-4: -2, -2: -1, -1: -2}
+    got = [(e.int_guard, e.int_updates) for e in net.procs[0].edges]
+    want = [([(1, OP_NE, 2), (0, OP_EQ, i), (1, OP_EQ, j)], [(1, 1), (0, f[i])]) for i in (-4, -2, -1) for j in (0, 1)]
+    assert got == want
+    # k := k + 2 on int[0,3] from 0: 0 -> 2 -> 4 leaves the range, so the copy at k = 2 is dropped
+    net = uppaal.parse_xml(XML.format(decl="clock x; int[0,3] k;", param="",
+                                      labels='<label kind="assignment">k = k + 2</label>'))
+    assert [(e.int_guard, e.int_updates) for e in net.procs[0].edges] == [([(0, OP_EQ, 0)], [(0, 2)])]
 
 
 def test_declaration_forms():
