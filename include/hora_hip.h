@@ -261,6 +261,7 @@ typedef struct hora_live_stats {
                                 (online_compact_key, src/liveness/construction.cpp:426-477)       */
   int64_t unions;            /* successful convex unions of a successor with an active record    */
   int64_t enlarged;          /* A[] not deadlock: unexpanded records enlarged in place            */
+  int64_t fed_hbm_nodes;     /* node launches whose federation lists outgrew LDS (device memory) */
   int32_t early;             /* 1: settled by a witness state during the construction           */
   int32_t capped;            /* 1: a federation outgrew its capacity; no verdict                */
   int32_t route;             /* covering of the graph decided on: 0 a_LU, 1 a_M                 */

@@ -1,4 +1,9 @@
-"""Build recipe of the HIP library: hipcc for gfx950, in-tree output ptctcl_amd/libhora_hip.so."""
+"""Build recipe of the HIP library: hipcc for gfx950, in-tree output ptctcl_amd/libhora_hip.so.
+
+Two translation units: csrc/hora_hip.hip (everything) and csrc/fed_hbm.hip (the same source
+compiled with HORA_FED_HBM: the liveness kernels' device-memory tier). They compile in parallel
+to objects, which link into one shared library.
+"""
 from __future__ import annotations
 
 import os
@@ -7,7 +12,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "hora_hip.hip")
+SRCS = [os.path.join(HERE, "csrc", "hora_hip.hip"), os.path.join(HERE, "csrc", "fed_hbm.hip")]
 OUT = os.path.join(HERE, "libhora_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -16,20 +21,36 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"),
 ]
 
 
 def build(verbose: bool = False) -> str:
-    cmd = [HIPCC] + FLAGS + ["-o", OUT + ".tmp", SRC]
+    objs = [OUT + f".{k}.o" for k in range(len(SRCS))]
+    procs = []
+    for src, obj in zip(SRCS, objs):
+        cmd = [HIPCC] + FLAGS + ["-c", "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
+    failed = False
+    for p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+    if failed:
+        raise RuntimeError("hipcc failed")
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp"] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
+    for obj in objs:
+        os.remove(obj)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed")
+        raise RuntimeError("hipcc link failed")
     os.replace(OUT + ".tmp", OUT)
     return OUT
 

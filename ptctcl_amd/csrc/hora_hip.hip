@@ -3723,6 +3723,9 @@ void open_ctx(const hora_net_enc* e, const hora_target* t, hora_hip_ctx* c) {
 }  // namespace
 
 #include "liveness.hpp"
+// Built as fed_hbm.hip (HORA_FED_HBM), this unit supplies only the liveness kernels' device-memory
+// tier; the serving loop, the static scratch and the C ABI belong to the main unit.
+#ifndef HORA_FED_HBM
 #include "serve.hpp"
 
 // ---------------------------------------------------------------- the allocation-free fixpoint
@@ -4116,3 +4119,4 @@ void hora_hip_server_close(hora_hip_server* s) {
 }
 
 }  // extern "C"
+#endif  // HORA_FED_HBM

@@ -622,10 +622,73 @@ __device__ __forceinline__ int fed_compact(int* F, int n, int zs, int d, int* dr
   return m;
 }
 
+// The reference's federation normalisation (fed_compact, src/liveness/operators.cpp:428-458): each
+// member in order is dropped when a kept member includes it, replaces the first kept member it
+// includes, or is joined to the first kept member whose union with it is one zone (z_try_union);
+// otherwise it is kept. One more pass when anything was reshaped. The set is unchanged; used when
+// a list is full after fed_compact (which only drops contained members). `fa`, `pa`: d*d ints of
+// LDS scratch each, `cnt` 2 ints. Returns the new length.
+__device__ __noinline__ int fed_compact_union(int* F, int n, int zs, int d, int* fa, int* pa, int* cnt) {
+  const int d2 = d * d;
+  int m = 0;
+  bool merged = false;
+  for (int k = 0; k < n; ++k) {
+    int* z = F + k * zs;
+    bool absorbed = false;
+    for (int o = 0; o < m && !absorbed; ++o) {
+      int* oz = F + o * zs;
+      if (z_includes(oz, z, d)) {
+        absorbed = true;
+      } else if (z_includes(z, oz, d)) {
+        zcopy_w(oz, z, d2);
+        absorbed = merged = true;
+      } else if (z_try_union(oz, z, d, fa, pa, cnt)) {
+        absorbed = merged = true;
+      }
+    }
+    if (!absorbed) {
+      if (m != k) zcopy_w(F + m * zs, z, d2);
+      ++m;
+    }
+  }
+  if (merged && m > 1) {
+    int s = 0;
+    for (int k = 0; k < m; ++k) {
+      int* z = F + k * zs;
+      bool absorbed = false;
+      for (int o = 0; o < s && !absorbed; ++o) {
+        int* oz = F + o * zs;
+        absorbed = z_includes(oz, z, d) || z_try_union(oz, z, d, fa, pa, cnt);
+      }
+      if (!absorbed) {
+        if (s != k) zcopy_w(F + s * zs, z, d2);
+        ++s;
+      }
+    }
+    m = s;
+  }
+  wsync();
+  return m;
+}
+
+// p minus y appended to out (capacity cap). With UNI, when out fills, the pieces from before this
+// call are normalised with unions (fed_compact_union) and the subtraction runs again; false when
+// there is no room (even then). `cnt`: 2 ints of LDS scratch (r and t serve as the union's).
+template <bool UNI>
+__device__ __forceinline__ bool subtract_room(const int* p, const int* y, int d, int* out, int zs, int* n, int cap,
+                                              int* r, int* t, int* cnt) {
+  const int n0 = *n;
+  if (z_subtract(p, y, d, out, zs, n, cap, r, t)) return true;
+  if (!UNI) return false;
+  *n = fed_compact_union(out, n0, zs, d, r, t, cnt);
+  return z_subtract(p, y, d, out, zs, n, cap, r, t);
+}
+
 // z covered by the union of the list? Subtract every member; `cur`/`nxt` are piece lists of
 // capacity kp. Returns 1 covered, 0 not, -1 capacity exceeded.
+template <bool UNI>
 __device__ __forceinline__ int fed_covers(const int* F, int n, int zs, const int* z, int d, int* cur, int* nxt,
-                                          int kp, int* r, int* t) {
+                                          int kp, int* r, int* t, int* cnt) {
   const int d2 = d * d;
   for (int c = 0; c < n; ++c)
     if (z_includes(F + c * zs, z, d)) return 1;
@@ -634,7 +697,7 @@ __device__ __forceinline__ int fed_covers(const int* F, int n, int zs, const int
   for (int c = 0; c < n && nc > 0; ++c) {
     int nn = 0;
     for (int k = 0; k < nc; ++k)
-      if (!z_subtract(cur + k * zs, F + c * zs, d, nxt, zs, &nn, kp, r, t)) return -1;
+      if (!subtract_room<UNI>(cur + k * zs, F + c * zs, d, nxt, zs, &nn, kp, r, t, cnt)) return -1;
     int* sw = cur; cur = nxt; nxt = sw;
     nc = nn;
   }
@@ -653,11 +716,18 @@ struct StopCaps {
 // (mode 2), seed test of v's discrete state (mode 3). Writes scnt (mode 0): -1 whole zone, else the
 // number of zones in sz; sets ctr->hit for a non-empty answer in modes 1 and 2; counts the seeds
 // in ctr->hit in mode 3.
+// UNI: the launch with the full lists, which normalises a full list with unions before it gives up.
+template <bool UNI>
 __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, const LiveArgs* __restrict__ Ap,
                                              const int* __restrict__ list, long n, long long first_id, int mode,
                                              StopCaps K, int* __restrict__ retry, int* __restrict__ scnt,
-                                             int* __restrict__ soff, int* __restrict__ sz, int scap, LiveCtr* lc) {
+                                             int* __restrict__ soff, int* __restrict__ sz, int scap, LiveCtr* lc,
+                                             int* __restrict__ gl) {
+#ifdef HORA_FED_HBM
+  int* sm = gl + (size_t)blockIdx.x * (size_t)K.ints(Np->d2);   // this node's lists in device memory
+#else
   extern __shared__ int sm[];
+#endif
   const DevNet& N = *Np;
   const LiveArgs& A = *Ap;
   const int d = N.d, d2 = N.d2, l = threadIdx.x;
@@ -827,6 +897,7 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
       ++m2;
     }
     nl = m2;
+    if (UNI && nl >= K.kl) nl = fed_compact_union(live, nl, d2, d, r, t, tmp);
     if (nl >= K.kl) { overflow(ERR_CAPPED); return; }
     zcopy_w(live + nl * d2, g, d2);
     ++nl;
@@ -843,7 +914,7 @@ __global__ __launch_bounds__(64) void k_stop(const DevNet* __restrict__ Np, cons
   for (int m = 0; m < nl && nc > 0; ++m) {
     int nn = 0;
     for (int k = 0; k < nc; ++k)
-      if (!z_subtract(c0 + k * d2, live + m * d2, d, c1, d2, &nn, K.kp, r, t)) {
+      if (!subtract_room<UNI>(c0 + k * d2, live + m * d2, d, c1, d2, &nn, K.kp, r, t, tmp)) {
         overflow(ERR_CAPPED);
         return;
       }
@@ -958,6 +1029,8 @@ struct GfpCaps {
   __host__ __device__ int ints(int d2) const {
     return (4 + kw + kc + 2 * kp) * d2 + 2 * (np + ni) + 2 * 3 * ic + drop();
   }
+  // with lists in device memory: a covered-member flag per value member follows
+  __host__ __device__ size_t hbm_ints(int d2) const { return (size_t)ints(d2) + (size_t)kw; }
 };
 
 // The nodes a Jacobi round must visit: a non-empty value with a successor whose value changed in
@@ -991,10 +1064,17 @@ __global__ void k_gdirty(const LiveArgs* __restrict__ Ap, const int* __restrict_
 // other bank, so the order of the nodes does not matter. With `retry` set, a node that outgrows a
 // capacity of K is appended to `retry` untouched (for a launch with the full capacities, which
 // reads the same banks); without it, that is the federation cap of the reference (no verdict).
+// UNI: the launch with the full lists, which normalises a full list with unions before it gives up.
+template <bool UNI>
 __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const LiveArgs* __restrict__ Ap,
                                             const int* __restrict__ slist, const int* __restrict__ list,
-                                            GfpCaps K, GfpBanks B, int* __restrict__ retry, LiveCtr* lc) {
+                                            GfpCaps K, GfpBanks B, int* __restrict__ retry, LiveCtr* lc,
+                                            int* __restrict__ gl) {
+#ifdef HORA_FED_HBM
+  int* sm = gl + (size_t)blockIdx.x * (size_t)K.hbm_ints(Np->d2);   // this node's lists in device memory
+#else
   extern __shared__ int sm[];
+#endif
   const DevNet& N = *Np;
   const LiveArgs& A = *Ap;
   const int d = N.d, d2 = N.d2, l = threadIdx.x;
@@ -1050,8 +1130,15 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
   wave_loc_info(N, locs, ints, &fzv, &amv, inv, &ninv, K.ic);
   if (ninv > K.ic) { overflow(ERR_RECIPES); return; }
   const u64 cm = ((1ull << N.nclocks) - 1) << 1;
+#ifdef HORA_FED_HBM
+  // values above 64 members: the covered members as flags after the lists
+  int* wcov = drop + K.drop();
+  for (int k = l; k < nW; k += kWave) wcov[k] = 0;
+  int ncov = 0;
+#else
   u64 covm = 0;
   const u64 full = (nW >= 64) ? ~0ull : ((1ull << nW) - 1);
+#endif
   int nc = 0;
   for (long long eb = e0; eb < e1; eb += kWave) {
     // One successor edge per lane: its target's value (count, pool offset) in the bank selected
@@ -1116,10 +1203,23 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
         }
         if (!fzv) z_down(pz, d);
         if (!z_apply_list(pz, d, inv, ninv)) continue;
+#ifdef HORA_FED_HBM
+        for (int k = 0; k < nW; ++k) {
+          wsync();
+          if (!wcov[k] && z_includes(pz, Wv + k * d2, d)) {
+            wsync();
+            if (l == 0) wcov[k] = 1;
+            ++ncov;
+          }
+        }
+        if (ncov == nW) { keep(); return; }   // every member covered: the value cannot move
+#else
         for (int k = 0; k < nW; ++k)
           if (!((covm >> k) & 1) && z_includes(pz, Wv + k * d2, d)) covm |= 1ull << k;
         if (covm == full) { keep(); return; }   // every member covered: the value cannot move
+#endif
         if (nc >= K.kc) nc = fed_compact(cand, nc, d2, d, drop);
+        if (UNI && nc >= K.kc) nc = fed_compact_union(cand, nc, d2, d, r, t, drop);
         if (nc >= K.kc) { overflow(ERR_CAPPED); return; }
         zcopy_w(cand + nc * d2, pz, d2);
         ++nc;
@@ -1130,7 +1230,7 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
   // does cand cover every member of W[v]? -1: capacity exceeded
   auto escapes = [&]() __attribute__((always_inline)) -> int {
     for (int k = 0; k < nW; ++k) {
-      const int cv = fed_covers(cand, nc, d2, Wv + k * d2, d, cur, nxt, K.kp, r, t);
+      const int cv = fed_covers<UNI>(cand, nc, d2, Wv + k * d2, d, cur, nxt, K.kp, r, t, drop);
       if (cv < 0) return -1;
       if (cv == 0) return 1;
     }
@@ -1143,6 +1243,7 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
   if (sc != 0) {
     const int add = sc < 0 ? 1 : sc;
     if (nc + add > K.kc) nc = fed_compact(cand, nc, d2, d, drop);
+    if (UNI && nc + add > K.kc) nc = fed_compact_union(cand, nc, d2, d, r, t, drop);
     if (nc + add > K.kc) { overflow(ERR_CAPPED); return; }
     if (sc < 0) zcopy_w(cand + nc * d2, A.gzone + (size_t)v * d2, d2);
     else
@@ -1169,17 +1270,21 @@ __global__ __launch_bounds__(64) void k_gfp(const DevNet* __restrict__ Np, const
       if (!z_intersect(tmp, z, cand + c * d2, d)) continue;
       ++prods;
       if (nn >= K.kp) nn = fed_compact(cur, nn, d2, d, drop);
+      if (UNI && nn >= K.kp) nn = fed_compact_union(cur, nn, d2, d, r, t, drop);
       if (nn >= K.kp) { overflow(ERR_CAPPED); return; }
       zcopy_w(cur + nn * d2, tmp, d2);
       ++nn;
     }
   }
   nn = fed_compact(cur, nn, d2, d, drop);
-  if (nn > A.kw) {                    // the bank's capacity, the same in every launch
-    if (l == 0) atomicExch(&lc->err, ERR_CAPPED);
-    keep();
-    return;
-  }
+  // the value's capacity: the LDS launches' bank width, or the federation cap in device memory
+#ifdef HORA_FED_HBM
+  const int vcap = K.kw;
+#else
+  const int vcap = A.kw;
+#endif
+  if (UNI && nn > vcap) nn = fed_compact_union(cur, nn, d2, d, r, t, drop);
+  if (nn > vcap) { overflow(ERR_CAPPED); return; }
   // fresh zones from the value pool; when it is full the caller compacts it and runs the round again
   int off = 0;
   if (l == 0) off = atomicAdd(&lc->ptop, nn);
@@ -1246,6 +1351,42 @@ __global__ void k_gverdict(const DevNet* __restrict__ Np, const LiveArgs* __rest
 
 }  // namespace
 
+// The device-memory tier of k_stop and k_gfp: the launch after the full LDS lists, for the nodes
+// whose federations outgrow them, with lists of up to the reference's federation cap (fedcap = 4096
+// zones, include/hora/detail/liveness_engine.hpp:420) in a per-workgroup slice of `gl`. Its kernels
+// are this source compiled a second time with HORA_FED_HBM (fed_hbm.hip); these launch them. The
+// small structs pass as bytes: each unit has its own copy of the same definitions.
+namespace hh_fed {
+constexpr int kFedCap = 4096;
+hipError_t launch_stop_hbm(const void* dnp, const void* argsp, const int* list, long n, int mode, const void* caps,
+                           int* scnt, int* soff, int* sz, int scap, void* lc, int* gl, hipStream_t s);
+hipError_t launch_gfp_hbm(const void* dnp, const void* argsp, const int* slist, const int* list, int n,
+                          const void* caps, const void* banks, void* lc, int* gl, hipStream_t s);
+}  // namespace hh_fed
+
+#ifdef HORA_FED_HBM
+namespace hh_fed {
+hipError_t launch_stop_hbm(const void* dnp, const void* argsp, const int* list, long n, int mode, const void* caps,
+                           int* scnt, int* soff, int* sz, int scap, void* lc, int* gl, hipStream_t s) {
+  StopCaps K;
+  std::memcpy(&K, caps, sizeof(K));
+  hipLaunchKernelGGL(k_stop<true>, dim3((unsigned)n), dim3(64), 0, s, (const DevNet*)dnp, (const LiveArgs*)argsp, list,
+                     n, (long long)0, mode, K, (int*)nullptr, scnt, soff, sz, scap, (LiveCtr*)lc, gl);
+  return hipGetLastError();
+}
+hipError_t launch_gfp_hbm(const void* dnp, const void* argsp, const int* slist, const int* list, int n,
+                          const void* caps, const void* banks, void* lc, int* gl, hipStream_t s) {
+  GfpCaps K;
+  GfpBanks B;
+  std::memcpy(&K, caps, sizeof(K));
+  std::memcpy(&B, banks, sizeof(B));
+  hipLaunchKernelGGL(k_gfp<true>, dim3((unsigned)n), dim3(64), 0, s, (const DevNet*)dnp, (const LiveArgs*)argsp, slist,
+                     list, K, B, (int*)nullptr, (LiveCtr*)lc, gl);
+  return hipGetLastError();
+}
+}  // namespace hh_fed
+#endif
+
 #include "online.hpp"
 
 // ---------------------------------------------------------------- host driver
@@ -1273,7 +1414,9 @@ struct LiveState {
   bool graph_valid = false;   // the last query built its graph to the end (readable by live_graph)
   // backward pass
   DBuf<int> in_s, slist, sidx, scnt, soff, sz, wcnt0, wcnt1, woff0, woff1, sel0, sel1, chg0, chg1, heads, dlist, rlist,
-      rlist2, wlist;
+      rlist2, rlist3, wlist;
+  DBuf<int> fedhbm;        // federation lists of the device-memory tier
+  long long hbm_nodes = 0; // node launches of the device-memory tier in the last query
   DBuf<int> wpool[2], wtmp;   // value pools (current, compaction target); per-node counts
   DBuf<long long> wnoff;      // compaction offsets
   int wcur = 0;
@@ -1285,7 +1428,9 @@ struct LiveState {
   }
 };
 
+#ifndef HORA_FED_HBM
 void live_free(LiveState* L) { delete L; }
+#endif
 
 namespace {
 
@@ -1356,12 +1501,43 @@ StopCaps stop_caps_small(const DevNet& dn) {
   return k;
 }
 
+// The full LDS lists; HORA_HIP_STOP_FULL_CAPS=kl,kp shrinks them (for tests: nodes that outgrow them
+// go on to the device-memory tier).
 StopCaps stop_caps_full(const LiveArgs& a, const DevNet& dn) {
-  return StopCaps{a.kl, a.skp, dn.icap, dn.rcap, dn.np, std::max(dn.ni, 1)};
+  StopCaps k{a.kl, a.skp, dn.icap, dn.rcap, dn.np, std::max(dn.ni, 1)};
+  if (const char* e = std::getenv("HORA_HIP_STOP_FULL_CAPS")) {
+    int kl = 0, kp = 0;
+    if (std::sscanf(e, "%d,%d", &kl, &kp) == 2 && kl >= 1 && kp >= 1 && kl <= k.kl && kp <= k.kp) {
+      k.kl = kl;
+      k.kp = kp;
+    }
+  }
+  return k;
 }
 
 // k_stop over `n` nodes (list[b], or first_id + b without a list) in `mode`; the caller reads the
 // counters afterwards. Mode 0 writes the stop sets of the closure into L's stop-set pool.
+// Sets the device-memory tier's node count of a query to zero, and copies it into the query's
+// statistics on every return.
+struct HbmCount {
+  LiveState& L;
+  hora_live_stats* st;
+  HbmCount(LiveState& l, hora_live_stats* s) : L(l), st(s) { L.hbm_nodes = 0; }
+  ~HbmCount() { st->fed_hbm_nodes = L.hbm_nodes; }
+};
+
+// HORA_HIP_FED_HBM=0 turns the device-memory tier off: a federation that outgrows the full LDS
+// lists then stops the query with HORA_HIP_E_CAPPED.
+bool fed_hbm_on() {
+  const char* e = std::getenv("HORA_HIP_FED_HBM");
+  return !(e && *e == '0');
+}
+
+// Nodes per launch of the device-memory tier: as many as keep its lists within 4 GiB.
+long fed_hbm_batch(long n, size_t ints_per_node) {
+  return std::max<long>(1, std::min<long>(n, (long)((size_t(4) << 30) / (ints_per_node * sizeof(int)))));
+}
+
 void launch_stop(hora_hip_ctx* c, LiveState& L, const int* list, long n, long long first_id, int mode) {
   if (n <= 0) return;
   const int d2 = c->d2;
@@ -1372,8 +1548,8 @@ void launch_stop(hora_hip_ctx* c, LiveState& L, const int* list, long n, long lo
   int* soff = mode == 0 ? L.soff.p : nullptr;
   int* sz = mode == 0 ? L.sz.p : nullptr;
   const int scap = mode == 0 ? (int)L.scap : 0;
-  hipLaunchKernelGGL(k_stop, dim3(n), dim3(64), KA.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp, list, n,
-                     first_id, mode, KA, L.rlist.p, scnt, soff, sz, scap, L.lc.p);
+  hipLaunchKernelGGL(k_stop<false>, dim3(n), dim3(64), KA.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp, list, n,
+                     first_id, mode, KA, L.rlist.p, scnt, soff, sz, scap, L.lc.p, (int*)nullptr);
   LiveCtr h = read_lc(c, L);
   if (h.rcnt <= 0 || h.err) return;
   // a middle tier between the small lists and the full ones: still several wavefronts per CU
@@ -1394,15 +1570,30 @@ void launch_stop(hora_hip_ctx* c, LiveState& L, const int* list, long n, long lo
     const int nm = h.rcnt;
     L.rlist2.ensure(nm);
     HIP_OK(hipMemsetAsync(&L.lc.p->rcnt, 0, sizeof(int), c->stream));
-    hipLaunchKernelGGL(k_stop, dim3(nm), dim3(64), KM.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+    hipLaunchKernelGGL(k_stop<false>, dim3(nm), dim3(64), KM.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
                        (const int*)L.rlist.p, (long)nm, (long long)0, mode, KM, L.rlist2.p, scnt, soff, sz, scap,
-                       L.lc.p);
+                       L.lc.p, (int*)nullptr);
     h = read_lc(c, L);
     if (h.rcnt <= 0 || h.err) return;
     full_list = L.rlist2.p;
   }
-  hipLaunchKernelGGL(k_stop, dim3(h.rcnt), dim3(64), KB.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
-                     full_list, (long)h.rcnt, (long long)0, mode, KB, (int*)nullptr, scnt, soff, sz, scap, L.lc.p);
+  // the full LDS lists; the nodes that outgrow them rerun with their lists in device memory
+  const int nf = h.rcnt;
+  L.rlist3.ensure(nf);
+  HIP_OK(hipMemsetAsync(&L.lc.p->rcnt, 0, sizeof(int), c->stream));
+  hipLaunchKernelGGL(k_stop<true>, dim3(nf), dim3(64), KB.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+                     full_list, (long)nf, (long long)0, mode, KB, fed_hbm_on() ? L.rlist3.p : (int*)nullptr, scnt,
+                     soff, sz, scap, L.lc.p, (int*)nullptr);
+  h = read_lc(c, L);
+  if (h.rcnt <= 0 || h.err) return;
+  const StopCaps KH{hh_fed::kFedCap, hh_fed::kFedCap, KB.ic, KB.rc, KB.np, KB.ni};
+  const size_t per = (size_t)KH.ints(d2);
+  const long batch = fed_hbm_batch(h.rcnt, per);
+  L.fedhbm.ensure(per * (size_t)batch);
+  L.hbm_nodes += h.rcnt;
+  for (long b0 = 0; b0 < h.rcnt; b0 += batch)
+    HIP_OK(hh_fed::launch_stop_hbm(c->dnp, L.argsp, L.rlist3.p + b0, std::min<long>(batch, h.rcnt - b0), mode, &KH,
+                                   scnt, soff, sz, scap, L.lc.p, L.fedhbm.p, c->stream));
 }
 
 // HORA_HIP_LIVE_POOL=tight: the stop-set and value pools start at the least they can hold and grow
@@ -1449,8 +1640,20 @@ GfpCaps gfp_caps_small(const DevNet& dn) {
   return k;
 }
 
+// The full LDS lists; HORA_HIP_GFP_FULL_CAPS=kw,kc,kp shrinks them (for tests: nodes that outgrow
+// them go on to the device-memory tier).
 GfpCaps gfp_caps_full(const LiveArgs& a, const DevNet& dn) {
-  return GfpCaps{a.kw, a.kc, a.kp, dn.icap, dn.np, std::max(dn.ni, 1)};
+  GfpCaps k{a.kw, a.kc, a.kp, dn.icap, dn.np, std::max(dn.ni, 1)};
+  if (const char* e = std::getenv("HORA_HIP_GFP_FULL_CAPS")) {
+    int kw = 0, kc = 0, kp = 0;
+    if (std::sscanf(e, "%d,%d,%d", &kw, &kc, &kp) == 3 && kw >= 1 && kc >= 1 && kp >= 2 && kw <= k.kw &&
+        kc <= k.kc && kp <= k.kp) {
+      k.kw = kw;
+      k.kc = kc;
+      k.kp = kp;
+    }
+  }
+  return k;
 }
 
 // Grows a per-record array to n entries, filling what is new with `fill` bytes.
@@ -1805,8 +2008,10 @@ int construct_online(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, ho
 // The query's tables and the kernels' arguments for one liveness query or fixpoint solve.
 int live_setup(hora_hip_ctx* c, LiveState& L, const hora_live_query* q) {
   // the full-capacity launches may ask for more than 64 KiB of LDS
-  HIP_OK(hipFuncSetAttribute((const void*)k_stop, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsWorkgroupMax));
-  HIP_OK(hipFuncSetAttribute((const void*)k_gfp, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsWorkgroupMax));
+  HIP_OK(hipFuncSetAttribute((const void*)k_stop<false>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsWorkgroupMax));
+  HIP_OK(hipFuncSetAttribute((const void*)k_stop<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsWorkgroupMax));
+  HIP_OK(hipFuncSetAttribute((const void*)k_gfp<false>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsWorkgroupMax));
+  HIP_OK(hipFuncSetAttribute((const void*)k_gfp<true>, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsWorkgroupMax));
   if (c->dn.ncc > 0) { c->err = "open the context with a target without clock constraints for liveness"; return HORA_HIP_E_ARG; }
   if (q->kind < HORA_LIVE_LEADSTO || q->kind > HORA_LIVE_DEADLOCK) return HORA_HIP_E_ARG;
   const int d2 = c->d2;
@@ -1877,7 +2082,11 @@ int solve_graph(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, long lo
     HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
     launch_stop(c, L, nullptr, (long)N, 0, 2);
     const LiveCtr h = read_lc(c, L);
-    if (h.err) { c->err = "federation capacity exceeded in a deadlock test"; return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY; }
+    if (h.err) {
+      c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a deadlock test" : "invariant or transition table too large";
+      st->capped = h.err == ERR_CAPPED;
+      return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
+    }
     *holds = h.hit ? 0 : 1;
   } else {
     // ---- closure S
@@ -1987,23 +2196,43 @@ int solve_graph(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, long lo
         LiveCtr h{};
         for (;;) {
           if (nd > 0)
-            hipLaunchKernelGGL(k_gfp, dim3(nd), dim3(64), KA.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
-                               L.slist.p, L.dlist.p, KA, banks(), L.rlist.p, L.lc.p);
+            hipLaunchKernelGGL(k_gfp<false>, dim3(nd), dim3(64), KA.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+                               L.slist.p, L.dlist.p, KA, banks(), L.rlist.p, L.lc.p, (int*)nullptr);
           h = read_lc(c, L);
           const int* full_list = L.rlist.p;
           if (h.rcnt > 0 && !h.err && !h.pfull && gfp_mid) {
             const int nm = h.rcnt;
             L.rlist2.ensure(nm);
             HIP_OK(hipMemsetAsync(&L.lc.p->rcnt, 0, sizeof(int), c->stream));
-            hipLaunchKernelGGL(k_gfp, dim3(nm), dim3(64), KM.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
-                               L.slist.p, L.rlist.p, KM, banks(), L.rlist2.p, L.lc.p);
+            hipLaunchKernelGGL(k_gfp<false>, dim3(nm), dim3(64), KM.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+                               L.slist.p, L.rlist.p, KM, banks(), L.rlist2.p, L.lc.p, (int*)nullptr);
             h = read_lc(c, L);
             full_list = L.rlist2.p;
           }
+          long long hbm_round = 0;
           if (h.rcnt > 0 && !h.err && !h.pfull) {
-            hipLaunchKernelGGL(k_gfp, dim3(h.rcnt), dim3(64), KB.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
-                               L.slist.p, full_list, KB, banks(), (int*)nullptr, L.lc.p);
+            // the full LDS lists; the nodes that outgrow them rerun with their lists in device memory
+            const int nf = h.rcnt;
+            L.rlist3.ensure(nf);
+            HIP_OK(hipMemsetAsync(&L.lc.p->rcnt, 0, sizeof(int), c->stream));
+            hipLaunchKernelGGL(k_gfp<true>, dim3(nf), dim3(64), KB.ints(d2) * sizeof(int), c->stream, c->dnp, L.argsp,
+                               L.slist.p, full_list, KB, banks(), fed_hbm_on() ? L.rlist3.p : (int*)nullptr,
+                               L.lc.p, (int*)nullptr);
             h = read_lc(c, L);
+            if (h.rcnt > 0 && !h.err && !h.pfull) {
+              const GfpCaps KH{hh_fed::kFedCap, hh_fed::kFedCap, hh_fed::kFedCap, KB.ic, KB.np, KB.ni};
+              const size_t per = KH.hbm_ints(d2);
+              const long batch = fed_hbm_batch(h.rcnt, per);
+              L.fedhbm.ensure(per * (size_t)batch);
+              hbm_round = h.rcnt;
+              L.hbm_nodes += h.rcnt;
+              const GfpBanks bk = banks();
+              for (long b0 = 0; b0 < h.rcnt; b0 += batch)
+                HIP_OK(hh_fed::launch_gfp_hbm(c->dnp, L.argsp, L.slist.p, L.rlist3.p + b0,
+                                              (int)std::min<long>(batch, h.rcnt - b0), &KH, &bk, L.lc.p, L.fedhbm.p,
+                                              c->stream));
+              h = read_lc(c, L);
+            }
           }
           if (!h.pfull || h.err) break;
           // the pool ran out: drop the round's writes, compact with room for every value at its
@@ -2015,7 +2244,9 @@ int solve_graph(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, long lo
           r.rcnt = 0;
           r.pfull = 0;
           HIP_OK(hipMemcpyAsync(L.lc.p, &r, sizeof(LiveCtr), hipMemcpyHostToDevice, c->stream));
-          compact_values(c, L, ns, sel_in, (long long)L.args.kw * nd + (tight ? 0 : ns / 2));
+          compact_values(c, L, ns, sel_in,
+                         (long long)L.args.kw * nd + (long long)(hh_fed::kFedCap - L.args.kw) * hbm_round +
+                             (tight ? 0 : ns / 2));
           ++compactions;
         }
         retried += h.rcnt;
@@ -2067,6 +2298,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
   }
   if (!c->live) c->live = new LiveState();
   LiveState& L = *c->live;
+  HbmCount hbm_count(L, st);
   L.graph_valid = false;
   const int d2 = c->d2;
   {
@@ -2304,6 +2536,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
         const LiveCtr h = read_lc(c, L);
         if (h.err) {
           c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a stop set" : "invariant or transition table too large";
+          st->capped = h.err == ERR_CAPPED;
           return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
         }
         if (h.hit) { early = q->kind == HORA_LIVE_EG ? 1 : 0; st->early = 1; }
@@ -2370,6 +2603,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
       const LiveCtr h = read_lc(c, L);
       if (h.err) {
         c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a seed test" : "invariant or transition table too large";
+        st->capped = h.err == ERR_CAPPED;
         return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
       }
       seeds = h.hit;
@@ -2417,6 +2651,7 @@ int gfp_solve(hora_hip_ctx* c, LiveState& L, const hora_gfp_graph* g, const hora
       return HORA_HIP_E_SCOPE;
     }
   }
+  HbmCount hbm_count(L, st);
   {
     const int rc = live_setup(c, L, q);
     if (rc != HORA_HIP_OK) return rc;

@@ -28,7 +28,7 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1
 // M = ceil(2^16 / d) = (2^16 + r) / d, r < d, floor(e*M / 2^16) = floor(e / d) since e*r < 2^16.
 // (A division by a run-time d costs a dozen vector instructions per entry; even the one division
 // that forms M costs twenty instructions where it is inlined, so M comes from a table.)
-__device__ __constant__ unsigned kEdivM[33] = {
+static __device__ __constant__ unsigned kEdivM[33] = {
     0,    65536, 32768, 21846, 16384, 13108, 10923, 9363, 8192, 7282, 6554, 5958, 5462, 5042, 4682, 4370, 4096,
     3856, 3641,  3450,  3277,  3121,  2979,  2850,  2731, 2622, 2521, 2428, 2341, 2260, 2185, 2115, 2048};
 __device__ __forceinline__ int ediv(int e, int d) { return (int)(((unsigned)e * kEdivM[d]) >> 16); }
@@ -54,10 +54,18 @@ __device__ __forceinline__ int r_neg(int r) {
 // keeps the accesses in program order and that earlier LDS operations have completed
 // (lgkmcnt(0)). A workgroup barrier (__syncthreads) would also wait for every outstanding global
 // load and store (vmcnt(0) counts both on CDNA), stalling each zone operation on HBM latency.
+//
+// Built with HORA_FED_HBM (the liveness kernels' device-memory tier, whose federation lists live in
+// global memory), the ordering point is a workgroup-scope fence instead: it also waits for the
+// wavefront's vector memory operations, so a lane reads what another lane stored to global memory.
 __device__ __forceinline__ void wsync() {
+#ifdef HORA_FED_HBM
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+#else
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
   __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0); vmcnt and expcnt left at their maxima
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
+#endif
 }
 
 // Full workgroup barrier with memory ordering: for global memory written by one lane and read

@@ -84,7 +84,7 @@ class hora_live_query(C.Structure):
 
 _STAT_I64 = ("outer_nodes", "fwd_edges", "disc_keys", "expanded_nodes", "generated", "covered", "levels",
              "closure", "nonempty", "iters", "pre_calls", "fed_max", "products", "chain_reads",
-             "retired", "unions", "enlarged")
+             "retired", "unions", "enlarged", "fed_hbm_nodes")
 
 
 class hora_live_stats(C.Structure):
@@ -127,6 +127,7 @@ class LiveStats:
     retired: int = 0
     unions: int = 0
     enlarged: int = 0
+    fed_hbm_nodes: int = 0
     early: int = 0
     capped: int = 0
     route: int = 0

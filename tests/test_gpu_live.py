@@ -156,6 +156,52 @@ def test_rerun_through_middle_tier(monkeypatch):
             assert r == want
 
 
+def test_device_memory_tier(monkeypatch):
+    """Lists of one or two zones in every LDS launch: the nodes that outgrow the full LDS lists run
+    again with their federation lists in device memory (up to 4096 zones, the reference's cap).
+    Graphs, stop sets and converged values must not change, and that tier must have run."""
+    monkeypatch.setenv("HORA_HIP_GFP_CAPS", "1,1,2,1")
+    monkeypatch.setenv("HORA_HIP_GFP_MID_CAPS", "1,1,2")
+    monkeypatch.setenv("HORA_HIP_GFP_FULL_CAPS", "1,1,2")
+    monkeypatch.setenv("HORA_HIP_STOP_CAPS", "1,1,1,1")
+    monkeypatch.setenv("HORA_HIP_STOP_FULL_CAPS", "1,1")
+    tier = 0
+    net = models.fischer(5)
+    for label, q, want in fischer_queries(net):
+        r, st = check(net, q, label)
+        assert r == want, label
+        tier += st.fed_hbm_nodes
+    for builder in live_models.ALL:
+        net, queries = builder()
+        for q, want in queries:
+            r, st = check(net, q)
+            assert r == want
+            tier += st.fed_hbm_nodes
+    assert tier > 0
+
+
+def test_capped_without_device_memory_tier(monkeypatch):
+    """With the device-memory tier off (HORA_HIP_FED_HBM=0) and full LDS lists of one or two zones,
+    a query whose federations outgrow them stops with HORA_HIP_E_CAPPED (the reference's `capped`,
+    no verdict), and leaves no graph to read."""
+    monkeypatch.setenv("HORA_HIP_FED_HBM", "0")
+    monkeypatch.setenv("HORA_HIP_GFP_FULL_CAPS", "1,1,2")
+    monkeypatch.setenv("HORA_HIP_STOP_FULL_CAPS", "1,1")
+    net = models.fischer(5)
+    capped = 0
+    with hip.HipContext(net, None) as ctx:
+        for label, q, want in fischer_queries(net):
+            try:
+                r, _ = ctx.live(q)
+                assert r == want, label
+            except hip.Capped as exc:
+                capped += 1
+                assert exc.stats.capped == 1
+                with pytest.raises(hip.HipError, match="hora_hip_live_graph"):
+                    ctx.live_graph()
+    assert capped > 0
+
+
 def test_pools_grow_and_compact(monkeypatch):
     """Stop sets and fixpoint values live in pools. With pools that start at the least they can
     hold and grow by exactly what is asked, the stop-set phase runs again after growing, and rounds
