@@ -410,18 +410,18 @@ __device__ __forceinline__ bool z_apply_list(int* z, int d, const int* cs, int n
 
 // Reset every clock in `mask` at once. Resetting x_k copies row 0 into row k and column 0 into
 // column k; for a set K that is z[i][j] := z[i'][j'] with i' = 0 if i is in K (else i), likewise
-// j' -- the composition of the single resets in any order. `tmp` is d*d LDS scratch.
+// j' -- the composition of the single resets in any order. In place, in one pass: every entry
+// read has both indices outside K (0 is never in K) and every entry written has one inside, so no
+// lane reads an entry any lane writes, whatever the order. (`tmp` is unused.)
 __device__ __forceinline__ void z_reset_many(int* z, int d, u64 mask, int* tmp) {
   if (!mask) return;
   wsync();
   const int l = lane_id(), d2 = d * d;
   for (int e = l; e < d2; e += kWave) {
     const int i = ediv(e, d), j = e - i * d;
-    const int si = ((mask >> i) & 1) ? 0 : i, sj = ((mask >> j) & 1) ? 0 : j;
-    tmp[e] = z[si * d + sj];
+    const bool ri = (mask >> i) & 1, rj = (mask >> j) & 1;
+    if (ri || rj) z[e] = z[(ri ? 0 : i) * d + (rj ? 0 : j)];
   }
-  wsync();
-  for (int e = l; e < d2; e += kWave) z[e] = tmp[e];
 }
 
 __device__ __forceinline__ u64 edge_reset_mask(const DevNet& N, int e) {
@@ -861,9 +861,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   auto finish = [&](int slot, int se, int re, u64 moved, int g1, int g2, int fz, u64 am, u64 rm, u64 key,
                     int tdisc) __attribute__((always_inline)) {
     wsync();
-    z_copy(z, src, d2);
     bool ok = true;
-    if (se >= 0) {
+    const bool guarded = se >= 0 && ((BC && re <= -2) || N.cg_off[se + 1] > N.cg_off[se] ||
+                                     (re >= 0 && N.cg_off[re + 1] > N.cg_off[re]));
+    if (se >= 0 && !guarded) {
+      // no clock guard: the copy and every reset in one pass (z_reset_many's gather)
+      for (int e = l; e < d2; e += kWave) {
+        const int i = ediv(e, d), j = e - i * d;
+        z[e] = src[(((rm >> i) & 1) ? 0 : i) * d + (((rm >> j) & 1) ? 0 : j)];
+      }
+    } else {
+      z_copy(z, src, d2);
+    }
+    if (guarded) {
       // guards (sender, then the receiver or each broadcast part), then every reset at once
       ok = apply_edge_guard(N, se, z);
       if (ok && re >= 0) ok = apply_edge_guard(N, re, z);
@@ -1013,8 +1023,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 
 // Exclusive scan, three phases: per-block totals, a scan of the totals, per-block downsweep.
 __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int* __restrict__ in, long n,
-                                                            long long* __restrict__ bsum) {
+                                                            long long* __restrict__ bsum, const int* __restrict__ np) {
   __shared__ long long s[kScanBlock];
+  if (np) n = min(n, (long)*np);   // n an upper bound, the length in device memory
   const long base = (long)blockIdx.x * kScanBlock * kScanItems;
   long long acc = 0;
   for (int k = 0; k < kScanItems; ++k) {
@@ -1055,8 +1066,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_sums(long long* __restrict_
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int* __restrict__ in, long n,
                                                           const long long* __restrict__ bsum,
-                                                          long long* __restrict__ out) {
+                                                          long long* __restrict__ out, const int* __restrict__ np) {
   __shared__ long long s[kScanBlock];
+  if (np) n = min(n, (long)*np);
   const long base = (long)blockIdx.x * kScanBlock * kScanItems;
   long long carry = bsum[blockIdx.x];
   for (int k = 0; k < kScanItems; ++k) {
@@ -1079,8 +1091,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int* __restrict_
 // The whole scan in one workgroup, for n <= kScanBlock * kScanItems: one launch instead of three.
 __global__ __launch_bounds__(kScanBlock) void k_scan_small(const int* __restrict__ in, long n,
                                                            long long* __restrict__ out,
-                                                           long long* __restrict__ total) {
+                                                           long long* __restrict__ total, const int* __restrict__ np) {
   __shared__ long long s[kScanBlock];
+  if (np) n = min(n, (long)*np);
   long long carry = 0;
   for (int k = 0; k < kScanItems; ++k) {
     const long idx = (long)k * kScanBlock + threadIdx.x;
@@ -1287,10 +1300,12 @@ __host__ __device__ inline u64 list_cap(u64 n) {
 // (twice the working chain, so that a working chain longer than the LDS copy fits beside its
 // cache-slot table).
 // (with `ctr`, also clears the light/heavy key counts)
+// T: the touched keys, or with Tp an upper bound and the count *Tp in device memory.
 __global__ void k_touched_caps(Passed P, const u32* __restrict__ touched, int T, int* __restrict__ caps,
-                               int* __restrict__ segs, Counters* ctr) {
+                               int* __restrict__ segs, Counters* ctr, const int* __restrict__ Tp) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t == 0 && ctr) { ctr->nlight = 0; ctr->nheavy = 0; ctr->npair = 0; }
+  if (Tp) T = min(T, *Tp);
   if (t >= T) return;
   const u32 h = touched[t];
   caps[t] = (int)list_cap(P.ncnt[h]);
@@ -1445,7 +1460,7 @@ __device__ __forceinline__ bool union_convex_lane(ZA a, ZB b, int d) {
 constexpr int kPairTile = 32;
 constexpr int kPairMaxN = 1 << 18;
 constexpr int kKillSlots = 4;
-constexpr int kWideCap = 1024;  // widened members tracked per key (more: full scans from then on)
+constexpr int kWideCap = 1024;  // widened members tracked per key in LDS (more: in device memory)
 constexpr u64 kPairMinWork = 8192;
 __host__ __device__ inline bool pair_eligible(u32 n, u32 old, int d) {
   return d <= 16 && n <= (u32)kPairMaxN && (u64)n * (u64)(old + n) >= kPairMinWork;
@@ -1505,7 +1520,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
                                               const int* __restrict__ pdrop, const int* __restrict__ kcnt,
                                               const u32* __restrict__ kslot, const long long* __restrict__ koff,
                                               const u32* __restrict__ klist, int* __restrict__ mpos,
-                                              int* __restrict__ zdead, u64 lvl_zbase) {
+                                              int* __restrict__ zdead, u64 lvl_zbase, int* __restrict__ wlg) {
   extern __shared__ int sm[];
   const DevNet& N = *Np;
   const int d = N.d, d2 = N.d2, l = threadIdx.x;
@@ -1634,6 +1649,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
   bool pairs_full = use_pairs;
   unsigned long long n_full = 0, n_wovf = 0;
   int nw = 0;
+  // the widened members' chain positions: in LDS, or, past its wide_cap entries, in the key's
+  // slice of wlg (one int per kept successor, as many as the key can widen)
+  int* wlp = wl;
   const u64 pbase = P.noff[h];
   if (use_pairs)
     for (u32 k = l; k < n; k += kWave) mpos[pbase + k] = -1;
@@ -1659,7 +1677,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
         for (int b0 = 0; b0 < nw && !covered; b0 += kWave) {
           bool c = false;
           if (b0 + l < nw) {
-            const int m = wl[b0 + l];
+            const int m = wlp[b0 + l];
             if (!(ch[m] & kDead))
               c = zwith(m, [&](auto Z) { return (PRE && d <= 16) ? alu_le_lane16<false>(S, Z, d, Lb, Ub) : alu_le_lane(S, Z, d, Lb, Ub); });
           }
@@ -1715,7 +1733,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
           bool kw = false;
           int mw = -1;
           if (b0 + l < nw) {
-            mw = wl[b0 + l];
+            mw = wlp[b0 + l];
             if (!(ch[mw] & kDead))
               kw = zwith(mw, [&](auto Z) { return (PRE && d <= 16) ? alu_le_lane16<true>(Z, S, d, Lb, Ub) : alu_le_lane(Z, S, d, Lb, Ub); });
           }
@@ -1834,7 +1852,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
       if (use_pairs) mpos[pbase + k] = len;
     }
     if (pairs_full && widened) {
-      if (nw == ml.wide_cap) {
+      if (wlp == wl && nw == ml.wide_cap) {
         // full: keep the members still live
         int w2 = 0;
         for (int b0 = 0; b0 < nw; b0 += kWave) {
@@ -1848,10 +1866,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
           w2 += __popcll(b);
         }
         nw = w2;
+        if (nw == ml.wide_cap && wlg) {
+          // still full: the list moves to the key's slice of wlg (an entry per kept successor, and
+          // each successor widens at most one member, its own)
+          int* gl = wlg + pbase;
+          for (int b0 = l; b0 < nw; b0 += kWave) gl[b0] = wl[b0];
+          bsync();
+          wlp = gl;
+        }
       }
-      if (nw < ml.wide_cap) {
-        if (l == 0) wl[nw] = len;
+      if (wlp != wl || nw < ml.wide_cap) {
+        if (l == 0) wlp[nw] = len;
         ++nw;
+        if (wlp != wl) bsync();   // every lane reads the list from device memory
       } else {
         pairs_full = false;
         n_wovf = 1;
@@ -2114,8 +2141,9 @@ __global__ void k_kill_need(const int* __restrict__ pdrop, const int* __restrict
 
 // light/heavy classification of touched keys by working-chain size
 __global__ void k_classify(Passed P, const u32* __restrict__ touched, int T, u32* __restrict__ light,
-                           u32* __restrict__ heavy, Counters* ctr, int light_max, int d) {
+                           u32* __restrict__ heavy, Counters* ctr, int light_max, int d, const int* __restrict__ Tp) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (Tp) T = min(T, *Tp);
   const u32 h = t < T ? touched[t] : 0u;
   const bool ok = t < T;
   const u32 w = ok ? P.len[h] + P.ncnt[h] : 0u;
@@ -2818,6 +2846,7 @@ struct hora_hip_ctx {
   // tile counts and offsets per heavy key
   DBuf<int> pdrop, kcnt, mpos, tcnt, kneed, kpos;
   DBuf<int> zdead;            // a chunked level's retirement marks for the zones it stored
+  DBuf<int> wlg;              // heavy keys' widened-member lists past their LDS capacity
   DBuf<long long> cuts;       // chunk boundaries of a level
   DBuf<u32> kslot, klist;
   DBuf<long long> toff, koff;
@@ -2858,24 +2887,21 @@ void read_dev(hora_hip_ctx* c, void* dst, const void* src, size_t n, void* dst2 
 
 namespace {
 
-// Exclusive scan of in[0, n) into out, the total into scan_total[slot] (no synchronisation).
-void scan_launch(hora_hip_ctx* c, const int* in, long n, long long* out, int slot) {
+// Exclusive scan of in[0, n) into out, the total into scan_total[slot] (no synchronisation). With
+// `np`, n is an upper bound and the length is *np, in device memory (written by an earlier kernel
+// of the stream, so the host need not read it first).
+void scan_launch(hora_hip_ctx* c, const int* in, long n, long long* out, int slot, const int* np = nullptr) {
   const long per = (long)kScanBlock * kScanItems;
   const long nb = std::max<long>(1, (n + per - 1) / per);
   c->scan_total.ensure(2);
   if (n <= per) {
-    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kScanBlock), 0, c->stream, in, n, out, c->scan_total.p + slot);
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kScanBlock), 0, c->stream, in, n, out, c->scan_total.p + slot, np);
     return;
   }
   c->bsum.ensure(nb);
-  if (n > 0) {
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, c->stream, in, n, c->bsum.p);
-  } else {
-    HIP_OK(hipMemsetAsync(c->bsum.p, 0, sizeof(long long), c->stream));
-  }
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, c->stream, in, n, c->bsum.p, np);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, c->stream, c->bsum.p, nb, c->scan_total.p + slot);
-  if (n > 0)
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanBlock), 0, c->stream, in, n, c->bsum.p, out);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanBlock), 0, c->stream, in, n, c->bsum.p, out, np);
 }
 
 long long scan_exclusive(hora_hip_ctx* c, const int* in, long n, long long* out) {
@@ -3230,29 +3256,37 @@ int search_once(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st, int n
       HIP_OK(hipEventRecord(c->ev[6], c->stream));
       hipLaunchKernelGGL(k_ticket, dim3(blocks_for(G, 1024)), dim3(1024), 0, c->stream, P, c->gbk.p, c->covered.p, G,
                          c->gticket.p, c->touched.p, c->ctr.p);
-      int T = 0;
-      read_dev(c, &T, &c->ctr.p->ntouched, sizeof(int));
-      c->caps.ensure(T);
-      c->segs.ensure(T);
-      hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
-                         c->caps.p, c->segs.p, c->ctr.p);
+      // The touched keys' count stays on the device: the per-key kernels and the two scans take G
+      // (at least the count) as their size and read the count itself, and the host reads it with
+      // the scans' totals, one synchronisation for all.
+      const int* Tdev = &c->ctr.p->ntouched;
+      c->caps.ensure(G);
+      c->segs.ensure(G);
+      hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, P, c->touched.p, (int)G,
+                         c->caps.p, c->segs.p, c->ctr.p, Tdev);
       // light and heavy keys (k_merge's two shapes) are known once the chains are counted: the
       // classification rides on the same synchronisation as the two scans
-      c->klight.ensure(T);
-      c->kheavy.ensure(T);
-      hipLaunchKernelGGL(k_classify, dim3(blocks_for(T, 1024)), dim3(1024), 0, c->stream, P, c->touched.p, T,
-                         c->klight.p, c->kheavy.p, c->ctr.p, kLightMax, c->d);
-      c->pos.ensure(T + 1);
-      c->pos2.ensure(T + 1);
+      c->klight.ensure(G);
+      c->kheavy.ensure(G);
+      hipLaunchKernelGGL(k_classify, dim3(blocks_for(G, 1024)), dim3(1024), 0, c->stream, P, c->touched.p, (int)G,
+                         c->klight.p, c->kheavy.p, c->ctr.p, kLightMax, c->d, Tdev);
+      c->pos.ensure(G + 1);
+      c->pos2.ensure(G + 1);
       long long W = 0, Sseg = 0;
+      int T = 0;
       int nlh[3] = {0, 0, 0};   // light keys, heavy keys, heavy keys for the pair pre-pass
       {
-        scan_launch(c, c->caps.p, T, c->pos.p, 0);
-        scan_launch(c, c->segs.p, T, c->pos2.p, 1);
+        scan_launch(c, c->caps.p, G, c->pos.p, 0, Tdev);
+        scan_launch(c, c->segs.p, G, c->pos2.p, 1, Tdev);
         long long tt[2] = {0, 0};
-        read_dev(c, tt, c->scan_total.p, sizeof(tt), nlh, &c->ctr.p->nlight, sizeof(nlh));
+        int tl[4] = {0, 0, 0, 0};   // ntouched, nlight, nheavy, npair: adjacent in Counters
+        read_dev(c, tt, c->scan_total.p, sizeof(tt), tl, &c->ctr.p->ntouched, sizeof(tl));
         W = tt[0];
         Sseg = tt[1];
+        T = tl[0];
+        nlh[0] = tl[1];
+        nlh[1] = tl[2];
+        nlh[2] = tl[3];
       }
       // capacity: the reserved chain segments, and zone ids for at most W surviving new zones (the
       // ids are handed out compactly after the merge; a chain entry holds an id below kWideTag)
@@ -3327,12 +3361,14 @@ int search_once(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st, int n
       if (nlh[0] > 0)
         hipLaunchKernelGGL(merge_plain, dim3(nlh[0]), dim3(64), lds_l, c->stream, c->dnp, P, c->klight.p, nlh[0],
                            c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 0, c->zscr.p, c->zused.p,
-                           c->flag_i.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, zdead, lvl_zbase);
+                           c->flag_i.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, zdead, lvl_zbase,
+                           (int*)nullptr);
+      if (pre) c->wlg.ensure(W + 1);
       if (nlh[1] > 0)
         hipLaunchKernelGGL(pre ? merge_pre : merge_plain, dim3(nlh[1]), dim3(64), lds_h, c->stream, c->dnp, P,
                            c->kheavy.p, nlh[1], c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 1,
                            c->zscr.p, c->zused.p, c->flag_i.p, c->pdrop.p, c->kcnt.p, c->kslot.p, c->koff.p,
-                           c->klist.p, c->mpos.p, zdead, lvl_zbase);
+                           c->klist.p, c->mpos.p, zdead, lvl_zbase, pre ? c->wlg.p : (int*)nullptr);
       HIP_OK(hipEventRecord(c->ev[4], c->stream));
       ncount += (u64)Sseg;
       // zone ids: the surviving new zones of the chunk, compact, in slot order (the count is read

@@ -1834,11 +1834,11 @@ int construct_online(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, ho
           c->caps.ensure(T);
           c->segs.ensure(T);
           hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
-                             c->caps.p, c->segs.p, c->ctr.p);
+                             c->caps.p, c->segs.p, c->ctr.p, (const int*)nullptr);
           c->klight.ensure(T);
           c->kheavy.ensure(T);
           hipLaunchKernelGGL(k_classify, dim3(blocks_for(T, 1024)), dim3(1024), 0, c->stream, P, c->touched.p, T,
-                             c->klight.p, c->kheavy.p, c->ctr.p, kLightMax, c->d);
+                             c->klight.p, c->kheavy.p, c->ctr.p, kLightMax, c->d, (const int*)nullptr);
           c->pos2.ensure(T + 1);
           L.segpos.ensure(T + 1);
           const long long W = scan_exclusive(c, c->caps.p, T, c->pos2.p);
@@ -2430,7 +2430,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
           c->caps.ensure(T);
           c->segs.ensure(T);
           hipLaunchKernelGGL(k_touched_caps, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T,
-                             c->caps.p, c->segs.p, (Counters*)nullptr);
+                             c->caps.p, c->segs.p, (Counters*)nullptr, (const int*)nullptr);
           c->pos2.ensure(T + 1);
           const long long Wl = scan_exclusive(c, c->caps.p, T, c->pos2.p);
           hipLaunchKernelGGL(k_gsegs, dim3(blocks_for(T, 256)), dim3(256), 0, c->stream, P, c->touched.p, T, c->segs.p);

@@ -229,6 +229,21 @@ __device__ __forceinline__ bool z_includes(const int* a, const int* b, int d) {
 __device__ __forceinline__ bool z_try_union(int* a, const int* b, int d, int* fa, int* pa, int* cnt) {
   wsync();
   const int l = lane_id(), d2 = d * d;
+  {
+    // A gap between the two zones' projections on some x_i - x_j (canonical: a's upper bound
+    // a[i][j], b's lower bound -b[j][i]) makes the union non-convex, as the projection of a convex
+    // set is an interval: the answer is "no" without the facet pairs.
+    bool gap = false;
+    for (int e = l; e < d2; e += kWave) {
+      const int i = ediv(e, d), j = e - i * d;
+      const int ra = a[e], rb = b[j * d + i];
+      if (i != j && ra < kInf && rb < kInf) {
+        const int s = r_bnd(ra) + r_bnd(rb);
+        if (s < 0 || (s == 0 && r_strict(ra) && r_strict(rb))) gap = true;
+      }
+    }
+    if (__any(gap)) return false;
+  }
   if (l == 0) { cnt[0] = 0; cnt[1] = 0; }
   wsync();
   // ballot compaction of the differing entries, in index order
