@@ -328,7 +328,8 @@ int hora_hip_gfp_solve(hora_hip_ctx* ctx, const hora_gfp_graph* g, const hora_li
  * buffer carved from one file-scope device array of 256 MiB per GPU (held until the process ends).
  * A graph of more nodes than fit (256 MiB / (80 + 16 d^2) bytes: 96,421 nodes at zone
  * dimension 13), or a run whose pools outgrow the array, returns HORA_HIP_E_MEMORY with the reason in
- * hora_hip_last_error. Every call on a GPU shares that GPU's array: calls that overlap on one GPU,
+ * hora_hip_last_error. Federations are held in LDS only here (no device-memory lists): one that
+ * outgrows them returns HORA_HIP_E_CAPPED. Every call on a GPU shares that GPU's array: calls that overlap on one GPU,
  * from any contexts, race and must be serialised by the caller. The context must have been opened
  * (hora_hip_open allocates, once). */
 int hora_hip_gfp_solve_noalloc(hora_hip_ctx* ctx, const hora_gfp_graph* g, const hora_live_query* q,

@@ -1527,8 +1527,10 @@ struct HbmCount {
 };
 
 // HORA_HIP_FED_HBM=0 turns the device-memory tier off: a federation that outgrows the full LDS
-// lists then stops the query with HORA_HIP_E_CAPPED.
+// lists then stops the query with HORA_HIP_E_CAPPED. So does the allocation-free entry point (a
+// fixed scratch, tl_arena), whose scratch is sized for the graph, not for lists of 4096 zones.
 bool fed_hbm_on() {
+  if (tl_arena) return false;
   const char* e = std::getenv("HORA_HIP_FED_HBM");
   return !(e && *e == '0');
 }
