@@ -226,8 +226,8 @@ typedef struct hora_live_query {
                                 state can seed a deadlock atom (online_seed_key, src/liveness/
                                 refinement.cpp:176-282), then the a_M graph; p --> q on a network
                                 where D is provably a_LU (structurally_empty_d, src/liveness/
-                                internal.cpp:59-103) takes the base construction (:863-1003), which
-                                the device does not run (HORA_HIP_E_SCOPE: the caller falls back).
+                                internal.cpp:59-103) takes the base construction (api.cpp:37-117,
+                                construction.cpp:863-1096), one node expanded per step.
                                 HORA_COVER_ONLINE: as AUTO, but the online construction there too.
                                 HORA_COVER_AM: the a_M graph always. */
   int32_t pad;

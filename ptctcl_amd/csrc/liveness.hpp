@@ -15,8 +15,9 @@
 //     bounds raised to M = max(L, U): a bisimulation, so deadlocks survive the quotient;
 //     construction.cpp:899-913), else it becomes a new node.
 // p --> q on a network where D is provably a_LU takes the reference's base construction
-// (construction.cpp:863-1003, with in-place enlargement), which this pass does not run: it returns
-// HORA_HIP_E_SCOPE, and the caller falls back.
+// (construction.cpp:863-1096: a FIFO of nodes, each successor covered by the oldest node of its
+// discrete state, else enlarging the oldest node it covers in place, else merged by a convex union,
+// else a new node), one node per step (base.hpp).
 // Node ids are creation order: frontier index, then transition index, as a sequential pass makes
 // them. The a_M graph's kernels:
 //
@@ -1388,6 +1389,7 @@ hipError_t launch_gfp_hbm(const void* dnp, const void* argsp, const int* slist, 
 #endif
 
 #include "online.hpp"
+#include "base.hpp"
 
 // ---------------------------------------------------------------- host driver
 
@@ -1415,6 +1417,12 @@ struct LiveState {
   // backward pass
   DBuf<int> in_s, slist, sidx, scnt, soff, sz, wcnt0, wcnt1, woff0, woff1, sel0, sel1, chg0, chg1, heads, dlist, rlist,
       rlist2, rlist3, wlist;
+  // base construction: its counters, node links, edge lists, key table and FIFO
+  DBuf<BaseCtr> bctr;
+  DBuf<int> bknext, behead, betail, benext, betgt, bese, bere, bqueue, bqueue2, bhhead, bhtail;
+  DBuf<u64> bhkey;
+  DBuf<signed char> binq;
+  int bhbits = 0;
   DBuf<int> fedhbm;        // federation lists of the device-memory tier
   long long hbm_nodes = 0; // node launches of the device-memory tier in the last query
   DBuf<int> wpool[2], wtmp;   // value pools (current, compaction target); per-node counts
@@ -1667,6 +1675,14 @@ void grow_rec(DBuf<T>& b, size_t n, int fill, hipStream_t s) {
   HIP_OK(hipMemsetAsync(b.p + old, fill, (b.cap - old) * sizeof(T), s));
 }
 
+// The packed discrete key of the initial state.
+u64 initial_key(const hora_hip_ctx* c) {
+  u64 dk0 = 0;
+  for (int p = 0; p < c->np; ++p) dk0 |= (u64)(unsigned)c->h_loc_init[p] << c->h_kshift[p];
+  for (int v = 0; v < c->ni; ++v) dk0 |= (u64)(unsigned)(c->h_int_init[v] - c->h_int_lo[v]) << c->h_kshift[c->np + v];
+  return dk0;
+}
+
 // One construction of the graph by the online rule (online.hpp). Returns HORA_HIP_OK or an error
 // code; *early receives the verdict of an early witness, or -1.
 int construct_online(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, hora_live_stats* st, int* early) {
@@ -1712,9 +1728,7 @@ int construct_online(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, ho
   L.args.recnt = L.recnt.p;
   L.args.whit = L.whit.p;
 
-  u64 dk0 = 0;
-  for (int p = 0; p < c->np; ++p) dk0 |= (u64)(unsigned)c->h_loc_init[p] << c->h_kshift[p];
-  for (int v = 0; v < c->ni; ++v) dk0 |= (u64)(unsigned)(c->h_int_init[v] - c->h_int_lo[v]) << c->h_kshift[c->np + v];
+  const u64 dk0 = initial_key(c);
   c->fr_dk.ensure(1);
   c->fr_zid.ensure(1);
   HIP_OK(hipMemcpyAsync(c->fr_dk.p, &dk0, sizeof(u64), hipMemcpyHostToDevice, c->stream));
@@ -2004,6 +2018,275 @@ int construct_online(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, ho
   }
   st->outer_nodes = L.nodes;
   st->fwd_edges = L.edges;
+  return HORA_HIP_OK;
+}
+
+BaseState base_view(hora_hip_ctx* c, LiveState& L) {
+  BaseState B;
+  B.nkey = L.gdk.p;
+  B.zone = c->zarena.p;
+  B.expanded = L.expanded.p;
+  B.inq = L.binq.p;
+  B.knext = L.bknext.p;
+  B.ehead = L.behead.p;
+  B.etail = L.betail.p;
+  B.recnt = L.recnt.p;
+  B.enext = L.benext.p;
+  B.etgt = L.betgt.p;
+  B.ese = L.bese.p;
+  B.ere = L.bere.p;
+  B.hkey = L.bhkey.p;
+  B.hhead = L.bhhead.p;
+  B.htail = L.bhtail.p;
+  B.queue = L.bqueue.p;
+  B.hbits = L.bhbits;
+  B.bc = L.bctr.p;
+  return B;
+}
+
+// A fresh key table of 2^bits slots (the nodes re-linked into it by k_base_rehash).
+void base_table(hora_hip_ctx* c, LiveState& L, int bits) {
+  const size_t H = (size_t)1 << bits;
+  L.bhbits = bits;
+  L.bhkey.ensure(H);
+  L.bhhead.ensure(H);
+  L.bhtail.ensure(H);
+  HIP_OK(hipMemsetAsync(L.bhkey.p, 0xff, H * sizeof(u64), c->stream));   // kEmptyKey
+}
+
+// Whether the initial discrete state can seed a deadlock atom (mode 3 of k_stop on the initial key,
+// online_seed_key, refinement.cpp:176-282): on a network with one possible discrete state, the
+// negation of structurally_empty_d's test (src/liveness/internal.cpp:59-103).
+int initial_key_seeds(hora_hip_ctx* c, LiveState& L, hora_live_stats* st, bool* seeds) {
+  const u64 dk0 = initial_key(c);
+  L.gdk.ensure(1);
+  HIP_OK(hipMemcpyAsync(L.gdk.p, &dk0, sizeof(u64), hipMemcpyHostToDevice, c->stream));
+  c->zarena.ensure(c->d2);
+  sync_graph_args(L);
+  L.args.gzone = c->zarena.p;
+  L.args.recnt = nullptr;
+  L.args.whit = nullptr;
+  upload_args(c, L);
+  HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
+  launch_stop(c, L, nullptr, 1, 0, 3);
+  const LiveCtr h = read_lc(c, L);
+  if (h.err) {
+    c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a seed test" : "invariant or transition table too large";
+    st->capped = h.err == ERR_CAPPED;
+    return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
+  }
+  *seeds = h.hit > 0;
+  return HORA_HIP_OK;
+}
+
+// The base construction (base.hpp): one node expanded per step, in FIFO order. Returns HORA_HIP_OK
+// or an error code; *early receives the verdict of an early witness, or -1.
+int construct_base(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, hora_live_stats* st, int* early) {
+  const int d2 = c->d2;
+  *early = -1;
+  {
+    const int32_t seeds = st->seed_keys;
+    std::memset(st, 0, sizeof(*st));
+    st->seed_keys = seeds;
+  }
+  HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
+  c->dn.lu_plus = 0;
+  c->dn.znar = 0;
+  HIP_OK(hipMemcpyAsync(c->dnp, &c->dn, sizeof(DevNet), hipMemcpyHostToDevice, c->stream));
+  c->ctr.ensure(1);
+  HIP_OK(hipMemsetAsync(c->ctr.p, 0, sizeof(Counters), c->stream));
+  {
+    const int ft = 0x7fffffff;
+    HIP_OK(hipMemcpyAsync(&c->ctr.p->first_target, &ft, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  }
+  // node, edge and FIFO storage (grown below as the steps need)
+  const size_t ncap0 = 4096, ecap0 = 16384;
+  L.gdk.ensure(ncap0);
+  L.expanded.ensure(ncap0);
+  L.recnt.ensure(ncap0);
+  L.binq.ensure(ncap0);
+  L.bknext.ensure(ncap0);
+  L.behead.ensure(ncap0);
+  L.betail.ensure(ncap0);
+  L.bqueue.ensure(ncap0);
+  L.benext.ensure(ecap0);
+  L.betgt.ensure(ecap0);
+  L.bese.ensure(ecap0);
+  L.bere.ensure(ecap0);
+  c->zarena.ensure(ncap0 * d2);
+  L.bctr.ensure(1);
+  HIP_OK(hipMemsetAsync(L.bctr.p, 0, sizeof(BaseCtr), c->stream));
+  base_table(c, L, 12);
+  L.nodes = 0;
+  L.edges = 0;
+  L.ns = 0;
+  auto upload_graph_args = [&]() {
+    sync_graph_args(L);
+    L.args.gzone = c->zarena.p;
+    L.args.recnt = L.recnt.p;   // the stop set of an early test reads the node's successor count
+    L.args.whit = nullptr;
+    upload_args(c, L);
+  };
+  upload_graph_args();
+
+  // the initial node: slot 0 of the initial expansion
+  const u64 dk0 = initial_key(c);
+  c->fr_dk.ensure(1);
+  c->fr_zid.ensure(1);
+  c->rec_cnt.ensure(1);
+  c->rec_off.ensure(1);
+  c->pos.ensure(2);
+  HIP_OK(hipMemcpyAsync(c->fr_dk.p, &dk0, sizeof(u64), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemsetAsync(c->fr_zid.p, 0, sizeof(u32), c->stream));
+  HIP_OK(hipMemsetAsync(c->rec_off.p, 0, sizeof(int), c->stream));
+  HIP_OK(hipMemsetAsync(c->pos.p, 0, sizeof(long long), c->stream));
+  auto outputs = [&](long R) {
+    c->s_valid.ensure(std::max<long>(R, 1));
+    c->s_tgt.ensure(std::max<long>(R, 1));
+    c->s_dk.ensure(std::max<long>(R, 1));
+    c->s_zone.ensure((size_t)std::max<long>(R, 1) * d2);
+    L.s_se.ensure(std::max<long>(R, 1));
+    L.s_re.ensure(std::max<long>(R, 1));
+    return ExpandOut{c->s_valid.p, c->s_dk.p, c->s_zone.p, c->s_tgt.p, L.s_se.p, L.s_re.p};
+  };
+  {
+    const ExpandOut eo = outputs(1);
+    launch_expand(c, expand_tuned(c, 1, 0, eo, true), 1, 0, eo, true);
+  }
+  BaseState B = base_view(c, L);
+  hipLaunchKernelGGL(k_base_init, dim3(1), dim3(64), 0, c->stream, B, (const int*)c->s_valid.p, (const u64*)c->s_dk.p,
+                     (const int*)c->s_zone.p, d2);
+  // each step's first half: the frontier of node u and its successor slot count
+  auto stage = [&]() {
+    hipLaunchKernelGGL(k_base_frontier, dim3(1), dim3(1), 0, c->stream, B, c->fr_dk.p, c->fr_zid.p);
+    hipLaunchKernelGGL(k_count, dim3(1), dim3(256), 0, c->stream, c->dnp, c->fr_dk.p, 1, c->rec_cnt.p, c->ctr.p);
+    scan_launch(c, c->rec_cnt.p, 1, c->pos.p, 0);
+  };
+  stage();
+  BaseCtr hb{};
+  long long R = 0;
+  read_dev(c, &hb, L.bctr.p, sizeof(BaseCtr), &R, c->scan_total.p, sizeof(R));
+  long long steps = 0;
+  while (hb.u >= 0) {
+    // room for the step: R more nodes and edges, the FIFO's pushes (each node at most once), R keys
+    bool moved = false;
+    const size_t nneed = (size_t)hb.nodes + (size_t)R + 1;
+    if (nneed > L.gdk.cap || nneed > L.expanded.cap || nneed > L.recnt.cap || nneed > L.binq.cap ||
+        nneed > L.bknext.cap || nneed > L.behead.cap || nneed > L.betail.cap || nneed * d2 > c->zarena.cap) {
+      const size_t n2 = std::max(nneed, 2 * (size_t)hb.nodes);
+      grow_keep_to(L.gdk, n2, c->stream);
+      grow_keep_to(L.expanded, n2, c->stream);
+      grow_keep_to(L.recnt, n2, c->stream);
+      grow_keep_to(L.binq, n2, c->stream);
+      grow_keep_to(L.bknext, n2, c->stream);
+      grow_keep_to(L.behead, n2, c->stream);
+      grow_keep_to(L.betail, n2, c->stream);
+      if (n2 * d2 > c->zarena.cap) c->zarena.grow_keep(std::max(n2 * d2, c->zarena.cap * 2), c->stream);
+      moved = true;
+    }
+    const size_t eneed = (size_t)hb.edges + (size_t)R;
+    if (eneed > L.benext.cap || eneed > L.betgt.cap || eneed > L.bese.cap || eneed > L.bere.cap) {
+      const size_t e2 = std::max(eneed, 2 * (size_t)hb.edges);
+      grow_keep_to(L.benext, e2, c->stream);
+      grow_keep_to(L.betgt, e2, c->stream);
+      grow_keep_to(L.bese, e2, c->stream);
+      grow_keep_to(L.bere, e2, c->stream);
+      moved = true;
+    }
+    if ((size_t)hb.qt + nneed > L.bqueue.cap) {
+      // the queued part moves to the front of a buffer with room for it and the step's pushes
+      const long long len = hb.qt - hb.qh;
+      L.bqueue2.ensure(std::max<size_t>(2 * ((size_t)len + nneed), 4096));
+      if (len > 0)
+        HIP_OK(hipMemcpyAsync(L.bqueue2.p, L.bqueue.p + hb.qh, (size_t)len * sizeof(int), hipMemcpyDeviceToDevice,
+                              c->stream));
+      std::swap(L.bqueue.p, L.bqueue2.p);
+      std::swap(L.bqueue.cap, L.bqueue2.cap);
+      const long long qq[2] = {0, len};
+      HIP_OK(hipMemcpyAsync(&L.bctr.p->qh, qq, sizeof(qq), hipMemcpyHostToDevice, c->stream));
+      HIP_OK(hipStreamSynchronize(c->stream));
+      hb.qh = 0;
+      hb.qt = len;
+      moved = true;
+    }
+    if ((double)(hb.keys + R) > 0.5 * (double)((size_t)1 << L.bhbits)) {
+      int bits = L.bhbits;
+      while ((double)(hb.keys + R) > 0.5 * (double)((size_t)1 << bits)) ++bits;
+      base_table(c, L, bits);
+      moved = true;
+      B = base_view(c, L);
+      hipLaunchKernelGGL(k_base_rehash, dim3(1), dim3(64), 0, c->stream, B);
+    }
+    if (moved) {
+      B = base_view(c, L);
+      upload_graph_args();
+    }
+    // the node's successors
+    const ExpandOut eo = outputs(R);
+    if (R > 0) launch_expand(c, expand_tuned(c, 1, 0, eo, false), 1, 0, eo, false);
+    hipLaunchKernelGGL(k_base_count, dim3(1), dim3(64), 0, c->stream, B, (const int*)c->s_valid.p,
+                       (const int*)c->rec_cnt.p);
+    if (q->early) {
+      launch_stop(c, L, &L.bctr.p->u, 1, 0, 1);
+      const LiveCtr h = read_lc(c, L);
+      if (h.err) {
+        c->err = h.err == ERR_CAPPED ? "federation capacity exceeded in a stop set" : "invariant or transition table too large";
+        st->capped = h.err == ERR_CAPPED;
+        return h.err == ERR_CAPPED ? HORA_HIP_E_CAPPED : HORA_HIP_E_MEMORY;
+      }
+    }
+    hipLaunchKernelGGL(k_base_step, dim3(1), dim3(64), 0, c->stream, c->dnp, B, (const int*)c->rec_cnt.p,
+                       (const int*)c->s_valid.p, (const u64*)c->s_dk.p, (const int*)c->s_zone.p, (const int*)L.s_se.p,
+                       (const int*)L.s_re.p, (const LiveCtr*)L.lc.p, (const Counters*)c->ctr.p, q->early ? 1 : 0);
+    stage();
+    read_dev(c, &hb, L.bctr.p, sizeof(BaseCtr), &R, c->scan_total.p, sizeof(R));
+    ++steps;
+    if (c->trace && (steps & 1023) == 0)
+      std::fprintf(stderr, "base step %lld nodes=%d edges=%d keys=%d queued=%lld\n", steps, hb.nodes, hb.edges, hb.keys,
+                   hb.qt - hb.qh);
+  }
+  if (hb.err) {
+    c->err = hb.err == ERR_HASH_FULL ? "hash table full"
+                                     : "a state has more transitions or invariant constraints than the device holds";
+    return HORA_HIP_E_MEMORY;
+  }
+  {
+    const Tally hc = read_counters(c);
+    if (hc.err) { c->err = "a state has more transitions or invariant constraints than the device holds"; return HORA_HIP_E_MEMORY; }
+  }
+  if (hb.stop) {
+    *early = 0;
+    st->early = 1;
+  }
+  // the graph in CSR form
+  const long N = hb.nodes;
+  grow_keep_to(L.grow, (size_t)N + 1, c->stream);
+  long long E = 0;
+  if (N > 0) {
+    L.tslot.ensure(N);
+    hipLaunchKernelGGL(k_base_rows, dim3(blocks_for(N, 256)), dim3(256), 0, c->stream, B, (int)N, L.tslot.p);
+    E = scan_exclusive(c, L.tslot.p, N, L.grow.p);
+  }
+  HIP_OK(hipMemcpyAsync(L.grow.p + N, &E, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  grow_keep_to(L.etgt, (size_t)std::max<long long>(E, 1), c->stream);
+  grow_keep_to(L.ese, (size_t)std::max<long long>(E, 1), c->stream);
+  grow_keep_to(L.ere, (size_t)std::max<long long>(E, 1), c->stream);
+  if (N > 0)
+    hipLaunchKernelGGL(k_base_fill, dim3(blocks_for(N, 256)), dim3(256), 0, c->stream, B, (int)N,
+                       (const long long*)L.grow.p, L.etgt.p, L.ese.p, L.ere.p);
+  L.nodes = N;
+  L.edges = E;
+  L.args.recnt = nullptr;   // the backward pass reads the rows
+  st->outer_nodes = N;
+  st->fwd_edges = E;
+  st->disc_keys = hb.keys;
+  st->expanded_nodes = (long long)hb.expanded;
+  st->generated = (long long)hb.generated;
+  st->covered = (long long)hb.covered;
+  st->enlarged = (long long)hb.enlarged;
+  st->unions = (long long)hb.unions;
+  if (c->trace)
+    std::fprintf(stderr, "base construction: %lld steps, %ld nodes, %lld edges, %d keys\n", steps, N, E, hb.keys);
   return HORA_HIP_OK;
 }
 
@@ -2347,9 +2630,7 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     L.ns = 0;
   
     const bool eg = q->kind == HORA_LIVE_EG;
-    u64 dk0 = 0;
-    for (int p = 0; p < c->np; ++p) dk0 |= (u64)(unsigned)c->h_loc_init[p] << c->h_kshift[p];
-    for (int v = 0; v < c->ni; ++v) dk0 |= (u64)(unsigned)(c->h_int_init[v] - c->h_int_lo[v]) << c->h_kshift[c->np + v];
+    const u64 dk0 = initial_key(c);
     c->fr_dk.ensure(1);
     c->fr_zid.ensure(1);
     HIP_OK(hipMemcpyAsync(c->fr_dk.p, &dk0, sizeof(u64), hipMemcpyHostToDevice, c->stream));
@@ -2577,14 +2858,21 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     return HORA_HIP_OK;
   };
   const bool auto_route = q->cover != HORA_COVER_AM;
+  // p --> q where D is provably a_LU (structurally_empty_d, src/liveness/internal.cpp:59-103): the
+  // base construction (src/liveness/api.cpp:37-117); a time-open network always, a network with one
+  // possible discrete state unless that state can seed a deadlock atom
+  bool base = false;
   if (q->cover == HORA_COVER_AUTO && q->kind == HORA_LIVE_LEADSTO && c->base_route) {
-    c->err = c->base_route == 1
-                 ? "time-open network: the reference decides p --> q by its base construction, not on the device"
-                 : "one possible discrete state: the reference may decide p --> q by its base construction, not on the device";
-    return HORA_HIP_E_SCOPE;
+    bool seeds0 = false;
+    if (c->base_route == 2) {
+      const int rc = initial_key_seeds(c, L, st, &seeds0);
+      if (rc != HORA_HIP_OK) return rc;
+    }
+    base = !seeds0;
   }
   {
-    const int rc = auto_route ? construct_online(c, L, q, st, &early) : construct(true);
+    const int rc = base ? construct_base(c, L, q, st, &early)
+                        : auto_route ? construct_online(c, L, q, st, &early) : construct(true);
     if (rc != HORA_HIP_OK) return rc;
   }
   if (auto_route && early < 0 && L.nodes > 0) {
@@ -2593,10 +2881,17 @@ int run_live(hora_hip_ctx* c, const hora_live_query* q, int32_t* holds, hora_liv
     L.args.gzone = c->zarena.p;
     upload_args(c, L);
     HIP_OK(hipMemsetAsync(L.lc.p, 0, sizeof(LiveCtr), c->stream));
-    L.heads.ensure((size_t)std::max<u64>(c->H, 1));
-    Passed P = passed_view(c);
-    hipLaunchKernelGGL(k_heads, dim3(blocks_for((long)c->H, 256)), dim3(256), 0, c->stream, P, (long)c->H, L.heads.p,
-                       L.lc.p);
+    if (base) {
+      const long H = 1L << L.bhbits;
+      L.heads.ensure((size_t)H);
+      hipLaunchKernelGGL(k_base_heads, dim3(blocks_for(H, 256)), dim3(256), 0, c->stream, base_view(c, L), H,
+                         L.heads.p, L.lc.p);
+    } else {
+      L.heads.ensure((size_t)std::max<u64>(c->H, 1));
+      Passed P = passed_view(c);
+      hipLaunchKernelGGL(k_heads, dim3(blocks_for((long)c->H, 256)), dim3(256), 0, c->stream, P, (long)c->H, L.heads.p,
+                         L.lc.p);
+    }
     const int nh = read_lc(c, L).ns;
     int seeds = 0;
     if (nh > 0) {

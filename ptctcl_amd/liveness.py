@@ -190,11 +190,11 @@ class LiveQueryEnc:
 
 
 def base_route(net: Network) -> int:
-    """Where the reference decides ``p --> q`` by its base construction instead of the online one
-    (structurally_empty_d, src/liveness/internal.cpp:59-103), which the device does not run: 1 for a
-    time-open network (no urgency, no upper invariant), 2 for a network with one possible discrete
-    state. The device refuses ``p --> q`` with COVER_AUTO there; COVER_ONLINE runs the online
-    construction instead. Same test as the library's at open."""
+    """Where the reference may decide ``p --> q`` by its base construction instead of the online one
+    (structurally_empty_d, src/liveness/internal.cpp:59-103): 1 for a time-open network (no
+    urgency, no upper invariant; always the base construction), 2 for a network with one possible
+    discrete state (the base construction unless that state can seed a deadlock atom), else 0. Same
+    test as the library's at open; COVER_ONLINE runs the online construction there instead."""
     time_open = not any(c.urgent for c in net.channels)
     for p in net.procs:
         for loc in p.locations:

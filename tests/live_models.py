@@ -3,8 +3,10 @@
 Each builder returns the network and a dict of queries -> expected verdict. The reasoning for each
 verdict is in the docstring; the answers do not depend on any engine.
 """
+import random
+
 from ptctcl_amd import liveness as lv
-from ptctcl_amd.network import Edge, Location, Network, Process, c_ge, c_le
+from ptctcl_amd.network import Channel, Edge, Location, Network, Process, c_ge, c_gt, c_le
 
 P = lv.DPred
 
@@ -84,3 +86,35 @@ def no_partial_deadlock():
 
 
 ALL = [timelock, progress, cycle, forced, partial_deadlock, no_partial_deadlock]
+
+
+def open_random(seed, nproc=4, nloc=4, nclk=2):
+    """A seeded random network with no invariant and no urgency (time-open, so `p --> q` takes the
+    base construction): each process has `nloc` locations and `nclk` clocks, one or two edges out of
+    each location with random lower, strict-lower and upper clock guards, random resets, and a
+    third of the edges synchronising on one of two channels. No verdict is fixed here: the tests
+    compare engines on it."""
+    rng = random.Random(seed)
+    chans = [Channel("c0"), Channel("c1")]
+    procs = []
+    for p in range(nproc):
+        clocks = [1 + p * nclk + k for k in range(nclk)]
+        edges = []
+        for i in range(nloc):
+            for _ in range(rng.randint(1, 2)):
+                g = []
+                for x in clocks:
+                    r = rng.random()
+                    if r < 0.3:
+                        g.append(c_ge(x, rng.randint(1, 5)))
+                    elif r < 0.5:
+                        g.append(c_le(x, rng.randint(2, 8)))
+                    elif r < 0.6:
+                        g.append(c_gt(x, rng.randint(0, 3)))
+                rs = [x for x in clocks if rng.random() < 0.4]
+                ch, d = -1, 0
+                if rng.random() < 0.3:
+                    ch, d = rng.randrange(2), rng.choice((1, -1))
+                edges.append(Edge(i, rng.randrange(nloc), guard=g, resets=rs, chan=ch, dir=d))
+        procs.append(Process(f"P{p}", [Location(f"l{i}") for i in range(nloc)], edges, 0))
+    return Network(nproc * nclk, [], chans, procs, name=f"open{seed}").finalize()

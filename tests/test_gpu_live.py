@@ -35,14 +35,6 @@ GRAPH = ("dk", "zones", "row_off", "etgt", "ese", "ere", "expanded", "in_s", "w_
 
 def check(net, query, label=""):
     print("case", net.name, label, query.kind, query.early, query.cover, flush=True)
-    if query.kind == lv.LEADSTO and query.cover == lv.COVER_AUTO and lv.base_route(net):
-        # the reference's base construction is not on the device: refused, then compared on the
-        # online construction
-        with hip.HipContext(net, None) as ctx:
-            with pytest.raises(hip.HipError, match="base construction"):
-                ctx.live(query)
-        query = copy.copy(query)
-        query.cover = lv.COVER_ONLINE
     r_o, st_o, g_o = binding.live(net, query, graph=True)
     with hip.HipContext(net, None) as ctx:
         r_h, st_h = ctx.live(query)
@@ -260,8 +252,7 @@ def _recorded_net(row):
     return live_xml_cases.load(name)
 
 
-@pytest.mark.parametrize("row", [r for r in RECORDED if r["states"] <= 5_000_000 and not r["model"].startswith("single")],
-                         ids=lambda r: r["model"])
+@pytest.mark.parametrize("row", [r for r in RECORDED if r["states"] <= 5_000_000], ids=lambda r: r["model"])
 def test_recorded_construction_counts(row):
     """The reference's recorded liveness rows at full size, on the device alone: the verdict and the
     number of records the construction expands (the `states` column; the source line is in the
@@ -284,17 +275,72 @@ def test_recorded_construction_counts(row):
     assert st.route == 0 and st.expanded_nodes == row["states"]
 
 
-def test_base_route_networks_are_refused():
-    """single is a one-key network: the reference decides `p --> q` there by its base construction,
-    which the device does not run; HORA_COVER_ONLINE runs the online construction instead."""
-    net = live_xml_cases.load("single4")
+@pytest.mark.parametrize("name", ["single4", "single5", "single6"])
+def test_base_construction_single(name):
+    """single has one possible discrete state and no partial deadlock region: the reference decides
+    `p --> q` by its base construction there (structurally_empty_d). The device's graph, node for
+    node and edge for edge, against the oracle's, with and without the early witness; the online
+    construction still answers under HORA_COVER_ONLINE."""
+    net = live_xml_cases.load(name)
+    assert lv.base_route(net) == 2
+    for early in (True, False):
+        q = live_xml_cases.leads_to(net, "P.A --> P.A", early=early)
+        r, st = check(net, q, "early" if early else "full")
+        assert r and st.route == 0 and st.levels == 0
     q = live_xml_cases.leads_to(net, "P.A --> P.A")
-    with hip.HipContext(net, None) as ctx:
-        with pytest.raises(hip.HipError, match="base construction"):
-            ctx.live(q)
-        q.cover = lv.COVER_ONLINE
-        r, st = ctx.live(q)
+    q.cover = lv.COVER_ONLINE
+    r, st = check(net, q, "online")
     assert r
+
+
+def _time_open(net):
+    """The network with every location invariant dropped: no upper invariant, so time-open when it
+    has no urgency (structurally_empty_d's first case)."""
+    r = copy.deepcopy(net)
+    for p in r.procs:
+        for loc in p.locations:
+            loc.invariant = []
+    r._analysed = False
+    r.name = net.name + "open"
+    return r.finalize()
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("N", [3, 5, 7])
+def test_base_construction_time_open_ring(N, seed):
+    """Time-open token rings: p --> q by the base construction (FIFO, in-place enlargement, convex
+    unions, re-expansion of rewritten nodes) against the oracle's, graph and verdict."""
+    net = _time_open(models.token_ring(N, seed))
+    assert lv.base_route(net) == 1
+    h0, h1, i0 = P.at(net, "P0", "hold"), P.at(net, "P1", "hold"), P.at(net, "P0", "idle")
+    for q in (lv.leads_to(h0, h1), lv.leads_to(h0, h1, early=False), lv.leads_to(i0, h0, early=False),
+              lv.leads_to(P.true(), i0, early=False)):
+        check(net, q)
+
+
+@pytest.mark.parametrize("seed", range(1, 16))
+def test_base_construction_random_time_open(seed):
+    """Seeded random time-open networks, where the base construction enlarges nodes in place,
+    takes convex unions and expands rewritten nodes again: graph, counts and verdict against the
+    oracle's, with and without the early witness."""
+    net = live_models.open_random(seed)
+    assert lv.base_route(net) == 1
+    for q in (lv.leads_to(P.at(net, "P0", "l1"), P.at(net, "P0", "l2"), early=False),
+              lv.leads_to(P.at(net, "P0", "l1"), P.at(net, "P1", "l2")),
+              lv.leads_to(P.at(net, "P2", "l0"), P.at(net, "P3", "l3"))):
+        check(net, q)
+
+
+@pytest.mark.parametrize("N", [3, 4, 5])
+def test_base_construction_time_open_fischer(N):
+    """Fischer's protocol with its invariants dropped (time-open, many discrete states with several
+    zones each): the base construction's graph and verdicts against the oracle's."""
+    net = _time_open(models.fischer(N))
+    assert lv.base_route(net) == 1
+    a = lambda loc: P.at(net, "P0", loc)   # noqa: E731
+    for q in (lv.leads_to(a("req"), a("wait")), lv.leads_to(a("req"), a("wait"), early=False),
+              lv.leads_to(a("wait"), a("cs"), early=False), lv.leads_to(a("cs"), a("req"))):
+        check(net, q)
 
 
 @pytest.mark.parametrize("case", live_xml_cases.CASES, ids=lambda c: f"{c[0]}:{c[1]}")
