@@ -1206,39 +1206,60 @@ __global__ void k_find(Passed P, const int* __restrict__ sj, const u64* __restri
 //
 // Four 16-lane groups per wavefront, one successor per group, lanes over matrix entries; the
 // group walks the key's previous-level chain newest first.
+//
+// The successor's side of the a_LU test is fixed for the whole walk, so each lane reduces its
+// entries (y, x) once: with c = (<, -L_y) strict and M[y][x] < S[y][x] <= inf, M[y][x] + c < S[0][x]
+// reads (M[y][x] & ~1) < S[0][x] - c. An entry the test skips gets S[y][x] = INT_MIN, which no
+// stored entry is below. Up to kProbeRegs entries per lane (d*d <= 192) stay in registers, and a
+// chain step is one load, one mask and two compares per entry. The per-successor bounds take one
+// load per (process, clock) pair, issued together.
 constexpr int kGroup = 16;
+constexpr int kProbeRegs = 12;
 
 template <bool NAR>
 __global__ __launch_bounds__(64) void k_probe(const DevNet* __restrict__ Np, Passed P, const long* __restrict__ gbk,
                                               const int* __restrict__ sj, const u64* __restrict__ sdk,
                                               const int* __restrict__ szone, long G, int* __restrict__ covered,
                                               Counters* ctr) {
-  __shared__ int Lg[4][HORA_MAX_DIM], Ug[4][HORA_MAX_DIM], lg[4][HORA_MAX_PROCS];
+  __shared__ int Lg[4][HORA_MAX_DIM], Ug[4][HORA_MAX_DIM], gl[4][HORA_MAX_PROCS];
   const DevNet& N = *Np;
   const int lane = threadIdx.x, q = lane / kGroup, t = lane % kGroup;
   const long g = (long)blockIdx.x * 4 + q;
   const bool active = g < G;
-  const int d = N.d, d2 = N.d2;
+  const int d = N.d, d2 = N.d2, np = N.np;
   long h = active ? gbk[g] : -1;
   const u32 len = (h >= 0) ? P.len[h] : 0u;
-  // per-successor a_LU bounds from its locations (state_bounds, engine.cpp:1487-1497)
-  if (active && len > 0) {
+  const bool work = active && len > 0;
+  // per-successor a_LU bounds from its locations (state_bounds, engine.cpp:1487-1497): the global
+  // location rows first, then each clock's maximum over the processes
+  if (work) {
     const u64 dk = sdk[sj[g]];
-    for (int p = t; p < N.np; p += kGroup) lg[q][p] = (int)((dk >> N.kshift[p]) & N.kmask[p]);
+    for (int p = t; p < np; p += kGroup) gl[q][p] = (N.locbase[p] + (int)((dk >> N.kshift[p]) & N.kmask[p])) * d;
   }
   __syncthreads();
-  if (active && len > 0) {
+  if (work) {
     for (int c = t; c < d; c += kGroup) {
       int lb = 0, ub = 0;
       if (c > 0) {
-        lb = ub = kAluNegInf;
-        for (int p = 0; p < N.np; ++p) {
-          const size_t b = (size_t)(N.locbase[p] + lg[q][p]) * d + c;
-          lb = max(lb, N.lloc[b]);
-          ub = max(ub, N.uloc[b]);
+        lb = max(kAluNegInf, N.Lq[c]);
+        ub = max(kAluNegInf, N.Uq[c]);
+        for (int p0 = 0; p0 < np; p0 += 8) {
+          int lv[8], uv[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            lv[k] = uv[k] = kAluNegInf;
+            if (p0 + k < np) {
+              const int b = gl[q][p0 + k] + c;
+              lv[k] = N.lloc[b];
+              uv[k] = N.uloc[b];
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            lb = max(lb, lv[k]);
+            ub = max(ub, uv[k]);
+          }
         }
-        lb = max(lb, N.Lq[c]);
-        ub = max(ub, N.Uq[c]);
       }
       Lg[q][c] = lb;
       Ug[q][c] = ub;
@@ -1251,22 +1272,56 @@ __global__ __launch_bounds__(64) void k_probe(const DevNet* __restrict__ Np, Pas
   int reads = 0;
   const int* L = Lg[q];
   const int* U = Ug[q];
-  for (int m = (int)len - 1; m >= 0; --m) {   // group-uniform loop
-    const auto M = StoredZone<NAR>::at(P, P.nodes[P.off[h] + m], d2);
-    ++reads;
-    bool wit = false;
-    for (int e = t; e < d2; e += kGroup) {
-      const int y = ediv(e, d), x = e - y * d;
-      if (x == y) continue;
-      const int ux = U[x], ly = L[y];
-      if (ux == kAluNegInf || ly == kAluNegInf) continue;
-      const int s0x = S[x];
-      if (s0x < r_le(-ux)) continue;
-      const int mp = M[e];
-      if (mp < S[e] && radd(mp, r_lt(-ly)) < s0x) wit = true;
+  // the successor's terms of entry (y, x): the bound M must stay below, and S[y][x] (INT_MIN: skip)
+  auto terms = [&](int e, int* se, int* th) __attribute__((always_inline)) {
+    *se = INT_MIN;
+    *th = 0;
+    if (!work || e >= d2) return;
+    const int y = ediv(e, d), x = e - y * d;
+    if (x == y) return;
+    const int ux = U[x], ly = L[y];
+    if (ux == kAluNegInf || ly == kAluNegInf) return;
+    const int s0x = S[x];
+    if (s0x < r_le(-ux)) return;
+    *se = S[e];
+    *th = s0x - r_lt(-ly);
+  };
+  if (d2 <= kProbeRegs * kGroup) {
+    const int nper = (d2 + kGroup - 1) / kGroup;   // group-uniform
+    int se[kProbeRegs], th[kProbeRegs], ec[kProbeRegs];
+#pragma unroll
+    for (int k = 0; k < kProbeRegs; ++k) {
+      const int e = t + k * kGroup;
+      ec[k] = min(e, d2 - 1);
+      terms(e, &se[k], &th[k]);
     }
-    const u64 b = __ballot(wit) & gmask;
-    if (b == 0) { cov = true; break; }
+    for (int m = (int)len - 1; m >= 0; --m) {   // group-uniform loop
+      const auto M = StoredZone<NAR>::at(P, P.nodes[P.off[h] + m], d2);
+      ++reads;
+      bool wit = false;
+#pragma unroll
+      for (int k = 0; k < kProbeRegs; ++k)
+        if (k < nper) {
+          const int mp = M[ec[k]];
+          wit |= (mp < se[k]) & ((mp & ~1) < th[k]);
+        }
+      const u64 b = __ballot(wit) & gmask;
+      if (b == 0) { cov = true; break; }
+    }
+  } else {
+    for (int m = (int)len - 1; m >= 0; --m) {   // group-uniform loop
+      const auto M = StoredZone<NAR>::at(P, P.nodes[P.off[h] + m], d2);
+      ++reads;
+      bool wit = false;
+      for (int e = t; e < d2; e += kGroup) {
+        int se, th;
+        terms(e, &se, &th);
+        const int mp = M[e];
+        wit |= (mp < se) & ((mp & ~1) < th);
+      }
+      const u64 b = __ballot(wit) & gmask;
+      if (b == 0) { cov = true; break; }
+    }
   }
   if (active && t == 0) covered[g] = cov ? 1 : 0;
   const bool lead = active && t == 0;
