@@ -16,19 +16,25 @@
 // (one wavefront) interns its successors in order, queues the new work and pops the next node. The
 // host launches the steps and reads back one small record per node (whether to go on, and the
 // sizes it must grow the buffers to).
+//
+// A key's nodes are an array of node ids in creation order, so the three searches of an intern
+// test 64 nodes at once, one per lane (the first hit in creation order is the lowest lane of the
+// first block with a hit), instead of walking a list one dependent load at a time. The arrays
+// live in one pool, each doubling its capacity when full (the old array is left behind).
 #pragma once
 
 namespace {
 
 // The construction's state in device memory. Nodes: key, zone (the arena, d*d ints each), flags,
-// the next node of the same key (creation order), the head and tail of the node's edge list.
-// Edges: target, transition, the next edge of the same node. A hash table of the keys (open
-// addressing, linear probing) holds each key's first and last node. The FIFO is an array read
-// from qh and appended at qt.
+// the head and tail of the node's edge list. Edges: target, transition, the next edge of the same
+// node. A hash table of the keys (open addressing, linear probing) holds each key's node array
+// (offset into the pool, length, capacity). The FIFO is an array read from qh and appended at qt.
 struct BaseCtr {
   int u;                      // node to expand next, or -1: the construction is over
   int nodes, edges, keys;
+  u64 ukey;                   // node u's discrete key (the host decides from it whether to test u)
   long long qh, qt;
+  long long ptop;             // pool ints handed out
   int stop;                   // an early witness ended the construction
   int err;                    // ERR_HASH_FULL, or the expansion's error
   unsigned long long expanded, generated, covered, enlarged, unions;
@@ -39,7 +45,6 @@ struct BaseState {
   int* zone;                  // [N * d2]
   signed char* expanded;      // [N]
   signed char* inq;           // [N]
-  int* knext;                 // [N]
   int* ehead;                 // [N]
   int* etail;                 // [N]
   int* recnt;                 // [N] successors of the node's last expansion (the stop-set test reads it)
@@ -48,8 +53,10 @@ struct BaseState {
   int* ese;                   // [E]
   int* ere;                   // [E]
   u64* hkey;                  // [H]
-  int* hhead;                 // [H]
-  int* htail;                 // [H]
+  long long* kptr;            // [H] the key's node array: offset into the pool
+  int* klen;                  // [H] ... nodes in it
+  int* kcap;                  // [H] ... its capacity
+  int* pool;                  // node arrays
   int* queue;                 // [Q]
   int hbits;
   BaseCtr* bc;
@@ -71,6 +78,23 @@ __device__ __forceinline__ int base_slot(const BaseState& B, u64 k, bool* fresh)
   return -1;
 }
 
+// Node v appended to the array of key slot h (lane 0; the caller orders the writes). A full array
+// moves to twice its capacity at the pool's top.
+__device__ __forceinline__ void base_append(const BaseState& B, int h, int v) {
+  BaseCtr& C = *B.bc;
+  const int n = B.klen[h];
+  if (n == B.kcap[h]) {
+    const int cap = max(8, 2 * n);
+    const long long off = C.ptop;
+    C.ptop += cap;
+    for (int k = 0; k < n; ++k) B.pool[off + k] = B.pool[B.kptr[h] + k];
+    B.kptr[h] = off;
+    B.kcap[h] = cap;
+  }
+  B.pool[B.kptr[h] + n] = v;
+  B.klen[h] = n + 1;
+}
+
 // The first node to expand: the initial zone (successor slot 0 of the initial expansion) becomes
 // node 0 and is popped.
 __global__ __launch_bounds__(64) void k_base_init(BaseState B, const int* __restrict__ svalid,
@@ -87,14 +111,15 @@ __global__ __launch_bounds__(64) void k_base_init(BaseState B, const int* __rest
     bool fresh = false;
     const int h = base_slot(B, sdk[0], &fresh);
     B.hkey[h] = sdk[0];
-    B.hhead[h] = 0;
-    B.htail[h] = 0;
+    B.klen[h] = 0;
+    B.kcap[h] = 0;
+    base_append(B, h, 0);
     B.nkey[0] = sdk[0];
     B.expanded[0] = 0;
     B.inq[0] = 0;
-    B.knext[0] = -1;
     B.ehead[0] = B.etail[0] = -1;
     C.u = 0;
+    C.ukey = sdk[0];
     C.nodes = 1;
     C.keys = 1;
     C.edges = 0;
@@ -117,7 +142,21 @@ __global__ __launch_bounds__(64) void k_base_count(BaseState B, const int* __res
   int c = 0;
   for (int r = threadIdx.x; r < n; r += kWave) c += svalid[r] ? 1 : 0;
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-  if (threadIdx.x == 0) B.recnt[B.bc->u] = c;
+  if (threadIdx.x == 0 && B.bc->u >= 0) B.recnt[B.bc->u] = c;
+}
+
+// The position of the first node of `ids` (in order) for which test(zone) holds, or -1: blocks of
+// 64 nodes, one per lane.
+template <class Test>
+__device__ __forceinline__ int base_first(const BaseState& B, const int* ids, int n, int d2, Test test) {
+  const int l = lane_id();
+  for (int b0 = 0; b0 < n; b0 += kWave) {
+    bool hit = false;
+    if (b0 + l < n) hit = test(B.zone + (size_t)ids[b0 + l] * d2);
+    const u64 m = __ballot(hit);
+    if (m) return b0 + __builtin_ctzll(m);
+  }
+  return -1;
 }
 
 // One node's turn: its successors interned in order (intern, construction.cpp:863-1003), the
@@ -177,32 +216,50 @@ __global__ __launch_bounds__(64) void k_base_step(const DevNet* __restrict__ Np,
     int v = -1;
     bool grew = false;
     if (!fresh) {
-      // oldest node that covers the successor
-      for (int j = B.hhead[h]; j >= 0 && v < 0; j = B.knext[j])
-        if (z_alu_le(S, B.zone + (size_t)j * d2, d, Lb, Ub)) v = j;
-      if (v >= 0) {
+      const int* ids = B.pool + B.kptr[h];
+      const int n = B.klen[h];
+      // the oldest node that covers the successor (a lane's zone is read a column at a time when
+      // d <= 16: one wait per column)
+      int j = base_first(B, ids, n, d2, [&](const int* Z) {
+        return d <= 16 ? alu_le_lane16<false>(S, Z, d, Lb, Ub) : alu_le_lane(S, Z, d, Lb, Ub);
+      });
+      if (j >= 0) {
+        v = ids[j];
         if (l == 0) ++C.covered;
       } else {
-        // oldest node the successor covers: enlarged to it in place
-        for (int j = B.hhead[h]; j >= 0 && v < 0; j = B.knext[j])
-          if (z_alu_le(B.zone + (size_t)j * d2, S, d, Lb, Ub)) v = j;
-        if (v >= 0) {
+        // the oldest node the successor covers: enlarged to it in place
+        j = base_first(B, ids, n, d2, [&](const int* Z) {
+          return d <= 16 ? alu_le_lane16<true>(Z, S, d, Lb, Ub) : alu_le_lane(Z, S, d, Lb, Ub);
+        });
+        if (j >= 0) {
+          v = ids[j];
           int* zj = B.zone + (size_t)v * d2;
           for (int e = l; e < d2; e += kWave) zj[e] = S[e];
           grew = true;
           if (l == 0) ++C.enlarged;
         } else {
-          // oldest node whose union with it is one zone: the hull
-          for (int j = B.hhead[h]; j >= 0 && v < 0; j = B.knext[j]) {
-            const int* zj = B.zone + (size_t)j * d2;
-            bsync();
-            for (int e = l; e < d2; e += kWave) T[e] = zj[e];
-            if (z_try_union(T, S, d, fa, pa, misc)) v = j;
+          // the oldest node whose union with it is one zone: the hull (entrywise maximum)
+          if (d <= 16) {
+            j = base_first(B, ids, n, d2, [&](const int* Z) { return union_convex_lane(S, Z, d); });
+            if (j >= 0) {
+              v = ids[j];
+              int* zj = B.zone + (size_t)v * d2;
+              for (int e = l; e < d2; e += kWave) zj[e] = max(zj[e], S[e]);
+            }
+          } else {
+            for (int m = 0; m < n && v < 0; ++m) {
+              const int* zj = B.zone + (size_t)ids[m] * d2;
+              bsync();
+              for (int e = l; e < d2; e += kWave) T[e] = zj[e];
+              if (z_try_union(T, S, d, fa, pa, misc)) v = ids[m];
+            }
+            if (v >= 0) {
+              int* zj = B.zone + (size_t)v * d2;
+              wsync();
+              for (int e = l; e < d2; e += kWave) zj[e] = T[e];
+            }
           }
           if (v >= 0) {
-            int* zj = B.zone + (size_t)v * d2;
-            wsync();
-            for (int e = l; e < d2; e += kWave) zj[e] = T[e];
             grew = true;
             if (l == 0) ++C.unions;
           }
@@ -218,16 +275,14 @@ __global__ __launch_bounds__(64) void k_base_step(const DevNet* __restrict__ Np,
         B.nkey[v] = k;
         B.expanded[v] = 0;
         B.inq[v] = 0;
-        B.knext[v] = -1;
         B.ehead[v] = B.etail[v] = -1;
         if (fresh) {
           B.hkey[h] = k;
-          B.hhead[h] = v;
+          B.klen[h] = 0;
+          B.kcap[h] = 0;
           ++C.keys;
-        } else {
-          B.knext[B.htail[h]] = v;
         }
-        B.htail[h] = v;
+        base_append(B, h, v);
         C.nodes = v + 1;
       }
     }
@@ -271,34 +326,32 @@ __global__ __launch_bounds__(64) void k_base_step(const DevNet* __restrict__ Np,
       B.inq[w] = 0;
       if (!B.expanded[w]) {
         C.u = w;
+        C.ukey = B.nkey[w];
         break;
       }
     }
   }
 }
 
-// Rebuild of the key table at a new size (creation order kept: the nodes in order, each appended
-// to its key's list).
-__global__ __launch_bounds__(64) void k_base_rehash(BaseState B) {
-  if (threadIdx.x != 0) return;
-  BaseCtr& C = *B.bc;
-  for (int v = 0; v < C.nodes; ++v) {
-    const u64 k = B.nkey[v];
-    bool fresh = false;
-    const int h = base_slot(B, k, &fresh);
-    if (h < 0) {
-      C.err = ERR_HASH_FULL;
+// The key table at a new size: every occupied slot of the old table (O) moved to the new one (B),
+// with its node array, one thread per old slot.
+__global__ void k_base_rehash(BaseState O, long Ho, BaseState B) {
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= Ho) return;
+  const u64 k = O.hkey[s];
+  if (k == kEmptyKey) return;
+  const u64 H = 1ull << B.hbits;
+  u64 h = key_mix(k) >> (64 - B.hbits);
+  for (u64 n = 0; n < H; ++n, h = (h + 1) & (H - 1)) {
+    const u64 prev = atomicCAS((unsigned long long*)&B.hkey[h], (unsigned long long)kEmptyKey, (unsigned long long)k);
+    if (prev == kEmptyKey) {
+      B.kptr[h] = O.kptr[s];
+      B.klen[h] = O.klen[s];
+      B.kcap[h] = O.kcap[s];
       return;
     }
-    B.knext[v] = -1;
-    if (fresh) {
-      B.hkey[h] = k;
-      B.hhead[h] = v;
-    } else {
-      B.knext[B.htail[h]] = v;
-    }
-    B.htail[h] = v;
   }
+  atomicExch(&B.bc->err, ERR_HASH_FULL);
 }
 
 // The graph in CSR form: a node's edges in list order, nodes in creation order.
@@ -327,7 +380,7 @@ __global__ void k_base_heads(BaseState B, long H, int* __restrict__ heads, LiveC
   const long h = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool ok = h < H && B.hkey[h] != kEmptyKey;
   const int idx = block_append(&lc->ns, ok);
-  if (ok) heads[idx] = B.hhead[h];
+  if (ok) heads[idx] = B.pool[B.kptr[h]];
 }
 
 }  // namespace

@@ -1419,10 +1419,12 @@ struct LiveState {
       rlist2, rlist3, wlist;
   // base construction: its counters, node links, edge lists, key table and FIFO
   DBuf<BaseCtr> bctr;
-  DBuf<int> bknext, behead, betail, benext, betgt, bese, bere, bqueue, bqueue2, bhhead, bhtail;
-  DBuf<u64> bhkey;
+  DBuf<int> behead, betail, benext, betgt, bese, bere, bqueue, bqueue2, bpool;
+  DBuf<u64> bhkey[2];
+  DBuf<long long> bkptr[2];
+  DBuf<int> bklen[2], bkcap[2];
   DBuf<signed char> binq;
-  int bhbits = 0;
+  int bhbits = 0, bht = 0;   // key table size (bits) and which of the two table buffers is current
   DBuf<int> fedhbm;        // federation lists of the device-memory tier
   long long hbm_nodes = 0; // node launches of the device-memory tier in the last query
   DBuf<int> wpool[2], wtmp;   // value pools (current, compaction target); per-node counts
@@ -2021,13 +2023,13 @@ int construct_online(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, ho
   return HORA_HIP_OK;
 }
 
-BaseState base_view(hora_hip_ctx* c, LiveState& L) {
+BaseState base_view(hora_hip_ctx* c, LiveState& L, int t = -1) {
+  if (t < 0) t = L.bht;
   BaseState B;
   B.nkey = L.gdk.p;
   B.zone = c->zarena.p;
   B.expanded = L.expanded.p;
   B.inq = L.binq.p;
-  B.knext = L.bknext.p;
   B.ehead = L.behead.p;
   B.etail = L.betail.p;
   B.recnt = L.recnt.p;
@@ -2035,23 +2037,41 @@ BaseState base_view(hora_hip_ctx* c, LiveState& L) {
   B.etgt = L.betgt.p;
   B.ese = L.bese.p;
   B.ere = L.bere.p;
-  B.hkey = L.bhkey.p;
-  B.hhead = L.bhhead.p;
-  B.htail = L.bhtail.p;
+  B.hkey = L.bhkey[t].p;
+  B.kptr = L.bkptr[t].p;
+  B.klen = L.bklen[t].p;
+  B.kcap = L.bkcap[t].p;
+  B.pool = L.bpool.p;
   B.queue = L.bqueue.p;
   B.hbits = L.bhbits;
   B.bc = L.bctr.p;
   return B;
 }
 
-// A fresh key table of 2^bits slots (the nodes re-linked into it by k_base_rehash).
-void base_table(hora_hip_ctx* c, LiveState& L, int bits) {
+// Table buffer t sized for 2^bits slots, every slot empty.
+void base_table(hora_hip_ctx* c, LiveState& L, int t, int bits) {
   const size_t H = (size_t)1 << bits;
-  L.bhbits = bits;
-  L.bhkey.ensure(H);
-  L.bhhead.ensure(H);
-  L.bhtail.ensure(H);
-  HIP_OK(hipMemsetAsync(L.bhkey.p, 0xff, H * sizeof(u64), c->stream));   // kEmptyKey
+  L.bhkey[t].ensure(H);
+  L.bkptr[t].ensure(H);
+  L.bklen[t].ensure(H);
+  L.bkcap[t].ensure(H);
+  HIP_OK(hipMemsetAsync(L.bhkey[t].p, 0xff, H * sizeof(u64), c->stream));   // kEmptyKey
+}
+
+// A discrete predicate on a packed key, on the host (the device's pred_eval).
+bool host_pred(const hora_hip_ctx* c, const hora_dpred& P, u64 dk) {
+  for (int t = 0; t < P.nterms; ++t) {
+    bool all = true;
+    for (int a = P.term_off[t]; a < P.term_off[t + 1] && all; ++a) {
+      const int kind = P.atoms[4 * a], idx = P.atoms[4 * a + 1], op = P.atoms[4 * a + 2], val = P.atoms[4 * a + 3];
+      const int f = kind == 0 ? idx : c->np + idx;
+      int v = (int)((dk >> c->h_kshift[f]) & c->h_kmask[f]);
+      if (kind != 0) v += c->h_int_lo[idx];
+      all = op == 0 ? v == val : op == 1 ? v != val : op == 2 ? v < val : op == 3 ? v <= val : op == 4 ? v > val : v >= val;
+    }
+    if (all) return true;
+  }
+  return false;
 }
 
 // Whether the initial discrete state can seed a deadlock atom (mode 3 of k_stop on the initial key,
@@ -2105,7 +2125,7 @@ int construct_base(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, hora
   L.expanded.ensure(ncap0);
   L.recnt.ensure(ncap0);
   L.binq.ensure(ncap0);
-  L.bknext.ensure(ncap0);
+  L.bpool.ensure(8 * ncap0);
   L.behead.ensure(ncap0);
   L.betail.ensure(ncap0);
   L.bqueue.ensure(ncap0);
@@ -2116,7 +2136,9 @@ int construct_base(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, hora
   c->zarena.ensure(ncap0 * d2);
   L.bctr.ensure(1);
   HIP_OK(hipMemsetAsync(L.bctr.p, 0, sizeof(BaseCtr), c->stream));
-  base_table(c, L, 12);
+  L.bht = 0;
+  L.bhbits = 12;
+  base_table(c, L, 0, 12);
   L.nodes = 0;
   L.edges = 0;
   L.ns = 0;
@@ -2172,13 +2194,12 @@ int construct_base(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, hora
     bool moved = false;
     const size_t nneed = (size_t)hb.nodes + (size_t)R + 1;
     if (nneed > L.gdk.cap || nneed > L.expanded.cap || nneed > L.recnt.cap || nneed > L.binq.cap ||
-        nneed > L.bknext.cap || nneed > L.behead.cap || nneed > L.betail.cap || nneed * d2 > c->zarena.cap) {
+        nneed > L.behead.cap || nneed > L.betail.cap || nneed * d2 > c->zarena.cap) {
       const size_t n2 = std::max(nneed, 2 * (size_t)hb.nodes);
       grow_keep_to(L.gdk, n2, c->stream);
       grow_keep_to(L.expanded, n2, c->stream);
       grow_keep_to(L.recnt, n2, c->stream);
       grow_keep_to(L.binq, n2, c->stream);
-      grow_keep_to(L.bknext, n2, c->stream);
       grow_keep_to(L.behead, n2, c->stream);
       grow_keep_to(L.betail, n2, c->stream);
       if (n2 * d2 > c->zarena.cap) c->zarena.grow_keep(std::max(n2 * d2, c->zarena.cap * 2), c->stream);
@@ -2209,13 +2230,28 @@ int construct_base(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, hora
       hb.qt = len;
       moved = true;
     }
+    {
+      // the key arrays' pool: a step's appends take at most twice the nodes it ends with, plus a
+      // first array of 8 per new key
+      const size_t pneed = (size_t)hb.ptop + 2 * nneed + 8 * (size_t)R + 64;
+      if (pneed > L.bpool.cap) {
+        grow_keep_to(L.bpool, std::max(pneed, 2 * (size_t)hb.ptop), c->stream);
+        moved = true;
+      }
+    }
     if ((double)(hb.keys + R) > 0.5 * (double)((size_t)1 << L.bhbits)) {
+      // a table twice as large (or more), filled from the current one
+      const long Ho = 1L << L.bhbits;
       int bits = L.bhbits;
       while ((double)(hb.keys + R) > 0.5 * (double)((size_t)1 << bits)) ++bits;
-      base_table(c, L, bits);
+      const int t2 = 1 - L.bht;
+      base_table(c, L, t2, bits);
+      const BaseState O = base_view(c, L, L.bht);
+      L.bhbits = bits;
+      const BaseState Bn = base_view(c, L, t2);
+      hipLaunchKernelGGL(k_base_rehash, dim3(blocks_for(Ho, 256)), dim3(256), 0, c->stream, O, Ho, Bn);
+      L.bht = t2;
       moved = true;
-      B = base_view(c, L);
-      hipLaunchKernelGGL(k_base_rehash, dim3(1), dim3(64), 0, c->stream, B);
     }
     if (moved) {
       B = base_view(c, L);
@@ -2226,7 +2262,7 @@ int construct_base(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, hora
     if (R > 0) launch_expand(c, expand_tuned(c, 1, 0, eo, false), 1, 0, eo, false);
     hipLaunchKernelGGL(k_base_count, dim3(1), dim3(64), 0, c->stream, B, (const int*)c->s_valid.p,
                        (const int*)c->rec_cnt.p);
-    if (q->early) {
+    if (q->early && host_pred(c, q->p, hb.ukey) && !host_pred(c, q->q, hb.ukey)) {
       launch_stop(c, L, &L.bctr.p->u, 1, 0, 1);
       const LiveCtr h = read_lc(c, L);
       if (h.err) {
