@@ -2120,7 +2120,10 @@ int construct_base(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, hora
     HIP_OK(hipMemcpyAsync(&c->ctr.p->first_target, &ft, sizeof(int), hipMemcpyHostToDevice, c->stream));
   }
   // node, edge and FIFO storage (grown below as the steps need)
-  const size_t ncap0 = 4096, ecap0 = 16384;
+  // HORA_HIP_LIVE_POOL=tight: small first sizes, so that every buffer and the key table grow (tests)
+  const bool tight = pool_tight();
+  const size_t ncap0 = tight ? 64 : 4096, ecap0 = tight ? 64 : 16384;
+  const int hbits0 = tight ? 4 : 12;
   L.gdk.ensure(ncap0);
   L.expanded.ensure(ncap0);
   L.recnt.ensure(ncap0);
@@ -2137,8 +2140,8 @@ int construct_base(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, hora
   L.bctr.ensure(1);
   HIP_OK(hipMemsetAsync(L.bctr.p, 0, sizeof(BaseCtr), c->stream));
   L.bht = 0;
-  L.bhbits = 12;
-  base_table(c, L, 0, 12);
+  L.bhbits = hbits0;
+  base_table(c, L, 0, hbits0);
   L.nodes = 0;
   L.edges = 0;
   L.ns = 0;

@@ -331,6 +331,22 @@ def test_base_construction_random_time_open(seed):
         check(net, q)
 
 
+def test_base_construction_buffers_grow(monkeypatch):
+    """The base construction's node, edge, key-array and FIFO buffers and its key table, started
+    small (HORA_HIP_LIVE_POOL=tight) so that each grows and the table is rebuilt several times
+    (time-open fischer 5: 3,552 nodes, 15,200 edges; single6: 844 nodes in one key): the graph
+    and verdict against the oracle's."""
+    monkeypatch.setenv("HORA_HIP_LIVE_POOL", "tight")
+    net = _time_open(models.fischer(5))
+    a = lambda loc: P.at(net, "P0", loc)   # noqa: E731
+    check(net, lv.leads_to(a("req"), a("wait"), early=False))
+    net = live_xml_cases.load("single6")
+    check(net, live_xml_cases.leads_to(net, "P.A --> P.A"))
+    for seed in (5, 14):
+        net = live_models.open_random(seed)
+        check(net, lv.leads_to(P.at(net, "P0", "l1"), P.at(net, "P0", "l2"), early=False))
+
+
 @pytest.mark.parametrize("N", [3, 4, 5])
 def test_base_construction_time_open_fischer(N):
     """Fischer's protocol with its invariants dropped (time-open, many discrete states with several
