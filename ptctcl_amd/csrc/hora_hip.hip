@@ -1566,8 +1566,69 @@ __host__ __device__ inline MergeLayout merge_layout(int d2, bool heavy) {
 // Two instances: without the pair pre-pass (light keys, and heavy keys when it did not run) six
 // wavefronts per SIMD measured fastest; with it, five leave the batched lane tests (used there,
 // where chains are long and mostly outside the LDS cache) their registers.
+// Heavy keys' member scans, shared by the wavefronts of a workgroup. Wavefront 0 posts a task in
+// LDS: test the successor S (in LDS) against n members, the widened-member list `wl` (W tasks) or
+// the chain positions n-1 .. 0 (C tasks); a cover task sets `flag` if some live member covers S,
+// a kill task retires every live member S covers and counts them. Wavefront w takes the blocks of
+// 64 members w, w + kMergeHelpWaves, ...; the members' answers are independent, so the split gives
+// the scans of one wavefront exactly.
+constexpr int kMergeHelpWaves = 8;
+enum { kTaskCoverW = 0, kTaskKillW = 1, kTaskCoverC = 2, kTaskKillC = 3, kTaskExit = 4 };
+
+struct MergeTask {
+  int kind, n, flag, kills;
+  const int* wl;   // widened members' chain positions (W tasks)
+  u32* ch;         // chain ids
+  const int* cs;   // chain cache slots (>= 0 LDS slot, <= -2 successor slot, -1 stored zone)
+};
+
+template <bool NAR>
+__device__ __forceinline__ void merge_slice(MergeTask& T, int w, const int* S, const int* Lb, const int* Ub, int d,
+                                            int d2, const int* cache, int zs, const int* szone, const Passed& P,
+                                            int* zdead, u64 lvl_zbase) {
+  const int l = lane_id(), kind = T.kind, n = T.n;
+  const bool cover = kind == kTaskCoverW || kind == kTaskCoverC;
+  const bool wlist = kind == kTaskCoverW || kind == kTaskKillW;
+  for (int b0 = w * kWave; b0 < n; b0 += kMergeHelpWaves * kWave) {
+    if (cover && *(volatile int*)&T.flag) break;
+    int m = -1;
+    if (b0 + l < n) m = wlist ? T.wl[b0 + l] : n - 1 - (b0 + l);
+    u32 id = 0;
+    bool hit = false;
+    if (m >= 0) {
+      id = T.ch[m];
+      if (!(id & kDead)) {
+        const int s = T.cs[m];
+        auto test = [&](auto Z) __attribute__((always_inline)) -> bool {
+          if (cover) return d <= 16 ? alu_le_lane16<false>(S, Z, d, Lb, Ub) : alu_le_lane(S, Z, d, Lb, Ub);
+          return d <= 16 ? alu_le_lane16<true>(Z, S, d, Lb, Ub) : alu_le_lane(Z, S, d, Lb, Ub);
+        };
+        if (NAR && s == -1) {
+          hit = test(Z16{(const short*)P.zarena + (size_t)(id & ~kDead) * nstride(d2)});
+        } else {
+          const int* z = s >= 0 ? cache + s * zs
+                         : s <= -2 ? szone + (size_t)(-(s + 2)) * d2
+                                   : P.zarena + (size_t)(id & ~kDead) * d2;
+          hit = test(z);
+        }
+      }
+    }
+    const u64 b = __ballot(hit);
+    if (cover) {
+      if (b && l == 0) atomicOr(&T.flag, 1);
+    } else {
+      if (hit) {
+        // a stored zone an earlier chunk of this level stored leaves the next frontier
+        if (zdead && !(id & (kNewTag | kDead)) && (u64)id >= lvl_zbase) zdead[id - lvl_zbase] = 1;
+        T.ch[m] = id | kDead;
+      }
+      if (b && l == 0) atomicAdd(&T.kills, __popcll(b));
+    }
+  }
+}
+
 template <bool PRE, bool NAR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6))) void k_merge(const DevNet* __restrict__ Np, Passed P, const u32* __restrict__ list, int nlist,
+__global__ __launch_bounds__(PRE ? kMergeHelpWaves * 64 : 64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6))) void k_merge(const DevNet* __restrict__ Np, Passed P, const u32* __restrict__ list, int nlist,
                                               int* __restrict__ wlist, const int* __restrict__ sj,
                                               int* __restrict__ szone, int* __restrict__ survived,
                                               Counters* ctr, int heavy, int* __restrict__ zscr,
@@ -1578,7 +1639,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
                                               int* __restrict__ zdead, u64 lvl_zbase, int* __restrict__ wlg) {
   extern __shared__ int sm[];
   const DevNet& N = *Np;
-  const int d = N.d, d2 = N.d2, l = threadIdx.x;
+  const int d = N.d, d2 = N.d2, l = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   const MergeLayout ml = merge_layout(d2, heavy != 0);
   int* S = sm;
   int* M = S + d2;
@@ -1597,6 +1658,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
   int* cache = wl + ml.wide_cap;
   const int zs = ml.zs, cz = ml.cz;
   if ((int)blockIdx.x >= nlist) return;
+  // Heavy keys (PRE): wavefront 0 runs the fold; the other wavefronts only take slices of its
+  // member scans (merge_slice), between two workgroup barriers per scan, and leave when wavefront
+  // 0 posts kTaskExit after the fold. Wavefront 0 therefore orders its own global memory with a
+  // fence (gsync), never with a workgroup barrier, outside those scans.
+  __shared__ MergeTask tk;
+  auto obsync = [&]() __attribute__((always_inline)) { if (PRE) gsync(); else bsync(); };
+  if (PRE && wid > 0) {
+    for (;;) {
+      __syncthreads();   // the task is posted
+      if (tk.kind == kTaskExit) return;
+      merge_slice<NAR>(tk, wid, S, Lb, Ub, d, d2, cache, zs, szone, P, zdead, lvl_zbase);
+      __syncthreads();   // every slice is done
+    }
+  }
 
   const u32 h = list[blockIdx.x];
   const u32 n = P.ncnt[h];
@@ -1658,7 +1733,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
     cs = (int*)(P.nodes + seg + old_len + n);
   }
   // ordering after a chain write: the chain may be in global memory
-  auto csync = [&]() __attribute__((always_inline)) { if (chg) bsync(); else wsync(); };
+  auto csync = [&]() __attribute__((always_inline)) { if (chg) obsync(); else wsync(); };
   // cache the newest min(old_len, cz) stored zones; the rest of the slots start free
   const int ncached = min((int)old_len, cz);
   for (u32 k = l; k < old_len; k += kWave) {
@@ -1710,8 +1785,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
   const u64 pbase = P.noff[h];
   if (use_pairs)
     for (u32 k = l; k < n; k += kWave) mpos[pbase + k] = -1;
-  bsync();
+  obsync();
 
+  // a member scan of the current successor split over the workgroup's wavefronts (PRE only):
+  // returns the cover flag or the number of members retired
+  auto helped = [&](int kind, int cnt) __attribute__((always_inline)) -> int {
+    wsync();
+    if (l == 0) {
+      tk.kind = kind;
+      tk.n = cnt;
+      tk.flag = 0;
+      tk.kills = 0;
+      tk.wl = wlp;
+      tk.ch = ch;
+      tk.cs = cs;
+    }
+    __syncthreads();   // the task is posted
+    merge_slice<NAR>(tk, 0, S, Lb, Ub, d, d2, cache, zs, szone, P, zdead, lvl_zbase);
+    __syncthreads();   // every slice is done
+    return (kind == kTaskCoverW || kind == kTaskCoverC) ? tk.flag : tk.kills;
+  };
   for (u32 k = 0; k < n; ++k) {
     if (use_pairs && pdrop[pbase + k]) {
       ++n_sub;
@@ -1728,17 +1821,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
       const bool wide = live > 8;
       // (1)+(2) covered by a live stored zone: drop
       bool covered = false;
-      if (pairs_full) {
-        for (int b0 = 0; b0 < nw && !covered; b0 += kWave) {
-          bool c = false;
-          if (b0 + l < nw) {
-            const int m = wlp[b0 + l];
-            if (!(ch[m] & kDead))
-              c = zwith(m, [&](auto Z) { return (PRE && d <= 16) ? alu_le_lane16<false>(S, Z, d, Lb, Ub) : alu_le_lane(S, Z, d, Lb, Ub); });
-          }
-          covered = __any(c);
-        }
-        n_reads += (unsigned long long)nw;
+      if (PRE && (pairs_full || wide)) {
+        covered = helped(pairs_full ? kTaskCoverW : kTaskCoverC, pairs_full ? nw : len) != 0;
+        n_reads += (unsigned long long)(pairs_full ? nw : live);
       } else if (!wide) {
         for (int m = len - 1; m >= 0 && !covered; --m) {
           if (ch[m] & kDead) continue;
@@ -1750,7 +1835,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
           const int m = base - l;
           bool c = false;
           if (m >= 0 && !(ch[m] & kDead))
-            c = zwith(m, [&](auto Z) { return (PRE && d <= 16) ? alu_le_lane16<false>(S, Z, d, Lb, Ub) : alu_le_lane(S, Z, d, Lb, Ub); });
+            c = zwith(m, [&](auto Z) { return alu_le_lane(S, Z, d, Lb, Ub); });
           covered = __any(c);
         }
         n_reads += (unsigned long long)live;
@@ -1760,7 +1845,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
       if (pairs_full) {
         // recorded kills among live unwidened members, then the widened members as they are now
         const int kc = kcnt[pbase + k];
-        if (kc > 0) bsync();   // member positions that lane 0 wrote to global memory
+        if (kc > 0) obsync();   // member positions that lane 0 wrote to global memory
         const u32* kl = kc <= kKillSlots ? kslot + (pbase + k) * kKillSlots : klist + koff[pbase + k];
         for (int b0 = 0; b0 < kc; b0 += kWave) {
           bool kill = false;
@@ -1784,25 +1869,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
           n_kill += __popcll(b);
         }
         n_reads += (unsigned long long)kc;
-        for (int b0 = 0; b0 < nw; b0 += kWave) {
-          bool kw = false;
-          int mw = -1;
-          if (b0 + l < nw) {
-            mw = wlp[b0 + l];
-            if (!(ch[mw] & kDead))
-              kw = zwith(mw, [&](auto Z) { return (PRE && d <= 16) ? alu_le_lane16<true>(Z, S, d, Lb, Ub) : alu_le_lane(Z, S, d, Lb, Ub); });
-          }
-          const u64 bw = __ballot(kw);
-          wsync();
-          if (kw) {
-            note_dead(ch[mw]);
-            ch[mw] |= kDead;
-          }
-          csync();
-          live -= __popcll(bw);
-          n_kill += __popcll(bw);
-        }
-        n_reads += (unsigned long long)nw;
       } else if (!wide) {
         for (int m = len - 1; m >= 0; --m) {
           const u32 id = ch[m];
@@ -1819,12 +1885,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
             ++n_kill;
           }
         }
+      } else if (PRE) {
+        // the chain's live members, in the helped scan below
       } else {
         for (int base = len - 1; base >= 0; base -= kWave) {
           const int m = base - l;
           bool kill = false;
           if (m >= 0 && !(ch[m] & kDead))
-            kill = zwith(m, [&](auto Z) { return (PRE && d <= 16) ? alu_le_lane16<true>(Z, S, d, Lb, Ub) : alu_le_lane(Z, S, d, Lb, Ub); });
+            kill = zwith(m, [&](auto Z) { return alu_le_lane(Z, S, d, Lb, Ub); });
           const u64 b = __ballot(kill);
           wsync();
           if (kill) {
@@ -1836,6 +1904,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
           n_kill += __popcll(b);
         }
         n_reads += (unsigned long long)live;
+      }
+      if (PRE && (pairs_full || wide)) {
+        // the widened members as they are now (pair pre-pass), or every live chain member
+        const int kw = helped(pairs_full ? kTaskKillW : kTaskKillC, pairs_full ? nw : len);
+        live -= kw;
+        n_kill += kw;
+        n_reads += (unsigned long long)(pairs_full ? nw : live);
       }
       // (4) convex union with the kMergeScan newest live zones, restarting after each merge
       for (bool merged = true; merged;) {
@@ -1899,7 +1974,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
       for (int e = l; e < d2; e += kWave) cache[slot * zs + e] = S[e];
     if (widened) {
       for (int e = l; e < d2; e += kWave) sz[e] = S[e];
-      bsync();   // the slot is read back by other lanes when the zone is not cached
+      obsync();   // the slot is read back by other lanes when the zone is not cached
     }
     if (l == 0) {
       ch[len] = kNewTag | k | ((use_pairs && widened) ? kWideTag : 0u);
@@ -1926,14 +2001,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
           // each successor widens at most one member, its own)
           int* gl = wlg + pbase;
           for (int b0 = l; b0 < nw; b0 += kWave) gl[b0] = wl[b0];
-          bsync();
+          obsync();
           wlp = gl;
         }
       }
       if (wlp != wl || nw < ml.wide_cap) {
         if (l == 0) wlp[nw] = len;
         ++nw;
-        if (wlp != wl) bsync();   // every lane reads the list from device memory
+        if (wlp != wl) obsync();   // every lane reads the list from device memory
       } else {
         pairs_full = false;
         n_wovf = 1;
@@ -1943,6 +2018,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 6)
     ++live;
     ++n_ins;
     csync();
+  }
+  if (PRE) {
+    // the helper wavefronts leave
+    wsync();
+    if (l == 0) tk.kind = kTaskExit;
+    __syncthreads();
   }
   // Live new zones go to the level's scratch slot of their work-list position (noff + k), named
   // kNewTag | slot in the chain and in `survived` until k_zassign gives them compact zone ids; the
@@ -3420,7 +3501,8 @@ int search_once(hora_hip_ctx* c, int32_t* reachable, hora_reach_stats* st, int n
                            (int*)nullptr);
       if (pre) c->wlg.ensure(W + 1);
       if (nlh[1] > 0)
-        hipLaunchKernelGGL(pre ? merge_pre : merge_plain, dim3(nlh[1]), dim3(64), lds_h, c->stream, c->dnp, P,
+        hipLaunchKernelGGL(pre ? merge_pre : merge_plain, dim3(nlh[1]), dim3(pre ? 64 * kMergeHelpWaves : 64), lds_h,
+                           c->stream, c->dnp, P,
                            c->kheavy.p, nlh[1], c->wlist.p, c->sj.p, c->s_zone.p, c->survived.p, c->ctr.p, 1,
                            c->zscr.p, c->zused.p, c->flag_i.p, c->pdrop.p, c->kcnt.p, c->kslot.p, c->koff.p,
                            c->klist.p, c->mpos.p, zdead, lvl_zbase, pre ? c->wlg.p : (int*)nullptr);

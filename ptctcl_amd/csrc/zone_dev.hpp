@@ -72,6 +72,14 @@ __device__ __forceinline__ void wsync() {
 // by another, and for multi-wavefront workgroups.
 __device__ __forceinline__ void bsync() { __syncthreads(); }
 
+// Ordering point for global memory between the lanes of one wavefront, without a workgroup
+// barrier (for a wavefront whose workgroup holds other wavefronts that do not take the same path):
+// a workgroup-scope fence, which also waits for the wavefront's outstanding memory operations.
+__device__ __forceinline__ void gsync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 __device__ __forceinline__ void z_copy(int* __restrict__ dst, const int* __restrict__ src, int d2) {
   for (int e = lane_id(); e < d2; e += kWave) dst[e] = src[e];
 }
