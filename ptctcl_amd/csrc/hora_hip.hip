@@ -1919,11 +1919,26 @@ __global__ __launch_bounds__(PRE ? kMergeHelpWaves * 64 : 64) __attribute__((amd
         merged = false;
         wsync();
         // window positions, newest first
-        if (l == 0) {
+        if (!PRE || len <= kWave) {
+          if (l == 0) {
+            int c = 0;
+            for (int m = len - 1; m >= 0 && c < kMergeScan; --m)
+              if (!(ch[m] & kDead)) cand[c++] = m;
+            misc[6] = c;
+          }
+        } else {
+          // a heavy key's long chain: the newest live members, 64 chain positions per step (lane l
+          // takes position base - l, so lower lanes are newer and the ranks keep newest-first order)
           int c = 0;
-          for (int m = len - 1; m >= 0 && c < kMergeScan; --m)
-            if (!(ch[m] & kDead)) cand[c++] = m;
-          misc[6] = c;
+          for (int base = len - 1; base >= 0 && c < kMergeScan; base -= kWave) {
+            const int m = base - l;
+            const bool lv = m >= 0 && !(ch[m] & kDead);
+            const u64 b = __ballot(lv);
+            const int r = c + __popcll(b & ((1ull << l) - 1ull));
+            if (lv && r < kMergeScan) cand[r] = m;
+            c += __popcll(b);
+          }
+          if (l == 0) misc[6] = min(c, kMergeScan);
         }
         wsync();
         const int nc = misc[6];
