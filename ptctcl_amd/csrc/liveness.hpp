@@ -2234,9 +2234,11 @@ int construct_base(hora_hip_ctx* c, LiveState& L, const hora_live_query* q, hora
       moved = true;
     }
     {
-      // the key arrays' pool: a step's appends take at most twice the nodes it ends with, plus a
-      // first array of 8 per new key
-      const size_t pneed = (size_t)hb.ptop + 2 * nneed + 8 * (size_t)R + 64;
+      // the key arrays' pool: in one step a key holding n >= r nodes, r of the step's appends,
+      // doubles at most once (2n <= 2 * nodes over all such keys); a smaller or new key takes
+      // capacities 8, 16, ... summing to less than twice its last, 4 * max(8, n + r) <= 32 + 8r;
+      // so a step takes at most 2 * nodes + 40 * R
+      const size_t pneed = (size_t)hb.ptop + 2 * nneed + 40 * (size_t)R + 64;
       if (pneed > L.bpool.cap) {
         grow_keep_to(L.bpool, std::max(pneed, 2 * (size_t)hb.ptop), c->stream);
         moved = true;
